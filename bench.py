@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark: micrographs/hour of the GisSPA picker on one MI355X.
+
+Workload (the full-size plain-method configuration): one 5760x4092 float32
+micrograph with 300 embedded particles (SNR 0.05), 180x180 templates of a
+60-blob synthetic density at 1649 views of a ~5 degree sphere, 72 in-plane
+steps (5 degrees), defocus 1.8 um, window 720 with overlap 180 (88 tiles).
+One step = one micrograph through the whole hot path: micrograph prep (stats,
+hot-pixel clamp, mean removal, 88 tile FFTs) and, for every psi, template
+rotation + FFT and all 88 x 1649 cross-correlation maps with their peak /
+score reduction and pick selection.  The micrograph is resident in device
+memory when the timed region starts.
+
+Prints one JSON line (rank 0).  --gpus N with N > 1 exits: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Hip:
+    """Minimal HIP runtime access for device-resident inputs."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL("libamdhip64.so")
+
+    def malloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        r = self.lib.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes))
+        if r != 0:
+            raise RuntimeError(f"hipMalloc failed: {r}")
+        return p.value
+
+    def h2d(self, dst: int, arr: np.ndarray) -> None:
+        r = self.lib.hipMemcpy(ctypes.c_void_p(dst), arr.ctypes.data_as(ctypes.c_void_p),
+                               ctypes.c_size_t(arr.nbytes), 1)
+        if r != 0:
+            raise RuntimeError(f"hipMemcpy failed: {r}")
+
+    def sync(self) -> None:
+        self.lib.hipDeviceSynchronize()
+
+    def free(self, p: int) -> None:
+        self.lib.hipFree(ctypes.c_void_p(p))
+
+
+def full_workload(cache_dir: str):
+    from gisspa_amd import synth
+    path = os.path.join(cache_dir, "gisspa_full_seed0.npz")
+    if os.path.exists(path):
+        d = np.load(path)
+        return d["templates"], d["image"], float(d["defocus"])
+    t = time.time()
+    wl = synth.full_workload(seed=0)
+    img, df = wl.micrographs[0]
+    log(f"[bench] generated full workload in {time.time() - t:.1f}s")
+    try:
+        os.makedirs(cache_dir, exist_ok=True)
+        np.savez(path, templates=wl.templates, image=img, defocus=df)
+    except OSError:
+        pass
+    return wl.templates, img, df
+
+
+FULL_PARAMS = dict(angpix=1.0, phistep=5.0, kk=3, energy=300, cs=2.7, Highres=8, Lowres=100,
+                   diameter=135.0, threshold=7.0, window_size=720, overlap=180, ampconst=0.1)
+
+
+def cpu_baseline(templates, image, defocus, nthreads: int, n_views: int, n_psi: int):
+    """FP64 oracle on a bounded sample: n_views views x n_psi psi x all tiles,
+    extrapolated linearly by the CC-map count (work after prep is linear in
+    views x psi x tiles)."""
+    from oracle import oracle as orc
+    p = orc.params(720, **FULL_PARAMS)
+    c = orc.ctf(defocus)
+    sub = np.ascontiguousarray(templates[:n_views])
+    t0 = time.time()
+    st = orc.pick_norm0(p, c, sub, image, 0, n_psi, nthreads=nthreads)
+    dt = time.time() - t0
+    T = st.shape[1]
+    maps = n_views * n_psi * T
+    total = templates.shape[0] * 72 * T
+    sec_per_mic = dt * total / maps
+    return dict(value=3600.0 / sec_per_mic, unit="micrographs/hour", cores=nthreads,
+                kind="port",
+                sample=f"FP64 oracle, {n_views} views x {n_psi} psi x {T} tiles = {maps} CC maps "
+                       f"(incl. template/micrograph prep) in {dt:.1f}s, extrapolated to "
+                       f"{total} maps/micrograph")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--psi-limit", type=int, default=0,
+                    help="exploration only: psi steps per micrograph (0 = all 72)")
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--cpu-psi", type=int, default=1)
+    ap.add_argument("--cache", default=os.environ.get("TMPDIR", "/tmp"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the picker on one GPU; --gpus {args.gpus} is not "
+              f"supported (shard micrographs across processes instead)", file=sys.stderr)
+        sys.exit(2)
+
+    import gisspa_amd
+    templates, image, defocus = full_workload(args.cache)
+    ny, nx = image.shape
+    hip = Hip()
+    d_img = hip.malloc(image.nbytes)
+    hip.h2d(d_img, np.ascontiguousarray(image, dtype=np.float32))
+
+    pk = gisspa_amd.Picker(0, **FULL_PARAMS)
+    if args.batch:
+        pk.set_batch(args.batch)
+    t = time.time()
+    pk.set_templates(templates, defocus)
+    log(f"[bench] template prep {time.time() - t:.2f}s")
+    npsi = 72 if args.psi_limit <= 0 else min(72, args.psi_limit)
+
+    def step():
+        pk.set_micrograph_device(d_img, nx, ny, defocus)
+        return pk.pick(0, npsi)
+
+    for i in range(args.warmup):
+        t = time.time()
+        picks = step()
+        log(f"[bench] warmup {i}: {time.time() - t:.2f}s, {len(picks)} picks")
+    hip.sync()
+    t0 = time.time()
+    for i in range(args.steps):
+        picks = step()
+        log(f"[bench] step {i}: {time.time() - t0:.2f}s cumulative, {len(picks)} picks")
+    hip.sync()
+    dt = time.time() - t0
+    sec_per_step = dt / args.steps
+    sec_per_mic = sec_per_step * (72 / npsi)
+    mph = 3600.0 / sec_per_mic
+    info = pk.info()
+    T = info.tiles_x * info.tiles_y
+    N = info.n_views
+
+    # roofline of the dominant correlation kernel, timed with hipEvents on the
+    # library's stream
+    p1, p2, mpl = pk.time_kernels(0, reps=5)
+    L = info.l
+    H = L // 2 + 1
+    bytes_p1 = 16.0 * L * H * mpl   # template spectrum read + intermediate write
+    bytes_p2 = 8.0 * L * H * mpl    # intermediate read
+    ach1 = bytes_p1 / (p1 * 1e-3) / 1e9
+    ach2 = bytes_p2 / (p2 * 1e-3) / 1e9
+    dom = ("corr_pass1", p1, bytes_p1, ach1) if p1 >= p2 else ("corr_pass2", p2, bytes_p2, ach2)
+    log(f"[bench] pass1 {p1:.3f} ms ({ach1:.0f} GB/s alg), pass2 {p2:.3f} ms ({ach2:.0f} GB/s alg), "
+        f"{mpl} maps/launch")
+    pk.close()
+    hip.free(d_img)
+
+    cpu = None
+    if not args.no_cpu:
+        nthreads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(templates, image, defocus, nthreads, args.cpu_views, args.cpu_psi)
+
+    out = {
+        "metric": "micrographs/hour at 5760x4092, 1650 views x72 psi; batched vs plain",
+        "value": mph,
+        "unit": "micrographs/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec_per_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain full-size: 5760x4092 micrograph, 1649 views x 72 psi, "
+                               "180px templates, window 720 overlap 180 (88 tiles)",
+                   "views": N, "psi": 72, "tiles": T, "window": L,
+                   "psi_timed": npsi, "maps_per_micrograph": N * 72 * T},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[3], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom[3] / HBM_PEAK_GBS, "traffic": None,
+                     "launch_ms": dom[1], "maps_per_launch": mpl},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,251 @@
+"""gisspa_amd -- MI355X-native GisSPA template-matching picker.
+
+Python view of the C ABI in include/gisspa.h (libgisspa.so, HIP kernels for
+gfx950).  The package is plumbing for tests and benchmarks; the product is
+the shared library and the ./main CLI built next to it.  Loading fails loudly
+if the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgisspa.so")
+MAIN_PATH = os.path.join(_HERE, "main")
+
+GS_OK = 0
+GS_ERR_CAPACITY = -6
+
+
+class GsParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("apix", "phi_step", "kk", "energy", "cs", "highres", "lowres", "diameter",
+                 "threshold")] + \
+               [(n, ctypes.c_int) for n in
+                ("window_size", "phase_flip", "overlap", "norm_type", "invert")] + \
+               [(n, ctypes.c_float) for n in
+                ("ampconst", "a", "b", "b2", "bfactor", "bfactor2", "bfactor3")]
+
+
+class GsCtf(ctypes.Structure):
+    _fields_ = [("defocus", ctypes.c_float), ("dfdiff", ctypes.c_float), ("dfang", ctypes.c_float)]
+
+
+class GsParticle(ctypes.Structure):
+    _fields_ = [("psi_index", ctypes.c_int), ("view", ctypes.c_int), ("tile", ctypes.c_int),
+                ("psi", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("score", ctypes.c_float)]
+
+
+class GsInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("l", "n_views", "n_psi", "nx", "ny", "tiles_x", "tiles_y", "box_x", "box_y")]
+
+
+class GsTiming(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("template_prep_ms", "micrograph_prep_ms", "rotate_ms", "corr_pass1_ms",
+                 "corr_pass2_ms", "finalize_ms", "pick_total_ms")] + \
+               [(n, ctypes.c_longlong) for n in
+                ("corr_maps", "corr_pass1_launches", "corr_pass2_launches")]
+
+
+MAPSTAT_DTYPE = np.dtype([("score", "<f4"), ("pos", "<i4"), ("peak", "<f4"), ("sd", "<f4"),
+                          ("sum", "<f8"), ("sum2", "<f8")])
+PARTICLE_DTYPE = np.dtype([("psi_index", "<i4"), ("view", "<i4"), ("tile", "<i4"),
+                           ("psi", "<f4"), ("cx", "<f4"), ("cy", "<f4"), ("score", "<f4")])
+
+_lib = None
+
+
+class GisspaError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        msg = load().gs_strerror(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{what}: {msg} ({code})")
+        self.code = code
+
+
+def load() -> ctypes.CDLL:
+    """Load libgisspa.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `make -C gisspa_amd` (or "
+                           f"__graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.gs_params_default.argtypes = [ctypes.POINTER(GsParams)]
+    lib.gs_params_default.restype = None
+    lib.gs_strerror.argtypes = [ctypes.c_int]
+    lib.gs_strerror.restype = ctypes.c_char_p
+    lib.gs_create.argtypes = [ctypes.POINTER(GsParams), ctypes.c_int, ctypes.POINTER(vp)]
+    lib.gs_set_templates.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(GsCtf)]
+    lib.gs_set_micrograph.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(GsCtf)]
+    lib.gs_set_micrograph_device.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(GsCtf)]
+    lib.gs_pick.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t,
+                            ctypes.POINTER(ctypes.c_size_t)]
+    lib.gs_get_mapstats.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    lib.gs_get_template_real.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.gs_get_tile_spectra.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.gs_get_rotated_spectra.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
+    lib.gs_get_info.argtypes = [vp, ctypes.POINTER(GsInfo)]
+    lib.gs_get_timing.argtypes = [vp, ctypes.POINTER(GsTiming)]
+    lib.gs_set_batch.argtypes = [vp, ctypes.c_int]
+    lib.gs_time_kernels.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
+    lib.gs_destroy.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Entry points declared in include/gisspa.h."""
+    hdr = os.path.join(_HERE, "..", "include", "gisspa.h")
+    import re
+    names = []
+    with open(hdr) as f:
+        for line in f:
+            m = re.match(r"^\s*(?:int|void|const char\*)\s+(gs_\w+)\s*\(", line)
+            if m:
+                names.append(m.group(1))
+    return names
+
+
+def make_params(**kw) -> GsParams:
+    """gs_params from config-key style names (angpix, phistep, Highres, ...)."""
+    p = GsParams()
+    load().gs_params_default(ctypes.byref(p))
+    alias = {"angpix": "apix", "phistep": "phi_step", "Highres": "highres", "Lowres": "lowres",
+             "threshold": "threshold"}
+    for k, v in kw.items():
+        name = alias.get(k, k)
+        if name in ("window_size", "phase_flip", "overlap", "norm_type", "invert"):
+            setattr(p, name, int(v))
+        elif hasattr(p, name):
+            setattr(p, name, float(v))
+    return p
+
+
+def _check(code: int, what: str) -> None:
+    if code != GS_OK:
+        raise GisspaError(code, what)
+
+
+class Picker:
+    """One picking context on one device (gs_create ... gs_destroy)."""
+
+    def __init__(self, device: int = 0, **params):
+        self.lib = load()
+        self.params = make_params(**params)
+        self.ctx = ctypes.c_void_p()
+        _check(self.lib.gs_create(ctypes.byref(self.params), device, ctypes.byref(self.ctx)),
+               "gs_create")
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.gs_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_templates(self, tm: np.ndarray, defocus: float, dfdiff: float = 0.0,
+                      dfang: float = 0.0) -> None:
+        tm = np.ascontiguousarray(tm, dtype=np.float32)
+        n, by, bx = tm.shape
+        c = GsCtf(defocus, dfdiff, dfang)
+        _check(self.lib.gs_set_templates(self.ctx, tm.ctypes.data, n, bx, by, ctypes.byref(c)),
+               "gs_set_templates")
+
+    def set_micrograph(self, img: np.ndarray, defocus: float, dfdiff: float = 0.0,
+                       dfang: float = 0.0) -> None:
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        ny, nx = img.shape
+        c = GsCtf(defocus, dfdiff, dfang)
+        _check(self.lib.gs_set_micrograph(self.ctx, img.ctypes.data, nx, ny, ctypes.byref(c)),
+               "gs_set_micrograph")
+
+    def set_micrograph_device(self, ptr: int, nx: int, ny: int, defocus: float,
+                              dfdiff: float = 0.0, dfang: float = 0.0) -> None:
+        c = GsCtf(defocus, dfdiff, dfang)
+        _check(self.lib.gs_set_micrograph_device(self.ctx, ctypes.c_void_p(ptr), nx, ny,
+                                                 ctypes.byref(c)), "gs_set_micrograph_device")
+
+    def pick(self, psi_begin: int = 0, psi_end: int = -1) -> np.ndarray:
+        n = ctypes.c_size_t(0)
+        cap = 4096
+        while True:
+            out = np.zeros(cap, dtype=PARTICLE_DTYPE)
+            r = self.lib.gs_pick(self.ctx, psi_begin, psi_end, out.ctypes.data, cap, ctypes.byref(n))
+            if r == GS_ERR_CAPACITY and n.value > cap:
+                cap = n.value
+                # the maps are already computed; a second call recomputes them
+                continue
+            _check(r, "gs_pick")
+            return out[: n.value].copy()
+
+    def mapstats(self) -> np.ndarray:
+        n = ctypes.c_size_t(0)
+        self.lib.gs_get_mapstats(self.ctx, None, 0, ctypes.byref(n))
+        out = np.zeros(n.value, dtype=MAPSTAT_DTYPE)
+        _check(self.lib.gs_get_mapstats(self.ctx, out.ctypes.data, n.value, ctypes.byref(n)),
+               "gs_get_mapstats")
+        return out
+
+    def info(self) -> GsInfo:
+        i = GsInfo()
+        _check(self.lib.gs_get_info(self.ctx, ctypes.byref(i)), "gs_get_info")
+        return i
+
+    def timing(self) -> GsTiming:
+        t = GsTiming()
+        _check(self.lib.gs_get_timing(self.ctx, ctypes.byref(t)), "gs_get_timing")
+        return t
+
+    def set_batch(self, n: int) -> None:
+        _check(self.lib.gs_set_batch(self.ctx, n), "gs_set_batch")
+
+    def time_kernels(self, psi_index: int = 0, reps: int = 10):
+        """(pass1_ms, pass2_ms, maps_per_launch): mean device time of one launch."""
+        ms = np.zeros(2, dtype=np.float64)
+        n = ctypes.c_int(0)
+        _check(self.lib.gs_time_kernels(self.ctx, psi_index, reps, ms.ctypes.data, ctypes.byref(n)),
+               "gs_time_kernels")
+        return float(ms[0]), float(ms[1]), n.value
+
+    def template_real(self) -> np.ndarray:
+        i = self.info()
+        out = np.zeros((i.n_views, i.l, i.l), dtype=np.float32)
+        _check(self.lib.gs_get_template_real(self.ctx, out.ctypes.data, out.size),
+               "gs_get_template_real")
+        return out
+
+    def tile_spectra(self) -> np.ndarray:
+        """[tile][kx][ky] complex64 half spectra."""
+        i = self.info()
+        T = i.tiles_x * i.tiles_y
+        out = np.zeros((T, i.l // 2 + 1, i.l), dtype=np.complex64)
+        _check(self.lib.gs_get_tile_spectra(self.ctx, out.ctypes.data, out.size * 2),
+               "gs_get_tile_spectra")
+        return out
+
+    def rotated_spectra(self, psi_index: int) -> np.ndarray:
+        i = self.info()
+        out = np.zeros((i.n_views, i.l // 2 + 1, i.l), dtype=np.complex64)
+        _check(self.lib.gs_get_rotated_spectra(self.ctx, psi_index, out.ctypes.data, out.size * 2),
+               "gs_get_rotated_spectra")
+        return out

@@ -1,0 +1,649 @@
+// gs_api.cpp -- context and C ABI of libgisspa (see include/gisspa.h).
+//
+// Host orchestration of the picking path.  Replaces the free functions of the
+// reference's main.cu (Autumn1998/GisSPA) with one context per device:
+// buffers are allocated once, every stage runs asynchronously on the
+// context's stream, and the only host synchronisations are the final copy of
+// the picks (the reference synchronises and reduces on the host once per tile
+// per psi, main.cu:852-866).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gisspa.h"
+#include "gs_kernels.hpp"
+
+using namespace gs;
+
+namespace {
+
+const double kRefPiD = 3.141592654;  // GPU_func.cuh:29
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int ensure(size_t n) {
+    if (n <= bytes) return GS_OK;
+    release();
+    if (n == 0) return GS_OK;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return GS_ERR_NOMEM;
+    }
+    bytes = n;
+    return GS_OK;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct SelRec {
+  long long m;
+  float cx, cy, score, pad;
+};
+
+struct Ctx {
+  gs_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int L = 0, L_config = 0;
+  Geometry geo{};
+  DevBuf twL, twM;
+  // templates
+  int N = 0, bx = 0, by = 0;
+  gs_ctf tctf{};
+  DevBuf treal, tspec, rot_part, bscale;
+  // micrograph
+  int nx = 0, ny = 0;
+  bool have_mic = false;
+  DevBuf img, img_part, img_stats, img_mean, offx_d, offy_d, tiles;
+  std::vector<int> offx, offy;
+  int bxn = 0, byn = 0, T = 0;
+  // correlation
+  int batch = 0;
+  DevBuf G, part, maps, stats, sel, sel_count;
+  int stats_psi_begin = 0, stats_npsi = 0;
+  gs_timing timing{};
+};
+
+int hip_ok(hipError_t e) {
+  if (e == hipSuccess) return GS_OK;
+  (void)hipGetLastError();
+  return GS_ERR_DEVICE;
+}
+#define HIPCHK(x)                       \
+  do {                                  \
+    int _r = hip_ok(x);                 \
+    if (_r != GS_OK) return _r;         \
+  } while (0)
+#define CHK(x)                          \
+  do {                                  \
+    int _r = (x);                       \
+    if (_r != GS_OK) return _r;         \
+  } while (0)
+
+int tile_offsets(int n, int l, int overlap, std::vector<int>& offs) {
+  // main.cu:228-250
+  const int interval = l - overlap;
+  if (interval <= 0) return GS_ERR_ARG;
+  offs.clear();
+  int off = 0;
+  offs.push_back(0);
+  while (off + l < n) {
+    off += interval;
+    if (off + l >= n) off = n - l;
+    offs.push_back(off);
+  }
+  return GS_OK;
+}
+
+float psi_value(double phi_step, int index) {
+  float e = 0.0f;  // float accumulator, main.cu:1230
+  for (int i = 0; i < index; ++i) e += (float)phi_step;
+  return e;
+}
+
+int num_psi(double phi_step) {
+  int n = 0;
+  for (float e = 0.0f; e < 360.0f; e += (float)phi_step) n++;
+  return n;
+}
+
+int upload_twiddles(Ctx* c) {
+  const int L = c->L, M = L / 2;
+  std::vector<float2> tl(L), tm(M);
+  for (int i = 0; i < L; ++i) {
+    const double a = -2.0 * M_PI * i / L;
+    tl[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  for (int i = 0; i < M; ++i) {
+    const double a = -2.0 * M_PI * i / M;
+    tm[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  CHK(c->twL.ensure(L * sizeof(float2)));
+  CHK(c->twM.ensure(M * sizeof(float2)));
+  HIPCHK(hipMemcpy(c->twL.p, tl.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->twM.p, tm.data(), M * sizeof(float2), hipMemcpyHostToDevice));
+  return GS_OK;
+}
+
+int set_window(Ctx* c, int L) {
+  if (!geometry_supported(L)) return GS_ERR_UNSUPPORTED;
+  if (L != c->L) {
+    c->L = L;
+    c->geo = geometry_of(L);
+    CHK(upload_twiddles(c));
+  }
+  return GS_OK;
+}
+
+WeightArgs weight_args(const Ctx* c, const gs_ctf& ctf) {
+  WeightArgs w{};
+  const gs_params& p = c->p;
+  w.apix = p.apix; w.highres = p.highres; w.lowres = p.lowres; w.kk = p.kk;
+  w.ds = (float)(1.0 / (p.apix * (double)c->L_config));  // main.cu:155 (before adjustment)
+  w.a = p.a; w.b = p.b; w.b2 = p.b2; w.bf = p.bfactor; w.bf2 = p.bfactor2; w.bf3 = p.bfactor3;
+  const float defocus = -ctf.defocus;  // negated at parse, DataReader.cpp:378
+  w.dfu = defocus + ctf.dfdiff;
+  w.dfv = defocus - ctf.dfdiff;
+  w.dfdiff = ctf.dfdiff;
+  w.dfang = ctf.dfang;
+  w.lambda = (float)(12.2639 / std::sqrt(p.energy * 1000.0 + 0.97845 * p.energy * p.energy));
+  w.cs = p.cs;
+  w.ampconst = p.ampconst;
+  return w;
+}
+
+int prep_templates(Ctx* c, const float* host, int n, int bx, int by) {
+  const int L = c->L, H = c->geo.H, HP = c->geo.HP;
+  hipStream_t s = c->stream;
+  const long L2 = (long)L * L;
+  DevBuf box, sums, scale, ra, rbins, wpart, binstart, binpix, gtmp;
+  CHK(box.ensure((size_t)n * bx * by * sizeof(float)));
+  HIPCHK(hipMemcpyAsync(box.p, host, (size_t)n * bx * by * sizeof(float), hipMemcpyHostToDevice, s));
+  CHK(sums.ensure((size_t)n * 2 * sizeof(double)));
+  CHK(scale.ensure((size_t)n * sizeof(float)));
+  launch_box_sums(box.as<float>(), bx * by, n, sums.as<double>(), s);
+  launch_sigma_scale(sums.as<double>(), n, L, scale.as<float>(), s);
+  CHK(c->tspec.ensure((size_t)n * H * L * sizeof(float2)));
+  float2* spec = c->tspec.as<float2>();
+  launch_rows_r2c_pad(L, box.as<float>(), bx, by, scale.as<float>(), n, spec, c->twL.as<float2>(),
+                      c->twM.as<float2>(), s);
+  launch_cols_fwd(L, spec, n, (float)(1.0 / (double)L2), nullptr, 0, c->twL.as<float2>(), s);
+  // radial bins of the half plane x <= l/2 (SQRSum_by_circle, GPU_func.cu:98-128)
+  const int NBIN = L / 2;
+  std::vector<std::vector<int>> bins(NBIN);
+  for (int kx = 0; kx <= L / 2; ++kx)
+    for (int ky = 0; ky < L; ++ky) {
+      const double dy = std::min(ky, L - ky), dx = std::min(kx, L - kx);
+      const int r = (int)std::floor(std::hypot(dy, dx) + 0.5) - 1;
+      if (r >= 0 && r < NBIN) bins[r].push_back(kx * L + ky);
+    }
+  std::vector<int> bstart(NBIN + 1, 0), bpix;
+  std::vector<float> rcount(NBIN);
+  double count_total = 0;
+  for (int r = 0; r < NBIN; ++r) {
+    bstart[r + 1] = bstart[r] + (int)bins[r].size();
+    bpix.insert(bpix.end(), bins[r].begin(), bins[r].end());
+    rcount[r] = (float)bins[r].size();
+    count_total += bins[r].size();
+  }
+  CHK(binstart.ensure(bstart.size() * sizeof(int)));
+  CHK(binpix.ensure(std::max<size_t>(1, bpix.size()) * sizeof(int)));
+  CHK(rbins.ensure(NBIN * sizeof(float)));
+  HIPCHK(hipMemcpyAsync(binstart.p, bstart.data(), bstart.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(binpix.p, bpix.data(), bpix.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(rbins.p, rcount.data(), NBIN * sizeof(float), hipMemcpyHostToDevice, s));
+  CHK(ra.ensure((size_t)n * NBIN * sizeof(double)));
+  launch_radial_power(L, spec, n, binstart.as<int>(), binpix.as<int>(), ra.as<double>(), s);
+  const long tot = (long)H * L;
+  const int nwp = (int)((tot + 255) / 256);
+  CHK(wpart.ensure((size_t)n * nwp * sizeof(double)));
+  launch_tmpl_weight(L, spec, n, ra.as<double>(), rbins.as<float>(), weight_args(c, c->tctf),
+                     wpart.as<double>(), s);
+  launch_tmpl_ampnorm(L, spec, n, wpart.as<double>(), count_total, s);
+  // inverse 2-D transform to the real-space templates, in chunks
+  CHK(c->treal.ensure((size_t)n * L2 * sizeof(float)));
+  const int chunk = std::max(1, std::min(n, (int)((512l << 20) / ((long)L * HP * 8))));
+  CHK(gtmp.ensure((size_t)chunk * L * HP * sizeof(float2)));
+  for (int b0 = 0; b0 < n; b0 += chunk) {
+    const int nb = std::min(chunk, n - b0);
+    launch_cols_inv_T(L, spec + (size_t)b0 * H * L, nb, gtmp.as<float2>(), c->twL.as<float2>(), s);
+    launch_rows_c2r(L, gtmp.as<float2>(), nb, c->treal.as<float>() + (size_t)b0 * L2,
+                    c->twL.as<float2>(), c->twM.as<float2>(), s);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipGetLastError());
+  return GS_OK;
+}
+
+int prep_micrograph(Ctx* c, const float* src, bool src_is_device, int nx, int ny) {
+  hipStream_t s = c->stream;
+  const long n = (long)nx * ny;
+  const int L = c->L;
+  if (!c->have_mic) {
+    CHK(tile_offsets(nx, L, c->p.overlap, c->offx));
+    CHK(tile_offsets(ny, L, c->p.overlap, c->offy));
+    c->bxn = (int)c->offx.size();
+    c->byn = (int)c->offy.size();
+    c->T = c->bxn * c->byn;
+    c->nx = nx;
+    c->ny = ny;
+    CHK(c->offx_d.ensure(c->offx.size() * sizeof(int)));
+    CHK(c->offy_d.ensure(c->offy.size() * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(c->offx_d.p, c->offx.data(), c->offx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->offy_d.p, c->offy.data(), c->offy.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  } else if (nx != c->nx || ny != c->ny) {
+    return GS_ERR_ARG;  // the reference silently assumes equal sizes (main.cu:1108)
+  }
+  CHK(c->img.ensure(n * sizeof(float)));
+  HIPCHK(hipMemcpyAsync(c->img.p, src, n * sizeof(float),
+                        src_is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  const int nblk = img_reduce_blocks(n);
+  CHK(c->img_part.ensure((size_t)nblk * 2 * sizeof(double)));
+  CHK(c->img_stats.ensure(8 * sizeof(double)));
+  CHK(c->img_mean.ensure(sizeof(double)));
+  launch_img_sums(c->img.as<float>(), n, c->img_part.as<double>(), nblk, s);
+  launch_img_bounds(c->img_part.as<double>(), nblk, n, c->img_stats.as<double>(), s);
+  launch_img_clamp(c->img.as<float>(), n, c->p.invert, c->img_stats.as<double>(),
+                   c->img_part.as<double>(), nblk, s);
+  launch_img_mean(c->img_part.as<double>(), nblk, n, c->img_mean.as<double>(), s);
+  const int H = c->geo.H;
+  CHK(c->tiles.ensure((size_t)c->T * H * L * sizeof(float2)));
+  launch_rows_r2c_tiles(L, c->img.as<float>(), nx, ny, c->offx_d.as<int>(), c->offy_d.as<int>(),
+                        c->bxn, c->T, c->img_mean.as<double>(), c->tiles.as<float2>(),
+                        c->twL.as<float2>(), c->twM.as<float2>(), s);
+  launch_cols_fwd(L, c->tiles.as<float2>(), c->T, (float)(1.0 / ((double)L * L)), nullptr, 0,
+                  c->twL.as<float2>(), s);
+  // the caller may release the host image on return
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipGetLastError());
+  c->have_mic = true;
+  return GS_OK;
+}
+
+// device -> host copy ordered after all work queued on the context stream
+int copy_out(Ctx* c, void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GS_OK;
+}
+
+int rotate_templates(Ctx* c, int psi_index) {
+  const int L = c->L;
+  const float e = psi_value(c->p.phi_step, psi_index);
+  // rotate_subIMG computes cos/sin in double and stores float (GPU_func.cu:498-499)
+  const float cose = (float)std::cos((double)e * kRefPiD / 180);
+  const float sine = (float)std::sin((double)e * kRefPiD / 180);
+  const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
+  CHK(c->rot_part.ensure((size_t)c->N * nrb * 2 * sizeof(double)));
+  CHK(c->bscale.ensure((size_t)c->N * sizeof(float)));
+  launch_rows_r2c_rot(L, c->treal.as<float>(), c->N, cose, sine, 1.5f * c->p.diameter,
+                      c->tspec.as<float2>(), c->rot_part.as<double>(), c->twL.as<float2>(),
+                      c->twM.as<float2>(), c->stream);
+  launch_rot_scale(L, c->rot_part.as<double>(), c->N, c->bscale.as<float>(), c->stream);
+  launch_cols_fwd(L, c->tspec.as<float2>(), c->N, 1.0f, c->bscale.as<float>(), 1,
+                  c->twL.as<float2>(), c->stream);
+  return GS_OK;
+}
+
+int ensure_corr_buffers(Ctx* c) {
+  const int L = c->L, HP = c->geo.HP;
+  const long per_map = (long)L * HP * sizeof(float2);
+  if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (256l << 20) / per_map));
+  const int nb = std::min(c->batch, c->T * c->N);
+  CHK(c->G.ensure((size_t)nb * per_map));
+  const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
+  CHK(c->part.ensure((size_t)nb * nrb * sizeof(CorrPartial)));
+  // canonical map order: tile-major, view-minor
+  const size_t nm = (size_t)c->T * c->N;
+  if (c->maps.bytes < nm * sizeof(MapRef)) {
+    std::vector<MapRef> mr(nm);
+    for (int t = 0; t < c->T; ++t)
+      for (int j = 0; j < c->N; ++j) mr[(size_t)t * c->N + j] = MapRef{t, j};
+    CHK(c->maps.ensure(nm * sizeof(MapRef)));
+    HIPCHK(hipMemcpyAsync(c->maps.p, mr.data(), nm * sizeof(MapRef), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));  // mr is released on return
+  }
+  return GS_OK;
+}
+
+int correlate_psi(Ctx* c, MapStat* out, hipEvent_t e_mid, hipEvent_t e_end) {
+  const int L = c->L;
+  const int nm = c->T * c->N;
+  const int nb = std::min(c->batch, nm);
+  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;  // compute_corner_CCG `of` rule
+  const float lo = c->p.diameter / 4.0f;
+  const float hi = (float)L - c->p.diameter / 4.0f;
+  for (int m0 = 0; m0 < nm; m0 += nb) {
+    const int cnt = std::min(nb, nm - m0);
+    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>() + m0,
+                      cnt, sgn, c->G.as<float2>(), c->twL.as<float2>(), c->stream);
+    launch_corr_pass2(L, c->G.as<float2>(), cnt, lo, hi, c->part.as<CorrPartial>(),
+                      c->twL.as<float2>(), c->twM.as<float2>(), c->stream);
+    launch_corr_finalize(L, c->part.as<CorrPartial>(), cnt, out + m0, c->stream);
+    c->timing.corr_pass1_launches++;
+    c->timing.corr_pass2_launches++;
+  }
+  c->timing.corr_maps += nm;
+  (void)e_mid;
+  (void)e_end;
+  return GS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// selection kernel lives here (host-side record layout)
+// ---------------------------------------------------------------------------
+__global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T, int N, int L,
+                         const int* __restrict__ offx, const int* __restrict__ offy, int bxn,
+                         float thres, float dmin, int nx, int ny, int* __restrict__ count,
+                         SelRec* __restrict__ out, int cap) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmaps) return;
+  const MapStat s = st[m];
+  if (!(s.score >= thres)) return;
+  const long long rem = m % ((long long)T * N);
+  const int t = (int)(rem / N);
+  const float cx = (float)(offx[t % bxn] + s.pos % L);
+  const float cy = (float)(offy[t / bxn] + s.pos / L);
+  if (cx >= dmin && cx < nx - dmin && cy >= dmin && cy < ny - dmin) {
+    const int i = atomicAdd(count, 1);
+    if (i < cap) {
+      SelRec r;
+      r.m = m; r.cx = cx; r.cy = cy; r.score = s.score; r.pad = 0;
+      out[i] = r;
+    }
+  }
+}
+
+extern "C" {
+
+void gs_params_default(gs_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->apix = -1; p->phi_step = -1; p->kk = -1; p->energy = -1; p->cs = -1;
+  p->highres = -1; p->lowres = -1; p->diameter = -1;
+  p->threshold = 7; p->window_size = 320; p->phase_flip = 0; p->overlap = -1;
+  p->norm_type = 0; p->invert = 0; p->ampconst = 0.07f;
+  p->a = -9.32f; p->b = 2.65f; p->b2 = 0.01908f;
+  p->bfactor = -78.7757f; p->bfactor2 = -12.9121f; p->bfactor3 = 1.28732f;
+}
+
+const char* gs_strerror(int code) {
+  switch (code) {
+    case GS_OK: return "ok";
+    case GS_ERR_ARG: return "invalid argument";
+    case GS_ERR_UNSUPPORTED: return "unsupported configuration";
+    case GS_ERR_DEVICE: return "HIP runtime error";
+    case GS_ERR_NOMEM: return "device memory allocation failed";
+    case GS_ERR_STATE: return "templates or micrograph not set";
+    case GS_ERR_CAPACITY: return "output buffer too small";
+    default: return "unknown error";
+  }
+}
+
+int gs_create(const gs_params* p, int device, void** out) {
+  if (!p || !out) return GS_ERR_ARG;
+  *out = nullptr;
+  if (p->apix <= 0 || p->phi_step <= 0 || p->window_size <= 0 || p->diameter < 0 ||
+      p->highres <= 0 || p->lowres <= 0)
+    return GS_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    (void)hipGetLastError();
+    return GS_ERR_DEVICE;
+  }
+  HIPCHK(hipSetDevice(device));
+  Ctx* c = new (std::nothrow) Ctx();
+  if (!c) return GS_ERR_NOMEM;
+  c->p = *p;
+  c->device = device;
+  c->L_config = p->window_size;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GS_ERR_DEVICE;
+  }
+  *out = c;
+  return GS_OK;
+}
+
+int gs_destroy(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return GS_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GS_OK;
+}
+
+int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const gs_ctf* tctf) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !host || n <= 0 || bx <= 0 || by <= 0 || !tctf) return GS_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  int L = c->p.window_size;
+  if (L < bx || L < by) {  // adjustPaddingSize, main.cu:158-178
+    L = 0;
+    while (L < bx || L < by) L += 32;
+  }
+  CHK(set_window(c, L));
+  c->N = n; c->bx = bx; c->by = by; c->tctf = *tctf;
+  c->have_mic = false;  // tiles depend on the window
+  return prep_templates(c, host, n, bx, by);
+}
+
+static int set_mic(Ctx* c, const float* src, bool dev, int nx, int ny, const gs_ctf* ctf) {
+  if (!c || !src || nx <= 0 || ny <= 0 || !ctf) return GS_ERR_ARG;
+  if (c->N == 0) return GS_ERR_STATE;
+  if (c->p.phase_flip != 0) return GS_ERR_UNSUPPORTED;
+  HIPCHK(hipSetDevice(c->device));
+  return prep_micrograph(c, src, dev, nx, ny);
+}
+
+int gs_set_micrograph(void* ctx, const float* host, int nx, int ny, const gs_ctf* ctf) {
+  return set_mic(static_cast<Ctx*>(ctx), host, false, nx, ny, ctf);
+}
+int gs_set_micrograph_device(void* ctx, const float* dev, int nx, int ny, const gs_ctf* ctf) {
+  return set_mic(static_cast<Ctx*>(ctx), dev, true, nx, ny, ctf);
+}
+
+int gs_set_batch(void* ctx, int maps_per_batch) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || maps_per_batch < 0) return GS_ERR_ARG;
+  c->batch = maps_per_batch;
+  c->G.release();
+  c->part.release();
+  return GS_OK;
+}
+
+int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap, size_t* n_out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !n_out) return GS_ERR_ARG;
+  *n_out = 0;
+  if (c->N == 0 || !c->have_mic) return GS_ERR_STATE;
+  if (c->p.norm_type != 0) return GS_ERR_UNSUPPORTED;
+  HIPCHK(hipSetDevice(c->device));
+  const int P = num_psi(c->p.phi_step);
+  if (psi_end < 0 || psi_end > P) psi_end = P;
+  if (psi_begin < 0 || psi_begin > psi_end) return GS_ERR_ARG;
+  const int npsi = psi_end - psi_begin;
+  const long long per_psi = (long long)c->T * c->N;
+  CHK(ensure_corr_buffers(c));
+  CHK(c->stats.ensure((size_t)std::max<long long>(1, npsi * per_psi) * sizeof(MapStat)));
+  c->stats_psi_begin = psi_begin;
+  c->stats_npsi = npsi;
+  c->timing.corr_maps = 0;
+  c->timing.corr_pass1_launches = 0;
+  c->timing.corr_pass2_launches = 0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int ip = psi_begin; ip < psi_end; ++ip) {
+    CHK(rotate_templates(c, ip));
+    CHK(correlate_psi(c, c->stats.as<MapStat>() + (size_t)(ip - psi_begin) * per_psi, nullptr,
+                      nullptr));
+  }
+  // device-side selection, then canonical order on the host
+  const long long nmaps = npsi * per_psi;
+  int capsel = (int)std::min<long long>(std::max<long long>(nmaps, 1), 1 << 22);
+  CHK(c->sel.ensure((size_t)capsel * sizeof(SelRec)));
+  CHK(c->sel_count.ensure(sizeof(int)));
+  HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), c->stream));
+  if (nmaps > 0)
+    hipLaunchKernelGGL(k_select, dim3((unsigned)((nmaps + 255) / 256)), dim3(256), 0, c->stream,
+                       c->stats.as<MapStat>(), nmaps, c->T, c->N, c->L, c->offx_d.as<int>(),
+                       c->offy_d.as<int>(), c->bxn, c->p.threshold, c->p.diameter, c->nx, c->ny,
+                       c->sel_count.as<int>(), c->sel.as<SelRec>(), capsel);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  int count = 0;
+  HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipGetLastError());
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  c->timing.pick_total_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (count > capsel) return GS_ERR_CAPACITY;
+  std::vector<SelRec> recs(count);
+  if (count)
+    CHK(copy_out(c, recs.data(), c->sel.p, (size_t)count * sizeof(SelRec)));
+  std::sort(recs.begin(), recs.end(), [](const SelRec& a, const SelRec& b) { return a.m < b.m; });
+  *n_out = (size_t)count;
+  if ((size_t)count > cap || (count && !out)) return GS_ERR_CAPACITY;
+  for (int i = 0; i < count; ++i) {
+    const long long m = recs[i].m;
+    const int ip = (int)(m / per_psi) + psi_begin;
+    const long long rem = m % per_psi;
+    gs_particle& pk = out[i];
+    pk.psi_index = ip;
+    pk.tile = (int)(rem / c->N);
+    pk.view = (int)(rem % c->N);
+    float cur_e = 360.0f - psi_value(c->p.phi_step, ip);  // main.cu:1253-1254
+    if (cur_e >= 360) cur_e -= 360;
+    pk.psi = cur_e;
+    pk.cx = recs[i].cx;
+    pk.cy = recs[i].cy;
+    pk.score = recs[i].score;
+  }
+  return GS_OK;
+}
+
+int gs_get_mapstats(void* ctx, gs_mapstat* out, size_t cap, size_t* n_out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !n_out) return GS_ERR_ARG;
+  const size_t n = (size_t)c->stats_npsi * c->T * c->N;
+  *n_out = n;
+  if (cap < n || (n && !out)) return GS_ERR_CAPACITY;
+  static_assert(sizeof(gs_mapstat) == sizeof(MapStat), "layout");
+  HIPCHK(hipSetDevice(c->device));
+  if (n) CHK(copy_out(c, out, c->stats.p, n * sizeof(MapStat)));
+  return GS_OK;
+}
+
+int gs_get_template_real(void* ctx, float* out, size_t cap) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !out) return GS_ERR_ARG;
+  const size_t n = (size_t)c->N * c->L * c->L;
+  if (c->N == 0) return GS_ERR_STATE;
+  if (cap < n) return GS_ERR_CAPACITY;
+  HIPCHK(hipSetDevice(c->device));
+  CHK(copy_out(c, out, c->treal.p, n * sizeof(float)));
+  return GS_OK;
+}
+
+int gs_get_tile_spectra(void* ctx, float* out, size_t cap) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !out) return GS_ERR_ARG;
+  if (!c->have_mic) return GS_ERR_STATE;
+  const size_t n = (size_t)c->T * c->geo.H * c->L * 2;
+  if (cap < n) return GS_ERR_CAPACITY;
+  HIPCHK(hipSetDevice(c->device));
+  CHK(copy_out(c, out, c->tiles.p, n * sizeof(float)));
+  return GS_OK;
+}
+
+int gs_get_rotated_spectra(void* ctx, int psi_index, float* out, size_t cap) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !out || psi_index < 0) return GS_ERR_ARG;
+  if (c->N == 0) return GS_ERR_STATE;
+  const size_t n = (size_t)c->N * c->geo.H * c->L * 2;
+  if (cap < n) return GS_ERR_CAPACITY;
+  HIPCHK(hipSetDevice(c->device));
+  CHK(rotate_templates(c, psi_index));
+  CHK(copy_out(c, out, c->tspec.p, n * sizeof(float)));
+  return GS_OK;
+}
+
+int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* maps_per_launch) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !out_ms || !maps_per_launch || reps <= 0 || psi_index < 0) return GS_ERR_ARG;
+  if (c->N == 0 || !c->have_mic) return GS_ERR_STATE;
+  HIPCHK(hipSetDevice(c->device));
+  CHK(ensure_corr_buffers(c));
+  CHK(rotate_templates(c, psi_index));
+  const int L = c->L;
+  const int nb = std::min(c->batch, c->T * c->N);
+  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;
+  const float lo = c->p.diameter / 4.0f, hi = (float)L - c->p.diameter / 4.0f;
+  hipEvent_t e[3];
+  for (auto& x : e) HIPCHK(hipEventCreate(&x));
+  HIPCHK(hipEventRecord(e[0], c->stream));
+  for (int i = 0; i < reps; ++i)
+    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>(), nb,
+                      sgn, c->G.as<float2>(), c->twL.as<float2>(), c->stream);
+  HIPCHK(hipEventRecord(e[1], c->stream));
+  for (int i = 0; i < reps; ++i)
+    launch_corr_pass2(L, c->G.as<float2>(), nb, lo, hi, c->part.as<CorrPartial>(),
+                      c->twL.as<float2>(), c->twM.as<float2>(), c->stream);
+  HIPCHK(hipEventRecord(e[2], c->stream));
+  HIPCHK(hipEventSynchronize(e[2]));
+  HIPCHK(hipGetLastError());
+  float a = 0, b = 0;
+  HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+  HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+  for (auto& x : e) (void)hipEventDestroy(x);
+  out_ms[0] = a / reps;
+  out_ms[1] = b / reps;
+  *maps_per_launch = nb;
+  return GS_OK;
+}
+
+int gs_get_info(void* ctx, gs_info* info) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !info) return GS_ERR_ARG;
+  info->l = c->L;
+  info->n_views = c->N;
+  info->n_psi = num_psi(c->p.phi_step);
+  info->nx = c->nx;
+  info->ny = c->ny;
+  info->tiles_x = c->bxn;
+  info->tiles_y = c->byn;
+  info->box_x = c->bx;
+  info->box_y = c->by;
+  return GS_OK;
+}
+
+int gs_get_timing(void* ctx, gs_timing* t) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !t) return GS_ERR_ARG;
+  *t = c->timing;
+  return GS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,917 @@
+// gs_kernels.hip -- HIP kernels of the MI355X GisSPA picker (gfx950, wave64).
+//
+// Reference (Autumn1998/GisSPA) correspondence is cited per kernel.  All
+// reductions are deterministic: fixed thread-to-element assignment, fixed
+// shuffle trees, FP64 where sums cross workgroups, no float atomics.
+#include <climits>
+#include <cmath>
+
+#include "gs_fft.hpp"
+#include "gs_kernels.hpp"
+
+namespace gs {
+
+constexpr int WG = 256;
+constexpr float kRefPi = 3.141592654f;  // the reference's PI (GPU_func.cuh:29)
+
+constexpr int pow2floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <int L>
+struct Geo {
+  static constexpr int M = L / 2;
+  static constexpr int H = M + 1;
+  static constexpr int HP = (H + 15) / 16 * 16;
+  static constexpr int RB = clampi(pow2floor(8192 / M), 8, 32);
+  static constexpr int CB = clampi(pow2floor(8192 / L), 8, 32);
+  static_assert(L % 2 == 0, "even tile size");
+};
+
+// ---------------------------------------------------------------------------
+// wave / block reductions (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// (value, index) max with lowest index on ties
+__device__ __forceinline__ void vmax_combine(float& v, int& p, float v2, int p2) {
+  if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
+}
+__device__ __forceinline__ void wave_max_idx(float& v, int& p) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float v2 = __shfl_xor(v, o, 64);
+    int p2 = __shfl_xor(p, o, 64);
+    vmax_combine(v, p, v2, p2);
+  }
+}
+
+// block-wide FP64 sum of two values; result valid in thread 0
+__device__ __forceinline__ void block_sum2_d(double& a, double& b, double* scratch) {
+  a = wave_sum_d(a);
+  b = wave_sum_d(b);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) { scratch[2 * w] = a; scratch[2 * w + 1] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0, sb = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) { sa += scratch[2 * i]; sb += scratch[2 * i + 1]; }
+    a = sa; b = sb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// real-FFT pre/post processing
+// ---------------------------------------------------------------------------
+// X[k] from Z = DFT_M(x[2m] + i x[2m+1]):  X[k] = E + W^k O,
+// E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / (2i), W = e^{-2 pi i / L}
+__device__ __forceinline__ float2 r2c_post(float2 zk, float2 zmk, float2 wk) {
+  float2 e = make_float2(0.5f * (zk.x + zmk.x), 0.5f * (zk.y - zmk.y));
+  float2 d = make_float2(zk.x - zmk.x, zk.y + zmk.y);  // Z[k] - conj Z[M-k]
+  float2 o = make_float2(0.5f * d.y, -0.5f * d.x);       // d / (2i)
+  return cadd(e, cmul(wk, o));
+}
+// Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) e^{+2 pi i k / L}
+__device__ __forceinline__ float2 c2r_pre(float2 xk, float2 xmk, float2 wk_conj) {
+  float2 s = make_float2(xk.x + xmk.x, xk.y - xmk.y);
+  float2 d = make_float2(xk.x - xmk.x, xk.y + xmk.y);
+  float2 t = cmul(d, wk_conj);
+  return make_float2(s.x - t.y, s.y + t.x);
+}
+
+// ---------------------------------------------------------------------------
+// Row R2C: rows y0..y0+RB-1 of image b -> columns [b][kx][y] (transposed)
+// ---------------------------------------------------------------------------
+enum { SRC_PAD = 0, SRC_TILE = 1, SRC_ROT = 2 };
+
+struct RowSrc {
+  // SRC_PAD
+  const float* box;
+  int bx, by;
+  const float* scale;
+  // SRC_TILE
+  const float* img;
+  int nx, ny;
+  const int* offx;
+  const int* offy;
+  int bxn;
+  const double* mean;
+  // SRC_ROT
+  const float* treal;
+  float cose, sine, dmask;
+  double* partials;
+};
+
+template <int L, int SRC>
+__device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x, float aux) {
+  if constexpr (SRC == SRC_PAD) {
+    const int sx = (L - s.bx) / 2, sy = (L - s.by) / 2;
+    const int i = x - sx, j = y - sy;
+    if (i < 0 || i >= s.bx || j < 0 || j >= s.by) return 0.0f;
+    return s.box[((long)b * s.by + j) * s.bx + i] * aux;
+  } else if constexpr (SRC == SRC_TILE) {
+    const int ix = b % s.bxn, iy = b / s.bxn;
+    const int gx = s.offx[ix] + x, gy = s.offy[iy] + y;
+    if (gx >= s.nx || gy >= s.ny) return 0.0f;
+    return (float)((double)s.img[(long)gy * s.nx + gx] - (double)aux);
+  } else {
+    // rotate_subIMG (GPU_func.cu:496-560) then apply_mask (GPU_func.cu:229-247)
+    const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
+    const float r = hypotf(xf, yf);
+    const float dm2 = s.dmask * 0.5f;  // d_m = 1.5 * diameter, radius d_m / 2
+    const float ehw = 6.0f;
+    if (r > dm2 + 2 * ehw) return 0.0f;
+    const float x2 = (s.cose * xf + s.sine * yf) + (float)(L / 2);
+    const float y2 = (-s.sine * xf + s.cose * yf) + (float)(L / 2);
+    float v = 0.0f;
+    if (!(x2 < 0 || x2 > L - 1.0f || y2 < 0 || y2 > L - 1.0f)) {
+      const int ii = (int)floorf(x2), jj = (int)floorf(y2);
+      long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
+      if (ii == L - 1) { k1--; k2--; }
+      if (jj == L - 1) { k2 -= L; k3 -= L; }
+      const float t = x2 - (float)ii, u = y2 - (float)jj;
+      const float tt = 1.0f - t, uu = 1.0f - u;
+      const float* src = s.treal + (long)b * L * L;
+      const float p0 = src[k0] * tt * uu, p1 = src[k1] * t * uu;
+      const float p3 = src[k3] * tt * u, p2 = src[k2] * t * u;
+      v = p0 + p1 + p2 + p3;
+    }
+    if (r >= dm2) v *= 0.5f * cosf(kRefPi * (r - dm2) / (2 * ehw)) + 0.5f;
+    return v;
+  }
+}
+
+template <int L, int SRC>
+__global__ __launch_bounds__(WG) void k_rows_r2c(RowSrc s, float2* __restrict__ out,
+                                                 const float2* __restrict__ twL,
+                                                 const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int M = G::M, H = G::H, RB = G::RB;
+  __shared__ float2 lds[RB * M];
+  __shared__ float2 tw[M];
+  __shared__ double red[2 * (WG / 64)];
+  const int b = blockIdx.y;
+  const int y0 = blockIdx.x * RB;
+  load_twiddles<M, WG>(tw, twM);
+  float aux = 0.0f;
+  if constexpr (SRC == SRC_PAD) aux = s.scale[b];
+  if constexpr (SRC == SRC_TILE) aux = (float)0;  // mean applied in double below
+  double acc1 = 0, acc2 = 0;
+  for (int g = threadIdx.x; g < RB * M; g += WG) {
+    const int r = g / M, m = g - r * M;
+    const int y = y0 + r;
+    float v0 = 0.0f, v1 = 0.0f;
+    if (y < L) {
+      if constexpr (SRC == SRC_TILE) {
+        const float mean = (float)0;
+        (void)mean;
+        const int ix = b % s.bxn, iy = b / s.bxn;
+        const int gy = s.offy[iy] + y;
+        const int gx0 = s.offx[ix] + 2 * m;
+        const double mu = *s.mean;
+        if (gy < s.ny) {
+          const float* row = s.img + (long)gy * s.nx;
+          if (gx0 < s.nx) v0 = (float)((double)row[gx0] - mu);
+          if (gx0 + 1 < s.nx) v1 = (float)((double)row[gx0 + 1] - mu);
+        }
+      } else {
+        v0 = row_value<L, SRC>(s, b, y, 2 * m, aux);
+        v1 = row_value<L, SRC>(s, b, y, 2 * m + 1, aux);
+      }
+    }
+    if constexpr (SRC == SRC_ROT) {
+      acc1 += (double)v0 + (double)v1;
+      acc2 += (double)v0 * v0 + (double)v1 * v1;
+    }
+    lds[r * M + m] = make_float2(v0, v1);
+  }
+  if constexpr (SRC == SRC_ROT) {
+    block_sum2_d(acc1, acc2, red);
+    if (threadIdx.x == 0) {
+      s.partials[2 * ((long)b * gridDim.x + blockIdx.x)] = acc1;
+      s.partials[2 * ((long)b * gridDim.x + blockIdx.x) + 1] = acc2;
+    }
+  }
+  __syncthreads();
+  lds_fft<M, RB, WG, -1, M>(lds, tw);
+  // post-process to X[k], k = 0..M, and write out[b][k][y0 + r]
+  for (int g = threadIdx.x; g < RB * H; g += WG) {
+    const int k = g / RB, r = g - k * RB;
+    const int y = y0 + r;
+    if (y >= L) continue;
+    const float2 zk = lds[r * M + (k % M)];
+    const float2 zmk = lds[r * M + ((M - k) % M)];
+    const float2 wk = twL[k];
+    out[((long)b * H + k) * L + y] = r2c_post(zk, zmk, wk);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column transforms on [b][kx][y]
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(WG) void k_cols_fwd(float2* __restrict__ spec, float scale,
+                                                 const float* __restrict__ bscale, int zero_dc,
+                                                 const float2* __restrict__ twL) {
+  using G = Geo<L>;
+  constexpr int H = G::H, CB = G::CB;
+  __shared__ float2 lds[CB * L];
+  __shared__ float2 tw[L];
+  const int b = blockIdx.y;
+  const int kx0 = blockIdx.x * CB;
+  load_twiddles<L, WG>(tw, twL);
+  float2* base = spec + ((long)b * H + kx0) * L;
+  const int ncol = min(CB, H - kx0);
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int c = g / L;
+    lds[g] = (c < ncol) ? base[g] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft<L, CB, WG, -1, L>(lds, tw);
+  const float sc = bscale ? bscale[b] : scale;
+  for (int g = threadIdx.x; g < ncol * L; g += WG) {
+    float2 v = cscale(lds[g], sc);
+    if (zero_dc && kx0 == 0 && g == 0) v = make_float2(0.f, 0.f);
+    base[g] = v;
+  }
+}
+
+// inverse columns, transposed write to G[b][y][kx] (pitch HP)
+template <int L>
+__global__ __launch_bounds__(WG) void k_cols_inv_T(const float2* __restrict__ spec,
+                                                   float2* __restrict__ Gout,
+                                                   const float2* __restrict__ twL) {
+  using G = Geo<L>;
+  constexpr int H = G::H, HP = G::HP, CB = G::CB;
+  __shared__ float2 lds[CB * L];
+  __shared__ float2 tw[L];
+  const int b = blockIdx.y;
+  const int kx0 = blockIdx.x * CB;
+  load_twiddles<L, WG>(tw, twL);
+  const float2* base = spec + ((long)b * H + kx0) * L;
+  const int ncol = min(CB, H - kx0);
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int c = g / L;
+    lds[g] = (c < ncol) ? base[g] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft<L, CB, WG, +1, L>(lds, tw);
+  float2* gb = Gout + (long)b * L * HP;
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int y = g / CB, c = g - y * CB;
+    if (c < ncol) gb[(long)y * HP + kx0 + c] = lds[c * L + y];
+  }
+}
+
+// row C2R of G[b][y][kx] into real out[b][y][x]
+template <int L>
+__global__ __launch_bounds__(WG) void k_rows_c2r(const float2* __restrict__ Gin,
+                                                 float* __restrict__ out,
+                                                 const float2* __restrict__ twL,
+                                                 const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int M = G::M, H = G::H, HP = G::HP, RB = G::RB;
+  __shared__ float2 lds[RB * H];
+  __shared__ float2 tw[M];
+  const int b = blockIdx.y;
+  const int y0 = blockIdx.x * RB;
+  load_twiddles<M, WG>(tw, twM);
+  const float2* gb = Gin + (long)b * L * HP;
+  for (int g = threadIdx.x; g < RB * H; g += WG) {
+    const int r = g / H, k = g - r * H;
+    const int y = y0 + r;
+    lds[g] = (y < L) ? gb[(long)y * HP + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  constexpr int NPAIR = M / 2 + 1;
+  for (int g = threadIdx.x; g < RB * NPAIR; g += WG) {
+    const int r = g / NPAIR, q = g - r * NPAIR;
+    float2* row = lds + r * H;
+    const int k1 = q, k2 = M - q;
+    const float2 x1 = row[k1], x2 = row[k2];
+    float2 w1 = twL[k1];
+    w1.y = -w1.y;
+    row[k1] = c2r_pre(x1, x2, w1);
+    if (q > 0 && k2 != k1) {
+      float2 w2 = twL[k2];
+      w2.y = -w2.y;
+      row[k2] = c2r_pre(x2, x1, w2);
+    }
+  }
+  __syncthreads();
+  lds_fft<M, RB, WG, +1, H>(lds, tw);
+  float* ob = out + (long)b * L * L;
+  for (int g = threadIdx.x; g < RB * M; g += WG) {
+    const int r = g / M, n = g - r * M;
+    const int y = y0 + r;
+    if (y >= L) continue;
+    const float2 z = lds[r * H + n];
+    *reinterpret_cast<float2*>(ob + (long)y * L + 2 * n) = z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Correlation (compute_corner_CCG GPU_func.cu:584-615, cuFFT inverse
+// main.cu:848, get_peak_and_SUM GPU_func.cu:667-708, host loop main.cu:856-881)
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ tiles,
+                                                   const float2* __restrict__ tspec,
+                                                   const MapRef* __restrict__ maps, float sgn,
+                                                   float2* __restrict__ Gout,
+                                                   const float2* __restrict__ twL) {
+  using G = Geo<L>;
+  constexpr int H = G::H, HP = G::HP, CB = G::CB;
+  __shared__ float2 lds[CB * L];
+  __shared__ float2 tw[L];
+  const int m = blockIdx.y;
+  const MapRef mr = maps[m];
+  const int kx0 = blockIdx.x * CB;
+  load_twiddles<L, WG>(tw, twL);
+  const float2* ib = tiles + ((long)mr.tile * H + kx0) * L;
+  const float2* tb = tspec + ((long)mr.view * H + kx0) * L;
+  const int ncol = min(CB, H - kx0);
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int c = g / L, y = g - c * L;
+    float2 v = make_float2(0.f, 0.f);
+    if (c < ncol) {
+      v = cmulc(ib[g], tb[g]);
+      const float sg = ((kx0 + c + y) & 1) ? -sgn : sgn;
+      v = cscale(v, sg);
+    }
+    lds[g] = v;
+  }
+  __syncthreads();
+  lds_fft<L, CB, WG, +1, L>(lds, tw);
+  float2* gb = Gout + (long)m * L * HP;
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int y = g / CB, c = g - y * CB;
+    if (c < ncol) gb[(long)y * HP + kx0 + c] = lds[c * L + y];
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gin, float lo,
+                                                   float hi, CorrPartial* __restrict__ part,
+                                                   const float2* __restrict__ twL,
+                                                   const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int M = G::M, H = G::H, HP = G::HP, RB = G::RB;
+  __shared__ float2 lds[RB * H];
+  __shared__ float2 tw[M];
+  __shared__ float red_v[WG / 64];
+  __shared__ int red_p[WG / 64];
+  __shared__ float red_s1[WG / 64], red_s2[WG / 64];
+  const int m = blockIdx.y;
+  const int y0 = blockIdx.x * RB;
+  load_twiddles<M, WG>(tw, twM);
+  const float2* gb = Gin + (long)m * L * HP;
+  for (int g = threadIdx.x; g < RB * H; g += WG) {
+    const int r = g / H, k = g - r * H;
+    const int y = y0 + r;
+    lds[g] = (y < L) ? gb[(long)y * HP + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  constexpr int NPAIR = M / 2 + 1;
+  for (int g = threadIdx.x; g < RB * NPAIR; g += WG) {
+    const int r = g / NPAIR, q = g - r * NPAIR;
+    float2* row = lds + r * H;
+    const int k1 = q, k2 = M - q;
+    const float2 x1 = row[k1], x2 = row[k2];
+    float2 w1 = twL[k1];
+    w1.y = -w1.y;
+    row[k1] = c2r_pre(x1, x2, w1);
+    if (q > 0 && k2 != k1) {
+      float2 w2 = twL[k2];
+      w2.y = -w2.y;
+      row[k2] = c2r_pre(x2, x1, w2);
+    }
+  }
+  __syncthreads();
+  lds_fft<M, RB, WG, +1, H>(lds, tw);
+  float vmax = 0.0f;
+  int pos = INT_MAX;
+  float s1 = 0.0f, s2 = 0.0f;
+  for (int g = threadIdx.x; g < RB * M; g += WG) {
+    const int r = g / M, n = g - r * M;
+    const int y = y0 + r;
+    if (y >= L) continue;
+    const float2 z = lds[r * H + n];
+    s1 += z.x + z.y;
+    s2 += z.x * z.x + z.y * z.y;
+    const bool yok = !((float)y < lo || (float)y > hi);
+    if (yok) {
+      const int x0 = 2 * n;
+      if (!((float)x0 < lo || (float)x0 > hi) && z.x > 0.0f) vmax_combine(vmax, pos, z.x, y * L + x0);
+      if (!((float)(x0 + 1) < lo || (float)(x0 + 1) > hi) && z.y > 0.0f)
+        vmax_combine(vmax, pos, z.y, y * L + x0 + 1);
+    }
+  }
+  wave_max_idx(vmax, pos);
+  s1 = wave_sum_f(s1);
+  s2 = wave_sum_f(s2);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { red_v[w] = vmax; red_p[w] = pos; red_s1[w] = s1; red_s2[w] = s2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float v = red_v[0];
+    int p = red_p[0];
+    float a = red_s1[0], q = red_s2[0];
+    for (int i = 1; i < WG / 64; ++i) {
+      vmax_combine(v, p, red_v[i], red_p[i]);
+      a += red_s1[i];
+      q += red_s2[i];
+    }
+    CorrPartial cp;
+    cp.vmax = v; cp.pos = p; cp.s1 = a; cp.s2 = q;
+    part[(long)m * gridDim.x + blockIdx.x] = cp;
+  }
+}
+
+__global__ void k_corr_finalize(const CorrPartial* __restrict__ part, int nparts, int nmaps,
+                                double rc, MapStat* __restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmaps) return;
+  const CorrPartial* p = part + (long)m * nparts;
+  float v = 0.0f;
+  int pos = INT_MAX;
+  double s1 = 0, s2 = 0;
+  for (int i = 0; i < nparts; ++i) {
+    vmax_combine(v, pos, p[i].vmax, p[i].pos);
+    s1 += p[i].s1;
+    s2 += p[i].s2;
+  }
+  const double peak = v;
+  if (!(v > 0.0f)) pos = 0;
+  const double ra = s1 - peak, rb = s2 - peak * peak;
+  const double var = rb / rc - (ra / rc) * (ra / rc);
+  const double sd = sqrt(var);
+  MapStat st;
+  st.peak = (float)peak;
+  st.pos = pos;
+  st.sd = (float)sd;
+  st.score = (sd > 0) ? (float)(peak / sd) : 0.0f;
+  st.sum = s1;
+  st.sum2 = s2;
+  out[m] = st;
+}
+
+// ---------------------------------------------------------------------------
+// Per-psi normalisation scale: mean/var of the rotated masked template over
+// L^2 (compute_sum_sqr + main.cu:686-699), divided_by_var, FFT scale 1/L^2.
+// ---------------------------------------------------------------------------
+__global__ void k_rot_scale(const double* __restrict__ partials, int nparts, int nb, double L2,
+                            float* __restrict__ bscale) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s1 = 0, s2 = 0;
+  for (int i = 0; i < nparts; ++i) {
+    s1 += partials[2 * ((long)b * nparts + i)];
+    s2 += partials[2 * ((long)b * nparts + i) + 1];
+  }
+  const double mean = s1 / L2;
+  const double var = s2 / L2 - mean * mean;
+  double sc = 1.0 / L2;
+  if (var != 0) sc /= sqrt(var);
+  bscale[b] = (float)sc;
+}
+
+// ---------------------------------------------------------------------------
+// Template preparation kernels (main.cu:311-474)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_box_sums(const float* __restrict__ src, int npix,
+                                                 double* __restrict__ sums) {
+  __shared__ double red[2 * (WG / 64)];
+  const int b = blockIdx.x;
+  const float* p = src + (long)b * npix;
+  double a = 0, q = 0;
+  for (int i = threadIdx.x; i < npix; i += WG) {
+    const double v = p[i];
+    a += v;
+    q += v * v;
+  }
+  block_sum2_d(a, q, red);
+  if (threadIdx.x == 0) { sums[2 * b] = a; sums[2 * b + 1] = q; }
+}
+
+__global__ void k_sigma_scale(const double* __restrict__ sums, int nb, double L2,
+                              float* __restrict__ scale) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double mean = sums[2 * b] / L2;
+  double sigma = sqrt(sums[2 * b + 1] / L2 - mean * mean);
+  if (sigma < 0 || !isfinite(sigma)) sigma = 0;
+  // scale_each (GPU_func.cu:85-94): skipped when |sigma| < 1e-8
+  scale[b] = (sigma < 1e-8 && sigma > -1e-8) ? 1.0f : (float)(1.0 / sigma);
+}
+
+template <int L>
+__global__ __launch_bounds__(WG) void k_radial_power(const float2* __restrict__ spec,
+                                                     const int* __restrict__ bin_start,
+                                                     const int* __restrict__ bin_pix,
+                                                     double* __restrict__ ra) {
+  using G = Geo<L>;
+  constexpr int H = G::H, NB = L / 2;
+  const int b = blockIdx.x;
+  const float2* sp = spec + (long)b * H * L;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = w; r < NB; r += WG / 64) {
+    double acc = 0;
+    for (int i = bin_start[r] + lane; i < bin_start[r + 1]; i += 64) {
+      const float2 v = sp[bin_pix[i]];
+      acc += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    acc = wave_sum_d(acc);
+    if (lane == 0) ra[(long)b * NB + r] = acc;
+  }
+}
+
+__device__ float ctf_ast_f(int x1, int y1, int nx, int ny, const WeightArgs& w, int mode) {
+  // CTF_AST, GPU_func.cu:292-315
+  y1 = x1 > nx / 2 ? ny - y1 : y1;
+  x1 = min(x1, nx - x1);
+  const float dx = (float)min(x1, nx - x1);
+  const float dy = (float)(y1 - ny / 2);
+  const float ss = (dx * dx / (float)(nx * nx) + dy * dy / (float)(ny * ny)) / (w.apix * w.apix);
+  const float ag = atan2f((float)(y1 - ny / 2), (float)x1);
+  const float df_ast = 0.5f * (w.dfu + w.dfv + 2 * w.dfdiff * cosf(2 * (w.dfang * kRefPi / 180 - ag)));
+  const float gamma = -2 * kRefPi * (w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * ss * ss +
+                                     df_ast * 5000.0f * w.lambda * ss);
+  const float v = sqrtf(1.0f - w.ampconst * w.ampconst) * sinf(gamma) + w.ampconst * cosf(gamma);
+  if (mode == 0) return v > 0 ? 1.0f : -1.0f;
+  if (mode == 2) return fabsf(v);
+  return v;
+}
+
+// whiten_Tmp (GPU_func.cu:130-152) + apply_weighting_function (249-290) +
+// compute_area_sum_ofSQR partials (318-362); spectrum pixel (kx, ky) with
+// kx <= L/2 is the reference's (x, y).
+template <int L>
+__global__ __launch_bounds__(WG) void k_tmpl_weight(float2* __restrict__ spec,
+                                                    const double* __restrict__ ra,
+                                                    const float* __restrict__ rbins, WeightArgs w,
+                                                    double* __restrict__ partials) {
+  using G = Geo<L>;
+  constexpr int H = G::H, NB = L / 2;
+  __shared__ double red[2 * (WG / 64)];
+  const int b = blockIdx.y;
+  const long i = (long)blockIdx.x * WG + threadIdx.x;
+  double acc = 0, dummy = 0;
+  if (i < (long)H * L) {
+    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
+    float2 v = spec[(long)b * H * L + i];
+    const float hy = (float)min(ky, L - ky), hx = (float)min(kx, L - kx);
+    const float rr = hypotf(hy, hx);
+    const int rbin = (int)floorf(rr + 0.5f) - 1;
+    if (rbin < NB && rbin >= 0) {
+      const double fb = ra[(long)b * NB + rbin] / (double)rbins[rbin];
+      v = cscale(v, (float)(1.0 / sqrt(fb)));
+    }
+    // low pass
+    const int r_round = rbin;
+    const float hi = L * w.apix / w.highres, lo = L * w.apix / w.lowres;
+    float f = 1.0f;
+    if (r_round < hi && r_round >= 0) {
+    } else if (r_round >= hi && r_round < hi + 8) {
+      f = 0.5f * cosf(kRefPi * (r_round - hi) / (2 * 8)) + 0.5f;
+    } else if (r_round >= (lo - 8) && r_round < lo && r_round >= 0) {
+      f = 0.5f * cosf(kRefPi * (lo - r_round) / (2 * 8)) + 0.5f;
+    } else {
+      f = 0.0f;
+    }
+    const float ss = rr * w.ds;
+    if (r_round < L / 2 && r_round >= 0) {
+      const float vv = ctf_ast_f(kx, (ky + L / 2) % L, L, L, w, 2);
+      const float signal = expf(w.bf * ss * ss + w.bf2 * ss + w.bf3) / (w.kk + 1);
+      const float ncurve = expf(w.a * ss * ss + w.b * ss + w.b2) / signal;
+      f = f * vv * sqrtf(1 / (ncurve + w.kk * vv * vv));
+    }
+    v = cscale(v, f);
+    spec[(long)b * H * L + i] = v;
+    if (rbin < NB && rbin >= 0) acc = (double)v.x * v.x + (double)v.y * v.y;
+  }
+  block_sum2_d(acc, dummy, red);
+  if (threadIdx.x == 0) partials[(long)b * gridDim.x + blockIdx.x] = acc;
+}
+
+template <int L>
+__global__ __launch_bounds__(WG) void k_tmpl_ampnorm(float2* __restrict__ spec,
+                                                     const double* __restrict__ partials,
+                                                     int nparts, double count) {
+  using G = Geo<L>;
+  constexpr int H = G::H;
+  __shared__ float inv_m;
+  const int b = blockIdx.y;
+  if (threadIdx.x == 0) {
+    double S = 0;
+    for (int i = 0; i < nparts; ++i) S += partials[(long)b * nparts + i];
+    const double m = sqrt(S / (count * count));
+    inv_m = (m != 0) ? (float)(1.0 / m) : 1.0f;
+  }
+  __syncthreads();
+  const long i = (long)blockIdx.x * WG + threadIdx.x;
+  if (i < (long)H * L) spec[(long)b * H * L + i] = cscale(spec[(long)b * H * L + i], inv_m);
+}
+
+// ---------------------------------------------------------------------------
+// Micrograph statistics (init_d_image, main.cu:523-545) in FP64
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_img_sums(const float* __restrict__ img, long n,
+                                                 double* __restrict__ partials) {
+  __shared__ double red[2 * (WG / 64)];
+  const long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long beg = blockIdx.x * chunk, end = min(n, beg + chunk);
+  double a = 0, q = 0;
+  for (long i = beg + threadIdx.x; i < end; i += WG) {
+    const double v = img[i];
+    a += v;
+    q += v * v;
+  }
+  block_sum2_d(a, q, red);
+  if (threadIdx.x == 0) { partials[2 * blockIdx.x] = a; partials[2 * blockIdx.x + 1] = q; }
+}
+
+// stats: [0] avg, [1] var (std), [2] up bound, [3] low bound, [4] clamp flag
+__global__ void k_img_bounds(const double* __restrict__ partials, int nblocks, long n,
+                             double* __restrict__ stats) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0, q = 0;
+  for (int i = 0; i < nblocks; ++i) { a += partials[2 * i]; q += partials[2 * i + 1]; }
+  const double avg = a / (double)n;
+  const double s2 = q / (double)n;
+  const double var = sqrt(s2 - avg * avg);
+  stats[0] = avg;
+  stats[1] = var;
+  stats[2] = (double)(int)(avg + 6 * var);  // int truncation, main.cu:539-540
+  stats[3] = (double)(int)(avg - 6 * var);
+  stats[4] = (s2 > avg * avg) ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(WG) void k_img_clamp(float* __restrict__ img, long n, int invert,
+                                                  const double* __restrict__ stats,
+                                                  double* __restrict__ partials) {
+  __shared__ double red[2 * (WG / 64)];
+  const double avg = stats[0], up = stats[2], low = stats[3];
+  const bool clamp = stats[4] != 0.0;
+  const long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long beg = blockIdx.x * chunk, end = min(n, beg + chunk);
+  double a = 0, dummy = 0;
+  for (long i = beg + threadIdx.x; i < end; i += WG) {
+    float v = img[i];
+    if (clamp && ((double)v > up || (double)v < low)) v = (float)avg;
+    if (invert == 1) v = -v;
+    img[i] = v;
+    a += (double)v;
+  }
+  block_sum2_d(a, dummy, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = a;
+}
+
+__global__ void k_img_mean(const double* __restrict__ partials, int nblocks, long n,
+                           double* __restrict__ mean) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0;
+  for (int i = 0; i < nblocks; ++i) a += partials[i];
+  *mean = a / (double)n;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers, dispatched over the supported tile sizes
+// ---------------------------------------------------------------------------
+#define GS_SIZES(X) X(64) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(448) X(512) X(576) X(640) X(720) X(768) X(960) X(1024)
+
+bool geometry_supported(int L) {
+  switch (L) {
+#define CASE(n) case n:
+    GS_SIZES(CASE)
+#undef CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+Geometry geometry_of(int L) {
+  Geometry g{};
+  switch (L) {
+#define CASE(n)                                                                        \
+  case n:                                                                              \
+    g.L = n; g.M = Geo<n>::M; g.H = Geo<n>::H; g.HP = Geo<n>::HP; g.RB = Geo<n>::RB;   \
+    g.CB = Geo<n>::CB;                                                                 \
+    break;
+    GS_SIZES(CASE)
+#undef CASE
+    default:
+      break;
+  }
+  return g;
+}
+
+#define DISPATCH(L, FN, ...)                 \
+  switch (L) {                               \
+    case 64: FN<64>(__VA_ARGS__); break;     \
+    case 96: FN<96>(__VA_ARGS__); break;     \
+    case 128: FN<128>(__VA_ARGS__); break;   \
+    case 160: FN<160>(__VA_ARGS__); break;   \
+    case 192: FN<192>(__VA_ARGS__); break;   \
+    case 256: FN<256>(__VA_ARGS__); break;   \
+    case 320: FN<320>(__VA_ARGS__); break;   \
+    case 384: FN<384>(__VA_ARGS__); break;   \
+    case 448: FN<448>(__VA_ARGS__); break;   \
+    case 512: FN<512>(__VA_ARGS__); break;   \
+    case 576: FN<576>(__VA_ARGS__); break;   \
+    case 640: FN<640>(__VA_ARGS__); break;   \
+    case 720: FN<720>(__VA_ARGS__); break;   \
+    case 768: FN<768>(__VA_ARGS__); break;   \
+    case 960: FN<960>(__VA_ARGS__); break;   \
+    case 1024: FN<1024>(__VA_ARGS__); break; \
+    default: break;                          \
+  }
+
+template <int L>
+static void rows_r2c_pad_t(const float* src, int bx, int by, const float* scale, int nb,
+                           float2* out, const float2* twL, const float2* twM, hipStream_t s) {
+  RowSrc rs{};
+  rs.box = src; rs.bx = bx; rs.by = by; rs.scale = scale;
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  hipLaunchKernelGGL((k_rows_r2c<L, SRC_PAD>), grid, dim3(WG), 0, s, rs, out, twL, twM);
+}
+void launch_rows_r2c_pad(int L, const float* src, int bx, int by, const float* scale, int nb,
+                         float2* out, const float2* twL, const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_r2c_pad_t, src, bx, by, scale, nb, out, twL, twM, s);
+}
+
+template <int L>
+static void rows_r2c_tiles_t(const float* img, int nx, int ny, const int* offx, const int* offy,
+                             int bxn, int ntiles, const double* mean, float2* out,
+                             const float2* twL, const float2* twM, hipStream_t s) {
+  RowSrc rs{};
+  rs.img = img; rs.nx = nx; rs.ny = ny; rs.offx = offx; rs.offy = offy; rs.bxn = bxn;
+  rs.mean = mean;
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, ntiles);
+  hipLaunchKernelGGL((k_rows_r2c<L, SRC_TILE>), grid, dim3(WG), 0, s, rs, out, twL, twM);
+}
+void launch_rows_r2c_tiles(int L, const float* img, int nx, int ny, const int* offx,
+                           const int* offy, int bxn, int ntiles, const double* mean,
+                           float2* out, const float2* twL, const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_r2c_tiles_t, img, nx, ny, offx, offy, bxn, ntiles, mean, out, twL, twM, s);
+}
+
+template <int L>
+static void rows_r2c_rot_t(const float* treal, int nb, float cose, float sine, float dmask,
+                           float2* out, double* partials, const float2* twL, const float2* twM,
+                           hipStream_t s) {
+  RowSrc rs{};
+  rs.treal = treal; rs.cose = cose; rs.sine = sine; rs.dmask = dmask; rs.partials = partials;
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  hipLaunchKernelGGL((k_rows_r2c<L, SRC_ROT>), grid, dim3(WG), 0, s, rs, out, twL, twM);
+}
+void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float sine, float dmask,
+                         float2* out, double* partials, const float2* twL, const float2* twM,
+                         hipStream_t s) {
+  DISPATCH(L, rows_r2c_rot_t, treal, nb, cose, sine, dmask, out, partials, twL, twM, s);
+}
+
+template <int L>
+static void cols_fwd_t(float2* spec, int nb, float scale, const float* bscale, int zero_dc,
+                       const float2* twL, hipStream_t s) {
+  dim3 grid((Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB, nb);
+  hipLaunchKernelGGL((k_cols_fwd<L>), grid, dim3(WG), 0, s, spec, scale, bscale, zero_dc, twL);
+}
+void launch_cols_fwd(int L, float2* spec, int nb, float scale, const float* bscale, int zero_dc,
+                     const float2* twL, hipStream_t s) {
+  DISPATCH(L, cols_fwd_t, spec, nb, scale, bscale, zero_dc, twL, s);
+}
+
+void launch_rot_scale(int L, const double* partials, int nb, float* bscale, hipStream_t s) {
+  const int nparts = (L + geometry_of(L).RB - 1) / geometry_of(L).RB;
+  hipLaunchKernelGGL(k_rot_scale, dim3((nb + 255) / 256), dim3(256), 0, s, partials, nparts, nb,
+                     (double)L * L, bscale);
+}
+
+template <int L>
+static void cols_inv_T_t(const float2* spec, int nb, float2* G, const float2* twL, hipStream_t s) {
+  dim3 grid((Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB, nb);
+  hipLaunchKernelGGL((k_cols_inv_T<L>), grid, dim3(WG), 0, s, spec, G, twL);
+}
+void launch_cols_inv_T(int L, const float2* spec, int nb, float2* G, const float2* twL,
+                       hipStream_t s) {
+  DISPATCH(L, cols_inv_T_t, spec, nb, G, twL, s);
+}
+
+template <int L>
+static void rows_c2r_t(const float2* G, int nb, float* out, const float2* twL, const float2* twM,
+                       hipStream_t s) {
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  hipLaunchKernelGGL((k_rows_c2r<L>), grid, dim3(WG), 0, s, G, out, twL, twM);
+}
+void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* twL,
+                     const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_c2r_t, G, nb, out, twL, twM, s);
+}
+
+template <int L>
+static void corr_pass1_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
+                         float sgn, float2* G, const float2* twL, hipStream_t s) {
+  dim3 grid((Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB, nmaps);
+  hipLaunchKernelGGL((k_corr_pass1<L>), grid, dim3(WG), 0, s, tiles, tspec, maps, sgn, G, twL);
+}
+void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
+                       int nmaps, float sgn, float2* G, const float2* twL, hipStream_t s) {
+  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, sgn, G, twL, s);
+}
+
+template <int L>
+static void corr_pass2_t(const float2* G, int nmaps, float lo, float hi, CorrPartial* part,
+                         const float2* twL, const float2* twM, hipStream_t s) {
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
+  hipLaunchKernelGGL((k_corr_pass2<L>), grid, dim3(WG), 0, s, G, lo, hi, part, twL, twM);
+}
+void launch_corr_pass2(int L, const float2* G, int nmaps, float lo, float hi, CorrPartial* part,
+                       const float2* twL, const float2* twM, hipStream_t s) {
+  DISPATCH(L, corr_pass2_t, G, nmaps, lo, hi, part, twL, twM, s);
+}
+
+void launch_corr_finalize(int L, const CorrPartial* part, int nmaps, MapStat* out,
+                          hipStream_t s) {
+  const int nparts = (L + geometry_of(L).RB - 1) / geometry_of(L).RB;
+  hipLaunchKernelGGL(k_corr_finalize, dim3((nmaps + 255) / 256), dim3(256), 0, s, part, nparts,
+                     nmaps, (double)L * L - 1.0, out);
+}
+
+void launch_box_sums(const float* src, int npix, int nb, double* sums, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_sums, dim3(nb), dim3(WG), 0, s, src, npix, sums);
+}
+void launch_sigma_scale(const double* sums, int nb, int L, float* scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_sigma_scale, dim3((nb + 255) / 256), dim3(256), 0, s, sums, nb,
+                     (double)L * L, scale);
+}
+
+template <int L>
+static void radial_power_t(const float2* spec, int nb, const int* bin_start, const int* bin_pix,
+                           double* ra, hipStream_t s) {
+  hipLaunchKernelGGL((k_radial_power<L>), dim3(nb), dim3(WG), 0, s, spec, bin_start, bin_pix, ra);
+}
+void launch_radial_power(int L, const float2* spec, int nb, const int* bin_start,
+                         const int* bin_pix, double* ra, hipStream_t s) {
+  DISPATCH(L, radial_power_t, spec, nb, bin_start, bin_pix, ra, s);
+}
+
+template <int L>
+static void tmpl_weight_t(float2* spec, int nb, const double* ra, const float* rb,
+                          const WeightArgs& w, double* partials, hipStream_t s) {
+  const long tot = (long)Geo<L>::H * L;
+  dim3 grid((unsigned)((tot + WG - 1) / WG), nb);
+  hipLaunchKernelGGL((k_tmpl_weight<L>), grid, dim3(WG), 0, s, spec, ra, rb, w, partials);
+}
+void launch_tmpl_weight(int L, float2* spec, int nb, const double* ra, const float* rb,
+                        const WeightArgs& w, double* partials, hipStream_t s) {
+  DISPATCH(L, tmpl_weight_t, spec, nb, ra, rb, w, partials, s);
+}
+
+template <int L>
+static void tmpl_ampnorm_t(float2* spec, int nb, const double* partials, double count,
+                           hipStream_t s) {
+  const long tot = (long)Geo<L>::H * L;
+  const int nparts = (int)((tot + WG - 1) / WG);
+  dim3 grid(nparts, nb);
+  hipLaunchKernelGGL((k_tmpl_ampnorm<L>), grid, dim3(WG), 0, s, spec, partials, nparts, count);
+}
+void launch_tmpl_ampnorm(int L, float2* spec, int nb, const double* partials, double count,
+                         hipStream_t s) {
+  DISPATCH(L, tmpl_ampnorm_t, spec, nb, partials, count, s);
+}
+
+int img_reduce_blocks(long n) {
+  long b = (n + 65535) / 65536;
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return (int)b;
+}
+void launch_img_sums(const float* img, long n, double* partials, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_img_sums, dim3(nblocks), dim3(WG), 0, s, img, n, partials);
+}
+void launch_img_bounds(const double* partials, int nblocks, long n, double* stats, hipStream_t s) {
+  hipLaunchKernelGGL(k_img_bounds, dim3(1), dim3(64), 0, s, partials, nblocks, n, stats);
+}
+void launch_img_clamp(float* img, long n, int invert, const double* stats, double* partials,
+                      int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_img_clamp, dim3(nblocks), dim3(WG), 0, s, img, n, invert, stats, partials);
+}
+void launch_img_mean(const double* partials, int nblocks, long n, double* mean, hipStream_t s) {
+  hipLaunchKernelGGL(k_img_mean, dim3(1), dim3(64), 0, s, partials, nblocks, n, mean);
+}
+
+}  // namespace gs

@@ -1,0 +1,117 @@
+// gs_kernels.hpp -- host-visible launch interface of the picker's HIP kernels.
+//
+// Spectrum layout ("column-major half spectrum"): an L x L real image has an
+// R2C half spectrum of H = L/2 + 1 columns kx; it is stored as [kx][ky], each
+// column of L complex values contiguous (ky fastest).  Row-major intermediate
+// of the 2-D inverse: [y][kx] with a padded row pitch HP.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gs {
+
+// per-workgroup partial of the correlation reduction
+struct CorrPartial {
+  float vmax;   // max over valid (border-excluded) pixels with value > 0, else 0
+  int pos;      // y*L + x of vmax (lowest index on ties), INT_MAX if none
+  float s1;     // sum of all pixels of the rows handled
+  float s2;     // sum of squares
+};
+
+// per-map result of the norm_type 0 reduction (main.cu:856-881 semantics)
+struct MapStat {
+  float score;
+  int pos;
+  float peak;
+  float sd;
+  double sum;
+  double sum2;
+};
+
+struct MapRef {
+  int tile;
+  int view;
+};
+
+// geometry shared by host and device code
+struct Geometry {
+  int L, M, H, HP;
+  int RB;  // rows per workgroup for M-point transforms
+  int CB;  // columns per workgroup for L-point transforms
+};
+
+bool geometry_supported(int L);
+Geometry geometry_of(int L);
+
+// ---- launches (all asynchronous on `stream`) ----
+
+// Real rows -> half spectrum [b][kx][y] (unscaled), from padded templates:
+// out[b] = R2C rows of pad(src[b], box bx x by centred) * scale[b]
+void launch_rows_r2c_pad(int L, const float* src, int bx, int by, const float* scale, int nb,
+                         float2* out, const float2* twL, const float2* twM, hipStream_t s);
+// Real rows -> half spectrum from micrograph tiles: tile (ix, iy) of the
+// image, minus `mean` (device scalar), zero outside the image.
+void launch_rows_r2c_tiles(int L, const float* img, int nx, int ny, const int* offx,
+                           const int* offy, int bxn, int ntiles, const double* mean,
+                           float2* out, const float2* twL, const float2* twM, hipStream_t s);
+// Real rows -> half spectrum from rotated + masked real templates, with
+// per-(template, row block) partial sums of value and value^2.
+void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float sine, float dmask,
+                         float2* out, double* partials, const float2* twL, const float2* twM,
+                         hipStream_t s);
+// In-place column transform of [b][kx][y]; multiplies by scale (global) or
+// per-b scale from `bscale` (may be null); zero_dc zeroes element (0,0).
+void launch_cols_fwd(int L, float2* spec, int nb, float scale, const float* bscale, int zero_dc,
+                     const float2* twL, hipStream_t s);
+// Per-template normalisation scale 1/(L^2 sd) of the rotated templates from
+// the row partials (main.cu:682-707 semantics).
+void launch_rot_scale(int L, const double* partials, int nb, float* bscale, hipStream_t s);
+// Column inverse of [b][kx][y] written row-major to G[b][y][kx] (pitch HP).
+void launch_cols_inv_T(int L, const float2* spec, int nb, float2* G, const float2* twL,
+                       hipStream_t s);
+// Row C2R of G[b][y][kx] into real images out[b][y][x].
+void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* twL,
+                     const float2* twM, hipStream_t s);
+
+// Correlation pass 1: P = tile[t] * conj(tmpl[J]) * (-1)^(kx+ky) * sgn,
+// column inverse, written to G[m][y][kx].
+void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
+                       int nmaps, float sgn, float2* G, const float2* twL, hipStream_t s);
+// Correlation pass 2: row C2R of G and the border-excluded max / sums,
+// one CorrPartial per (map, row block).
+void launch_corr_pass2(int L, const float2* G, int nmaps, float dm4_lo, float dm4_hi,
+                       CorrPartial* part, const float2* twL, const float2* twM, hipStream_t s);
+// Combine partials into MapStat (score in FP64).
+void launch_corr_finalize(int L, const CorrPartial* part, int nmaps, MapStat* out,
+                          hipStream_t s);
+
+// ---- template preparation ----
+// per-template sums of value and value^2 of the raw box (FP64)
+void launch_box_sums(const float* src, int npix, int nb, double* sums, hipStream_t s);
+// 1/sigma per template (sigma over the L^2 window, em = 0)
+void launch_sigma_scale(const double* sums, int nb, int L, float* scale, hipStream_t s);
+// radial power bins (deterministic CSR gather): ra[b][r] = sum |F|^2, r < L/2
+void launch_radial_power(int L, const float2* spec, int nb, const int* bin_start,
+                         const int* bin_pix, double* ra, hipStream_t s);
+// whiten + low-pass + |CTF| weighting, in place on the half spectrum, and the
+// per-(template, column block) partial sums of |F|^2 inside the radius.
+struct WeightArgs {
+  float apix, highres, lowres, kk, ds;
+  float a, b, b2, bf, bf2, bf3;
+  float dfu, dfv, dfdiff, dfang, lambda, cs, ampconst;
+};
+void launch_tmpl_weight(int L, float2* spec, int nb, const double* ra, const float* rb,
+                        const WeightArgs& w, double* partials, hipStream_t s);
+void launch_tmpl_ampnorm(int L, float2* spec, int nb, const double* partials, double count,
+                         hipStream_t s);
+
+// ---- micrograph ----
+void launch_img_sums(const float* img, long n, double* partials, int nblocks, hipStream_t s);
+void launch_img_bounds(const double* partials, int nblocks, long n, double* stats, hipStream_t s);
+void launch_img_clamp(float* img, long n, int invert, const double* stats, double* partials,
+                      int nblocks, hipStream_t s);
+void launch_img_mean(const double* partials, int nblocks, long n, double* mean, hipStream_t s);
+
+int img_reduce_blocks(long n);
+
+}  // namespace gs

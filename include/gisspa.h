@@ -1,0 +1,175 @@
+/*
+ * gisspa.h -- C ABI of the MI355X GisSPA picker (libgisspa.so).
+ *
+ * The reference (Autumn1998/GisSPA) has no library or FFI: its picking path
+ * is a set of free functions in main.cu called from main() with raw device
+ * pointers (GPU_func.cuh:43-77).  This header replaces that host<->kernel
+ * seam with a context API.  Each entry point names the reference code it
+ * replaces.  Conventions:
+ *   - every function returns GS_OK (0) or a negative gs_status; nothing
+ *     calls exit() (the reference's CUDA_CALL/CUFFT_CALL exit(-1),
+ *     GPU_func.cuh:33-41);
+ *   - the caller owns host buffers; the context owns device memory, the
+ *     stream and the FFT tables;
+ *   - a context is used by one host thread at a time; several contexts may
+ *     coexist (one per device or stream);
+ *   - outputs are in a deterministic, canonical order.
+ */
+#ifndef GISSPA_H
+#define GISSPA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gs_status {
+  GS_OK = 0,
+  GS_ERR_ARG = -1,          /* invalid argument */
+  GS_ERR_UNSUPPORTED = -2,  /* e.g. a window size without a compiled kernel */
+  GS_ERR_DEVICE = -3,       /* HIP runtime error */
+  GS_ERR_NOMEM = -4,        /* device allocation failed */
+  GS_ERR_STATE = -5,        /* call order (templates / micrograph not set) */
+  GS_ERR_CAPACITY = -6      /* output buffer too small (n_out holds the need) */
+} gs_status;
+
+/* Run parameters: the reference's Parameters (EMReader/DataReader.h:14-42)
+ * with the same config-key meanings.  gs_params_default() fills the
+ * reference's defaults. */
+typedef struct gs_params {
+  /* required keys */
+  float apix;        /* angpix, A/pixel */
+  float phi_step;    /* phistep, degrees */
+  float kk;          /* kk */
+  float energy;      /* energy, kV */
+  float cs;          /* cs, mm */
+  float highres;     /* Highres, A */
+  float lowres;      /* Lowres, A */
+  float diameter;    /* diameter, pixels */
+  /* optional keys (reference defaults) */
+  float threshold;   /* threshold = 7 */
+  int window_size;   /* window_size = 320 */
+  int phase_flip;    /* phase_flip = 0 */
+  int overlap;       /* overlap = -1 (stride window_size + 1) */
+  int norm_type;     /* norm_type = 0 */
+  int invert;        /* invert = 0 */
+  float ampconst;    /* ampconst = 0.07 (fixed in the reference, a key here) */
+  /* fixed SNR model (DataReader.h:25) */
+  float a, b, b2, bfactor, bfactor2, bfactor3;
+} gs_params;
+
+/* Micrograph CTF values as written in the micrograph .lst
+ * (defocus in um, positive underfocus; dfdiff um; dfang degrees),
+ * EMReader/DataReader.cpp:366-387. */
+typedef struct gs_ctf {
+  float defocus;
+  float dfdiff;
+  float dfang;
+} gs_ctf;
+
+/* One picked particle: a line of the reference's output (main.cu:985). */
+typedef struct gs_particle {
+  int psi_index;   /* index of the in-plane step (euler3 = psi_index * phistep) */
+  int view;        /* Euler-list line J */
+  int tile;        /* tile index ix + iy * bx */
+  float psi;       /* printed psi = 360 - euler3, wrapped to [0, 360) */
+  float cx, cy;    /* centre in micrograph pixels */
+  float score;
+} gs_particle;
+
+/* Per (psi, tile, view) correlation result of norm_type 0
+ * (main.cu:856-881): peak of the border-excluded CC map, its position
+ * y*l + x in the tile, sums over the whole map, and the score. */
+typedef struct gs_mapstat {
+  float score;
+  int pos;
+  float peak;
+  float sd;
+  double sum;
+  double sum2;
+} gs_mapstat;
+
+typedef struct gs_info {
+  int l;           /* window (tile) size after adjustment */
+  int n_views;
+  int n_psi;
+  int nx, ny;      /* current micrograph */
+  int tiles_x, tiles_y;
+  int box_x, box_y;
+} gs_info;
+
+/* Per-stage device time of the last calls, milliseconds (hipEvent). */
+typedef struct gs_timing {
+  double template_prep_ms;
+  double micrograph_prep_ms;
+  double rotate_ms;          /* per-psi template rotation + FFT, summed */
+  double corr_pass1_ms;      /* product + column inverse FFT, summed */
+  double corr_pass2_ms;      /* row C2R + reduction, summed */
+  double finalize_ms;
+  double pick_total_ms;      /* whole gs_pick, device time */
+  long long corr_maps;       /* CC maps evaluated by the last gs_pick */
+  long long corr_pass1_launches;
+  long long corr_pass2_launches;
+} gs_timing;
+
+/* Fill p with the reference's defaults (DataReader.h:16-42); required keys
+ * are set to -1 like the reference. */
+void gs_params_default(gs_params* p);
+
+/* Create a context on `device`.  Replaces main.cu:1071 (cudaSetDevice),
+ * the stream of main.cu:274 and the plans of main.cu:293-307. */
+int gs_create(const gs_params* p, int device, void** ctx);
+
+/* Upload and prepare the templates (replaces readAndPaddingTemplate,
+ * edgeNormalize and handleTemplate, main.cu:182-474).  host: n images of
+ * bx x by floats, x fastest.  tctf: CTF used for the |CTF| weighting --
+ * the reference uses the first micrograph's values (main.cu:1106, 1133).
+ * If bx or by exceeds window_size the window is enlarged to the next
+ * multiple of 32 (adjustPaddingSize, main.cu:158-178). */
+int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const gs_ctf* tctf);
+
+/* Upload and prepare a micrograph (replaces init_d_image,
+ * split_normalize_image and fft_on_img, main.cu:523-655, 758-769).
+ * host: ny rows of nx floats.  The first call fixes nx, ny. */
+int gs_set_micrograph(void* ctx, const float* host, int nx, int ny, const gs_ctf* c);
+/* Same, with the image already in device memory (left unmodified). */
+int gs_set_micrograph_device(void* ctx, const float* dev, int nx, int ny, const gs_ctf* c);
+
+/* Run the in-plane steps [psi_begin, psi_end) of the current micrograph
+ * (replaces the psi loop of main.cu:1230-1257 with pickPartcles_IMG_NORM,
+ * main.cu:824-894, for norm_type 0; main.cu:1170-1229 for norm_type 1).
+ * Picks above the threshold with a centre at least `diameter` from the
+ * edges are returned in the reference's order: psi, then tile (y outer,
+ * x inner), then view (then block for norm_type 1).  If cap is too small,
+ * returns GS_ERR_CAPACITY with *n_out set to the required count.
+ * psi_end < 0 means all steps. */
+int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap, size_t* n_out);
+
+/* Per-map results of the last norm_type 0 gs_pick, [psi][tile][view]. */
+int gs_get_mapstats(void* ctx, gs_mapstat* out, size_t cap, size_t* n_out);
+
+/* Intermediate arrays for stage-level parity checks. */
+int gs_get_template_real(void* ctx, float* out, size_t cap);            /* n*l*l */
+int gs_get_tile_spectra(void* ctx, float* out, size_t cap);             /* T*(l/2+1)*l complex, [t][kx][ky] */
+int gs_get_rotated_spectra(void* ctx, int psi_index, float* out, size_t cap); /* n*(l/2+1)*l complex */
+
+/* Kernel timing probe: prepares the rotated templates of psi_index, then
+ * launches the two correlation kernels `reps` times each on the context
+ * stream over the first batch of maps, timed with hipEvents.  out_ms[0],
+ * out_ms[1] = mean duration of one pass-1 / pass-2 launch; *maps_per_launch
+ * = CC maps one launch covers.  No reference counterpart (measurement). */
+int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* maps_per_launch);
+
+int gs_get_info(void* ctx, gs_info* info);
+int gs_get_timing(void* ctx, gs_timing* t);
+/* Number of CC-map evaluations one gs_pick launch batch holds (tuning). */
+int gs_set_batch(void* ctx, int maps_per_batch);
+int gs_destroy(void* ctx);
+const char* gs_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GISSPA_H */

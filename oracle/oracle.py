@@ -1,0 +1,190 @@
+"""ctypes view of the FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The picker never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class OrParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("apix", "phi_step", "kk", "energy", "cs", "highres", "lowres", "diameter",
+                 "threshold", "ampconst", "a", "b", "b2", "bfactor", "bfactor2", "bfactor3")] + \
+               [(n, ctypes.c_int) for n in
+                ("l", "l_config", "overlap", "phase_flip", "invert", "norm_type")]
+
+
+class OrCtf(ctypes.Structure):
+    _fields_ = [("defocus_um", ctypes.c_double), ("dfdiff", ctypes.c_double),
+                ("dfang", ctypes.c_double)]
+
+
+MAPSTAT = np.dtype([("peak", "<f8"), ("sum", "<f8"), ("sum2", "<f8"), ("score", "<f8"),
+                    ("pos", "<i4"), ("pad", "<i4")])
+PICK1 = np.dtype([("psi_index", "<i4"), ("view", "<i4"), ("cx", "<f8"), ("cy", "<f8"),
+                  ("score", "<f8")])
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        P = ctypes.POINTER(OrParams)
+        C = ctypes.POINTER(OrCtf)
+        lib.or_fft1d.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.or_fft2d.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.or_ctf_ast.argtypes = [ctypes.c_int] * 4 + [P, C, ctypes.c_int]
+        lib.or_ctf_ast.restype = ctypes.c_double
+        lib.or_tile_offsets.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
+        lib.or_template_prep_ctf.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        lib.or_rotate_normalize.argtypes = [P, vp, ctypes.c_int, ctypes.c_float, vp]
+        lib.or_micrograph_prep.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
+        lib.or_correlate_norm0.argtypes = [P, vp, vp, vp]
+        lib.or_pick_norm0.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                      ctypes.c_int]
+        lib.or_pick_norm1.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                      ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int]
+        lib.or_select_norm0.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
+        lib.or_num_psi.argtypes = [ctypes.c_double]
+        lib.or_psi_value.argtypes = [ctypes.c_double, ctypes.c_int]
+        lib.or_psi_value.restype = ctypes.c_float
+        _lib = lib
+    return _lib
+
+
+def params(l: int, **kw) -> OrParams:
+    """Oracle parameters from config-key names, with the reference defaults."""
+    p = OrParams()
+    d = dict(apix=-1, phi_step=-1, kk=-1, energy=-1, cs=-1, highres=-1, lowres=-1, diameter=-1,
+             threshold=7, ampconst=0.07, a=-9.32, b=2.65, b2=0.01908, bfactor=-78.7757,
+             bfactor2=-12.9121, bfactor3=1.28732, overlap=-1, phase_flip=0, invert=0, norm_type=0)
+    alias = {"angpix": "apix", "phistep": "phi_step", "Highres": "highres", "Lowres": "lowres"}
+    for k, v in kw.items():
+        k = alias.get(k, k)
+        if k in d:
+            d[k] = v
+    # the reference's fixed constants are float (DataReader.h:23-25)
+    for k in ("ampconst", "a", "b", "b2", "bfactor", "bfactor2", "bfactor3"):
+        d[k] = float(np.float32(d[k]))
+    for k, v in d.items():
+        if isinstance(getattr(p, k), float):
+            v = float(np.float32(v))  # the reference stores every parameter as float
+        setattr(p, k, type(getattr(p, k))(v))
+    p.l = l
+    p.l_config = kw.get("window_size", l)
+    return p
+
+
+def ctf(defocus: float, dfdiff: float = 0.0, dfang: float = 0.0) -> OrCtf:
+    return OrCtf(defocus, dfdiff, dfang)
+
+
+def fft2(a: np.ndarray, sign: int = -1) -> np.ndarray:
+    x = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    ny, nx = x.shape
+    load().or_fft2d(x.ctypes.data, nx, ny, sign)
+    return x
+
+
+def template_prep(p: OrParams, c: OrCtf, tm: np.ndarray) -> np.ndarray:
+    tm = np.ascontiguousarray(tm, dtype=np.float32)
+    n, by, bx = tm.shape
+    out = np.zeros((n, p.l, p.l), dtype=np.complex128)
+    r = load().or_template_prep_ctf(ctypes.byref(p), ctypes.byref(c), tm.ctypes.data, n, bx, by,
+                                    out.ctypes.data)
+    assert r == 0, r
+    return out
+
+
+def rotate_normalize(p: OrParams, treal: np.ndarray, euler3: float) -> np.ndarray:
+    treal = np.ascontiguousarray(treal, dtype=np.complex128)
+    out = np.zeros_like(treal)
+    load().or_rotate_normalize(ctypes.byref(p), treal.ctypes.data, treal.shape[0], euler3,
+                               out.ctypes.data)
+    return out
+
+
+def tile_offsets(n: int, l: int, overlap: int) -> list[int]:
+    buf = np.zeros(8192, dtype=np.int32)
+    k = load().or_tile_offsets(n, l, overlap, buf.ctypes.data, 8192)
+    return list(buf[:k])
+
+
+def micrograph_prep(p: OrParams, c: OrCtf, img: np.ndarray) -> np.ndarray:
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    ny, nx = img.shape
+    T = len(tile_offsets(nx, p.l, p.overlap)) * len(tile_offsets(ny, p.l, p.overlap))
+    out = np.zeros((T, p.l, p.l), dtype=np.complex128)
+    r = load().or_micrograph_prep(ctypes.byref(p), ctypes.byref(c), img.ctypes.data, nx, ny,
+                                  out.ctypes.data, T)
+    assert r == T, r
+    return out
+
+
+def num_psi(phi_step: float) -> int:
+    return load().or_num_psi(phi_step)
+
+
+def psi_value(phi_step: float, i: int) -> float:
+    return load().or_psi_value(phi_step, i)
+
+
+def pick_norm0(p: OrParams, c: OrCtf, tm: np.ndarray, img: np.ndarray, psi_begin: int = 0,
+               psi_end: int | None = None, nthreads: int = 0) -> np.ndarray:
+    """Per-map results [psi][tile][view] (MAPSTAT records)."""
+    tm = np.ascontiguousarray(tm, dtype=np.float32)
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    n, by, bx = tm.shape
+    ny, nx = img.shape
+    if psi_end is None:
+        psi_end = num_psi(p.phi_step)
+    T = len(tile_offsets(nx, p.l, p.overlap)) * len(tile_offsets(ny, p.l, p.overlap))
+    out = np.zeros((psi_end - psi_begin, T, n), dtype=MAPSTAT)
+    r = load().or_pick_norm0(ctypes.byref(p), ctypes.byref(c), tm.ctypes.data, n, bx, by,
+                             img.ctypes.data, nx, ny, psi_begin, psi_end, out.ctypes.data, nthreads)
+    assert r == T, r
+    return out
+
+
+def select_norm0(p: OrParams, st: np.ndarray, psi_begin: int, nx: int, ny: int) -> np.ndarray:
+    st = np.ascontiguousarray(st)
+    npsi, T, n = st.shape
+    cap = st.size
+    out = np.zeros(max(cap, 1), dtype=PICK1)
+    k = load().or_select_norm0(ctypes.byref(p), st.ctypes.data, npsi, psi_begin, n, nx, ny,
+                               out.ctypes.data, cap)
+    assert k >= 0
+    return out[:k]
+
+
+def pick_norm1(p: OrParams, c: OrCtf, tm: np.ndarray, img: np.ndarray, cap: int = 1 << 20,
+               nthreads: int = 0) -> np.ndarray:
+    tm = np.ascontiguousarray(tm, dtype=np.float32)
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    n, by, bx = tm.shape
+    ny, nx = img.shape
+    out = np.zeros(cap, dtype=PICK1)
+    k = load().or_pick_norm1(ctypes.byref(p), ctypes.byref(c), tm.ctypes.data, n, bx, by,
+                             img.ctypes.data, nx, ny, out.ctypes.data, cap, nthreads)
+    assert k >= 0
+    return out[:min(k, cap)]
