@@ -1,0 +1,393 @@
+// gs_io.cpp -- config / Euler / lst / MRC / output formats of the reference's
+// drop-in surface (Autumn1998/GisSPA EMReader/ and main.cu).
+#include "gs_io.hpp"
+
+#include <sys/stat.h>
+
+#include <cctype>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace gsio {
+
+namespace {
+
+std::string rstrip(std::string s) {
+  while (!s.empty() && (s.back() == '\r' || s.back() == '\n' || s.back() == ' ' || s.back() == '\t'))
+    s.pop_back();
+  return s;
+}
+
+// parse (DataReader.cpp:128-247): option = text up to the first ' ' or '=';
+// value = the rest after skipping ' ' and '='.
+void parse_line(const std::string& line, Config& c) {
+  size_t i = 0;
+  while (i < line.size() && line[i] != ' ' && line[i] != '=') ++i;
+  const std::string option = line.substr(0, i);
+  while (i < line.size() && (line[i] == ' ' || line[i] == '=')) ++i;
+  const std::string value = rstrip(line.substr(i));
+  const char* v = value.c_str();
+  gs_params& p = c.p;
+  if (option == "input") c.input = value;
+  else if (option == "template") c.templ = value;
+  else if (option == "eulerfile") c.eulerf = value;
+  else if (option == "kk") p.kk = (float)atof(v);
+  else if (option == "angpix") p.apix = (float)atof(v);
+  else if (option == "phistep") p.phi_step = (float)atof(v);
+  else if (option == "energy") p.energy = (float)atof(v);
+  else if (option == "cs") p.cs = (float)atof(v);
+  else if (option == "Highres") p.highres = (float)atof(v);
+  else if (option == "Lowres") p.lowres = (float)atof(v);
+  else if (option == "diameter") p.diameter = (float)atof(v);
+  else if (option == "output") c.output = value;
+  else if (option == "first") c.first = atoi(v);
+  else if (option == "last") c.last = atoi(v);
+  else if (option == "threshold") p.threshold = (float)atof(v);
+  else if (option == "GPU_ID") c.device_id = atoi(v);
+  else if (option == "window_size") p.window_size = atoi(v);
+  else if (option == "phase_flip") p.phase_flip = atoi(v);
+  else if (option == "overlap") p.overlap = atoi(v);
+  else if (option == "norm_type") p.norm_type = atoi(v);
+  else if (option == "invert") p.invert = atoi(v);
+  else if (option == "ampconst") p.ampconst = (float)atof(v);  // not a reference key (fixed 0.07 there)
+  else {
+    c.unknown.push_back(value);
+    printf("Undefined option: %s .Abort.\n", value.c_str());
+  }
+}
+
+bool is_machine_big_endian() {
+  const uint32_t one = 1;
+  return reinterpret_cast<const unsigned char*>(&one)[0] == 0;
+}
+
+void swap4(void* p) {
+  unsigned char* c = static_cast<unsigned char*>(p);
+  std::swap(c[0], c[3]);
+  std::swap(c[1], c[2]);
+}
+
+}  // namespace
+
+bool read_config(const std::string& path, Config& c) {
+  gs_params_default(&c.p);
+  std::ifstream in(path);
+  if (!in) {
+    printf("Error => Open config file failed : %s\n\n", path.c_str());
+    return false;
+  }
+  std::string buf;
+  while (std::getline(in, buf)) {
+    buf = rstrip(buf);
+    if (buf.empty() || buf[0] == '#') continue;
+    parse_line(buf, c);
+  }
+  // checkRequestPara (DataReader.cpp:302-315)
+  auto req = [&](bool bad, const char* msg) {
+    if (bad) {
+      c.errors.push_back(msg);
+      printf("Error : %s\n", msg);
+    }
+  };
+  req(c.input.empty(), "lst/star file is requested.");
+  req(c.templ.empty(), "template file is requested.");
+  req(c.eulerf.empty(), "euler file is requested.");
+  req(c.p.apix < 0, "angpix is requested.");
+  req(c.p.phi_step < 0, "phistep is requested.");
+  req(c.p.kk < 0, "kk is requested.");
+  req(c.p.energy < 0, "energy is requested.");
+  req(c.p.cs < 0, "cs is requested.");
+  req(c.p.highres < 0, "highres is requested.");
+  req(c.p.lowres < 0, "lowres is requested.");
+  req(c.p.diameter < 0, "diameter is requested.");
+  return true;
+}
+
+void print_config(const Config& c, FILE* o) {
+  // printAllPara, DataReader.h:44-77
+  const gs_params& p = c.p;
+  fprintf(o, "Parameter requested:\n");
+  fprintf(o, "--input            %s\n", c.input.c_str());
+  fprintf(o, "--template         %s\n", c.templ.c_str());
+  fprintf(o, "--eulerfile        %s\n", c.eulerf.c_str());
+  fprintf(o, "--angpix           %f\n", p.apix);
+  fprintf(o, "--phistep          %f\n", p.phi_step);
+  fprintf(o, "--kk               %f\n", p.kk);
+  fprintf(o, "--energy           %f\n", p.energy);
+  fprintf(o, "--cs               %f\n", p.cs);
+  fprintf(o, "--Highres          %f\n", p.highres);
+  fprintf(o, "--Lowres           %f\n", p.lowres);
+  fprintf(o, "--diameter         %f\n", p.diameter);
+  fprintf(o, "Parameter optional:\n");
+  fprintf(o, "--threshold        %f\n", p.threshold);
+  fprintf(o, "--output           %s\n", c.output.c_str());
+  fprintf(o, "--first            %d\n", c.first);
+  fprintf(o, "--last             %d\n", c.last);
+  fprintf(o, "--window_size      %d\n", p.window_size);
+  fprintf(o, "--GPU_ID           %d\n", c.device_id);
+  fprintf(o, "--phase_flip       %d\n", p.phase_flip);
+  fprintf(o, "--overlap          %d\n", p.overlap);
+  fprintf(o, "--norm_type        %d\n", p.norm_type);
+  fprintf(o, "--invert           %d\n", p.invert);
+  fprintf(o, "--ampconst         %f\n", p.ampconst);
+  fprintf(o, "Parameter fixed:\n");
+  fprintf(o, "--a                %f\n", p.a);
+  fprintf(o, "--b                %f\n", p.b);
+  fprintf(o, "--b2               %f\n", p.b2);
+  fprintf(o, "--bfactor          %f\n", p.bfactor);
+  fprintf(o, "--bfactor2         %f\n", p.bfactor2);
+  fprintf(o, "--bfactor3         %f\n\n", p.bfactor3);
+}
+
+void print_help(FILE* o) {
+  // printHelpMsg, EMReader/util_func.cpp:78-110 (same keys)
+  fprintf(o, "This program detect targets with orientations and tanslations.\n");
+  fprintf(o, "How to use this problem:  \"./main config_file\"  \n");
+  fprintf(o, "All parameters should be set on the config file.\n");
+  fprintf(o, "(for .eg,  \"input  =  /Data/inputfile\" )\n");
+  fprintf(o, "Requested Parameters:\n");
+  fprintf(o, "input            = input filtered images lstfile with ctf information\n");
+  fprintf(o, "template         = input 2D projections templates (MRC stack)\n");
+  fprintf(o, "eulerfile        = euler file with euler values\n");
+  fprintf(o, "angpix           = input pixel size in angstroms\n");
+  fprintf(o, "phistep          = inplane rotation sampling\n");
+  fprintf(o, "kk               = overlapping density parameter, default is 3.\n");
+  fprintf(o, "energy           = accerlerating voltage in kV.\n");
+  fprintf(o, "cs               = spherical aberration in mm.\n");
+  fprintf(o, "Highres          = high resolution cut \n");
+  fprintf(o, "Lowres           = low resolution cut\n");
+  fprintf(o, "diameter         = target diameter in pixels\n");
+  fprintf(o, "Optional Parameters:\n");
+  fprintf(o, "norm_type        = CCG norm type, 0/ 1\n");
+  fprintf(o, "invert           = 0 / 1, choose whether to invert contrast, 0 for skipping and 1 for invert\n");
+  fprintf(o, "threshold        = cc threshold value, only output score beyond this value\n");
+  fprintf(o, "output           = output lstfile filaname\n");
+  fprintf(o, "first            = the first image id to process.\n");
+  fprintf(o, "last             = the last image id to process.\n");
+  fprintf(o, "window_size      = the window size which is splitted from raw IMG.\n");
+  fprintf(o, "GPU_ID           = ID of GPU device.\n");
+  fprintf(o, "phase_flip       = Whether use phase filp operation(1) or not(0).\n");
+  fprintf(o, "overlap          = size of overlap between diff window.\n");
+  fprintf(o, "ampconst         = amplitude contrast of the CTF (default 0.07).\n");
+}
+
+bool read_euler(const std::string& path, Euler& e) {
+  std::ifstream in(path);
+  if (!in) {
+    printf("Error => Open euler file failed : %s\n\n", path.c_str());
+    return false;
+  }
+  std::string buf;
+  while (std::getline(in, buf)) {
+    if (buf.empty() || buf[0] == '#') continue;
+    float a = 0, b = 0;
+    sscanf(buf.c_str(), "%f %f", &a, &b);
+    e.alt.push_back(a);
+    e.az.push_back(b);
+  }
+  return true;
+}
+
+int read_lst_entry(const std::string& lst, int n, MicEntry& out) {
+  const int MAXLEN = 1024 + 10;
+  std::vector<char> buf(MAXLEN + 1), t(MAXLEN + 1), u(MAXLEN + 1);
+  FILE* in = fopen(lst.c_str(), "rb");
+  if (!in) {
+    printf("'%s' not found\n", lst.c_str());
+    return -3;
+  }
+  int nn = 0;
+  t[0] = 0;
+  u[0] = 0;
+  if (!fgets(buf.data(), MAXLEN, in)) {
+    fclose(in);
+    return -1;
+  }
+  if (strncmp(buf.data(), "#LSX", 4) == 0) {
+    int step = 0;
+    if (!fgets(buf.data(), MAXLEN, in) || !fgets(buf.data(), MAXLEN, in)) {
+      fclose(in);
+      return -1;
+    }
+    sscanf(buf.data() + 1, " %d", &step);
+    if (step == 0) {
+      fclose(in);
+      return -1;
+    }
+    long tl = ftell(in);
+    if (n > 0) fseek(in, (long)step * n, SEEK_CUR);
+    if (!fgets(buf.data(), MAXLEN, in)) {
+      fclose(in);
+      return -2;
+    }
+    if ((ftell(in) - tl) % step != 0) {
+      printf("Error in #LSX file detected. Please rerun lstfast.py\n");
+      fclose(in);
+      return -3;
+    }
+    sscanf(buf.data(), " %d %1034s %1034[^\n]", &nn, t.data(), u.data());
+  } else {
+    // the first line is a header and is skipped (DataReader.cpp:351-360)
+    if (n < 0) n = 0;
+    int i;
+    for (i = 0; i <= n; i++) {
+      u[0] = 0;
+      if (!fgets(buf.data(), MAXLEN, in)) break;
+      if (buf[0] == '#') i--;
+      else sscanf(buf.data(), " %d %1034s %1034[^\n]", &nn, t.data(), u.data());
+    }
+    if (i <= n) {
+      fclose(in);
+      return -1;
+    }
+  }
+  fclose(in);
+  out.nn = nn;
+  // addPrefix, main.cu:125-138: the lst file's directory
+  const size_t slash = lst.rfind('/');
+  std::string prefix = (slash == std::string::npos) ? "" : lst.substr(0, slash + 1);
+  out.path = prefix + std::string(t.data());
+  out.defocus = out.dfdiff = out.dfang = 0;
+  // Split on whitespace + parsePairs (DataReader.cpp:366-387)
+  std::istringstream ss(std::string(u.data()));
+  std::string tok;
+  while (ss >> tok) {
+    if (tok.compare(0, 8, "defocus=") == 0) sscanf(tok.c_str() + 8, "%f", &out.defocus);
+    else if (tok.compare(0, 7, "dfdiff=") == 0) sscanf(tok.c_str() + 7, "%f", &out.dfdiff);
+    else if (tok.compare(0, 6, "dfang=") == 0) sscanf(tok.c_str() + 6, "%f", &out.dfang);
+  }
+  return 1;
+}
+
+namespace {
+struct MrcHeader {
+  int32_t w[56];
+  char labels[10][80];
+};
+static_assert(sizeof(MrcHeader) == 1024, "MRC header is 1024 bytes");
+
+int open_mrc(const std::string& path, FILE** fp, MrcHeader& h, bool& swap, std::string* err) {
+  *fp = fopen(path.c_str(), "rb");
+  if (!*fp) {
+    if (err) *err = "Cannot open MRC file " + path;
+    return -1;
+  }
+  if (fread(&h, sizeof(h), 1, *fp) != 1) {
+    if (err) *err = "short MRC header " + path;
+    fclose(*fp);
+    return -1;
+  }
+  const unsigned char* m2 = reinterpret_cast<const unsigned char*>(&h);
+  const bool ord = (m2[0] == 0 && m2[1] == 0);
+  swap = ord ^ is_machine_big_endian();
+  if (swap)
+    for (int i = 0; i < 56; ++i) swap4(&h.w[i]);
+  const int nx = h.w[0], ny = h.w[1], nz = h.w[2], mode = h.w[3], nlabl = h.w[55];
+  if (ny > 0xffff || nz > 0xffff || mode > 4 || mode < 0 || nlabl > 11 || nx <= 0 || ny <= 0 ||
+      nz <= 0) {
+    if (err) *err = "Invalid MRC file " + path;
+    fclose(*fp);
+    return -1;
+  }
+  if (ny == 1) {
+    if (err) *err = "This appears to be a 1d file. I only read 2d files.";
+    fclose(*fp);
+    return -4;
+  }
+  return 0;
+}
+}  // namespace
+
+int mrc_sections(const std::string& path) {
+  FILE* fp;
+  MrcHeader h;
+  bool sw;
+  if (open_mrc(path, &fp, h, sw, nullptr)) return -1;
+  fclose(fp);
+  return h.w[2];
+}
+
+int read_mrc(const std::string& path, int n, Image& img, std::string* err) {
+  FILE* fp;
+  MrcHeader h;
+  bool swap;
+  int r = open_mrc(path, &fp, h, swap, err);
+  if (r) return r;
+  const int nx = h.w[0], ny = h.w[1], nzf = h.w[2], mode = h.w[3], nsym = h.w[23];
+  if (mode == 3 || mode == 4) {
+    if (err) *err = "complex MRC modes are not images";
+    fclose(fp);
+    return -1;
+  }
+  int nz, startz;
+  if (n >= 0 && n < nzf) { nz = 1; startz = n; }
+  else { nz = nzf; startz = 0; }
+  const int bpp = (mode == 0) ? 1 : (mode == 1 ? 2 : 4);
+  const size_t npix = (size_t)nx * ny * nz;
+  std::vector<unsigned char> raw(npix * bpp);
+  if (fseeko(fp, (off_t)1024 + nsym + (off_t)startz * nx * ny * bpp, SEEK_SET) != 0 ||
+      fread(raw.data(), 1, raw.size(), fp) != raw.size()) {
+    if (err) *err = "Incomplete data read " + path;
+    fclose(fp);
+    return -1;
+  }
+  fclose(fp);
+  img.nx = nx; img.ny = ny; img.nz = nz;
+  img.data.resize(npix);
+  for (size_t i = 0; i < npix; ++i) {
+    if (mode == 0) {
+      img.data[i] = (float)raw[i] / 100.0f - 1.28f;  // emdata_.cpp:510
+    } else if (mode == 1) {
+      unsigned char b[2] = {raw[2 * i], raw[2 * i + 1]};
+      if (swap) std::swap(b[0], b[1]);
+      int16_t s;
+      memcpy(&s, b, 2);
+      img.data[i] = (float)s;
+    } else {
+      unsigned char b[4] = {raw[4 * i], raw[4 * i + 1], raw[4 * i + 2], raw[4 * i + 3]};
+      if (swap) swap4(b);
+      memcpy(&img.data[i], b, 4);
+    }
+  }
+  return 0;
+}
+
+int write_mrc(const std::string& path, const float* data, int nx, int ny, int nz) {
+  MrcHeader h;
+  memset(&h, 0, sizeof(h));
+  h.w[0] = nx; h.w[1] = ny; h.w[2] = nz; h.w[3] = 2;
+  h.w[7] = nx; h.w[8] = ny; h.w[9] = nz;
+  float cell[3] = {(float)nx, (float)ny, (float)nz};
+  memcpy(&h.w[10], cell, sizeof(cell));
+  float ang[3] = {90, 90, 90};
+  memcpy(&h.w[13], ang, sizeof(ang));
+  h.w[16] = 1; h.w[17] = 2; h.w[18] = 3;
+  memcpy(&h.w[52], "MAP ", 4);
+  h.w[53] = 0x00004444;
+  FILE* fp = fopen(path.c_str(), "wb");
+  if (!fp) return -1;
+  fwrite(&h, sizeof(h), 1, fp);
+  fwrite(data, sizeof(float), (size_t)nx * ny * nz, fp);
+  fclose(fp);
+  return 0;
+}
+
+void write_pick_line(FILE* fp, int nn, const std::string& path, const MicEntry& m, float euler1,
+                     float euler2, float psi, float cx, float cy, float score) {
+  // main.cu:985-987: defocus is printed as (-1) * (negated lst value)
+  const float neg = -m.defocus;
+  fprintf(fp, "%d\t%s\tdefocus=%f\tdfdiff=%f\tdfang=%f\teuler=%f,%f,%f\tcenter=%f,%f\tscore=%f\n",
+          nn, path.c_str(), (-1) * neg, m.dfdiff, m.dfang, euler1, euler2, psi, cx, cy, score);
+}
+
+void create_outpath(const std::string& path) {
+  for (size_t pos = 0; pos < path.size(); ++pos) {
+    if (path[pos] != '/' || pos == 0) continue;
+    mkdir(path.substr(0, pos).c_str(), S_IRWXU | S_IRWXG | S_IROTH | S_IXOTH);
+  }
+}
+
+}  // namespace gsio

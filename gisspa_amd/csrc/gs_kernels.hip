@@ -133,8 +133,11 @@ __device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x,
     const float dm2 = s.dmask * 0.5f;  // d_m = 1.5 * diameter, radius d_m / 2
     const float ehw = 6.0f;
     if (r > dm2 + 2 * ehw) return 0.0f;
-    const float x2 = (s.cose * xf + s.sine * yf) + (float)(L / 2);
-    const float y2 = (-s.sine * xf + s.cose * yf) + (float)(L / 2);
+    // the reference's float expression, rounded step by step (no FMA), so the
+    // out-of-window test and the interpolation cell are those of
+    // fl(fl(fl(c x) + fl(s y)) + L/2)
+    const float x2 = __fadd_rn(__fadd_rn(__fmul_rn(s.cose, xf), __fmul_rn(s.sine, yf)), (float)(L / 2));
+    const float y2 = __fadd_rn(__fadd_rn(__fmul_rn(-s.sine, xf), __fmul_rn(s.cose, yf)), (float)(L / 2));
     float v = 0.0f;
     if (!(x2 < 0 || x2 > L - 1.0f || y2 < 0 || y2 > L - 1.0f)) {
       const int ii = (int)floorf(x2), jj = (int)floorf(y2);

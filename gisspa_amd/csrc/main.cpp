@@ -1,0 +1,139 @@
+// main.cpp -- `./main config_file`: the reference's command line
+// (Autumn1998/GisSPA main.cu:1025-1299) over the libgisspa C ABI.
+//
+// Same config keys, Euler .lst, micrograph .lst and output lines.  Errors end
+// the run with a message and a non-zero status instead of continuing on a
+// 10x10 zero image (emdata_.cpp:184-203) or garbage defaults.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gisspa.h"
+#include "gs_io.hpp"
+
+using namespace gsio;
+
+static int fail(const char* what, int code) {
+  fprintf(stderr, "Error: %s: %s (%d)\n", what, gs_strerror(code), code);
+  return 1;
+}
+
+int main(int argc, char** argv) {
+  if (argc == 1 || (argv[1][0] == '-' && argv[1][1] == 'h')) {
+    print_help(stdout);
+    return 0;
+  }
+  Config cfg;
+  if (!read_config(argv[1], cfg)) return 1;
+  print_config(cfg, stdout);
+  if (!cfg.errors.empty()) return 1;
+
+  Euler euler;
+  if (!read_euler(cfg.eulerf, euler)) return 1;
+  const int N = (int)euler.alt.size();
+  if (N == 0) {
+    fprintf(stderr, "Error: no views in %s\n", cfg.eulerf.c_str());
+    return 1;
+  }
+
+  create_outpath(cfg.output);
+  FILE* fp = fopen(cfg.output.c_str(), "wb");
+  if (!fp) {
+    printf("Out file create failed! %s\n", cfg.output.c_str());
+    printf("Please confirm the path exists!\n");
+    return 1;
+  }
+
+  // first image: size and the CTF used for the template weighting (main.cu:1106)
+  MicEntry first;
+  int r = read_lst_entry(cfg.input, cfg.first, first);
+  if (r < 0) {
+    printf("Error %d occured in readRawImage!\n", r);
+    return 1;
+  }
+  Image img;
+  std::string err;
+  if (read_mrc(first.path, first.nn, img, &err)) {
+    fprintf(stderr, "Error: %s\n", err.c_str());
+    return 1;
+  }
+  const int nx = img.nx, ny = img.ny;
+
+  // templates: the first N sections of the template stack
+  const int nsec = mrc_sections(cfg.templ);
+  if (nsec < N) {
+    fprintf(stderr, "Error: %s holds %d templates, the Euler file lists %d\n", cfg.templ.c_str(),
+            nsec, N);
+    return 1;
+  }
+  Image t0;
+  if (read_mrc(cfg.templ, 0, t0, &err)) {
+    fprintf(stderr, "Error: %s\n", err.c_str());
+    return 1;
+  }
+  const int bx = t0.nx, by = t0.ny;
+  std::vector<float> tm((size_t)N * bx * by);
+  {
+    Image all;
+    if (read_mrc(cfg.templ, -1, all, &err)) {
+      fprintf(stderr, "Error: %s\n", err.c_str());
+      return 1;
+    }
+    memcpy(tm.data(), all.data.data(), tm.size() * sizeof(float));
+  }
+
+  void* ctx = nullptr;
+  if ((r = gs_create(&cfg.p, cfg.device_id, &ctx)) != GS_OK) return fail("gs_create", r);
+  gs_ctf tctf{first.defocus, first.dfdiff, first.dfang};
+  if ((r = gs_set_templates(ctx, tm.data(), N, bx, by, &tctf)) != GS_OK)
+    return fail("gs_set_templates", r);
+  gs_info info;
+  gs_get_info(ctx, &info);
+  if (info.l != cfg.p.window_size)
+    printf("Aujust padding size to %d\n", info.l);
+
+  const auto t_start = std::chrono::steady_clock::now();
+  std::vector<gs_particle> picks(1 << 16);
+  for (int n = cfg.first; n < cfg.last; ++n) {
+    MicEntry e;
+    if ((r = read_lst_entry(cfg.input, n, e)) < 0) {
+      printf("Error %d occured in readRawImage!\n", r);
+      return 1;
+    }
+    Image im;
+    if (read_mrc(e.path, e.nn, im, &err)) {
+      fprintf(stderr, "Error: %s\n", err.c_str());
+      return 1;
+    }
+    if (im.nx != nx || im.ny != ny) {
+      fprintf(stderr, "Error: %s is %dx%d, the first micrograph is %dx%d\n", e.path.c_str(), im.nx,
+              im.ny, nx, ny);
+      return 1;
+    }
+    gs_ctf c{e.defocus, e.dfdiff, e.dfang};
+    if ((r = gs_set_micrograph(ctx, im.data.data(), nx, ny, &c)) != GS_OK)
+      return fail("gs_set_micrograph", r);
+    size_t np = 0;
+    r = gs_pick(ctx, 0, -1, picks.data(), picks.size(), &np);
+    if (r == GS_ERR_CAPACITY) {
+      picks.resize(np);
+      r = gs_pick(ctx, 0, -1, picks.data(), picks.size(), &np);
+    }
+    if (r != GS_OK) return fail("gs_pick", r);
+    for (size_t i = 0; i < np; ++i) {
+      const gs_particle& pk = picks[i];
+      write_pick_line(fp, e.nn, e.path, e, euler.alt[pk.view], euler.az[pk.view], pk.psi, pk.cx,
+                      pk.cy, pk.score);
+    }
+    fflush(fp);
+    printf("------------------------Image %d finished--------------------------------\n", n);
+  }
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  printf("Total consumed time is: %f seconds\n", secs);
+  gs_destroy(ctx);
+  fclose(fp);
+  return 0;
+}

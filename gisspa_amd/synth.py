@@ -153,11 +153,14 @@ class Micrograph:
 def make_micrograph(nx: int, ny: int, dens: BlobDensity, view_eulers: np.ndarray,
                     n_particles: int, snr: float, seed: int, apix: float, defocus_um: float,
                     cs_mm: float = 2.7, kv: float = 300.0, ampconst: float = 0.1,
-                    box: int = 128, psi_step: float = 10.0, margin: int | None = None) -> Micrograph:
+                    box: int = 128, psi_step: float = 10.0, margin: int | None = None,
+                    phase_flipped: bool = True) -> Micrograph:
     """Gaussian noise (sigma 1) plus n CTF-modulated particles at SNR `snr`
     (signal variance over noise variance inside one box).  Particle views are
     drawn from view_eulers and in-plane angles from the psi grid, so each
-    particle is an exact template match up to interpolation."""
+    particle is an exact template match up to interpolation.  With
+    phase_flipped (the input the picker expects when phase_flip = 0) the
+    signal carries |CTF|; otherwise the signed CTF."""
     rng = np.random.default_rng(seed)
     if margin is None:
         margin = box
@@ -177,11 +180,15 @@ def make_micrograph(nx: int, ny: int, dens: BlobDensity, view_eulers: np.ndarray
         parts.append(Particle(x, y, v, e))
     if n_particles:
         C = ctf_2d(nx, ny, apix, defocus_um, cs_mm, kv, ampconst)
+        if phase_flipped:
+            C = np.abs(C)
         sig = np.real(np.fft.ifft2(np.fft.fft2(sig) * C))
         # scale: variance of one isolated particle inside its box
         c2, s, a = projected_blobs(dens, view_eulers[0][0], view_eulers[0][1])
         one = render_blobs(c2, s, a, 2 * box, 2 * box, box, box)
         Cb = ctf_2d(2 * box, 2 * box, apix, defocus_um, cs_mm, kv, ampconst)
+        if phase_flipped:
+            Cb = np.abs(Cb)
         one = np.real(np.fft.ifft2(np.fft.fft2(one) * Cb))[box // 2: box // 2 + box, box // 2: box // 2 + box]
         v1 = float(one.var())
         if v1 > 0:

