@@ -90,6 +90,7 @@ def load() -> ctypes.CDLL:
                                              ctypes.POINTER(GsCtf)]
     lib.gs_pick.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t,
                             ctypes.POINTER(ctypes.c_size_t)]
+    lib.gs_get_picks.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     lib.gs_get_mapstats.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     lib.gs_get_template_real.argtypes = [vp, vp, ctypes.c_size_t]
     lib.gs_get_tile_spectra.argtypes = [vp, vp, ctypes.c_size_t]
@@ -186,17 +187,16 @@ class Picker:
                                                  ctypes.byref(c)), "gs_set_micrograph_device")
 
     def pick(self, psi_begin: int = 0, psi_end: int = -1) -> np.ndarray:
+        """Picks of in-plane steps [psi_begin, psi_end) (PARTICLE_DTYPE records)."""
         n = ctypes.c_size_t(0)
-        cap = 4096
-        while True:
-            out = np.zeros(cap, dtype=PARTICLE_DTYPE)
-            r = self.lib.gs_pick(self.ctx, psi_begin, psi_end, out.ctypes.data, cap, ctypes.byref(n))
-            if r == GS_ERR_CAPACITY and n.value > cap:
-                cap = n.value
-                # the maps are already computed; a second call recomputes them
-                continue
+        r = self.lib.gs_pick(self.ctx, psi_begin, psi_end, None, 0, ctypes.byref(n))
+        if r != GS_ERR_CAPACITY:
             _check(r, "gs_pick")
-            return out[: n.value].copy()
+        out = np.zeros(n.value, dtype=PARTICLE_DTYPE)
+        if n.value:
+            _check(self.lib.gs_get_picks(self.ctx, out.ctypes.data, n.value, ctypes.byref(n)),
+                   "gs_get_picks")
+        return out
 
     def mapstats(self) -> np.ndarray:
         n = ctypes.c_size_t(0)

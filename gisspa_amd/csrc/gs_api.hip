@@ -72,9 +72,16 @@ struct Ctx {
   DevBuf img, img_part, img_stats, img_mean, offx_d, offy_d, tiles;
   std::vector<int> offx, offy;
   int bxn = 0, byn = 0, T = 0;
-  // correlation
+  // correlation: two lanes (stream + intermediate + partials) so that pass 1
+  // of one batch overlaps pass 2 of the previous one
+  static constexpr int NLANE = 2;
   int batch = 0;
-  DevBuf G, part, maps, stats, sel, sel_count;
+  hipStream_t lane_stream[NLANE] = {nullptr, nullptr};
+  hipEvent_t ev_ready = nullptr, ev_lane[NLANE] = {nullptr, nullptr};
+  DevBuf G[NLANE];
+  DevBuf part;  // row-block partials of every map of one psi
+  DevBuf maps, stats, sel, sel_count;
+  std::vector<gs_particle> picks;  // of the last gs_pick
   int stats_psi_begin = 0, stats_npsi = 0;
   gs_timing timing{};
 };
@@ -303,11 +310,11 @@ int rotate_templates(Ctx* c, int psi_index) {
 int ensure_corr_buffers(Ctx* c) {
   const int L = c->L, HP = c->geo.HP;
   const long per_map = (long)L * HP * sizeof(float2);
-  if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (256l << 20) / per_map));
+  if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (128l << 20) / per_map));
   const int nb = std::min(c->batch, c->T * c->N);
-  CHK(c->G.ensure((size_t)nb * per_map));
   const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
-  CHK(c->part.ensure((size_t)nb * nrb * sizeof(CorrPartial)));
+  for (int i = 0; i < Ctx::NLANE; ++i) CHK(c->G[i].ensure((size_t)nb * per_map));
+  CHK(c->part.ensure((size_t)c->T * c->N * nrb * sizeof(CorrPartial)));
   // canonical map order: tile-major, view-minor
   const size_t nm = (size_t)c->T * c->N;
   if (c->maps.bytes < nm * sizeof(MapRef)) {
@@ -321,26 +328,36 @@ int ensure_corr_buffers(Ctx* c) {
   return GS_OK;
 }
 
-int correlate_psi(Ctx* c, MapStat* out, hipEvent_t e_mid, hipEvent_t e_end) {
+int correlate_psi(Ctx* c, MapStat* out) {
   const int L = c->L;
   const int nm = c->T * c->N;
   const int nb = std::min(c->batch, nm);
   const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;  // compute_corner_CCG `of` rule
   const float lo = c->p.diameter / 4.0f;
   const float hi = (float)L - c->p.diameter / 4.0f;
-  for (int m0 = 0; m0 < nm; m0 += nb) {
+  // the lanes start after the rotated templates of this psi are ready
+  HIPCHK(hipEventRecord(c->ev_ready, c->stream));
+  for (int i = 0; i < Ctx::NLANE; ++i) HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
+  int lane = 0;
+  for (int m0 = 0; m0 < nm; m0 += nb, lane = (lane + 1) % Ctx::NLANE) {
     const int cnt = std::min(nb, nm - m0);
+    hipStream_t ls = c->lane_stream[lane];
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>() + m0,
-                      cnt, sgn, c->G.as<float2>(), c->twL.as<float2>(), c->stream);
-    launch_corr_pass2(L, c->G.as<float2>(), cnt, lo, hi, c->part.as<CorrPartial>(),
-                      c->twL.as<float2>(), c->twM.as<float2>(), c->stream);
-    launch_corr_finalize(L, c->part.as<CorrPartial>(), cnt, out + m0, c->stream);
+                      cnt, sgn, c->G[lane].as<float2>(), c->twL.as<float2>(), ls);
+    launch_corr_pass2(L, c->G[lane].as<float2>(), cnt, lo, hi,
+                      c->part.as<CorrPartial>() + (size_t)m0 * corr_nparts(L),
+                      c->twL.as<float2>(), c->twM.as<float2>(), ls);
     c->timing.corr_pass1_launches++;
     c->timing.corr_pass2_launches++;
   }
+  // the context stream (finalisation, next psi's rotation, selection) waits
+  // for both lanes
+  for (int i = 0; i < Ctx::NLANE; ++i) {
+    HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_lane[i], 0));
+  }
+  launch_corr_finalize(L, c->part.as<CorrPartial>(), nm, out, c->stream);
   c->timing.corr_maps += nm;
-  (void)e_mid;
-  (void)e_end;
   return GS_OK;
 }
 
@@ -413,7 +430,13 @@ int gs_create(const gs_params* p, int device, void** out) {
   c->p = *p;
   c->device = device;
   c->L_config = p->window_size;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int i = 0; ok && i < Ctx::NLANE; ++i)
+    ok = hipStreamCreateWithFlags(&c->lane_stream[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&c->ev_lane[i], hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
     delete c;
     return GS_ERR_DEVICE;
   }
@@ -425,7 +448,12 @@ int gs_destroy(void* ctx) {
   Ctx* c = static_cast<Ctx*>(ctx);
   if (!c) return GS_ERR_ARG;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  (void)hipDeviceSynchronize();
+  for (int i = 0; i < Ctx::NLANE; ++i) {
+    if (c->lane_stream[i]) (void)hipStreamDestroy(c->lane_stream[i]);
+    if (c->ev_lane[i]) (void)hipEventDestroy(c->ev_lane[i]);
+  }
+  if (c->ev_ready) (void)hipEventDestroy(c->ev_ready);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return GS_OK;
@@ -464,9 +492,10 @@ int gs_set_micrograph_device(void* ctx, const float* dev, int nx, int ny, const 
 int gs_set_batch(void* ctx, int maps_per_batch) {
   Ctx* c = static_cast<Ctx*>(ctx);
   if (!c || maps_per_batch < 0) return GS_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
   c->batch = maps_per_batch;
-  c->G.release();
-  c->part.release();
+  for (int i = 0; i < Ctx::NLANE; ++i) c->G[i].release();
   return GS_OK;
 }
 
@@ -495,8 +524,7 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   HIPCHK(hipEventRecord(e0, c->stream));
   for (int ip = psi_begin; ip < psi_end; ++ip) {
     CHK(rotate_templates(c, ip));
-    CHK(correlate_psi(c, c->stats.as<MapStat>() + (size_t)(ip - psi_begin) * per_psi, nullptr,
-                      nullptr));
+    CHK(correlate_psi(c, c->stats.as<MapStat>() + (size_t)(ip - psi_begin) * per_psi));
   }
   // device-side selection, then canonical order on the host
   const long long nmaps = npsi * per_psi;
@@ -524,13 +552,12 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   if (count)
     CHK(copy_out(c, recs.data(), c->sel.p, (size_t)count * sizeof(SelRec)));
   std::sort(recs.begin(), recs.end(), [](const SelRec& a, const SelRec& b) { return a.m < b.m; });
-  *n_out = (size_t)count;
-  if ((size_t)count > cap || (count && !out)) return GS_ERR_CAPACITY;
+  c->picks.resize(count);
   for (int i = 0; i < count; ++i) {
     const long long m = recs[i].m;
     const int ip = (int)(m / per_psi) + psi_begin;
     const long long rem = m % per_psi;
-    gs_particle& pk = out[i];
+    gs_particle& pk = c->picks[i];
     pk.psi_index = ip;
     pk.tile = (int)(rem / c->N);
     pk.view = (int)(rem % c->N);
@@ -541,6 +568,15 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
     pk.cy = recs[i].cy;
     pk.score = recs[i].score;
   }
+  return gs_get_picks(ctx, out, cap, n_out);
+}
+
+int gs_get_picks(void* ctx, gs_particle* out, size_t cap, size_t* n_out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !n_out) return GS_ERR_ARG;
+  *n_out = c->picks.size();
+  if (c->picks.size() > cap || (!c->picks.empty() && !out)) return GS_ERR_CAPACITY;
+  if (!c->picks.empty()) std::memcpy(out, c->picks.data(), c->picks.size() * sizeof(gs_particle));
   return GS_OK;
 }
 
@@ -604,12 +640,13 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
   hipEvent_t e[3];
   for (auto& x : e) HIPCHK(hipEventCreate(&x));
   HIPCHK(hipEventRecord(e[0], c->stream));
+  CHK(c->stats.ensure((size_t)nb * sizeof(MapStat)));
   for (int i = 0; i < reps; ++i)
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>(), nb,
-                      sgn, c->G.as<float2>(), c->twL.as<float2>(), c->stream);
+                      sgn, c->G[0].as<float2>(), c->twL.as<float2>(), c->stream);
   HIPCHK(hipEventRecord(e[1], c->stream));
   for (int i = 0; i < reps; ++i)
-    launch_corr_pass2(L, c->G.as<float2>(), nb, lo, hi, c->part.as<CorrPartial>(),
+    launch_corr_pass2(L, c->G[0].as<float2>(), nb, lo, hi, c->part.as<CorrPartial>(),
                       c->twL.as<float2>(), c->twM.as<float2>(), c->stream);
   HIPCHK(hipEventRecord(e[2], c->stream));
   HIPCHK(hipEventSynchronize(e[2]));

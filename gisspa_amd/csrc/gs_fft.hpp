@@ -242,46 +242,109 @@ struct Dft {
 struct Plan {
   int count;
   int r[16];
+  int pad_shift = 4;  // LDS layout: element i at i + (i >> pad_shift) * pad_mult
+  int pad_mult = 1;
 };
 
-constexpr Plan make_plan(int n) {
-  Plan p{0, {0}};
-  int a = 0, b = 0, c = 0, d = 0;
-  while (n % 2 == 0) { n /= 2; ++a; }
-  while (n % 3 == 0) { n /= 3; ++b; }
-  while (n % 5 == 0) { n /= 5; ++c; }
-  while (n % 7 == 0) { n /= 7; ++d; }
-  // n must now be 1 (checked by the static_assert in the user)
-  int rad[16] = {0};
-  int k = 0;
-  // odd radices first, pairing a factor 2 with each 5 and 7 (radix 10/14),
-  // and 3*3 -> 9
-  while (b >= 2) { rad[k++] = 9; b -= 2; }
-  while (c > 0) {
-    if (a > 0 && a % 3 != 0) { rad[k++] = 10; --a; } else rad[k++] = 5;
-    --c;
+constexpr double ct_log2(int r) {
+  double v = 0;
+  double x = r;
+  while (x >= 2) { x /= 2; v += 1; }
+  // linear interpolation of the fractional part is enough for a cost model
+  return v + (x - 1);
+}
+
+// Cost of running radices rad[0..n) over NB transforms of length N with WG
+// threads: each pass costs (butterflies per thread, rounded up) * R log2 R
+// register work, and every LDS exchange between passes costs a read and a
+// write of the whole working set.
+constexpr double plan_cost(int N, int NB, int WG, const int* rad, int n) {
+  double c = 0;
+  for (int i = 0; i < n; ++i) {
+    const int bfly = NB * (N / rad[i]);
+    const int per = (bfly + WG - 1) / WG;
+    c += per * rad[i] * ct_log2(rad[i]);
   }
-  while (d > 0) {
-    if (a > 0 && a % 3 != 0) { rad[k++] = 14; --a; } else rad[k++] = 7;
-    --d;
+  c += (n - 1) * 4.0 * N * NB / WG;
+  return c;
+}
+
+// Fewest-work factorisation of N into at most three radices <= 32 (each a
+// product of 2, 3, 5, 7 handled by the register DFTs); ties keep the first
+// found (larger leading radix).
+constexpr Plan make_plan(int N, int NB = 1, int WG = 256) {
+  Plan best{0, {0}};
+  double bc = 1e300;
+  for (int a = 32; a >= 2; --a) {
+    if (N % a) continue;
+    const int na = N / a;
+    if (na == 1) {
+      int r[1] = {a};
+      const double c = plan_cost(N, NB, WG, r, 1);
+      if (c < bc) { bc = c; best = Plan{1, {a}}; }
+      continue;
+    }
+    for (int b = 32; b >= 2; --b) {
+      if (na % b) continue;
+      const int nb = na / b;
+      if (nb == 1) {
+        int r[2] = {a, b};
+        const double c = plan_cost(N, NB, WG, r, 2);
+        if (c < bc) { bc = c; best = Plan{2, {a, b}}; }
+        continue;
+      }
+      if (nb <= 32) {
+        int r[3] = {a, b, nb};
+        const double c = plan_cost(N, NB, WG, r, 3);
+        if (c < bc) { bc = c; best = Plan{3, {a, b, nb}}; }
+      }
+    }
   }
-  if (b == 1) {
-    if (a >= 2 && a % 3 != 0) { rad[k++] = 12; a -= 2; }
-    else if (a >= 1 && a % 3 != 0) { rad[k++] = 6; a -= 1; }
-    else rad[k++] = 3;
+  if (best.count == 0) {
+    // long transforms: greedy radix-16/8 then the remainder
+    int n = N, k = 0;
+    for (int r : {16, 8, 9, 10, 12, 15, 14, 7, 6, 5, 4, 3, 2})
+      while (n % r == 0 && n > 1 && k < 15) { best.r[k++] = r; n /= r; }
+    best.count = k;
   }
-  // powers of two: 8s, with a 16 or 4/2 for the remainder
-  while (a >= 3) {
-    if (a == 4) { rad[k++] = 16; a -= 4; break; }
-    rad[k++] = 8; a -= 3;
-  }
-  if (a == 2) rad[k++] = 4;
-  if (a == 1) rad[k++] = 2;
-  // order: largest power-of-two radix first (fewest conflicts on the first
-  // write pass), then the rest
-  for (int i = 0; i < k; ++i) p.r[i] = rad[k - 1 - i];
-  p.count = k;
-  return p;
+  return best;
+}
+
+// Radix order and LDS padding per (length, transforms per workgroup, last-pass
+// mapping), chosen offline with a bank-conflict model of the executor's LDS
+// traffic (32 banks, 16 lanes of b64 accesses per cycle) among the plans whose
+// register work is within 5% of the best.
+struct TunedPlan {
+  int n, nb, mapl, count, r0, r1, r2, shift, mult;
+};
+constexpr TunedPlan kTuned[] = {
+    {64, 32, 1, 2, 8, 8, 0, 2, 1},     {32, 32, 0, 2, 4, 8, 0, 1, 3},
+    {96, 32, 1, 2, 12, 8, 0, 1, 1},    {48, 32, 0, 2, 6, 8, 0, 2, 2},
+    {128, 32, 1, 2, 8, 16, 0, 3, 1},   {64, 32, 0, 2, 8, 8, 0, 2, 1},
+    {160, 32, 1, 2, 20, 8, 0, 1, 1},   {80, 32, 0, 2, 10, 8, 0, 1, 0},
+    {192, 32, 1, 2, 24, 8, 0, 2, 1},   {96, 32, 0, 2, 12, 8, 0, 1, 1},
+    {256, 32, 1, 2, 8, 32, 0, 3, 1},   {128, 32, 0, 2, 16, 8, 0, 4, 2},
+    {320, 16, 1, 2, 20, 16, 0, 2, 1},  {160, 32, 0, 2, 20, 8, 0, 1, 1},
+    {384, 16, 1, 2, 24, 16, 0, 3, 1},  {192, 32, 0, 2, 24, 8, 0, 2, 1},
+    {448, 16, 1, 2, 28, 16, 0, 2, 1},  {224, 32, 0, 2, 8, 28, 0, 5, 1},
+    {512, 16, 1, 2, 16, 32, 0, 4, 1},  {256, 32, 0, 2, 16, 16, 0, 4, 1},
+    {576, 8, 1, 2, 24, 24, 0, 6, 1},   {288, 16, 0, 2, 16, 18, 0, 4, 1},
+    {640, 8, 1, 2, 20, 32, 0, 2, 1},   {320, 16, 0, 2, 16, 20, 0, 4, 1},
+    {720, 8, 1, 2, 30, 24, 0, 6, 3},   {360, 16, 0, 2, 30, 12, 0, 1, 0},
+    {768, 8, 1, 2, 24, 32, 0, 3, 1},   {384, 16, 0, 2, 16, 24, 0, 4, 1},
+    {960, 8, 1, 2, 30, 32, 0, 6, 3},   {480, 16, 0, 2, 16, 30, 0, 4, 1},
+    {1024, 8, 1, 2, 32, 32, 0, 5, 1},  {512, 16, 0, 2, 16, 32, 0, 4, 1},
+};
+
+constexpr Plan tuned_plan(int N, int NB, int WG, int MAPL) {
+  for (const TunedPlan& t : kTuned)
+    if (t.n == N && t.nb == NB && t.mapl == MAPL && WG == 256) {
+      Plan p{t.count, {t.r0, t.r1, t.r2}};
+      p.pad_shift = t.shift;
+      p.pad_mult = t.mult;
+      return p;
+    }
+  return make_plan(N, NB, WG);
 }
 
 constexpr bool plan_ok(int n) {
@@ -341,7 +404,7 @@ __device__ __forceinline__ void lds_pass(float2* lds, const float2* tw) {
 
 template <int N, int PASS, int NS, int NB, int WG, int DIR, int STRIDE>
 __device__ __forceinline__ void lds_passes(float2* lds, const float2* tw) {
-  constexpr Plan P = make_plan(N);
+  constexpr Plan P = make_plan(N, NB, WG);
   if constexpr (PASS < P.count) {
     constexpr int R = P.r[PASS];
     lds_pass<N, R, NS, NB, WG, DIR, STRIDE>(lds, tw);
@@ -355,6 +418,107 @@ template <int N, int NB, int WG, int DIR, int STRIDE>
 __device__ __forceinline__ void lds_fft(float2* lds, const float2* tw) {
   static_assert(plan_ok(N), "FFT length must factor into 2, 3, 5, 7");
   lds_passes<N, 0, 1, NB, WG, DIR, STRIDE>(lds, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Fused executor: the first radix pass reads its inputs through a load
+// functor (global memory, products, pre-processing) and the last pass hands
+// its outputs to a store functor (global writes, reductions); only the
+// passes in between go through LDS.  LDS holds transform b element i at
+// lds[b * padded_len(N) + pidx(i)] with one float2 of padding every 16
+// elements, which spreads the stride-R Stockham writes over the banks.
+//
+// Thread mapping: butterfly g -> (b, j); MAP_J: j fastest (coalesced loads of
+// contiguous columns), MAP_B: b fastest (coalesced transposed stores).
+// ---------------------------------------------------------------------------
+template <int SHIFT, int MULT>
+__host__ __device__ constexpr int padded_len_sm(int n) { return n + (n >> SHIFT) * MULT + 1; }
+template <int SHIFT, int MULT>
+__device__ __forceinline__ int pidx_sm(int i) { return i + (i >> SHIFT) * MULT; }
+
+// LDS elements (per transform, padding included) the executor needs
+template <int N, int NB, int WG, int MAPL>
+__host__ __device__ constexpr int exec_stride() {
+  constexpr Plan P = tuned_plan(N, NB, WG, MAPL);
+  return padded_len_sm<P.pad_shift, P.pad_mult>(N);
+}
+
+enum { MAP_J = 0, MAP_B = 1 };
+
+template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int PASS, int NS, class LoadF,
+          class StoreF>
+__device__ __forceinline__ void fft_exec_pass(float2* lds, const float2* tw, LoadF& load,
+                                              StoreF& store) {
+  constexpr Plan P = tuned_plan(N, NB, WG, MAPL);
+  constexpr int R = P.r[PASS];
+  constexpr bool FIRST = PASS == 0, LAST = PASS == P.count - 1;
+  constexpr int M = N / R;
+  constexpr int TOT = NB * M;
+  constexpr int PER = (TOT + WG - 1) / WG;
+  constexpr int TWS = N / (NS * R);
+  constexpr int STRIDE = padded_len_sm<P.pad_shift, P.pad_mult>(N);
+  auto pidx = [](int i) { return pidx_sm<P.pad_shift, P.pad_mult>(i); };
+  // which butterfly a thread owns in this pass (the same for its reads and writes)
+  constexpr int MAPP = FIRST ? MAP0 : (LAST ? MAPL : MAP_J);
+  float2 v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int g = tid + p * WG;
+    if (TOT % WG == 0 || g < TOT) {
+      const int b = (MAPP == MAP_J) ? g / M : g % NB;
+      const int j = (MAPP == MAP_J) ? g % M : g / NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (FIRST) v[p][r] = load(b, j + r * M);
+        else v[p][r] = lds[b * STRIDE + pidx(j + r * M)];
+      }
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[k * r * TWS];
+          if constexpr (DIR > 0) w.y = -w.y;
+          v[p][r] = cmul(v[p][r], w);
+        }
+      }
+      Dft<R, DIR>::run(v[p]);
+    }
+  }
+  if constexpr (!FIRST && !LAST) __syncthreads();  // reads of this pass done before writes
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int g = tid + p * WG;
+    if (TOT % WG == 0 || g < TOT) {
+      const int b = (MAPP == MAP_J) ? g / M : g % NB;
+      const int j = (MAPP == MAP_J) ? g % M : g / NB;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (LAST) store(b, base + r * NS, v[p][r]);
+        else lds[b * STRIDE + pidx(base + r * NS)] = v[p][r];
+      }
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+    fft_exec_pass<N, NB, WG, DIR, MAP0, MAPL, PASS + 1, NS * R>(lds, tw, load, store);
+  }
+}
+
+// Run the whole N-point transform of NB sequences.  load(b, i) supplies input
+// element i of sequence b; store(b, i, v) receives output element i.  The
+// mapping of the first pass is MAP0 and of the last pass MAPL (a single-pass
+// plan uses MAP0).  Requires tw to be visible (barrier) before the call.
+template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, class LoadF, class StoreF>
+__device__ __forceinline__ void fft_exec(float2* lds, const float2* tw, LoadF& load,
+                                         StoreF& store) {
+  static_assert(plan_ok(N), "FFT length must factor into 2, 3, 5, 7");
+  constexpr Plan P = tuned_plan(N, NB, WG, MAPL);
+  static_assert(P.count != 1 || MAP0 == MAPL, "single-pass plans use one mapping");
+  constexpr int MAPL_EFF = MAPL;
+  fft_exec_pass<N, NB, WG, DIR, MAP0, MAPL_EFF, 0, 1>(lds, tw, load, store);
 }
 
 // fill an LDS twiddle table exp(-2 pi i m / N) from a global table

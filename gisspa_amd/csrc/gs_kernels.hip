@@ -337,32 +337,67 @@ __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ ti
                                                    const float2* __restrict__ twL) {
   using G = Geo<L>;
   constexpr int H = G::H, HP = G::HP, CB = G::CB;
-  __shared__ float2 lds[CB * L];
-  __shared__ float2 tw[L];
-  const int m = blockIdx.y;
+  __shared__ float2 lds[CB * exec_stride<L, CB, WG, MAP_B>()];
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
+  // w % 8 picks the XCD.  Each XCD owns the column blocks cb = xcd + 8 i of
+  // every map, which keeps its slice of the tile spectrum resident in its L2
+  // while the template spectra stream through.
+  constexpr int NCB = (H + CB - 1) / CB;
+  constexpr int NCB8 = (NCB + 7) / 8;
+  const int w = blockIdx.x;
+  const int xcd = w & 7, k = w >> 3;
+  const int cb = xcd + 8 * (k % NCB8);
+  const int m = k / NCB8;
+  if (cb >= NCB) return;
   const MapRef mr = maps[m];
-  const int kx0 = blockIdx.x * CB;
-  load_twiddles<L, WG>(tw, twL);
+  const int kx0 = cb * CB;
+  const int ncol = min(CB, H - kx0);
+  const float2* tw = twL;  // twiddles through the L1/L2 caches, LDS kept for data
   const float2* ib = tiles + ((long)mr.tile * H + kx0) * L;
   const float2* tb = tspec + ((long)mr.view * H + kx0) * L;
-  const int ncol = min(CB, H - kx0);
-  for (int g = threadIdx.x; g < CB * L; g += WG) {
-    const int c = g / L, y = g - c * L;
-    float2 v = make_float2(0.f, 0.f);
-    if (c < ncol) {
-      v = cmulc(ib[g], tb[g]);
-      const float sg = ((kx0 + c + y) & 1) ? -sgn : sgn;
-      v = cscale(v, sg);
-    }
-    lds[g] = v;
+  float2* gb = Gout + (long)m * L * HP + kx0;
+  // P = tile * conj(template) * (-1)^(kx+ky) * sgn, read straight into the
+  // first radix pass
+  // branch-free so the compiler issues all loads of a butterfly back to back
+  auto load = [&](int c, int y) -> float2 {
+    const bool ok = c < ncol;
+    const int cc = ok ? c : 0;
+    const float2 v = cmulc(ib[cc * L + y], tb[cc * L + y]);
+    const float sg = ok ? (((kx0 + c + y) & 1) ? -sgn : sgn) : 0.0f;
+    return cscale(v, sg);
+  };
+  // last pass writes row-major G[y][kx] (columns fastest across lanes)
+  auto store = [&](int c, int y, float2 v) {
+    if (c < ncol) gb[(long)y * HP + c] = v;
+  };
+  fft_exec<L, CB, WG, +1, MAP_J, MAP_B>(lds, tw, load, store);
+}
+
+// per-map finalisation (host loop of main.cu:856-881 in FP64): combine the
+// row-block partials in index order, so the result does not depend on which
+// workgroup finished last
+__device__ void corr_finalize_map(const CorrPartial* p, int nparts, double rc, MapStat* out) {
+  float v = 0.0f;
+  int pos = INT_MAX;
+  double s1 = 0, s2 = 0;
+  for (int i = 0; i < nparts; ++i) {
+    vmax_combine(v, pos, p[i].vmax, p[i].pos);
+    s1 += p[i].s1;
+    s2 += p[i].s2;
   }
-  __syncthreads();
-  lds_fft<L, CB, WG, +1, L>(lds, tw);
-  float2* gb = Gout + (long)m * L * HP;
-  for (int g = threadIdx.x; g < CB * L; g += WG) {
-    const int y = g / CB, c = g - y * CB;
-    if (c < ncol) gb[(long)y * HP + kx0 + c] = lds[c * L + y];
-  }
+  const double peak = v;
+  if (!(v > 0.0f)) pos = 0;
+  const double ra = s1 - peak, rb = s2 - peak * peak;
+  const double var = rb / rc - (ra / rc) * (ra / rc);
+  const double sd = sqrt(var);
+  MapStat st;
+  st.peak = (float)peak;
+  st.pos = pos;
+  st.sd = (float)sd;
+  st.score = (sd > 0) ? (float)(peak / sd) : 0.0f;
+  st.sum = s1;
+  st.sum2 = s2;
+  *out = st;
 }
 
 template <int L>
@@ -371,57 +406,44 @@ __global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gi
                                                    const float2* __restrict__ twL,
                                                    const float2* __restrict__ twM) {
   using G = Geo<L>;
-  constexpr int M = G::M, H = G::H, HP = G::HP, RB = G::RB;
-  __shared__ float2 lds[RB * H];
-  __shared__ float2 tw[M];
+  constexpr int M = G::M, HP = G::HP, RB = G::RB;
+  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
   __shared__ float red_v[WG / 64];
   __shared__ int red_p[WG / 64];
   __shared__ float red_s1[WG / 64], red_s2[WG / 64];
   const int m = blockIdx.y;
   const int y0 = blockIdx.x * RB;
-  load_twiddles<M, WG>(tw, twM);
+  const float2* tw = twM;
   const float2* gb = Gin + (long)m * L * HP;
-  for (int g = threadIdx.x; g < RB * H; g += WG) {
-    const int r = g / H, k = g - r * H;
+  // C2R pre-processing fused into the first radix pass:
+  // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) e^{+2 pi i k / L}
+  // branch-free (rows past L only exist when RB does not divide L)
+  auto load = [&](int r, int k) -> float2 {
     const int y = y0 + r;
-    lds[g] = (y < L) ? gb[(long)y * HP + k] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  constexpr int NPAIR = M / 2 + 1;
-  for (int g = threadIdx.x; g < RB * NPAIR; g += WG) {
-    const int r = g / NPAIR, q = g - r * NPAIR;
-    float2* row = lds + r * H;
-    const int k1 = q, k2 = M - q;
-    const float2 x1 = row[k1], x2 = row[k2];
-    float2 w1 = twL[k1];
-    w1.y = -w1.y;
-    row[k1] = c2r_pre(x1, x2, w1);
-    if (q > 0 && k2 != k1) {
-      float2 w2 = twL[k2];
-      w2.y = -w2.y;
-      row[k2] = c2r_pre(x2, x1, w2);
-    }
-  }
-  __syncthreads();
-  lds_fft<M, RB, WG, +1, H>(lds, tw);
+    const bool ok = y < L;
+    const float2* row = gb + (long)(ok ? y : 0) * HP;
+    float2 w = twL[k];
+    w.y = -w.y;
+    const float2 z = c2r_pre(row[k], row[M - k], w);
+    return ok ? z : make_float2(0.f, 0.f);
+  };
   float vmax = 0.0f;
   int pos = INT_MAX;
   float s1 = 0.0f, s2 = 0.0f;
-  for (int g = threadIdx.x; g < RB * M; g += WG) {
-    const int r = g / M, n = g - r * M;
+  // z'[n] = (c[y][2n], c[y][2n+1]): reduce straight from the last pass
+  auto store = [&](int r, int n, float2 z) {
     const int y = y0 + r;
-    if (y >= L) continue;
-    const float2 z = lds[r * H + n];
+    if (y >= L) return;
     s1 += z.x + z.y;
     s2 += z.x * z.x + z.y * z.y;
-    const bool yok = !((float)y < lo || (float)y > hi);
-    if (yok) {
+    if (!((float)y < lo || (float)y > hi)) {
       const int x0 = 2 * n;
       if (!((float)x0 < lo || (float)x0 > hi) && z.x > 0.0f) vmax_combine(vmax, pos, z.x, y * L + x0);
       if (!((float)(x0 + 1) < lo || (float)(x0 + 1) > hi) && z.y > 0.0f)
         vmax_combine(vmax, pos, z.y, y * L + x0 + 1);
     }
-  }
+  };
+  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, tw, load, store);
   wave_max_idx(vmax, pos);
   s1 = wave_sum_f(s1);
   s2 = wave_sum_f(s2);
@@ -443,32 +465,12 @@ __global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gi
   }
 }
 
+// one thread per map: combine the row-block partials of every map of a psi
 __global__ void k_corr_finalize(const CorrPartial* __restrict__ part, int nparts, int nmaps,
                                 double rc, MapStat* __restrict__ out) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nmaps) return;
-  const CorrPartial* p = part + (long)m * nparts;
-  float v = 0.0f;
-  int pos = INT_MAX;
-  double s1 = 0, s2 = 0;
-  for (int i = 0; i < nparts; ++i) {
-    vmax_combine(v, pos, p[i].vmax, p[i].pos);
-    s1 += p[i].s1;
-    s2 += p[i].s2;
-  }
-  const double peak = v;
-  if (!(v > 0.0f)) pos = 0;
-  const double ra = s1 - peak, rb = s2 - peak * peak;
-  const double var = rb / rc - (ra / rc) * (ra / rc);
-  const double sd = sqrt(var);
-  MapStat st;
-  st.peak = (float)peak;
-  st.pos = pos;
-  st.sd = (float)sd;
-  st.score = (sd > 0) ? (float)(peak / sd) : 0.0f;
-  st.sum = s1;
-  st.sum2 = s2;
-  out[m] = st;
+  corr_finalize_map(part + (long)m * nparts, nparts, rc, out + m);
 }
 
 // ---------------------------------------------------------------------------
@@ -828,7 +830,9 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
 template <int L>
 static void corr_pass1_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
                          float sgn, float2* G, const float2* twL, hipStream_t s) {
-  dim3 grid((Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB, nmaps);
+  constexpr int NCB = (Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB;
+  constexpr int NCB8 = (NCB + 7) / 8;
+  dim3 grid(8 * NCB8 * nmaps);
   hipLaunchKernelGGL((k_corr_pass1<L>), grid, dim3(WG), 0, s, tiles, tspec, maps, sgn, G, twL);
 }
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
@@ -847,11 +851,12 @@ void launch_corr_pass2(int L, const float2* G, int nmaps, float lo, float hi, Co
   DISPATCH(L, corr_pass2_t, G, nmaps, lo, hi, part, twL, twM, s);
 }
 
+int corr_nparts(int L) { return (L + geometry_of(L).RB - 1) / geometry_of(L).RB; }
+
 void launch_corr_finalize(int L, const CorrPartial* part, int nmaps, MapStat* out,
                           hipStream_t s) {
-  const int nparts = (L + geometry_of(L).RB - 1) / geometry_of(L).RB;
-  hipLaunchKernelGGL(k_corr_finalize, dim3((nmaps + 255) / 256), dim3(256), 0, s, part, nparts,
-                     nmaps, (double)L * L - 1.0, out);
+  hipLaunchKernelGGL(k_corr_finalize, dim3((nmaps + 255) / 256), dim3(256), 0, s, part,
+                     corr_nparts(L), nmaps, (double)L * L - 1.0, out);
 }
 
 void launch_box_sums(const float* src, int npix, int nb, double* sums, hipStream_t s) {

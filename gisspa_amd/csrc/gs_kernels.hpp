@@ -77,11 +77,12 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
 // column inverse, written to G[m][y][kx].
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
                        int nmaps, float sgn, float2* G, const float2* twL, hipStream_t s);
-// Correlation pass 2: row C2R of G and the border-excluded max / sums,
-// one CorrPartial per (map, row block).
+// Correlation pass 2: row C2R of G and the border-excluded max / sums, one
+// CorrPartial per (map, row block): part[m * corr_nparts(L) + block].
 void launch_corr_pass2(int L, const float2* G, int nmaps, float dm4_lo, float dm4_hi,
                        CorrPartial* part, const float2* twL, const float2* twM, hipStream_t s);
-// Combine partials into MapStat (score in FP64).
+int corr_nparts(int L);
+// Combine the partials of nmaps maps into MapStat (score in FP64).
 void launch_corr_finalize(int L, const CorrPartial* part, int nmaps, MapStat* out,
                           hipStream_t s);
 

@@ -119,7 +119,7 @@ int main(int argc, char** argv) {
     r = gs_pick(ctx, 0, -1, picks.data(), picks.size(), &np);
     if (r == GS_ERR_CAPACITY) {
       picks.resize(np);
-      r = gs_pick(ctx, 0, -1, picks.data(), picks.size(), &np);
+      r = gs_get_picks(ctx, picks.data(), picks.size(), &np);
     }
     if (r != GS_OK) return fail("gs_pick", r);
     for (size_t i = 0; i < np; ++i) {

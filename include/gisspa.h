@@ -143,9 +143,13 @@ int gs_set_micrograph_device(void* ctx, const float* dev, int nx, int ny, const 
  * Picks above the threshold with a centre at least `diameter` from the
  * edges are returned in the reference's order: psi, then tile (y outer,
  * x inner), then view (then block for norm_type 1).  If cap is too small,
- * returns GS_ERR_CAPACITY with *n_out set to the required count.
+ * returns GS_ERR_CAPACITY with *n_out set to the required count; the picks
+ * stay in the context and gs_get_picks returns them without recomputing.
  * psi_end < 0 means all steps. */
 int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap, size_t* n_out);
+/* The picks of the last gs_pick (no reference counterpart: the reference
+ * writes them to the output file as it goes, main.cu:1255). */
+int gs_get_picks(void* ctx, gs_particle* out, size_t cap, size_t* n_out);
 
 /* Per-map results of the last norm_type 0 gs_pick, [psi][tile][view]. */
 int gs_get_mapstats(void* ctx, gs_mapstat* out, size_t cap, size_t* n_out);

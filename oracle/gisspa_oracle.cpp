@@ -60,7 +60,8 @@ namespace {
 struct Plan {
   int n;
   std::vector<int> radices;
-  std::vector<cd> tw_fwd;  /* exp(-2 pi i m / n) */
+  std::vector<cd> tw_fwd;                 /* exp(-2 pi i m / n) */
+  std::map<int, std::vector<cd>> small;   /* per radix R: exp(-2 pi i m / R) */
 };
 
 std::mutex g_plan_mu;
@@ -84,11 +85,19 @@ std::shared_ptr<Plan> get_plan(int n) {
     double a = -2.0 * M_PI * (double)k / (double)n;
     p->tw_fwd[k] = cd(std::cos(a), std::sin(a));
   }
+  for (int R : p->radices) {
+    std::vector<cd>& t = p->small[R];
+    t.resize(R);
+    for (int m = 0; m < R; ++m) {
+      double a = -2.0 * M_PI * (double)m / (double)R;
+      t[m] = cd(std::cos(a), std::sin(a));
+    }
+  }
   g_plans[n] = p;
   return p;
 }
 
-void dft_small(cd* v, int R, int sign, cd* tmp) {
+void dft_small(cd* v, int R, int sign, cd* tmp, const cd* tab) {
   if (R == 2) {
     cd a = v[0], b = v[1];
     v[0] = a + b; v[1] = a - b;
@@ -104,9 +113,8 @@ void dft_small(cd* v, int R, int sign, cd* tmp) {
   for (int q = 0; q < R; ++q) {
     cd s = 0;
     for (int r = 0; r < R; ++r) {
-      long long m = ((long long)r * q) % R;
-      double a = sign * 2.0 * M_PI * (double)m / (double)R;
-      s += v[r] * cd(std::cos(a), std::sin(a));
+      const cd w = tab[((long long)r * q) % R];
+      s += v[r] * (sign < 0 ? w : std::conj(w));
     }
     tmp[q] = s;
   }
@@ -123,6 +131,7 @@ void fft1d_strided(cd* data, int n, long stride, int sign, const Plan& pl,
   std::vector<cd> v, tmp;
   for (int R : pl.radices) {
     v.resize(R); tmp.resize(R);
+    const cd* tab = pl.small.at(R).data();
     int m = n / R;
     for (int j = 0; j < m; ++j) {
       int k = j % Ns;
@@ -134,7 +143,7 @@ void fft1d_strided(cd* data, int n, long stride, int sign, const Plan& pl,
         if (sign > 0) w = std::conj(w);
         v[r] = x * w;
       }
-      dft_small(v.data(), R, sign, tmp.data());
+      dft_small(v.data(), R, sign, tmp.data(), tab);
       int base = (j / Ns) * Ns * R + k;
       for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
     }
@@ -754,4 +763,26 @@ extern "C" int or_select_norm0(const or_params* p, const or_mapstat* st, int nps
         }
       }
   return count;
+}
+
+/* All T x N maps of one psi (the correlation loop of main.cu:826-893), OpenMP
+ * over maps; out[t * n + J]. */
+extern "C" int or_correlate_many(const or_params* p, const double* tile_spec, int T,
+                                 const double* tspec, int n, or_mapstat* out, int nthreads) {
+  set_threads(nthreads);
+  const int l = p->l;
+  const long L2 = (long)l * l;
+  const cd* tiles = reinterpret_cast<const cd*>(tile_spec);
+  const cd* ts = reinterpret_cast<const cd*>(tspec);
+#pragma omp parallel
+  {
+    std::vector<cd> ccg(L2);
+#pragma omp for schedule(dynamic) collapse(2)
+    for (int t = 0; t < T; ++t)
+      for (int J = 0; J < n; ++J) {
+        cc_map(&tiles[t * L2], &ts[J * L2], l, ccg.data());
+        out[(long)t * n + J] = reduce_norm0(ccg.data(), l, p->diameter);
+      }
+  }
+  return 0;
 }

@@ -72,6 +72,8 @@ int or_cc_map(const or_params* p, const double* tile_spec, const double* tspec, 
 int or_rotate_bilinear(const double* in, int l, float e, double* out);
 int or_correlate_norm0(const or_params* p, const double* tile_spec, const double* tspec,
                        or_mapstat* out);
+int or_correlate_many(const or_params* p, const double* tile_spec, int T, const double* tspec,
+                      int n, or_mapstat* out, int nthreads);
 /* ---- whole pipeline ---- */
 int or_pick_norm0(const or_params* p, const or_ctf* c, const float* tmpl, int ntmpl, int bx, int by,
                   const float* img, int nx, int ny, int psi_begin, int psi_end,

@@ -59,6 +59,7 @@ def load() -> ctypes.CDLL:
         lib.or_micrograph_prep.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
         lib.or_correlate_norm0.argtypes = [P, vp, vp, vp]
         lib.or_cc_map.argtypes = [P, vp, vp, vp]
+        lib.or_correlate_many.argtypes = [P, vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int]
         lib.or_rotate_bilinear.argtypes = [vp, ctypes.c_int, ctypes.c_float, vp]
         lib.or_pick_norm0.argtypes = [P, C, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
@@ -217,3 +218,12 @@ def rotate_bilinear(img: np.ndarray, e: float) -> np.ndarray:
 
 def ctf_ast(p: OrParams, c: OrCtf, x: int, y: int, nx: int, ny: int, mode: int) -> float:
     return load().or_ctf_ast(x, y, nx, ny, ctypes.byref(p), ctypes.byref(c), mode)
+
+
+def correlate_many(p: OrParams, tile_spec: np.ndarray, tspec: np.ndarray, nthreads: int = 0):
+    a = np.ascontiguousarray(tile_spec, dtype=np.complex128)
+    b = np.ascontiguousarray(tspec, dtype=np.complex128)
+    out = np.zeros((a.shape[0], b.shape[0]), dtype=MAPSTAT)
+    load().or_correlate_many(ctypes.byref(p), a.ctypes.data, a.shape[0], b.ctypes.data,
+                             b.shape[0], out.ctypes.data, nthreads)
+    return out
