@@ -80,6 +80,7 @@ struct Ctx {
   hipEvent_t ev_ready = nullptr, ev_lane[NLANE] = {nullptr, nullptr};
   DevBuf G[NLANE];
   DevBuf part;  // row-block partials of every map of one psi
+  DevBuf psum;  // column-block Parseval sums of every map of one psi
   DevBuf maps, stats, sel, sel_count;
   std::vector<gs_particle> picks;  // of the last gs_pick
   int stats_psi_begin = 0, stats_npsi = 0;
@@ -308,19 +309,23 @@ int rotate_templates(Ctx* c, int psi_index) {
 }
 
 int ensure_corr_buffers(Ctx* c) {
-  const int L = c->L, HP = c->geo.HP;
-  const long per_map = (long)L * HP * sizeof(float2);
+  const int L = c->L;
+  const long per_map = c->geo.gmap * (long)sizeof(float2);
   if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (128l << 20) / per_map));
   const int nb = std::min(c->batch, c->T * c->N);
   const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
   for (int i = 0; i < Ctx::NLANE; ++i) CHK(c->G[i].ensure((size_t)nb * per_map));
   CHK(c->part.ensure((size_t)c->T * c->N * nrb * sizeof(CorrPartial)));
-  // canonical map order: tile-major, view-minor
+  CHK(c->psum.ensure((size_t)c->T * c->N * corr_ncb(L) * sizeof(float2)));
+  // processing order: view-major, so consecutive maps share one template
+  // spectrum (kept in cache across all tiles) while the tile spectra
+  // (T * 2 MB) stay resident in the last-level cache; results are stored in
+  // the canonical [tile][view] order by the finalisation
   const size_t nm = (size_t)c->T * c->N;
   if (c->maps.bytes < nm * sizeof(MapRef)) {
     std::vector<MapRef> mr(nm);
-    for (int t = 0; t < c->T; ++t)
-      for (int j = 0; j < c->N; ++j) mr[(size_t)t * c->N + j] = MapRef{t, j};
+    for (int j = 0; j < c->N; ++j)
+      for (int t = 0; t < c->T; ++t) mr[(size_t)j * c->T + t] = MapRef{t, j};
     CHK(c->maps.ensure(nm * sizeof(MapRef)));
     HIPCHK(hipMemcpyAsync(c->maps.p, mr.data(), nm * sizeof(MapRef), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));  // mr is released on return
@@ -343,7 +348,8 @@ int correlate_psi(Ctx* c, MapStat* out) {
     const int cnt = std::min(nb, nm - m0);
     hipStream_t ls = c->lane_stream[lane];
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>() + m0,
-                      cnt, sgn, c->G[lane].as<float2>(), c->twL.as<float2>(), ls);
+                      cnt, sgn, c->G[lane].as<float2>(),
+                      c->psum.as<float2>() + (size_t)m0 * corr_ncb(L), c->twL.as<float2>(), ls);
     launch_corr_pass2(L, c->G[lane].as<float2>(), cnt, lo, hi,
                       c->part.as<CorrPartial>() + (size_t)m0 * corr_nparts(L),
                       c->twL.as<float2>(), c->twM.as<float2>(), ls);
@@ -356,7 +362,8 @@ int correlate_psi(Ctx* c, MapStat* out) {
     HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
     HIPCHK(hipStreamWaitEvent(c->stream, c->ev_lane[i], 0));
   }
-  launch_corr_finalize(L, c->part.as<CorrPartial>(), nm, out, c->stream);
+  launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
+                       c->maps.as<MapRef>(), c->N, out, c->stream);
   c->timing.corr_maps += nm;
   return GS_OK;
 }
@@ -528,7 +535,8 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   }
   // device-side selection, then canonical order on the host
   const long long nmaps = npsi * per_psi;
-  int capsel = (int)std::min<long long>(std::max<long long>(nmaps, 1), 1 << 22);
+  // room for every map: a low threshold can select most of them
+  const int capsel = (int)std::max<long long>(nmaps, 1);
   CHK(c->sel.ensure((size_t)capsel * sizeof(SelRec)));
   CHK(c->sel_count.ensure(sizeof(int)));
   HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), c->stream));
@@ -547,7 +555,7 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   c->timing.pick_total_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (count > capsel) return GS_ERR_CAPACITY;
+  if (count > capsel) return GS_ERR_DEVICE;  // cannot happen: one record per map at most
   std::vector<SelRec> recs(count);
   if (count)
     CHK(copy_out(c, recs.data(), c->sel.p, (size_t)count * sizeof(SelRec)));
@@ -643,7 +651,8 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
   CHK(c->stats.ensure((size_t)nb * sizeof(MapStat)));
   for (int i = 0; i < reps; ++i)
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>(), nb,
-                      sgn, c->G[0].as<float2>(), c->twL.as<float2>(), c->stream);
+                      sgn, c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(),
+                      c->stream);
   HIPCHK(hipEventRecord(e[1], c->stream));
   for (int i = 0; i < reps; ++i)
     launch_corr_pass2(L, c->G[0].as<float2>(), nb, lo, hi, c->part.as<CorrPartial>(),

@@ -334,6 +334,7 @@ __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ ti
                                                    const float2* __restrict__ tspec,
                                                    const MapRef* __restrict__ maps, float sgn,
                                                    float2* __restrict__ Gout,
+                                                   float2* __restrict__ psum,
                                                    const float2* __restrict__ twL) {
   using G = Geo<L>;
   constexpr int H = G::H, HP = G::HP, CB = G::CB;
@@ -355,36 +356,59 @@ __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ ti
   const float2* tw = twL;  // twiddles through the L1/L2 caches, LDS kept for data
   const float2* ib = tiles + ((long)mr.tile * H + kx0) * L;
   const float2* tb = tspec + ((long)mr.view * H + kx0) * L;
-  float2* gb = Gout + (long)m * L * HP + kx0;
+  // blocked intermediate G[m][cb][y][c] (c < CB): every workgroup writes one
+  // contiguous block, rows of CB columns at a time
+  float2* gb = Gout + ((long)m * NCB + cb) * L * CB;
   // P = tile * conj(template) * (-1)^(kx+ky) * sgn, read straight into the
   // first radix pass
   // branch-free so the compiler issues all loads of a butterfly back to back
+  //
+  // The map's sums follow from Parseval on the way in: sum_u c(u)^2 =
+  // L^2 sum_k |P(k)|^2 over the Hermitian-completed spectrum (columns
+  // 0 < kx < L/2 count twice) and sum_u c(u) = L^2 P(0,0).
+  float sumsq = 0.0f, dc = 0.0f;
   auto load = [&](int c, int y) -> float2 {
     const bool ok = c < ncol;
     const int cc = ok ? c : 0;
     const float2 v = cmulc(ib[cc * L + y], tb[cc * L + y]);
     const float sg = ok ? (((kx0 + c + y) & 1) ? -sgn : sgn) : 0.0f;
-    return cscale(v, sg);
+    const float2 p = cscale(v, sg);
+    const int kx = kx0 + c;
+    const float wgt = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
+    sumsq += wgt * (p.x * p.x + p.y * p.y);
+    if (kx == 0 && y == 0) dc = p.x;
+    return p;
   };
-  // last pass writes row-major G[y][kx] (columns fastest across lanes)
+  // last pass: columns fastest across lanes, consecutive rows -> contiguous
   auto store = [&](int c, int y, float2 v) {
-    if (c < ncol) gb[(long)y * HP + c] = v;
+    if (c < ncol) gb[y * CB + c] = v;
   };
   fft_exec<L, CB, WG, +1, MAP_J, MAP_B>(lds, tw, load, store);
+  __shared__ float red_q[WG / 64], red_d[WG / 64];
+  sumsq = wave_sum_f(sumsq);
+  dc = wave_sum_f(dc);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { red_q[wv] = sumsq; red_d[wv] = dc; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float q = 0.0f, d = 0.0f;
+    for (int i = 0; i < WG / 64; ++i) { q += red_q[i]; d += red_d[i]; }
+    psum[(long)m * NCB + cb] = make_float2(q, d);
+  }
 }
 
 // per-map finalisation (host loop of main.cu:856-881 in FP64): combine the
 // row-block partials in index order, so the result does not depend on which
 // workgroup finished last
-__device__ void corr_finalize_map(const CorrPartial* p, int nparts, double rc, MapStat* out) {
+__device__ void corr_finalize_map(const CorrPartial* p, int nparts, const float2* ps, int ncb,
+                                  double L2, double rc, MapStat* out) {
   float v = 0.0f;
   int pos = INT_MAX;
-  double s1 = 0, s2 = 0;
-  for (int i = 0; i < nparts; ++i) {
-    vmax_combine(v, pos, p[i].vmax, p[i].pos);
-    s1 += p[i].s1;
-    s2 += p[i].s2;
-  }
+  for (int i = 0; i < nparts; ++i) vmax_combine(v, pos, p[i].vmax, p[i].pos);
+  double q = 0;
+  for (int i = 0; i < ncb; ++i) q += ps[i].x;
+  const double s1 = L2 * (double)ps[0].y;
+  const double s2 = L2 * q;
   const double peak = v;
   if (!(v > 0.0f)) pos = 0;
   const double ra = s1 - peak, rb = s2 - peak * peak;
@@ -401,76 +425,78 @@ __device__ void corr_finalize_map(const CorrPartial* p, int nparts, double rc, M
 }
 
 template <int L>
-__global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gin, float lo,
-                                                   float hi, CorrPartial* __restrict__ part,
+__global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gin, int blo,
+                                                   int bhi, CorrPartial* __restrict__ part,
                                                    const float2* __restrict__ twL,
                                                    const float2* __restrict__ twM) {
   using G = Geo<L>;
-  constexpr int M = G::M, HP = G::HP, RB = G::RB;
+  constexpr int M = G::M, H = G::H, RB = G::RB, CB = G::CB;
+  constexpr int NCB = (H + CB - 1) / CB;
   __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
   __shared__ float red_v[WG / 64];
   __shared__ int red_p[WG / 64];
-  __shared__ float red_s1[WG / 64], red_s2[WG / 64];
   const int m = blockIdx.y;
   const int y0 = blockIdx.x * RB;
   const float2* tw = twM;
-  const float2* gb = Gin + (long)m * L * HP;
+  const float2* gb = Gin + (long)m * NCB * L * CB;
   // C2R pre-processing fused into the first radix pass:
   // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) e^{+2 pi i k / L}
-  // branch-free (rows past L only exist when RB does not divide L)
+  // with X[k] of row y at G[m][k / CB][y][k % CB]; branch-free (rows past L
+  // only exist when RB does not divide L)
+  auto at = [&](int y, int k) { return gb[((k / CB) * L + y) * CB + (k % CB)]; };
   auto load = [&](int r, int k) -> float2 {
-    const int y = y0 + r;
-    const bool ok = y < L;
-    const float2* row = gb + (long)(ok ? y : 0) * HP;
+    const int y0r = y0 + r;
+    const bool ok = y0r < L;
+    const int y = ok ? y0r : 0;
     float2 w = twL[k];
     w.y = -w.y;
-    const float2 z = c2r_pre(row[k], row[M - k], w);
+    const float2 z = c2r_pre(at(y, k), at(y, M - k), w);
     return ok ? z : make_float2(0.f, 0.f);
   };
+  // z'[n] = (c[y][2n], c[y][2n+1]).  Border pixels (x or y outside
+  // [blo, bhi], the integer form of get_peak_and_SUM's d/4 test) count as 0;
+  // the max keeps the lowest index among equal values.  The common case is a
+  // value below the running max: one compare per pixel.
   float vmax = 0.0f;
   int pos = INT_MAX;
-  float s1 = 0.0f, s2 = 0.0f;
-  // z'[n] = (c[y][2n], c[y][2n+1]): reduce straight from the last pass
+  const unsigned brange = (unsigned)(bhi - blo);
   auto store = [&](int r, int n, float2 z) {
     const int y = y0 + r;
-    if (y >= L) return;
-    s1 += z.x + z.y;
-    s2 += z.x * z.x + z.y * z.y;
-    if (!((float)y < lo || (float)y > hi)) {
-      const int x0 = 2 * n;
-      if (!((float)x0 < lo || (float)x0 > hi) && z.x > 0.0f) vmax_combine(vmax, pos, z.x, y * L + x0);
-      if (!((float)(x0 + 1) < lo || (float)(x0 + 1) > hi) && z.y > 0.0f)
-        vmax_combine(vmax, pos, z.y, y * L + x0 + 1);
+    if ((unsigned)(y - blo) > brange) return;
+    const int x0 = 2 * n;
+    const float a = ((unsigned)(x0 - blo) <= brange) ? z.x : 0.0f;
+    const float b = ((unsigned)(x0 + 1 - blo) <= brange) ? z.y : 0.0f;
+    if (a >= vmax || b >= vmax) {
+      if (a > 0.0f) vmax_combine(vmax, pos, a, y * L + x0);
+      if (b > 0.0f) vmax_combine(vmax, pos, b, y * L + x0 + 1);
     }
   };
   fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, tw, load, store);
   wave_max_idx(vmax, pos);
-  s1 = wave_sum_f(s1);
-  s2 = wave_sum_f(s2);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) { red_v[w] = vmax; red_p[w] = pos; red_s1[w] = s1; red_s2[w] = s2; }
+  if (lane == 0) { red_v[w] = vmax; red_p[w] = pos; }
   __syncthreads();
   if (threadIdx.x == 0) {
     float v = red_v[0];
     int p = red_p[0];
-    float a = red_s1[0], q = red_s2[0];
-    for (int i = 1; i < WG / 64; ++i) {
-      vmax_combine(v, p, red_v[i], red_p[i]);
-      a += red_s1[i];
-      q += red_s2[i];
-    }
+    for (int i = 1; i < WG / 64; ++i) vmax_combine(v, p, red_v[i], red_p[i]);
     CorrPartial cp;
-    cp.vmax = v; cp.pos = p; cp.s1 = a; cp.s2 = q;
+    cp.vmax = v; cp.pos = p; cp.s1 = 0.0f; cp.s2 = 0.0f;
     part[(long)m * gridDim.x + blockIdx.x] = cp;
   }
 }
 
-// one thread per map: combine the row-block partials of every map of a psi
-__global__ void k_corr_finalize(const CorrPartial* __restrict__ part, int nparts, int nmaps,
-                                double rc, MapStat* __restrict__ out) {
+// one thread per map: combine the row-block partials of every map of a psi;
+// map m (processing order) lands at out[tile * nviews + view]
+__global__ void k_corr_finalize(const CorrPartial* __restrict__ part, int nparts,
+                                const float2* __restrict__ psum, int ncb, int nmaps,
+                                const MapRef* __restrict__ maps, int nviews, double L2,
+                                MapStat* __restrict__ out) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nmaps) return;
-  corr_finalize_map(part + (long)m * nparts, nparts, rc, out + m);
+  const MapRef mr = maps[m];
+  corr_finalize_map(part + (long)m * nparts, nparts, psum + (long)m * ncb, ncb, L2, L2 - 1.0,
+                    out + (long)mr.tile * nviews + mr.view);
 }
 
 // ---------------------------------------------------------------------------
@@ -715,6 +741,7 @@ Geometry geometry_of(int L) {
   case n:                                                                              \
     g.L = n; g.M = Geo<n>::M; g.H = Geo<n>::H; g.HP = Geo<n>::HP; g.RB = Geo<n>::RB;   \
     g.CB = Geo<n>::CB;                                                                 \
+    g.gmap = (long)((Geo<n>::H + Geo<n>::CB - 1) / Geo<n>::CB) * Geo<n>::CB * n;       \
     break;
     GS_SIZES(CASE)
 #undef CASE
@@ -829,34 +856,44 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
 
 template <int L>
 static void corr_pass1_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
-                         float sgn, float2* G, const float2* twL, hipStream_t s) {
+                         float sgn, float2* G, float2* psum, const float2* twL, hipStream_t s) {
   constexpr int NCB = (Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB;
   constexpr int NCB8 = (NCB + 7) / 8;
   dim3 grid(8 * NCB8 * nmaps);
-  hipLaunchKernelGGL((k_corr_pass1<L>), grid, dim3(WG), 0, s, tiles, tspec, maps, sgn, G, twL);
+  hipLaunchKernelGGL((k_corr_pass1<L>), grid, dim3(WG), 0, s, tiles, tspec, maps, sgn, G, psum,
+                     twL);
 }
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float sgn, float2* G, const float2* twL, hipStream_t s) {
-  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, sgn, G, twL, s);
+                       int nmaps, float sgn, float2* G, float2* psum, const float2* twL,
+                       hipStream_t s) {
+  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, sgn, G, psum, twL, s);
 }
 
 template <int L>
-static void corr_pass2_t(const float2* G, int nmaps, float lo, float hi, CorrPartial* part,
+static void corr_pass2_t(const float2* G, int nmaps, int blo, int bhi, CorrPartial* part,
                          const float2* twL, const float2* twM, hipStream_t s) {
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
-  hipLaunchKernelGGL((k_corr_pass2<L>), grid, dim3(WG), 0, s, G, lo, hi, part, twL, twM);
+  hipLaunchKernelGGL((k_corr_pass2<L>), grid, dim3(WG), 0, s, G, blo, bhi, part, twL, twM);
 }
 void launch_corr_pass2(int L, const float2* G, int nmaps, float lo, float hi, CorrPartial* part,
                        const float2* twL, const float2* twM, hipStream_t s) {
-  DISPATCH(L, corr_pass2_t, G, nmaps, lo, hi, part, twL, twM, s);
+  // integer form of get_peak_and_SUM's exclusion: x < lo || x > hi
+  int blo = (int)ceilf(lo), bhi = (int)floorf(hi);
+  if (bhi < blo) { blo = L + 1; bhi = L + 1; }  // nothing valid
+  DISPATCH(L, corr_pass2_t, G, nmaps, blo, bhi, part, twL, twM, s);
 }
 
 int corr_nparts(int L) { return (L + geometry_of(L).RB - 1) / geometry_of(L).RB; }
 
-void launch_corr_finalize(int L, const CorrPartial* part, int nmaps, MapStat* out,
-                          hipStream_t s) {
+int corr_ncb(int L) {
+  const Geometry g = geometry_of(L);
+  return (g.H + g.CB - 1) / g.CB;
+}
+
+void launch_corr_finalize(int L, const CorrPartial* part, const float2* psum, int nmaps,
+                          const MapRef* maps, int nviews, MapStat* out, hipStream_t s) {
   hipLaunchKernelGGL(k_corr_finalize, dim3((nmaps + 255) / 256), dim3(256), 0, s, part,
-                     corr_nparts(L), nmaps, (double)L * L - 1.0, out);
+                     corr_nparts(L), psum, corr_ncb(L), nmaps, maps, nviews, (double)L * L, out);
 }
 
 void launch_box_sums(const float* src, int npix, int nb, double* sums, hipStream_t s) {

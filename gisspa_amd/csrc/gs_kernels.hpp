@@ -38,6 +38,7 @@ struct Geometry {
   int L, M, H, HP;
   int RB;  // rows per workgroup for M-point transforms
   int CB;  // columns per workgroup for L-point transforms
+  long gmap;  // float2 elements of one map's correlation intermediate
 };
 
 bool geometry_supported(int L);
@@ -74,17 +75,22 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
                      const float2* twM, hipStream_t s);
 
 // Correlation pass 1: P = tile[t] * conj(tmpl[J]) * (-1)^(kx+ky) * sgn,
-// column inverse, written to G[m][y][kx].
+// column inverse, written to the blocked G[m][kx / CB][y][kx % CB]
+// (Geometry::gmap elements per map).
+// psum[m * corr_ncb(L) + cb] = (sum of weighted |P|^2, P(0,0).re) of the block
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float sgn, float2* G, const float2* twL, hipStream_t s);
+                       int nmaps, float sgn, float2* G, float2* psum, const float2* twL,
+                       hipStream_t s);
+int corr_ncb(int L);
 // Correlation pass 2: row C2R of G and the border-excluded max / sums, one
 // CorrPartial per (map, row block): part[m * corr_nparts(L) + block].
 void launch_corr_pass2(int L, const float2* G, int nmaps, float dm4_lo, float dm4_hi,
                        CorrPartial* part, const float2* twL, const float2* twM, hipStream_t s);
 int corr_nparts(int L);
-// Combine the partials of nmaps maps into MapStat (score in FP64).
-void launch_corr_finalize(int L, const CorrPartial* part, int nmaps, MapStat* out,
-                          hipStream_t s);
+// Combine the partials of nmaps maps (processing order `maps`) into
+// out[tile * nviews + view] (score in FP64).
+void launch_corr_finalize(int L, const CorrPartial* part, const float2* psum, int nmaps,
+                          const MapRef* maps, int nviews, MapStat* out, hipStream_t s);
 
 // ---- template preparation ----
 // per-template sums of value and value^2 of the raw box (FP64)
