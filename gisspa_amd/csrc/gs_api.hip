@@ -70,6 +70,11 @@ struct Ctx {
   int nx = 0, ny = 0;
   bool have_mic = false;
   DevBuf img, img_part, img_stats, img_mean, offx_d, offy_d, tiles;
+  // phase_flip = 1: whole-image spectra and their radial bins (fixed by nx, ny)
+  DevBuf cimg, ctmp, twx, twy, rbin_start, rbin_pix, rbin_count, rbin_ra, zero_mean;
+  DevBuf tile_part, tile_scale;
+  int nbin_img = 0;
+  double half_count = 0;  // pixels of the x <= l/2 half plane inside radius l/2
   std::vector<int> offx, offy;
   int bxn = 0, byn = 0, T = 0;
   // correlation: two lanes (stream + intermediate + partials) so that pass 1
@@ -82,6 +87,7 @@ struct Ctx {
   DevBuf part;  // row-block partials of every map of one psi
   DevBuf psum;  // column-block Parseval sums of every map of one psi
   DevBuf maps, stats, sel, sel_count;
+  DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_z, n1_rec;  // norm_type 1
   std::vector<gs_particle> picks;  // of the last gs_pick
   int stats_psi_begin = 0, stats_npsi = 0;
   gs_timing timing{};
@@ -223,6 +229,7 @@ int prep_templates(Ctx* c, const float* host, int n, int bx, int by) {
   launch_tmpl_weight(L, spec, n, ra.as<double>(), rbins.as<float>(), weight_args(c, c->tctf),
                      wpart.as<double>(), s);
   launch_tmpl_ampnorm(L, spec, n, wpart.as<double>(), count_total, s);
+  c->half_count = count_total;
   // inverse 2-D transform to the real-space templates, in chunks
   CHK(c->treal.ensure((size_t)n * L2 * sizeof(float)));
   const int chunk = std::max(1, std::min(n, (int)((512l << 20) / ((long)L * HP * 8))));
@@ -238,7 +245,74 @@ int prep_templates(Ctx* c, const float* host, int n, int bx, int by) {
   return GS_OK;
 }
 
-int prep_micrograph(Ctx* c, const float* src, bool src_is_device, int nx, int ny) {
+int upload_table(DevBuf& buf, int n, hipStream_t s) {
+  std::vector<float2> t(n);
+  for (int i = 0; i < n; ++i) {
+    const double a = -2.0 * M_PI * i / n;
+    t[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  CHK(buf.ensure(n * sizeof(float2)));
+  HIPCHK(hipMemcpyAsync(buf.p, t.data(), n * sizeof(float2), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return GS_OK;
+}
+
+// Whole-micrograph phase flip, whitening, weighting and band pass, DC := 0
+// (init_d_image with phase_flip = 1, main.cu:548-586), in place on c->img.
+int phase_flip_image(Ctx* c, int nx, int ny, const gs_ctf& ctf) {
+  hipStream_t s = c->stream;
+  const long n = (long)nx * ny;
+  CHK(c->cimg.ensure(n * sizeof(float2)));
+  CHK(c->ctmp.ensure(n * sizeof(float2)));
+  if (c->nbin_img == 0) {
+    CHK(upload_table(c->twx, nx, s));
+    CHK(upload_table(c->twy, ny, s));
+    // SQRSum_by_circle mode 1 bins (GPU_func.cu:113-126): x <= nx/2,
+    // 0 <= r < max(nx, ny)/2, r = floor(hypot(min(y,ny-y), min(x,nx-x)) + .5) - 1
+    const int nbin = std::max(nx, ny) / 2;
+    std::vector<std::vector<long>> bins(nbin);
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x <= nx / 2; ++x) {
+        const double dx = std::min(x, nx - x), dy = std::min(y, ny - y);
+        const int r = (int)std::floor(std::hypot(dy, dx) + 0.5) - 1;
+        if (r >= 0 && r < nbin) bins[r].push_back((long)y * nx + x);
+      }
+    std::vector<int> start(nbin + 1, 0);
+    std::vector<long> pix;
+    std::vector<float> cnt(nbin);
+    for (int r = 0; r < nbin; ++r) {
+      start[r + 1] = start[r] + (int)bins[r].size();
+      pix.insert(pix.end(), bins[r].begin(), bins[r].end());
+      cnt[r] = (float)bins[r].size();
+    }
+    CHK(c->rbin_start.ensure(start.size() * sizeof(int)));
+    CHK(c->rbin_pix.ensure(std::max<size_t>(1, pix.size()) * sizeof(long)));
+    CHK(c->rbin_count.ensure(nbin * sizeof(float)));
+    CHK(c->rbin_ra.ensure(nbin * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(c->rbin_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->rbin_pix.p, pix.data(), pix.size() * sizeof(long), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->rbin_count.p, cnt.data(), nbin * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->nbin_img = nbin;
+  }
+  const WeightArgs w = weight_args(c, ctf);
+  // FFT of the image scaled by 1/n (cuFFT is unnormalised; main.cu:553-554)
+  launch_real_to_complex(c->img.as<float>(), c->cimg.as<float2>(), n, (float)(1.0 / (double)n), s);
+  rt_fft_2d(c->cimg.as<float2>(), c->ctmp.as<float2>(), nx, ny, c->twx.as<float2>(),
+            c->twy.as<float2>(), -1, s);
+  launch_phase_flip(c->cimg.as<float2>(), nx, ny, w, s);
+  launch_img_radial(c->cimg.as<float2>(), c->rbin_start.as<int>(), c->rbin_pix.as<long>(),
+                    c->nbin_img, c->rbin_ra.as<double>(), s);
+  launch_img_weight(c->cimg.as<float2>(), nx, ny, c->rbin_ra.as<double>(),
+                    c->rbin_count.as<float>(), w, s);
+  rt_fft_2d(c->cimg.as<float2>(), c->ctmp.as<float2>(), nx, ny, c->twx.as<float2>(),
+            c->twy.as<float2>(), +1, s);
+  launch_complex_to_real(c->cimg.as<float2>(), c->img.as<float>(), n, s);
+  return GS_OK;
+}
+
+int prep_micrograph(Ctx* c, const float* src, bool src_is_device, int nx, int ny,
+                    const gs_ctf& ctf) {
   hipStream_t s = c->stream;
   const long n = (long)nx * ny;
   const int L = c->L;
@@ -269,13 +343,30 @@ int prep_micrograph(Ctx* c, const float* src, bool src_is_device, int nx, int ny
   launch_img_clamp(c->img.as<float>(), n, c->p.invert, c->img_stats.as<double>(),
                    c->img_part.as<double>(), nblk, s);
   launch_img_mean(c->img_part.as<double>(), nblk, n, c->img_mean.as<double>(), s);
+  const double* mean = c->img_mean.as<double>();
+  if (c->p.phase_flip == 1) {
+    CHK(phase_flip_image(c, nx, ny, ctf));
+    // the filtered image already has zero DC; split_IMG copies it as is
+    if (!c->zero_mean.p) {
+      CHK(c->zero_mean.ensure(sizeof(double)));
+      HIPCHK(hipMemsetAsync(c->zero_mean.p, 0, sizeof(double), s));
+    }
+    mean = c->zero_mean.as<double>();
+  }
   const int H = c->geo.H;
   CHK(c->tiles.ensure((size_t)c->T * H * L * sizeof(float2)));
   launch_rows_r2c_tiles(L, c->img.as<float>(), nx, ny, c->offx_d.as<int>(), c->offy_d.as<int>(),
-                        c->bxn, c->T, c->img_mean.as<double>(), c->tiles.as<float2>(),
-                        c->twL.as<float2>(), c->twM.as<float2>(), s);
+                        c->bxn, c->T, mean, c->tiles.as<float2>(), c->twL.as<float2>(),
+                        c->twM.as<float2>(), s);
   launch_cols_fwd(L, c->tiles.as<float2>(), c->T, (float)(1.0 / ((double)L * L)), nullptr, 0,
                   c->twL.as<float2>(), s);
+  if (c->p.phase_flip == 1) {
+    // per-tile amplitude normalisation (main.cu:617-653)
+    CHK(c->tile_part.ensure((size_t)c->T * tile_ampnorm_parts(L) * sizeof(double)));
+    CHK(c->tile_scale.ensure((size_t)c->T * sizeof(float)));
+    launch_tile_ampnorm(c->tiles.as<float2>(), L, c->T, c->half_count, c->tile_part.as<double>(),
+                        c->tile_scale.as<float>(), s);
+  }
   // the caller may release the host image on return
   HIPCHK(hipStreamSynchronize(s));
   HIPCHK(hipGetLastError());
@@ -365,6 +456,94 @@ int correlate_psi(Ctx* c, MapStat* out) {
   launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
                        c->maps.as<MapRef>(), c->N, out, c->stream);
   c->timing.corr_maps += nm;
+  return GS_OK;
+}
+
+// norm_type 1 (main.cu:1170-1229): pass 1 accumulates per-pixel sums of c/L^2
+// and its square over every view x psi, pass 2 turns each map into z-scores
+// and reports 1024-pixel runs whose maximum reaches the threshold.  One
+// stream, and batches that never repeat a tile, so the FP64 accumulators see
+// a fixed order of additions.
+int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& picks) {
+  const int L = c->L;
+  const long L2 = (long)L * L;
+  const int P = num_psi(c->p.phi_step);
+  const int nm = c->T * c->N;
+  const int nb = std::min({c->batch, nm, c->T});
+  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;
+  const size_t npix = (size_t)c->T * L2;
+  CHK(c->n1_s1.ensure(npix * sizeof(double)));
+  CHK(c->n1_s2.ensure(npix * sizeof(double)));
+  CHK(c->n1_avg.ensure(npix * sizeof(float)));
+  CHK(c->n1_sd.ensure(npix * sizeof(float)));
+  CHK(c->n1_z.ensure((size_t)nb * L2 * sizeof(float)));
+  const int cap = 1 << 22;
+  CHK(c->n1_rec.ensure((size_t)cap * sizeof(N1Rec)));
+  CHK(c->sel_count.ensure(sizeof(int)));
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemsetAsync(c->n1_s1.p, 0, npix * sizeof(double), s));
+  HIPCHK(hipMemsetAsync(c->n1_s2.p, 0, npix * sizeof(double), s));
+  const MapRef* maps = c->maps.as<MapRef>();
+  auto run_pass1 = [&](int m0, int cnt) {
+    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), maps + m0, cnt, sgn,
+                      c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(), s);
+  };
+  for (int ip = 0; ip < P; ++ip) {
+    CHK(rotate_templates(c, ip));
+    for (int m0 = 0; m0 < nm; m0 += nb) {
+      const int cnt = std::min(nb, nm - m0);
+      run_pass1(m0, cnt);
+      launch_norm1_rows(L, 0, c->G[0].as<float2>(), maps + m0, cnt, c->n1_s1.as<double>(),
+                        c->n1_s2.as<double>(), nullptr, nullptr, nullptr, c->twL.as<float2>(),
+                        c->twM.as<float2>(), s);
+    }
+  }
+  launch_norm1_stats(c->n1_s1.as<double>(), c->n1_s2.as<double>(), (long)npix, (double)c->N * P,
+                     c->n1_avg.as<float>(), c->n1_sd.as<float>(), s);
+  HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), s));
+  for (int ip = psi_begin; ip < psi_end; ++ip) {
+    CHK(rotate_templates(c, ip));
+    for (int m0 = 0; m0 < nm; m0 += nb) {
+      const int cnt = std::min(nb, nm - m0);
+      run_pass1(m0, cnt);
+      launch_norm1_rows(L, 1, c->G[0].as<float2>(), maps + m0, cnt, nullptr, nullptr,
+                        c->n1_avg.as<float>(), c->n1_sd.as<float>(), c->n1_z.as<float>(),
+                        c->twL.as<float2>(), c->twM.as<float2>(), s);
+      launch_norm1_blockmax(c->n1_z.as<float>(), L, cnt, (ip - psi_begin) * nm + m0, maps + m0,
+                            c->offx_d.as<int>(), c->offy_d.as<int>(), c->bxn, c->p.threshold,
+                            c->p.diameter, c->nx, c->ny, c->sel_count.as<int>(), c->n1_rec.p, cap, s);
+    }
+  }
+  int count = 0;
+  HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  HIPCHK(hipGetLastError());
+  if (count > cap) return GS_ERR_CAPACITY;
+  std::vector<N1Rec> recs(count);
+  if (count) CHK(copy_out(c, recs.data(), c->n1_rec.p, (size_t)count * sizeof(N1Rec)));
+  // canonical order: psi, tile, view, block (main.cu:784-819)
+  const int nblk = (int)((L2 + 1023) / 1024);
+  auto key = [&](const N1Rec& r) {
+    const int ipl = r.m / nm, mm = r.m % nm;
+    const MapRef mr{mm % c->T, mm / c->T};  // view-major processing order
+    return (((long long)ipl * c->T + mr.tile) * c->N + mr.view) * nblk + r.block;
+  };
+  std::sort(recs.begin(), recs.end(), [&](const N1Rec& a, const N1Rec& b) { return key(a) < key(b); });
+  picks.resize(count);
+  for (int i = 0; i < count; ++i) {
+    const N1Rec& r = recs[i];
+    const int ip = r.m / nm + psi_begin, mm = r.m % nm;
+    gs_particle& pk = picks[i];
+    pk.psi_index = ip;
+    pk.tile = mm % c->T;
+    pk.view = mm / c->T;
+    float cur_e = 360.0f - psi_value(c->p.phi_step, ip);
+    if (cur_e >= 360) cur_e -= 360;
+    pk.psi = cur_e;
+    pk.cx = (float)(c->offx[pk.tile % c->bxn] + r.pos % L);
+    pk.cy = (float)(c->offy[pk.tile / c->bxn] + r.pos / L);
+    pk.score = r.z;
+  }
   return GS_OK;
 }
 
@@ -484,9 +663,9 @@ int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const 
 static int set_mic(Ctx* c, const float* src, bool dev, int nx, int ny, const gs_ctf* ctf) {
   if (!c || !src || nx <= 0 || ny <= 0 || !ctf) return GS_ERR_ARG;
   if (c->N == 0) return GS_ERR_STATE;
-  if (c->p.phase_flip != 0) return GS_ERR_UNSUPPORTED;
+  if (c->p.phase_flip != 0 && c->p.phase_flip != 1) return GS_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
-  return prep_micrograph(c, src, dev, nx, ny);
+  return prep_micrograph(c, src, dev, nx, ny, *ctf);
 }
 
 int gs_set_micrograph(void* ctx, const float* host, int nx, int ny, const gs_ctf* ctf) {
@@ -511,11 +690,17 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   if (!c || !n_out) return GS_ERR_ARG;
   *n_out = 0;
   if (c->N == 0 || !c->have_mic) return GS_ERR_STATE;
-  if (c->p.norm_type != 0) return GS_ERR_UNSUPPORTED;
+  if (c->p.norm_type != 0 && c->p.norm_type != 1) return GS_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
   const int P = num_psi(c->p.phi_step);
   if (psi_end < 0 || psi_end > P) psi_end = P;
   if (psi_begin < 0 || psi_begin > psi_end) return GS_ERR_ARG;
+  if (c->p.norm_type == 1) {
+    CHK(ensure_corr_buffers(c));
+    c->stats_npsi = 0;
+    CHK(pick_norm1(c, psi_begin, psi_end, c->picks));
+    return gs_get_picks(ctx, out, cap, n_out);
+  }
   const int npsi = psi_end - psi_begin;
   const long long per_psi = (long long)c->T * c->N;
   CHK(ensure_corr_buffers(c));

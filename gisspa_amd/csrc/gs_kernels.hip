@@ -500,6 +500,118 @@ __global__ void k_corr_finalize(const CorrPartial* __restrict__ part, int nparts
 }
 
 // ---------------------------------------------------------------------------
+// norm_type 1 ("global" per-pixel normalisation, main.cu:1170-1229):
+// the same row C2R as pass 2, then either accumulate c/L^2 and its square per
+// tile pixel over all views x psi (add_CCG_to_sum, GPU_func.cu:618-633), or
+// write the z-score map (update_CCG, GPU_func.cu:649-662).  Accumulators are
+// FP64; a batch never holds two maps of one tile and batches run in stream
+// order, so every accumulator sees a fixed order of additions.
+// ---------------------------------------------------------------------------
+enum { N1_ACCUM = 0, N1_ZMAP = 1 };
+
+template <int L, int MODE>
+__global__ __launch_bounds__(WG) void k_norm1_rows(const float2* __restrict__ Gin,
+                                                   const MapRef* __restrict__ maps,
+                                                   double* __restrict__ s1,
+                                                   double* __restrict__ s2,
+                                                   const float* __restrict__ avg,
+                                                   const float* __restrict__ sd,
+                                                   float* __restrict__ zout,
+                                                   const float2* __restrict__ twL,
+                                                   const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int M = G::M, H = G::H, RB = G::RB, CB = G::CB;
+  constexpr int NCB = (H + CB - 1) / CB;
+  constexpr long L2 = (long)L * L;
+  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  const int m = blockIdx.y;
+  const int y0 = blockIdx.x * RB;
+  const int t = maps[m].tile;
+  const float2* gb = Gin + (long)m * NCB * L * CB;
+  auto at = [&](int y, int k) { return gb[((k / CB) * L + y) * CB + (k % CB)]; };
+  auto load = [&](int r, int k) -> float2 {
+    const int y0r = y0 + r;
+    const bool ok = y0r < L;
+    const int y = ok ? y0r : 0;
+    float2 w = twL[k];
+    w.y = -w.y;
+    const float2 z = c2r_pre(at(y, k), at(y, M - k), w);
+    return ok ? z : make_float2(0.f, 0.f);
+  };
+  const float inv = 1.0f / (float)L2;
+  auto store = [&](int r, int n, float2 z) {
+    const int y = y0 + r;
+    if (y >= L) return;
+    const long pix = (long)y * L + 2 * n;
+    if constexpr (MODE == N1_ACCUM) {
+      const double a = (double)(z.x * inv), b = (double)(z.y * inv);
+      double* p1 = s1 + t * L2 + pix;
+      double* p2 = s2 + t * L2 + pix;
+      p1[0] += a; p1[1] += b;
+      p2[0] += a * a; p2[1] += b * b;
+    } else {
+      const float* av = avg + t * L2 + pix;
+      const float* sv = sd + t * L2 + pix;
+      const float a = z.x * inv, b = z.y * inv;
+      float2 o;
+      o.x = sv[0] > 0 ? (a - av[0]) / sv[0] : 0.0f;
+      o.y = sv[1] > 0 ? (b - av[1]) / sv[1] : 0.0f;
+      *reinterpret_cast<float2*>(zout + (long)m * L2 + pix) = o;
+    }
+  };
+  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+}
+
+// set_CCG_mean (GPU_func.cu:635-646): mean and standard deviation per pixel
+__global__ void k_norm1_stats(const double* __restrict__ s1, const double* __restrict__ s2,
+                              long n, double total, float* __restrict__ avg,
+                              float* __restrict__ sd) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = s1[i] / total;
+  avg[i] = (float)a;
+  sd[i] = (float)sqrt(s2[i] / total - a * a);
+}
+
+// get_peak_pos (GPU_func.cu:711-745) + host loop (main.cu:796-819): maximum
+// of each run of 1024 pixels of a z-map (lowest index on ties); runs above
+// the threshold whose centre is at least `dmin` from the micrograph edges
+// are appended to `out` (order restored on the host).
+
+__global__ __launch_bounds__(256) void k_norm1_blockmax(const float* __restrict__ z, long L2,
+                                                        int nblk, int L, int mbase,
+                                                        const MapRef* maps,
+                                                        const int* __restrict__ offx,
+                                                        const int* __restrict__ offy, int bxn,
+                                                        float thres, float dmin, int nx, int ny,
+                                                        int* __restrict__ count,
+                                                        N1Rec* __restrict__ out, int cap) {
+  __shared__ float rv[4];
+  __shared__ int rp[4];
+  const int m = blockIdx.y, b = blockIdx.x;
+  const long beg = (long)b * 1024;
+  float v = -INFINITY;
+  int p = INT_MAX;
+  for (int i = threadIdx.x; i < 1024; i += 256) {
+    const long k = beg + i;
+    if (k < L2) vmax_combine(v, p, z[(long)m * L2 + k], (int)k);
+  }
+  wave_max_idx(v, p);
+  if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = v; rp[threadIdx.x >> 6] = p; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  v = rv[0]; p = rp[0];
+  for (int i = 1; i < 4; ++i) vmax_combine(v, p, rv[i], rp[i]);
+  if (!(v >= thres)) return;
+  const int t = maps[m].tile;
+  const int cx = offx[t % bxn] + p % L, cy = offy[t / bxn] + p / L;
+  if (cx >= dmin && cx < nx - dmin && cy >= dmin && cy < ny - dmin) {
+    const int i = atomicAdd(count, 1);
+    if (i < cap) { N1Rec r; r.m = mbase + m; r.block = b; r.pos = p; r.z = v; out[i] = r; }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Per-psi normalisation scale: mean/var of the rotated masked template over
 // L^2 (compute_sum_sqr + main.cu:686-699), divided_by_var, FFT scale 1/L^2.
 // ---------------------------------------------------------------------------
@@ -958,5 +1070,38 @@ void launch_img_clamp(float* img, long n, int invert, const double* stats, doubl
 void launch_img_mean(const double* partials, int nblocks, long n, double* mean, hipStream_t s) {
   hipLaunchKernelGGL(k_img_mean, dim3(1), dim3(64), 0, s, partials, nblocks, n, mean);
 }
+
+template <int L>
+static void norm1_rows_t(int mode, const float2* G, const MapRef* maps, int nmaps, double* s1,
+                         double* s2, const float* avg, const float* sd, float* z,
+                         const float2* twL, const float2* twM, hipStream_t s) {
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
+  if (mode == N1_ACCUM)
+    hipLaunchKernelGGL((k_norm1_rows<L, N1_ACCUM>), grid, dim3(WG), 0, s, G, maps, s1, s2, avg, sd,
+                       z, twL, twM);
+  else
+    hipLaunchKernelGGL((k_norm1_rows<L, N1_ZMAP>), grid, dim3(WG), 0, s, G, maps, s1, s2, avg, sd,
+                       z, twL, twM);
+}
+void launch_norm1_rows(int L, int mode, const float2* G, const MapRef* maps, int nmaps,
+                       double* s1, double* s2, const float* avg, const float* sd, float* z,
+                       const float2* twL, const float2* twM, hipStream_t s) {
+  DISPATCH(L, norm1_rows_t, mode, G, maps, nmaps, s1, s2, avg, sd, z, twL, twM, s);
+}
+void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
+                        float* sd, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm1_stats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, s1, s2, n,
+                     total, avg, sd);
+}
+void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, const MapRef* maps,
+                           const int* offx, const int* offy, int bxn, float thres, float dmin,
+                           int nx, int ny, int* count, void* out, int cap, hipStream_t s) {
+  const long L2 = (long)L * L;
+  const int nblk = (int)((L2 + 1023) / 1024);
+  hipLaunchKernelGGL(k_norm1_blockmax, dim3(nblk, nmaps), dim3(256), 0, s, z, L2, nblk, L, mbase,
+                     maps,
+                     offx, offy, bxn, thres, dmin, nx, ny, count, static_cast<N1Rec*>(out), cap);
+}
+int norm1_record_bytes() { return (int)sizeof(N1Rec); }
 
 }  // namespace gs

@@ -92,6 +92,23 @@ int corr_nparts(int L);
 void launch_corr_finalize(int L, const CorrPartial* part, const float2* psum, int nmaps,
                           const MapRef* maps, int nviews, MapStat* out, hipStream_t s);
 
+// ---- norm_type 1 ----
+// mode 0: s1/s2[tile][pixel] += c / L^2, (c / L^2)^2 ; mode 1: z[m][pixel]
+void launch_norm1_rows(int L, int mode, const float2* G, const MapRef* maps, int nmaps,
+                       double* s1, double* s2, const float* avg, const float* sd, float* z,
+                       const float2* twL, const float2* twM, hipStream_t s);
+void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
+                        float* sd, hipStream_t s);
+// records {int mbase + m, block, pos; float z} of 1024-pixel runs above threshold
+struct N1Rec {
+  int m, block, pos;
+  float z;
+};
+void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, const MapRef* maps,
+                           const int* offx, const int* offy, int bxn, float thres, float dmin,
+                           int nx, int ny, int* count, void* out, int cap, hipStream_t s);
+int norm1_record_bytes();
+
 // ---- template preparation ----
 // per-template sums of value and value^2 of the raw box (FP64)
 void launch_box_sums(const float* src, int npix, int nb, double* sums, hipStream_t s);
@@ -120,5 +137,19 @@ void launch_img_clamp(float* img, long n, int invert, const double* stats, doubl
 void launch_img_mean(const double* partials, int nblocks, long n, double* mean, hipStream_t s);
 
 int img_reduce_blocks(long n);
+
+// ---- runtime-length FFTs and the phase-flip micrograph path (gs_rtfft.hip) ----
+void rt_fft_2d(float2* data, float2* tmp, int nx, int ny, const float2* twx, const float2* twy,
+               int dir, hipStream_t s);
+void launch_real_to_complex(const float* in, float2* out, long n, float scale, hipStream_t s);
+void launch_complex_to_real(const float2* in, float* out, long n, hipStream_t s);
+void launch_phase_flip(float2* F, int nx, int ny, const WeightArgs& w, hipStream_t s);
+void launch_img_radial(const float2* F, const int* start, const long* pix, int nbin, double* ra,
+                       hipStream_t s);
+void launch_img_weight(float2* F, int nx, int ny, const double* ra, const float* rb,
+                       const WeightArgs& w, hipStream_t s);
+void launch_tile_ampnorm(float2* spec, int L, int ntiles, double count, double* part,
+                         float* scale, hipStream_t s);
+int tile_ampnorm_parts(int L);
 
 }  // namespace gs

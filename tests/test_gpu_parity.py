@@ -96,3 +96,65 @@ def test_mapstats_and_picks(gpu, tiny_case):
     assert np.array_equal(picks["cx"], sel["cx"].astype(np.float32))
     assert np.array_equal(picks["cy"], sel["cy"].astype(np.float32))
     assert np.allclose(picks["score"], sel["score"], rtol=1e-4)
+
+
+def test_norm_type1_picks(gpu):
+    """norm_type 1: per-pixel mean/sd over every view x psi, z-scores, and one
+    candidate per 1024-pixel run (main.cu:1170-1229).  Picks must match the
+    oracle's record for record; a record whose oracle z-score lies within
+    1e-4 (relative) of the threshold may be missing on either side."""
+    import gisspa_amd
+    c = _cases.tiny(seed=3, l=64, box=48, n_views=4, size=(200, 176), phistep=45.0, overlap=16,
+                    threshold=3.0)
+    c.params["norm_type"] = 1
+    op = _oparams(c)
+    ref = orc.pick_norm1(op, orc.ctf(c.defocus), c.templates, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates(c.templates, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        got = pk.pick()
+    thr = c.params["threshold"]
+    key = lambda r: (int(r["psi_index"]), int(r["view"]), float(r["cx"]), float(r["cy"]))
+    rk = {key(r): float(r["score"]) for r in ref}
+    gk = {key(r): float(r["score"]) for r in got}
+    print("norm1 picks gpu", len(got), "oracle", len(ref))
+    assert len(ref) > 10
+    for k in set(rk) ^ set(gk):
+        s = rk.get(k, gk.get(k))
+        assert abs(s - thr) <= 1e-4 * thr, (k, s)
+    for k in set(rk) & set(gk):
+        assert abs(rk[k] - gk[k]) <= 1e-4 * max(1.0, abs(rk[k])), (k, rk[k], gk[k])
+    # canonical order: psi, then tile, view, block
+    order = [(int(r["psi_index"]), int(r["tile"]), int(r["view"])) for r in got]
+    assert order == sorted(order)
+
+
+@pytest.mark.parametrize("size", [(320, 288), (330, 294)])
+def test_phase_flip_path(gpu, size):
+    """phase_flip = 1: whole-micrograph CTF sign flip, radial whitening,
+    weighting and band pass (init_d_image, main.cu:548-586) on sizes with
+    factors 3, 5, 7, 11 (330 = 2 3 5 11, 294 = 2 3 7^2), then per-tile
+    amplitude normalisation (main.cu:617-653) and the norm_type 0 picks."""
+    import gisspa_amd
+    c = _cases.tiny(seed=4, l=96, size=size, threshold=4.0)
+    c.params["phase_flip"] = 1
+    op = _oparams(c)
+    ref_tiles = orc.micrograph_prep(op, orc.ctf(c.defocus), c.image)
+    ref = orc.pick_norm0(op, orc.ctf(c.defocus), c.templates, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates(c.templates, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        tiles = pk.tile_spectra()
+        picks = pk.pick()
+        got = pk.mapstats().reshape(ref.shape)
+    H = c.l // 2 + 1
+    rh = np.transpose(ref_tiles, (0, 2, 1))[:, :H, :]
+    err = np.abs(tiles - rh).max() / np.abs(rh).max()
+    print("phase-flip tile spectra rel err", err)
+    assert err < 2e-5
+    rel = np.abs(got["score"] - ref["score"]) / np.maximum(np.abs(ref["score"]), 1.0)
+    print("phase-flip score max rel err", rel.max())
+    assert rel.max() < 1e-4
+    assert np.array_equal(got["pos"], ref["pos"])
+    sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
+    assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
