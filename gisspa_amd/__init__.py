@@ -37,12 +37,13 @@ class GsCtf(ctypes.Structure):
 class GsParticle(ctypes.Structure):
     _fields_ = [("psi_index", ctypes.c_int), ("view", ctypes.c_int), ("tile", ctypes.c_int),
                 ("psi", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
-                ("score", ctypes.c_float)]
+                ("score", ctypes.c_float), ("mic", ctypes.c_int)]
 
 
 class GsInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in
-                ("l", "n_views", "n_psi", "nx", "ny", "tiles_x", "tiles_y", "box_x", "box_y")]
+                ("l", "n_views", "n_psi", "nx", "ny", "tiles_x", "tiles_y", "box_x", "box_y",
+                 "n_mics", "n_tiles")]
 
 
 class GsTiming(ctypes.Structure):
@@ -56,7 +57,11 @@ class GsTiming(ctypes.Structure):
 MAPSTAT_DTYPE = np.dtype([("score", "<f4"), ("pos", "<i4"), ("peak", "<f4"), ("sd", "<f4"),
                           ("sum", "<f8"), ("sum2", "<f8")])
 PARTICLE_DTYPE = np.dtype([("psi_index", "<i4"), ("view", "<i4"), ("tile", "<i4"),
-                           ("psi", "<f4"), ("cx", "<f4"), ("cy", "<f4"), ("score", "<f4")])
+                           ("psi", "<f4"), ("cx", "<f4"), ("cy", "<f4"), ("score", "<f4"),
+                           ("mic", "<i4")])
+
+GS_MODE_FUSED = 0
+GS_MODE_PLAIN = 1
 
 _lib = None
 
@@ -88,6 +93,11 @@ def load() -> ctypes.CDLL:
     lib.gs_set_micrograph.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(GsCtf)]
     lib.gs_set_micrograph_device.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int,
                                              ctypes.POINTER(GsCtf)]
+    lib.gs_set_micrographs.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(GsCtf), ctypes.c_int]
+    lib.gs_set_mode.argtypes = [vp, ctypes.c_int]
+    lib.gs_set_graphs.argtypes = [vp, ctypes.c_int]
     lib.gs_pick.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t,
                             ctypes.POINTER(ctypes.c_size_t)]
     lib.gs_get_picks.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
@@ -186,6 +196,26 @@ class Picker:
         _check(self.lib.gs_set_micrograph_device(self.ctx, ctypes.c_void_p(ptr), nx, ny,
                                                  ctypes.byref(c)), "gs_set_micrograph_device")
 
+    def set_micrographs(self, imgs, ctfs) -> None:
+        """A batch of micrographs (any sizes); ctfs: one (defocus, dfdiff, dfang)
+        tuple or defocus per micrograph.  Picks then carry `mic`."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in imgs]
+        n = len(arrs)
+        ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in arrs])
+        nx = (ctypes.c_int * n)(*[a.shape[1] for a in arrs])
+        ny = (ctypes.c_int * n)(*[a.shape[0] for a in arrs])
+        cs = (GsCtf * n)()
+        for i, c in enumerate(ctfs):
+            c = tuple(np.atleast_1d(c)) + (0.0, 0.0)
+            cs[i] = GsCtf(float(c[0]), float(c[1]), float(c[2]))
+        _check(self.lib.gs_set_micrographs(self.ctx, n, ptrs, nx, ny, cs, 0), "gs_set_micrographs")
+
+    def set_mode(self, mode: int) -> None:
+        _check(self.lib.gs_set_mode(self.ctx, mode), "gs_set_mode")
+
+    def set_graphs(self, enable: bool) -> None:
+        _check(self.lib.gs_set_graphs(self.ctx, int(bool(enable))), "gs_set_graphs")
+
     def pick(self, psi_begin: int = 0, psi_end: int = -1) -> np.ndarray:
         """Picks of in-plane steps [psi_begin, psi_end) (PARTICLE_DTYPE records)."""
         n = ctypes.c_size_t(0)
@@ -237,8 +267,7 @@ class Picker:
     def tile_spectra(self) -> np.ndarray:
         """[tile][kx][ky] complex64 half spectra."""
         i = self.info()
-        T = i.tiles_x * i.tiles_y
-        out = np.zeros((T, i.l // 2 + 1, i.l), dtype=np.complex64)
+        out = np.zeros((i.n_tiles, i.l // 2 + 1, i.l), dtype=np.complex64)
         _check(self.lib.gs_get_tile_spectra(self.ctx, out.ctypes.data, out.size * 2),
                "gs_get_tile_spectra")
         return out

@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -50,9 +52,18 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
-struct SelRec {
-  long long m;
-  float cx, cy, score, pad;
+// One micrograph of the current batch.
+struct MicSlot {
+  int nx = 0, ny = 0;
+  gs_ctf ctf{};
+  std::vector<int> offx, offy;
+  int tile0 = 0, ntiles = 0;
+};
+
+// Whole-image phase-flip state for one micrograph size.
+struct FlipCache {
+  DevBuf twx, twy, rbin_start, rbin_pix, rbin_count, rbin_ra;
+  int nbin = 0;
 };
 
 struct Ctx {
@@ -62,23 +73,23 @@ struct Ctx {
   int L = 0, L_config = 0;
   Geometry geo{};
   DevBuf twL, twM;
+  int mode = GS_MODE_FUSED;
+  bool use_graphs = false;
   // templates
   int N = 0, bx = 0, by = 0;
   gs_ctf tctf{};
   DevBuf treal, tspec, rot_part, bscale;
-  // micrograph
-  int nx = 0, ny = 0;
-  bool have_mic = false;
-  DevBuf img, img_part, img_stats, img_mean, offx_d, offy_d, tiles;
-  // phase_flip = 1: whole-image spectra and their radial bins (fixed by nx, ny)
-  DevBuf cimg, ctmp, twx, twy, rbin_start, rbin_pix, rbin_count, rbin_ra, zero_mean;
-  DevBuf tile_part, tile_scale;
-  int nbin_img = 0;
   double half_count = 0;  // pixels of the x <= l/2 half plane inside radius l/2
-  std::vector<int> offx, offy;
-  int bxn = 0, byn = 0, T = 0;
-  // correlation: two lanes (stream + intermediate + partials) so that pass 1
-  // of one batch overlaps pass 2 of the previous one
+  // micrograph batch: tile pool of every micrograph, tile descriptors
+  std::vector<MicSlot> mics;
+  bool have_mic = false;
+  int T = 0;
+  DevBuf tiles, tdesc;
+  DevBuf img, img_part, img_stats, img_mean, offx_d, offy_d, zero_mean;
+  DevBuf cimg, ctmp, tile_part, tile_scale;
+  std::vector<std::pair<long long, std::unique_ptr<FlipCache>>> flips;
+  // correlation: two lanes (stream + intermediate) so that pass 1 of one
+  // batch overlaps pass 2 of the previous one
   static constexpr int NLANE = 2;
   int batch = 0;
   hipStream_t lane_stream[NLANE] = {nullptr, nullptr};
@@ -87,7 +98,13 @@ struct Ctx {
   DevBuf part;  // row-block partials of every map of one psi
   DevBuf psum;  // column-block Parseval sums of every map of one psi
   DevBuf maps, stats, sel, sel_count;
+  int maps_T = -1, maps_N = -1;
   DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_z, n1_rec;  // norm_type 1
+  // plain mode buffers
+  DevBuf pl_tspec, pl_tiles, pl_ccg, pl_part, pl_rotpart, pl_mean, pl_var;
+  // per-psi graphs of the fused correlation (gs_set_graphs)
+  std::vector<hipGraphExec_t> graphs;
+  std::vector<const void*> graph_key;  // buffers the graphs were captured with
   std::vector<gs_particle> picks;  // of the last gs_pick
   int stats_psi_begin = 0, stats_npsi = 0;
   gs_timing timing{};
@@ -257,6 +274,45 @@ int upload_table(DevBuf& buf, int n, hipStream_t s) {
   return GS_OK;
 }
 
+FlipCache* flip_cache(Ctx* c, int nx, int ny) {
+  const long long key = ((long long)nx << 32) | (unsigned)ny;
+  for (auto& e : c->flips)
+    if (e.first == key) return e.second.get();
+  std::unique_ptr<FlipCache> fc(new FlipCache());
+  hipStream_t s = c->stream;
+  if (upload_table(fc->twx, nx, s) || upload_table(fc->twy, ny, s)) return nullptr;
+  // SQRSum_by_circle mode 1 bins (GPU_func.cu:113-126): x <= nx/2,
+  // 0 <= r < max(nx, ny)/2, r = floor(hypot(min(y,ny-y), min(x,nx-x)) + .5) - 1
+  const int nbin = std::max(nx, ny) / 2;
+  std::vector<std::vector<long>> bins(nbin);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x <= nx / 2; ++x) {
+      const double dx = std::min(x, nx - x), dy = std::min(y, ny - y);
+      const int r = (int)std::floor(std::hypot(dy, dx) + 0.5) - 1;
+      if (r >= 0 && r < nbin) bins[r].push_back((long)y * nx + x);
+    }
+  std::vector<int> start(nbin + 1, 0);
+  std::vector<long> pix;
+  std::vector<float> cnt(nbin);
+  for (int r = 0; r < nbin; ++r) {
+    start[r + 1] = start[r] + (int)bins[r].size();
+    pix.insert(pix.end(), bins[r].begin(), bins[r].end());
+    cnt[r] = (float)bins[r].size();
+  }
+  if (fc->rbin_start.ensure(start.size() * sizeof(int)) ||
+      fc->rbin_pix.ensure(std::max<size_t>(1, pix.size()) * sizeof(long)) ||
+      fc->rbin_count.ensure(nbin * sizeof(float)) || fc->rbin_ra.ensure(nbin * sizeof(double)))
+    return nullptr;
+  if (hipMemcpyAsync(fc->rbin_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(fc->rbin_pix.p, pix.data(), pix.size() * sizeof(long), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(fc->rbin_count.p, cnt.data(), nbin * sizeof(float), hipMemcpyHostToDevice, s) ||
+      hipStreamSynchronize(s))
+    return nullptr;
+  fc->nbin = nbin;
+  c->flips.emplace_back(key, std::move(fc));
+  return c->flips.back().second.get();
+}
+
 // Whole-micrograph phase flip, whitening, weighting and band pass, DC := 0
 // (init_d_image with phase_flip = 1, main.cu:548-586), in place on c->img.
 int phase_flip_image(Ctx* c, int nx, int ny, const gs_ctf& ctf) {
@@ -264,73 +320,32 @@ int phase_flip_image(Ctx* c, int nx, int ny, const gs_ctf& ctf) {
   const long n = (long)nx * ny;
   CHK(c->cimg.ensure(n * sizeof(float2)));
   CHK(c->ctmp.ensure(n * sizeof(float2)));
-  if (c->nbin_img == 0) {
-    CHK(upload_table(c->twx, nx, s));
-    CHK(upload_table(c->twy, ny, s));
-    // SQRSum_by_circle mode 1 bins (GPU_func.cu:113-126): x <= nx/2,
-    // 0 <= r < max(nx, ny)/2, r = floor(hypot(min(y,ny-y), min(x,nx-x)) + .5) - 1
-    const int nbin = std::max(nx, ny) / 2;
-    std::vector<std::vector<long>> bins(nbin);
-    for (int y = 0; y < ny; ++y)
-      for (int x = 0; x <= nx / 2; ++x) {
-        const double dx = std::min(x, nx - x), dy = std::min(y, ny - y);
-        const int r = (int)std::floor(std::hypot(dy, dx) + 0.5) - 1;
-        if (r >= 0 && r < nbin) bins[r].push_back((long)y * nx + x);
-      }
-    std::vector<int> start(nbin + 1, 0);
-    std::vector<long> pix;
-    std::vector<float> cnt(nbin);
-    for (int r = 0; r < nbin; ++r) {
-      start[r + 1] = start[r] + (int)bins[r].size();
-      pix.insert(pix.end(), bins[r].begin(), bins[r].end());
-      cnt[r] = (float)bins[r].size();
-    }
-    CHK(c->rbin_start.ensure(start.size() * sizeof(int)));
-    CHK(c->rbin_pix.ensure(std::max<size_t>(1, pix.size()) * sizeof(long)));
-    CHK(c->rbin_count.ensure(nbin * sizeof(float)));
-    CHK(c->rbin_ra.ensure(nbin * sizeof(double)));
-    HIPCHK(hipMemcpyAsync(c->rbin_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->rbin_pix.p, pix.data(), pix.size() * sizeof(long), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->rbin_count.p, cnt.data(), nbin * sizeof(float), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
-    c->nbin_img = nbin;
-  }
+  FlipCache* fc = flip_cache(c, nx, ny);
+  if (!fc) return GS_ERR_DEVICE;
   const WeightArgs w = weight_args(c, ctf);
   // FFT of the image scaled by 1/n (cuFFT is unnormalised; main.cu:553-554)
   launch_real_to_complex(c->img.as<float>(), c->cimg.as<float2>(), n, (float)(1.0 / (double)n), s);
-  rt_fft_2d(c->cimg.as<float2>(), c->ctmp.as<float2>(), nx, ny, c->twx.as<float2>(),
-            c->twy.as<float2>(), -1, s);
+  rt_fft_2d(c->cimg.as<float2>(), c->ctmp.as<float2>(), nx, ny, fc->twx.as<float2>(),
+            fc->twy.as<float2>(), -1, s);
   launch_phase_flip(c->cimg.as<float2>(), nx, ny, w, s);
-  launch_img_radial(c->cimg.as<float2>(), c->rbin_start.as<int>(), c->rbin_pix.as<long>(),
-                    c->nbin_img, c->rbin_ra.as<double>(), s);
-  launch_img_weight(c->cimg.as<float2>(), nx, ny, c->rbin_ra.as<double>(),
-                    c->rbin_count.as<float>(), w, s);
-  rt_fft_2d(c->cimg.as<float2>(), c->ctmp.as<float2>(), nx, ny, c->twx.as<float2>(),
-            c->twy.as<float2>(), +1, s);
+  launch_img_radial(c->cimg.as<float2>(), fc->rbin_start.as<int>(), fc->rbin_pix.as<long>(),
+                    fc->nbin, fc->rbin_ra.as<double>(), s);
+  launch_img_weight(c->cimg.as<float2>(), nx, ny, fc->rbin_ra.as<double>(),
+                    fc->rbin_count.as<float>(), w, s);
+  rt_fft_2d(c->cimg.as<float2>(), c->ctmp.as<float2>(), nx, ny, fc->twx.as<float2>(),
+            fc->twy.as<float2>(), +1, s);
   launch_complex_to_real(c->cimg.as<float2>(), c->img.as<float>(), n, s);
   return GS_OK;
 }
 
-int prep_micrograph(Ctx* c, const float* src, bool src_is_device, int nx, int ny,
-                    const gs_ctf& ctf) {
+// Prepare micrograph `mi` of the batch into its slice of the tile pool
+// (init_d_image, split_normalize_image, fft_on_img: main.cu:523-655, 758-769).
+int prep_micrograph(Ctx* c, int mi, const float* src, bool src_is_device) {
   hipStream_t s = c->stream;
+  MicSlot& ms = c->mics[mi];
+  const int nx = ms.nx, ny = ms.ny;
   const long n = (long)nx * ny;
   const int L = c->L;
-  if (!c->have_mic) {
-    CHK(tile_offsets(nx, L, c->p.overlap, c->offx));
-    CHK(tile_offsets(ny, L, c->p.overlap, c->offy));
-    c->bxn = (int)c->offx.size();
-    c->byn = (int)c->offy.size();
-    c->T = c->bxn * c->byn;
-    c->nx = nx;
-    c->ny = ny;
-    CHK(c->offx_d.ensure(c->offx.size() * sizeof(int)));
-    CHK(c->offy_d.ensure(c->offy.size() * sizeof(int)));
-    HIPCHK(hipMemcpyAsync(c->offx_d.p, c->offx.data(), c->offx.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->offy_d.p, c->offy.data(), c->offy.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  } else if (nx != c->nx || ny != c->ny) {
-    return GS_ERR_ARG;  // the reference silently assumes equal sizes (main.cu:1108)
-  }
   CHK(c->img.ensure(n * sizeof(float)));
   HIPCHK(hipMemcpyAsync(c->img.p, src, n * sizeof(float),
                         src_is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
@@ -345,7 +360,7 @@ int prep_micrograph(Ctx* c, const float* src, bool src_is_device, int nx, int ny
   launch_img_mean(c->img_part.as<double>(), nblk, n, c->img_mean.as<double>(), s);
   const double* mean = c->img_mean.as<double>();
   if (c->p.phase_flip == 1) {
-    CHK(phase_flip_image(c, nx, ny, ctf));
+    CHK(phase_flip_image(c, nx, ny, ms.ctf));
     // the filtered image already has zero DC; split_IMG copies it as is
     if (!c->zero_mean.p) {
       CHK(c->zero_mean.ensure(sizeof(double)));
@@ -353,40 +368,90 @@ int prep_micrograph(Ctx* c, const float* src, bool src_is_device, int nx, int ny
     }
     mean = c->zero_mean.as<double>();
   }
+  CHK(c->offx_d.ensure(ms.offx.size() * sizeof(int)));
+  CHK(c->offy_d.ensure(ms.offy.size() * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(c->offx_d.p, ms.offx.data(), ms.offx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->offy_d.p, ms.offy.data(), ms.offy.size() * sizeof(int), hipMemcpyHostToDevice, s));
   const int H = c->geo.H;
-  CHK(c->tiles.ensure((size_t)c->T * H * L * sizeof(float2)));
+  float2* pool = c->tiles.as<float2>() + (size_t)ms.tile0 * H * L;
   launch_rows_r2c_tiles(L, c->img.as<float>(), nx, ny, c->offx_d.as<int>(), c->offy_d.as<int>(),
-                        c->bxn, c->T, mean, c->tiles.as<float2>(), c->twL.as<float2>(),
+                        (int)ms.offx.size(), ms.ntiles, mean, pool, c->twL.as<float2>(),
                         c->twM.as<float2>(), s);
-  launch_cols_fwd(L, c->tiles.as<float2>(), c->T, (float)(1.0 / ((double)L * L)), nullptr, 0,
+  launch_cols_fwd(L, pool, ms.ntiles, (float)(1.0 / ((double)L * L)), nullptr, 0,
                   c->twL.as<float2>(), s);
   if (c->p.phase_flip == 1) {
     // per-tile amplitude normalisation (main.cu:617-653)
-    CHK(c->tile_part.ensure((size_t)c->T * tile_ampnorm_parts(L) * sizeof(double)));
-    CHK(c->tile_scale.ensure((size_t)c->T * sizeof(float)));
-    launch_tile_ampnorm(c->tiles.as<float2>(), L, c->T, c->half_count, c->tile_part.as<double>(),
+    CHK(c->tile_part.ensure((size_t)ms.ntiles * tile_ampnorm_parts(L) * sizeof(double)));
+    CHK(c->tile_scale.ensure((size_t)ms.ntiles * sizeof(float)));
+    launch_tile_ampnorm(pool, L, ms.ntiles, c->half_count, c->tile_part.as<double>(),
                         c->tile_scale.as<float>(), s);
   }
-  // the caller may release the host image on return
+  if (c->mode == GS_MODE_PLAIN) {
+    // plain mode keeps full complex row-major tile spectra (fft_on_img)
+    const long L2 = (long)L * L;
+    CHK(c->pl_tiles.ensure((size_t)c->T * L2 * sizeof(float2)));
+    float2* pt = c->pl_tiles.as<float2>() + (size_t)ms.tile0 * L2;
+    launch_plain_tiles(L, c->img.as<float>(), nx, ny, c->tdesc.as<TileDesc>(), ms.tile0,
+                       ms.ntiles, mean, pt, s);
+    launch_c2c_2d(L, pt, ms.ntiles, -1, (float)(1.0 / (double)L2), c->twL.as<float2>(), s);
+  }
+  // the caller may release the source image on return
   HIPCHK(hipStreamSynchronize(s));
   HIPCHK(hipGetLastError());
+  return GS_OK;
+}
+
+// Lay out the tile pool of a batch and prepare every micrograph.
+int set_batch(Ctx* c, int n, const float* const* images, const int* nx, const int* ny,
+              const gs_ctf* ctfs, bool on_device) {
+  const int L = c->L;
+  std::vector<MicSlot> mics(n);
+  std::vector<TileDesc> desc;
+  int T = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!images[i] || nx[i] <= 0 || ny[i] <= 0) return GS_ERR_ARG;
+    MicSlot& ms = mics[i];
+    ms.nx = nx[i];
+    ms.ny = ny[i];
+    ms.ctf = ctfs[i];
+    CHK(tile_offsets(nx[i], L, c->p.overlap, ms.offx));
+    CHK(tile_offsets(ny[i], L, c->p.overlap, ms.offy));
+    ms.tile0 = T;
+    ms.ntiles = (int)(ms.offx.size() * ms.offy.size());
+    for (int iy = 0; iy < (int)ms.offy.size(); ++iy)
+      for (int ix = 0; ix < (int)ms.offx.size(); ++ix)
+        desc.push_back(TileDesc{i, ms.offx[ix], ms.offy[iy], nx[i], ny[i]});
+    T += ms.ntiles;
+  }
+  c->mics = std::move(mics);
+  c->T = T;
+  c->have_mic = false;
+  const int H = c->geo.H;
+  CHK(c->tiles.ensure((size_t)T * H * L * sizeof(float2)));
+  CHK(c->tdesc.ensure(desc.size() * sizeof(TileDesc)));
+  HIPCHK(hipMemcpyAsync(c->tdesc.p, desc.data(), desc.size() * sizeof(TileDesc), hipMemcpyHostToDevice, c->stream));
+  for (int i = 0; i < n; ++i) CHK(prep_micrograph(c, i, images[i], on_device));
   c->have_mic = true;
   return GS_OK;
 }
 
-// device -> host copy ordered after all work queued on the context stream
 int copy_out(Ctx* c, void* dst, const void* src, size_t bytes) {
   HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return GS_OK;
 }
 
-int rotate_templates(Ctx* c, int psi_index) {
-  const int L = c->L;
+void psi_rotation(const Ctx* c, int psi_index, float* cose, float* sine) {
   const float e = psi_value(c->p.phi_step, psi_index);
   // rotate_subIMG computes cos/sin in double and stores float (GPU_func.cu:498-499)
-  const float cose = (float)std::cos((double)e * kRefPiD / 180);
-  const float sine = (float)std::sin((double)e * kRefPiD / 180);
+  *cose = (float)std::cos((double)e * kRefPiD / 180);
+  *sine = (float)std::sin((double)e * kRefPiD / 180);
+}
+
+int rotate_templates(Ctx* c, int psi_index) {
+  const int L = c->L;
+  float cose, sine;
+  psi_rotation(c, psi_index, &cose, &sine);
   const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
   CHK(c->rot_part.ensure((size_t)c->N * nrb * 2 * sizeof(double)));
   CHK(c->bscale.ensure((size_t)c->N * sizeof(float)));
@@ -399,6 +464,13 @@ int rotate_templates(Ctx* c, int psi_index) {
   return GS_OK;
 }
 
+void drop_graphs(Ctx* c) {
+  for (auto g : c->graphs)
+    if (g) (void)hipGraphExecDestroy(g);
+  c->graphs.clear();
+  c->graph_key.clear();
+}
+
 int ensure_corr_buffers(Ctx* c) {
   const int L = c->L;
   const long per_map = c->geo.gmap * (long)sizeof(float2);
@@ -409,21 +481,25 @@ int ensure_corr_buffers(Ctx* c) {
   CHK(c->part.ensure((size_t)c->T * c->N * nrb * sizeof(CorrPartial)));
   CHK(c->psum.ensure((size_t)c->T * c->N * corr_ncb(L) * sizeof(float2)));
   // processing order: view-major, so consecutive maps share one template
-  // spectrum (kept in cache across all tiles) while the tile spectra
-  // (T * 2 MB) stay resident in the last-level cache; results are stored in
-  // the canonical [tile][view] order by the finalisation
+  // spectrum (kept in cache across all tiles) while the tile spectra stay
+  // resident in the last-level cache; results are stored in the canonical
+  // [tile][view] order by the finalisation
   const size_t nm = (size_t)c->T * c->N;
-  if (c->maps.bytes < nm * sizeof(MapRef)) {
+  if (c->maps_T != c->T || c->maps_N != c->N) {
     std::vector<MapRef> mr(nm);
     for (int j = 0; j < c->N; ++j)
       for (int t = 0; t < c->T; ++t) mr[(size_t)j * c->T + t] = MapRef{t, j};
     CHK(c->maps.ensure(nm * sizeof(MapRef)));
     HIPCHK(hipMemcpyAsync(c->maps.p, mr.data(), nm * sizeof(MapRef), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));  // mr is released on return
+    c->maps_T = c->T;
+    c->maps_N = c->N;
   }
   return GS_OK;
 }
 
+// Correlate all maps of one psi (the templates of this psi are in tspec):
+// batches alternate between the two lanes; stats[tile][view] <- scores.
 int correlate_psi(Ctx* c, MapStat* out) {
   const int L = c->L;
   const int nm = c->T * c->N;
@@ -444,8 +520,6 @@ int correlate_psi(Ctx* c, MapStat* out) {
     launch_corr_pass2(L, c->G[lane].as<float2>(), cnt, lo, hi,
                       c->part.as<CorrPartial>() + (size_t)m0 * corr_nparts(L),
                       c->twL.as<float2>(), c->twM.as<float2>(), ls);
-    c->timing.corr_pass1_launches++;
-    c->timing.corr_pass2_launches++;
   }
   // the context stream (finalisation, next psi's rotation, selection) waits
   // for both lanes
@@ -455,7 +529,46 @@ int correlate_psi(Ctx* c, MapStat* out) {
   }
   launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
                        c->maps.as<MapRef>(), c->N, out, c->stream);
-  c->timing.corr_maps += nm;
+  return GS_OK;
+}
+
+// One psi of the fused path: rotation + correlation, replayed from a
+// captured graph when graphs are on.  The graph of a psi is keyed on every
+// buffer it touches; any reallocation drops all graphs.
+int fused_psi(Ctx* c, int ip, MapStat* out) {
+  if (!c->use_graphs) {
+    CHK(rotate_templates(c, ip));
+    return correlate_psi(c, out);
+  }
+  const std::vector<const void*> key = {c->tiles.p, c->tspec.p, c->treal.p, c->G[0].p, c->G[1].p,
+                                        c->part.p, c->psum.p, c->maps.p, c->rot_part.p,
+                                        c->bscale.p, out, (const void*)(long)c->T,
+                                        (const void*)(long)c->batch};
+  const int P = num_psi(c->p.phi_step);
+  if ((int)c->graphs.size() != P) {
+    drop_graphs(c);
+    c->graphs.assign(P, nullptr);
+    c->graph_key.assign((size_t)P * key.size(), nullptr);
+  }
+  const void** gk = c->graph_key.data() + (size_t)ip * key.size();
+  if (!c->graphs[ip] || !std::equal(key.begin(), key.end(), gk)) {
+    if (c->graphs[ip]) (void)hipGraphExecDestroy(c->graphs[ip]);
+    c->graphs[ip] = nullptr;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = rotate_templates(c, ip);
+    if (r == GS_OK) r = correlate_psi(c, out);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != GS_OK) return r;
+    HIPCHK(e);
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ie);
+    c->graphs[ip] = ge;
+    std::copy(key.begin(), key.end(), gk);
+  }
+  HIPCHK(hipGraphLaunch(c->graphs[ip], c->stream));
   return GS_OK;
 }
 
@@ -510,8 +623,8 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
                         c->n1_avg.as<float>(), c->n1_sd.as<float>(), c->n1_z.as<float>(),
                         c->twL.as<float2>(), c->twM.as<float2>(), s);
       launch_norm1_blockmax(c->n1_z.as<float>(), L, cnt, (ip - psi_begin) * nm + m0, maps + m0,
-                            c->offx_d.as<int>(), c->offy_d.as<int>(), c->bxn, c->p.threshold,
-                            c->p.diameter, c->nx, c->ny, c->sel_count.as<int>(), c->n1_rec.p, cap, s);
+                            c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
+                            c->sel_count.as<int>(), c->n1_rec.p, cap, s);
     }
   }
   int count = 0;
@@ -521,7 +634,7 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   if (count > cap) return GS_ERR_CAPACITY;
   std::vector<N1Rec> recs(count);
   if (count) CHK(copy_out(c, recs.data(), c->n1_rec.p, (size_t)count * sizeof(N1Rec)));
-  // canonical order: psi, tile, view, block (main.cu:784-819)
+  // canonical order: micrograph, psi, tile, view, block (main.cu:784-819)
   const int nblk = (int)((L2 + 1023) / 1024);
   auto key = [&](const N1Rec& r) {
     const int ipl = r.m / nm, mm = r.m % nm;
@@ -533,46 +646,112 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   for (int i = 0; i < count; ++i) {
     const N1Rec& r = recs[i];
     const int ip = r.m / nm + psi_begin, mm = r.m % nm;
+    const int t = mm % c->T;
+    int mi = 0;
+    while (mi + 1 < (int)c->mics.size() && c->mics[mi + 1].tile0 <= t) ++mi;
+    const MicSlot& ms = c->mics[mi];
+    const int lt = t - ms.tile0;
     gs_particle& pk = picks[i];
     pk.psi_index = ip;
-    pk.tile = mm % c->T;
+    pk.tile = lt;
     pk.view = mm / c->T;
+    pk.mic = mi;
     float cur_e = 360.0f - psi_value(c->p.phi_step, ip);
     if (cur_e >= 360) cur_e -= 360;
     pk.psi = cur_e;
-    pk.cx = (float)(c->offx[pk.tile % c->bxn] + r.pos % L);
-    pk.cy = (float)(c->offy[pk.tile / c->bxn] + r.pos / L);
+    pk.cx = (float)(ms.offx[lt % ms.offx.size()] + r.pos % L);
+    pk.cy = (float)(ms.offy[lt / ms.offx.size()] + r.pos / L);
     pk.score = r.z;
+  }
+  std::stable_sort(picks.begin(), picks.end(),
+                   [](const gs_particle& a, const gs_particle& b) { return a.mic < b.mic; });
+  return GS_OK;
+}
+
+// Stage-by-stage picker (main.cu:669-716, 824-894): per psi, rotation with a
+// host-side mean/variance, a full complex FFT of every template; per tile,
+// product, full complex inverse FFT, per-1024-element partials copied to the
+// host and reduced there.  Scores land in stats[tile][view] like the fused
+// path's.
+int plain_psi(Ctx* c, int ip, MapStat* out) {
+  const int L = c->L;
+  const long L2 = (long)L * L;
+  const int N = c->N, T = c->T;
+  hipStream_t s = c->stream;
+  const int nblk = plain_blocks(L);
+  CHK(c->pl_tspec.ensure((size_t)N * L2 * sizeof(float2)));
+  CHK(c->pl_ccg.ensure((size_t)N * L2 * sizeof(float2)));
+  CHK(c->pl_rotpart.ensure((size_t)N * nblk * 2 * sizeof(double)));
+  CHK(c->pl_part.ensure((size_t)N * nblk * sizeof(PeakPartHost)));
+  CHK(c->pl_mean.ensure((size_t)N * sizeof(float)));
+  CHK(c->pl_var.ensure((size_t)N * sizeof(float)));
+  float cose, sine;
+  psi_rotation(c, ip, &cose, &sine);
+  launch_plain_rotate(L, c->treal.as<float>(), N, cose, sine, 1.5f * c->p.diameter,
+                      c->pl_tspec.as<float2>(), c->pl_rotpart.as<double>(), s);
+  std::vector<double> rp((size_t)N * nblk * 2);
+  CHK(copy_out(c, rp.data(), c->pl_rotpart.p, rp.size() * sizeof(double)));
+  std::vector<float> mean(N), var(N);
+  for (int j = 0; j < N; ++j) {  // main.cu:688-699
+    double s1 = 0, s2 = 0;
+    for (int b = 0; b < nblk; ++b) { s1 += rp[2 * ((size_t)j * nblk + b)]; s2 += rp[2 * ((size_t)j * nblk + b) + 1]; }
+    const double m = s1 / L2;
+    mean[j] = (float)m;
+    var[j] = (float)(s2 / L2 - m * m);
+  }
+  HIPCHK(hipMemcpyAsync(c->pl_mean.p, mean.data(), N * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->pl_var.p, var.data(), N * sizeof(float), hipMemcpyHostToDevice, s));
+  launch_plain_normalize(L, c->pl_tspec.as<float2>(), N, c->pl_mean.as<float>(),
+                         c->pl_var.as<float>(), s);
+  launch_c2c_2d(L, c->pl_tspec.as<float2>(), N, -1, (float)(1.0 / (double)L2), c->twL.as<float2>(), s);
+  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;
+  const float lo = c->p.diameter / 4.0f, hi = (float)L - c->p.diameter / 4.0f;
+  std::vector<PeakPartHost> pp((size_t)N * nblk);
+  for (int t = 0; t < T; ++t) {
+    launch_plain_ccg(L, c->pl_tiles.as<float2>() + (size_t)t * L2, c->pl_tspec.as<float2>(), N,
+                     sgn, c->pl_ccg.as<float2>(), s);
+    launch_c2c_2d(L, c->pl_ccg.as<float2>(), N, +1, 1.0f, c->twL.as<float2>(), s);
+    launch_plain_peak(L, c->pl_ccg.as<float2>(), N, lo, hi, c->pl_part.p, s);
+    CHK(copy_out(c, pp.data(), c->pl_part.p, pp.size() * sizeof(PeakPartHost)));
+    std::vector<MapStat> st(N);
+    for (int j = 0; j < N; ++j) {  // main.cu:856-881, FP64 sums
+      float peak = 0.0f;
+      int pos = 0;
+      double s1 = 0, s2 = 0;
+      for (int b = 0; b < nblk; ++b) {
+        const PeakPartHost& q = pp[(size_t)j * nblk + b];
+        if (q.peak > peak || (q.peak == peak && peak > 0 && q.pos < pos)) { peak = q.peak; pos = q.pos; }
+        s1 += q.s1;
+        s2 += q.s2;
+      }
+      const double rc = (double)L2 - 1, ra = s1 - peak, rb = s2 - (double)peak * peak;
+      const double sd = std::sqrt(rb / rc - (ra / rc) * (ra / rc));
+      MapStat& m = st[j];
+      m.peak = peak;
+      m.pos = pos;
+      m.sd = (float)sd;
+      m.score = (sd > 0) ? (float)(peak / sd) : 0.0f;
+      m.sum = s1;
+      m.sum2 = s2;
+    }
+    HIPCHK(hipMemcpyAsync(out + (size_t)t * N, st.data(), N * sizeof(MapStat), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
   }
   return GS_OK;
 }
 
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// selection kernel lives here (host-side record layout)
-// ---------------------------------------------------------------------------
-__global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T, int N, int L,
-                         const int* __restrict__ offx, const int* __restrict__ offy, int bxn,
-                         float thres, float dmin, int nx, int ny, int* __restrict__ count,
-                         SelRec* __restrict__ out, int cap) {
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nmaps) return;
-  const MapStat s = st[m];
-  if (!(s.score >= thres)) return;
-  const long long rem = m % ((long long)T * N);
-  const int t = (int)(rem / N);
-  const float cx = (float)(offx[t % bxn] + s.pos % L);
-  const float cy = (float)(offy[t / bxn] + s.pos / L);
-  if (cx >= dmin && cx < nx - dmin && cy >= dmin && cy < ny - dmin) {
-    const int i = atomicAdd(count, 1);
-    if (i < cap) {
-      SelRec r;
-      r.m = m; r.cx = cx; r.cy = cy; r.score = s.score; r.pad = 0;
-      out[i] = r;
-    }
+// Map the pool tile index t to (micrograph, tile within it).
+void tile_owner(const Ctx* c, int t, int* mic, int* local) {
+  int lo = 0, hi = (int)c->mics.size() - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (c->mics[mid].tile0 <= t) lo = mid; else hi = mid - 1;
   }
+  *mic = lo;
+  *local = t - c->mics[lo].tile0;
 }
+
+}  // namespace
 
 extern "C" {
 
@@ -635,6 +814,7 @@ int gs_destroy(void* ctx) {
   if (!c) return GS_ERR_ARG;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
+  drop_graphs(c);
   for (int i = 0; i < Ctx::NLANE; ++i) {
     if (c->lane_stream[i]) (void)hipStreamDestroy(c->lane_stream[i]);
     if (c->ev_lane[i]) (void)hipEventDestroy(c->ev_lane[i]);
@@ -649,30 +829,61 @@ int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const 
   Ctx* c = static_cast<Ctx*>(ctx);
   if (!c || !host || n <= 0 || bx <= 0 || by <= 0 || !tctf) return GS_ERR_ARG;
   HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
   int L = c->p.window_size;
   if (L < bx || L < by) {  // adjustPaddingSize, main.cu:158-178
     L = 0;
     while (L < bx || L < by) L += 32;
   }
   CHK(set_window(c, L));
+  drop_graphs(c);
   c->N = n; c->bx = bx; c->by = by; c->tctf = *tctf;
   c->have_mic = false;  // tiles depend on the window
   return prep_templates(c, host, n, bx, by);
 }
 
-static int set_mic(Ctx* c, const float* src, bool dev, int nx, int ny, const gs_ctf* ctf) {
-  if (!c || !src || nx <= 0 || ny <= 0 || !ctf) return GS_ERR_ARG;
+static int set_mics(Ctx* c, int n, const float* const* images, const int* nx, const int* ny,
+                    const gs_ctf* ctfs, bool dev) {
+  if (!c || n <= 0 || !images || !nx || !ny || !ctfs) return GS_ERR_ARG;
   if (c->N == 0) return GS_ERR_STATE;
   if (c->p.phase_flip != 0 && c->p.phase_flip != 1) return GS_ERR_UNSUPPORTED;
+  if (c->mode == GS_MODE_PLAIN && c->p.phase_flip != 0) return GS_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
-  return prep_micrograph(c, src, dev, nx, ny, *ctf);
+  HIPCHK(hipDeviceSynchronize());  // graphs and lanes may still read the pool
+  return set_batch(c, n, images, nx, ny, ctfs, dev);
 }
 
 int gs_set_micrograph(void* ctx, const float* host, int nx, int ny, const gs_ctf* ctf) {
-  return set_mic(static_cast<Ctx*>(ctx), host, false, nx, ny, ctf);
+  return set_mics(static_cast<Ctx*>(ctx), 1, &host, &nx, &ny, ctf, false);
 }
 int gs_set_micrograph_device(void* ctx, const float* dev, int nx, int ny, const gs_ctf* ctf) {
-  return set_mic(static_cast<Ctx*>(ctx), dev, true, nx, ny, ctf);
+  return set_mics(static_cast<Ctx*>(ctx), 1, &dev, &nx, &ny, ctf, true);
+}
+int gs_set_micrographs(void* ctx, int n, const float* const* images, const int* nx, const int* ny,
+                       const gs_ctf* ctfs, int on_device) {
+  return set_mics(static_cast<Ctx*>(ctx), n, images, nx, ny, ctfs, on_device != 0);
+}
+
+int gs_set_mode(void* ctx, int mode) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || (mode != GS_MODE_FUSED && mode != GS_MODE_PLAIN)) return GS_ERR_ARG;
+  if (mode == GS_MODE_PLAIN && (c->p.norm_type != 0 || c->p.phase_flip != 0))
+    return GS_ERR_UNSUPPORTED;
+  if (mode != c->mode) {
+    c->mode = mode;
+    c->have_mic = false;  // the two modes keep different tile spectra
+  }
+  return GS_OK;
+}
+
+int gs_set_graphs(void* ctx, int enable) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return GS_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
+  c->use_graphs = enable != 0;
+  if (!c->use_graphs) drop_graphs(c);
+  return GS_OK;
 }
 
 int gs_set_batch(void* ctx, int maps_per_batch) {
@@ -681,6 +892,7 @@ int gs_set_batch(void* ctx, int maps_per_batch) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipDeviceSynchronize());
   c->batch = maps_per_batch;
+  drop_graphs(c);
   for (int i = 0; i < Ctx::NLANE; ++i) c->G[i].release();
   return GS_OK;
 }
@@ -707,7 +919,7 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   CHK(c->stats.ensure((size_t)std::max<long long>(1, npsi * per_psi) * sizeof(MapStat)));
   c->stats_psi_begin = psi_begin;
   c->stats_npsi = npsi;
-  c->timing.corr_maps = 0;
+  c->timing.corr_maps = npsi * per_psi;
   c->timing.corr_pass1_launches = 0;
   c->timing.corr_pass2_launches = 0;
   hipEvent_t e0, e1;
@@ -715,8 +927,9 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c->stream));
   for (int ip = psi_begin; ip < psi_end; ++ip) {
-    CHK(rotate_templates(c, ip));
-    CHK(correlate_psi(c, c->stats.as<MapStat>() + (size_t)(ip - psi_begin) * per_psi));
+    MapStat* out = c->stats.as<MapStat>() + (size_t)(ip - psi_begin) * per_psi;
+    if (c->mode == GS_MODE_PLAIN) CHK(plain_psi(c, ip, out));
+    else CHK(fused_psi(c, ip, out));
   }
   // device-side selection, then canonical order on the host
   const long long nmaps = npsi * per_psi;
@@ -725,11 +938,9 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   CHK(c->sel.ensure((size_t)capsel * sizeof(SelRec)));
   CHK(c->sel_count.ensure(sizeof(int)));
   HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), c->stream));
-  if (nmaps > 0)
-    hipLaunchKernelGGL(k_select, dim3((unsigned)((nmaps + 255) / 256)), dim3(256), 0, c->stream,
-                       c->stats.as<MapStat>(), nmaps, c->T, c->N, c->L, c->offx_d.as<int>(),
-                       c->offy_d.as<int>(), c->bxn, c->p.threshold, c->p.diameter, c->nx, c->ny,
-                       c->sel_count.as<int>(), c->sel.as<SelRec>(), capsel);
+  launch_select(c->stats.as<MapStat>(), nmaps, c->T, c->N, c->L, c->tdesc.as<TileDesc>(),
+                c->p.threshold, c->p.diameter, c->sel_count.as<int>(), c->sel.as<SelRec>(), capsel,
+                c->stream);
   HIPCHK(hipEventRecord(e1, c->stream));
   int count = 0;
   HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -744,7 +955,6 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   std::vector<SelRec> recs(count);
   if (count)
     CHK(copy_out(c, recs.data(), c->sel.p, (size_t)count * sizeof(SelRec)));
-  std::sort(recs.begin(), recs.end(), [](const SelRec& a, const SelRec& b) { return a.m < b.m; });
   c->picks.resize(count);
   for (int i = 0; i < count; ++i) {
     const long long m = recs[i].m;
@@ -752,7 +962,7 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
     const long long rem = m % per_psi;
     gs_particle& pk = c->picks[i];
     pk.psi_index = ip;
-    pk.tile = (int)(rem / c->N);
+    tile_owner(c, (int)(rem / c->N), &pk.mic, &pk.tile);
     pk.view = (int)(rem % c->N);
     float cur_e = 360.0f - psi_value(c->p.phi_step, ip);  // main.cu:1253-1254
     if (cur_e >= 360) cur_e -= 360;
@@ -761,6 +971,16 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
     pk.cy = recs[i].cy;
     pk.score = recs[i].score;
   }
+  // one micrograph after the other, each in the reference's psi, tile, view order
+  std::vector<long long> order(count);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](long long a, long long b) {
+    if (c->picks[a].mic != c->picks[b].mic) return c->picks[a].mic < c->picks[b].mic;
+    return recs[a].m < recs[b].m;
+  });
+  std::vector<gs_particle> sorted(count);
+  for (int i = 0; i < count; ++i) sorted[i] = c->picks[order[i]];
+  c->picks.swap(sorted);
   return gs_get_picks(ctx, out, cap, n_out);
 }
 
@@ -861,10 +1081,13 @@ int gs_get_info(void* ctx, gs_info* info) {
   info->l = c->L;
   info->n_views = c->N;
   info->n_psi = num_psi(c->p.phi_step);
-  info->nx = c->nx;
-  info->ny = c->ny;
-  info->tiles_x = c->bxn;
-  info->tiles_y = c->byn;
+  const MicSlot* ms = c->mics.empty() ? nullptr : &c->mics[0];
+  info->nx = ms ? ms->nx : 0;
+  info->ny = ms ? ms->ny : 0;
+  info->tiles_x = ms ? (int)ms->offx.size() : 0;
+  info->tiles_y = ms ? (int)ms->offy.size() : 0;
+  info->n_mics = (int)c->mics.size();
+  info->n_tiles = c->T;
   info->box_x = c->bx;
   info->box_y = c->by;
   return GS_OK;

@@ -581,9 +581,8 @@ __global__ void k_norm1_stats(const double* __restrict__ s1, const double* __res
 __global__ __launch_bounds__(256) void k_norm1_blockmax(const float* __restrict__ z, long L2,
                                                         int nblk, int L, int mbase,
                                                         const MapRef* maps,
-                                                        const int* __restrict__ offx,
-                                                        const int* __restrict__ offy, int bxn,
-                                                        float thres, float dmin, int nx, int ny,
+                                                        const TileDesc* __restrict__ tdesc,
+                                                        float thres, float dmin,
                                                         int* __restrict__ count,
                                                         N1Rec* __restrict__ out, int cap) {
   __shared__ float rv[4];
@@ -603,9 +602,9 @@ __global__ __launch_bounds__(256) void k_norm1_blockmax(const float* __restrict_
   v = rv[0]; p = rp[0];
   for (int i = 1; i < 4; ++i) vmax_combine(v, p, rv[i], rp[i]);
   if (!(v >= thres)) return;
-  const int t = maps[m].tile;
-  const int cx = offx[t % bxn] + p % L, cy = offy[t / bxn] + p / L;
-  if (cx >= dmin && cx < nx - dmin && cy >= dmin && cy < ny - dmin) {
+  const TileDesc td = tdesc[maps[m].tile];
+  const int cx = td.ox + p % L, cy = td.oy + p / L;
+  if (cx >= dmin && cx < td.nx - dmin && cy >= dmin && cy < td.ny - dmin) {
     const int i = atomicAdd(count, 1);
     if (i < cap) { N1Rec r; r.m = mbase + m; r.block = b; r.pos = p; r.z = v; out[i] = r; }
   }
@@ -1094,13 +1093,40 @@ void launch_norm1_stats(const double* s1, const double* s2, long n, double total
                      total, avg, sd);
 }
 void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, const MapRef* maps,
-                           const int* offx, const int* offy, int bxn, float thres, float dmin,
-                           int nx, int ny, int* count, void* out, int cap, hipStream_t s) {
+                           const TileDesc* tdesc, float thres, float dmin, int* count, void* out,
+                           int cap, hipStream_t s) {
   const long L2 = (long)L * L;
   const int nblk = (int)((L2 + 1023) / 1024);
   hipLaunchKernelGGL(k_norm1_blockmax, dim3(nblk, nmaps), dim3(256), 0, s, z, L2, nblk, L, mbase,
-                     maps,
-                     offx, offy, bxn, thres, dmin, nx, ny, count, static_cast<N1Rec*>(out), cap);
+                     maps, tdesc, thres, dmin, count, static_cast<N1Rec*>(out), cap);
+}
+
+__global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T, int N, int L,
+                         const TileDesc* __restrict__ tdesc, float thres, float dmin,
+                         int* __restrict__ count, SelRec* __restrict__ out, int cap) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmaps) return;
+  const MapStat s = st[m];
+  if (!(s.score >= thres)) return;
+  const int t = (int)((m % ((long long)T * N)) / N);
+  const TileDesc td = tdesc[t];
+  const float cx = (float)(td.ox + s.pos % L);
+  const float cy = (float)(td.oy + s.pos / L);
+  if (cx >= dmin && cx < td.nx - dmin && cy >= dmin && cy < td.ny - dmin) {
+    const int i = atomicAdd(count, 1);
+    if (i < cap) {
+      SelRec r;
+      r.m = m; r.cx = cx; r.cy = cy; r.score = s.score; r.pad = 0;
+      out[i] = r;
+    }
+  }
+}
+
+void launch_select(const MapStat* st, long long nmaps, int T, int N, int L, const TileDesc* tdesc,
+                   float thres, float dmin, int* count, SelRec* out, int cap, hipStream_t s) {
+  if (nmaps <= 0) return;
+  hipLaunchKernelGGL(k_select, dim3((unsigned)((nmaps + 255) / 256)), dim3(256), 0, s, st, nmaps, T,
+                     N, L, tdesc, thres, dmin, count, out, cap);
 }
 int norm1_record_bytes() { return (int)sizeof(N1Rec); }
 

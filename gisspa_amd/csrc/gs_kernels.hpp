@@ -33,6 +33,19 @@ struct MapRef {
   int view;
 };
 
+// where a tile of the (multi-micrograph) tile pool comes from
+struct TileDesc {
+  int mic;     // micrograph of the batch
+  int ox, oy;  // offset of the tile in its micrograph
+  int nx, ny;  // micrograph size
+};
+
+// one selected correlation map (norm_type 0)
+struct SelRec {
+  long long m;  // (psi - psi_begin) * T * N + tile * N + view
+  float cx, cy, score, pad;
+};
+
 // geometry shared by host and device code
 struct Geometry {
   int L, M, H, HP;
@@ -105,8 +118,11 @@ struct N1Rec {
   float z;
 };
 void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, const MapRef* maps,
-                           const int* offx, const int* offy, int bxn, float thres, float dmin,
-                           int nx, int ny, int* count, void* out, int cap, hipStream_t s);
+                           const TileDesc* tdesc, float thres, float dmin, int* count, void* out,
+                           int cap, hipStream_t s);
+// threshold + centre-bounds selection of norm_type 0 maps (main.cu:871-891)
+void launch_select(const MapStat* st, long long nmaps, int T, int N, int L, const TileDesc* tdesc,
+                   float thres, float dmin, int* count, SelRec* out, int cap, hipStream_t s);
 int norm1_record_bytes();
 
 // ---- template preparation ----
@@ -151,5 +167,25 @@ void launch_img_weight(float2* F, int nx, int ny, const double* ra, const float*
 void launch_tile_ampnorm(float2* spec, int L, int ntiles, double count, double* part,
                          float* scale, hipStream_t s);
 int tile_ampnorm_parts(int L);
+
+// ---- stage-by-stage ("plain") picker (gs_plain.hip) ----
+struct PeakPartHost {
+  float peak;
+  int pos;
+  float s1, s2;
+};
+int plain_blocks(int L);
+void launch_plain_rotate(int L, const float* treal, int n, float cose, float sine, float dmask,
+                         float2* out, double* part, hipStream_t s);
+void launch_plain_normalize(int L, float2* data, int n, const float* mean, const float* var,
+                            hipStream_t s);
+void launch_c2c_2d(int L, float2* data, int nimg, int dir, float scale, const float2* twL,
+                   hipStream_t s);
+void launch_plain_tiles(int L, const float* img, int nx, int ny, const TileDesc* td, int tile0,
+                        int ntiles, const double* mean, float2* out, hipStream_t s);
+void launch_plain_ccg(int L, const float2* tile, const float2* tspec, int n, float sgn,
+                      float2* ccg, hipStream_t s);
+void launch_plain_peak(int L, const float2* ccg, int n, float lo, float hi, void* part,
+                       hipStream_t s);
 
 }  // namespace gs

@@ -73,11 +73,20 @@ typedef struct gs_ctf {
 typedef struct gs_particle {
   int psi_index;   /* index of the in-plane step (euler3 = psi_index * phistep) */
   int view;        /* Euler-list line J */
-  int tile;        /* tile index ix + iy * bx */
+  int tile;        /* tile index ix + iy * bx within its micrograph */
   float psi;       /* printed psi = 360 - euler3, wrapped to [0, 360) */
   float cx, cy;    /* centre in micrograph pixels */
   float score;
+  int mic;         /* micrograph of the batch (0 for gs_set_micrograph) */
 } gs_particle;
+
+/* Execution modes (gs_set_mode). */
+enum {
+  GS_MODE_FUSED = 0,  /* half-spectrum correlation, fused product/transform/reduction,
+                         all micrographs of a batch in one tile pool (default) */
+  GS_MODE_PLAIN = 1   /* stage-by-stage like the reference: full complex spectra, one
+                         kernel per stage, host reduction after every tile */
+};
 
 /* Per (psi, tile, view) correlation result of norm_type 0
  * (main.cu:856-881): peak of the border-excluded CC map, its position
@@ -95,9 +104,11 @@ typedef struct gs_info {
   int l;           /* window (tile) size after adjustment */
   int n_views;
   int n_psi;
-  int nx, ny;      /* current micrograph */
+  int nx, ny;      /* first micrograph of the current batch */
   int tiles_x, tiles_y;
   int box_x, box_y;
+  int n_mics;      /* micrographs in the current batch */
+  int n_tiles;     /* tiles of all of them */
 } gs_info;
 
 /* Per-stage device time of the last calls, milliseconds (hipEvent). */
@@ -136,6 +147,22 @@ int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const 
 int gs_set_micrograph(void* ctx, const float* host, int nx, int ny, const gs_ctf* c);
 /* Same, with the image already in device memory (left unmodified). */
 int gs_set_micrograph_device(void* ctx, const float* dev, int nx, int ny, const gs_ctf* c);
+
+/* Upload and prepare a batch of micrographs of any sizes (each with its own
+ * CTF); their tiles form one pool that gs_pick correlates in the same
+ * launches.  images[i] points to ny[i] rows of nx[i] floats, in host memory
+ * (on_device = 0) or device memory (on_device = 1).  No reference
+ * counterpart: the reference processes one micrograph per loop iteration
+ * (main.cu:1149-1269). */
+int gs_set_micrographs(void* ctx, int n, const float* const* images, const int* nx, const int* ny,
+                       const gs_ctf* ctfs, int on_device);
+
+/* GS_MODE_FUSED or GS_MODE_PLAIN (norm_type 0; plain mode is the
+ * stage-by-stage baseline). */
+int gs_set_mode(void* ctx, int mode);
+/* Capture the per-psi launch sequence of the fused mode in HIP graphs
+ * (built on first use, replayed afterwards; 0 turns them off). */
+int gs_set_graphs(void* ctx, int enable);
 
 /* Run the in-plane steps [psi_begin, psi_end) of the current micrograph
  * (replaces the psi loop of main.cu:1230-1257 with pickPartcles_IMG_NORM,

@@ -1,0 +1,130 @@
+"""GPU parity of the batched, plain and graph-replay paths.
+
+  * a batch of micrographs of different sizes and CTFs (gs_set_micrographs)
+    gives, micrograph by micrograph, the oracle's per-map scores and the
+    oracle's pick list (the reference picks one micrograph per loop
+    iteration, main.cu:1149-1269);
+  * the stage-by-stage mode (GS_MODE_PLAIN) gives the oracle's scores;
+  * per-psi HIP graphs replay to the same bits as direct launches.
+
+Tolerances as in test_gpu_parity: scores within 1e-4 * max(|oracle|, 1),
+identical peak positions.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_scores(got, ref, what):
+    rel = np.abs(got["score"].astype(np.float64) - ref["score"]) / np.maximum(
+        np.abs(ref["score"]), 1.0)
+    print(what, "score max rel err", rel.max())
+    assert rel.max() < 1e-4
+    assert np.array_equal(got["pos"], ref["pos"]), what
+
+
+def _oracle_maps(op, tctf, mctf, tm, img):
+    """[psi][tile][view] scores with the templates weighted by the first
+    micrograph's CTF and each micrograph phase-flipped by its own (the
+    reference prepares the templates once, from the first micrograph's
+    header, main.cu:1104-1129, and reads each micrograph's CTF in the loop,
+    main.cu:1159-1163)."""
+    treal = orc.template_prep(op, tctf, tm)
+    tiles = orc.micrograph_prep(op, mctf, img)
+    P = orc.num_psi(op.phi_step)
+    return np.stack([orc.correlate_many(op, tiles, orc.rotate_normalize(
+        op, treal, orc.psi_value(op.phi_step, ip))) for ip in range(P)])
+
+
+def _batch_cases():
+    sizes = [(320, 288), (256, 352), (400, 200)]
+    out = []
+    for i, sz in enumerate(sizes):
+        c = _cases.tiny(seed=20 + i, l=96, size=sz, threshold=4.0)
+        out.append(c)
+    # one template set (case 0's), three micrographs with their own defocus
+    return out
+
+
+@pytest.mark.parametrize("phase_flip", [0, 1])
+def test_batch_of_micrographs(gpu, phase_flip):
+    import gisspa_amd
+    cases = _batch_cases()
+    c0 = cases[0]
+    params = dict(c0.params, phase_flip=phase_flip)
+    defs = [1.5, 1.2, 1.9]
+    op = orc.params(c0.l, **params)
+    with gisspa_amd.Picker(0, **params) as pk:
+        pk.set_templates(c0.templates, defs[0])
+        pk.set_micrographs([c.image for c in cases], defs)
+        info = pk.info()
+        picks = pk.pick()
+        st = pk.mapstats()
+    assert info.n_mics == 3
+    P, N = info.n_psi, info.n_views
+    st = st.reshape(P, info.n_tiles, N)
+    t0 = 0
+    for i, c in enumerate(cases):
+        ny, nx = c.image.shape
+        ref = _oracle_maps(op, orc.ctf(defs[0]), orc.ctf(defs[i]), c0.templates, c.image)
+        nt = ref.shape[1]
+        got = st[:, t0:t0 + nt, :]
+        t0 += nt
+        _check_scores(got, ref, f"mic {i} ({nx}x{ny})")
+        sel = orc.select_norm0(op, ref, 0, nx, ny)
+        mine = picks[picks["mic"] == i]
+        print("mic", i, "picks gpu", len(mine), "oracle", len(sel))
+        assert len(mine) == len(sel)
+        assert np.array_equal(mine["psi_index"], sel["psi_index"])
+        assert np.array_equal(mine["view"], sel["view"])
+        assert np.array_equal(mine["cx"], sel["cx"].astype(np.float32))
+        assert np.array_equal(mine["cy"], sel["cy"].astype(np.float32))
+    assert t0 == info.n_tiles
+    # one micrograph after the other
+    assert np.all(np.diff(picks["mic"]) >= 0)
+
+
+def test_plain_mode_matches_oracle(gpu):
+    import gisspa_amd
+    c = _cases.tiny(seed=5, l=128)
+    op = orc.params(c.l, **c.params)
+    ref = orc.pick_norm0(op, orc.ctf(c.defocus), c.templates, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_mode(gisspa_amd.GS_MODE_PLAIN)
+        pk.set_templates(c.templates, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        picks = pk.pick()
+        got = pk.mapstats().reshape(ref.shape)
+    _check_scores(got, ref, "plain")
+    sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
+    assert len(picks) == len(sel) and np.array_equal(picks["view"], sel["view"])
+
+
+def test_graph_replay_is_bit_exact(gpu):
+    import gisspa_amd
+    c = _cases.tiny(seed=6, l=96)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates(c.templates, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        base_picks = pk.pick()
+        base = pk.mapstats()
+        pk.set_graphs(True)
+        for rep in range(3):   # capture, then two replays
+            picks = pk.pick()
+            st = pk.mapstats()
+            assert np.array_equal(st, base), f"replay {rep}"
+            assert np.array_equal(picks, base_picks), f"replay {rep}"
+        # a new micrograph into the same buffers: replays must see it
+        c2 = _cases.tiny(seed=7, l=96)
+        pk.set_micrograph(c2.image, c2.defocus)
+        g = pk.pick()
+        gst = pk.mapstats()
+        pk.set_graphs(False)
+        d = pk.pick()
+        dst = pk.mapstats()
+    assert np.array_equal(gst, dst)
+    assert np.array_equal(g, d)
