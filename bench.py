@@ -30,6 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per CC map at window 720 from rocprofv3 FETCH_SIZE (x2.0) and
+# WRITE_SIZE (x1.0) counters, scaled by a 1 GiB copy kernel's readings
+# (profiles/corr_720_full_workload.md)
+TRAFFIC_PER_MAP_720 = {"k_corr_pass1": 4.257e6, "k_corr_pass2": 2.251e6}
 
 
 def log(*a):
@@ -52,6 +56,11 @@ class Hip:
     def h2d(self, dst: int, arr: np.ndarray) -> None:
         r = self.lib.hipMemcpy(ctypes.c_void_p(dst), arr.ctypes.data_as(ctypes.c_void_p),
                                ctypes.c_size_t(arr.nbytes), 1)
+        if r != 0:
+            raise RuntimeError(f"hipMemcpy failed: {r}")
+
+    def d2d(self, dst: int, src: int, nbytes: int) -> None:
+        r = self.lib.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(nbytes), 3)
         if r != 0:
             raise RuntimeError(f"hipMemcpy failed: {r}")
 
@@ -113,16 +122,44 @@ def cpu_baseline(templates, image, defocus, nthreads: int, n_views: int):
                        f"({nthreads} threads); extrapolated to {N} views x 72 psi")
 
 
+def plain_rate(templates, d_img, nx, ny, defocus, npsi: int):
+    """Stage-by-stage picker (GS_MODE_PLAIN) on the same micrograph: prep +
+    npsi in-plane steps, timed on the wall clock, extrapolated to 72 steps."""
+    import gisspa_amd
+    pk = gisspa_amd.Picker(0, **FULL_PARAMS)
+    pk.set_mode(gisspa_amd.GS_MODE_PLAIN)
+    pk.set_templates(templates, defocus)
+    pk.set_micrograph_device(d_img, nx, ny, defocus)   # warm-up of every kernel
+    pk.pick(0, 1)
+    t0 = time.time()
+    pk.set_micrograph_device(d_img, nx, ny, defocus)
+    t_prep = time.time() - t0
+    t0 = time.time()
+    pk.pick(0, npsi)
+    t_psi = (time.time() - t0) / npsi
+    pk.close()
+    sec = t_prep + 72 * t_psi
+    return dict(value=3600.0 / sec, unit="micrographs/hour",
+                sample=f"GS_MODE_PLAIN: micrograph prep {t_prep:.2f}s + {npsi} psi at "
+                       f"{t_psi:.2f}s each, extrapolated to 72 psi")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--psi-limit", type=int, default=0,
-                    help="exploration only: psi steps per micrograph (0 = all 72)")
-    ap.add_argument("--batch", type=int, default=0)
+                    help="exploration/profiling only: psi steps per micrograph (0 = all 72)")
+    ap.add_argument("--batch", type=int, default=0, help="CC maps per launch (0 = library default)")
+    ap.add_argument("--graphs", action="store_true", help="replay per-psi HIP graphs")
+    ap.add_argument("--threshold", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-psi", type=int, default=1)
     ap.add_argument("--cpu-views", type=int, default=8)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--cache", default=os.environ.get("TMPDIR", "/tmp"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -131,15 +168,26 @@ def main():
         sys.exit(2)
 
     import gisspa_amd
+    FULL_PARAMS["threshold"] = args.threshold
     templates, image, defocus = full_workload(args.cache)
     ny, nx = image.shape
     hip = Hip()
     d_img = hip.malloc(image.nbytes)
     hip.h2d(d_img, np.ascontiguousarray(image, dtype=np.float32))
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        a, b = hip.malloc(nb), hip.malloc(nb)
+        hip.d2d(b, a, nb)
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+        log(f"[bench] calibration copy of {nb} bytes done")
 
     pk = gisspa_amd.Picker(0, **FULL_PARAMS)
     if args.batch:
         pk.set_batch(args.batch)
+    pk.set_graphs(args.graphs)
+    pk.set_profiling(True)
     t = time.time()
     pk.set_templates(templates, defocus)
     log(f"[bench] template prep {time.time() - t:.2f}s")
@@ -155,31 +203,46 @@ def main():
         log(f"[bench] warmup {i}: {time.time() - t:.2f}s, {len(picks)} picks")
     hip.sync()
     t0 = time.time()
+    p1 = p2 = 0.0
+    n1 = n2 = nmaps = 0
     for i in range(args.steps):
         picks = step()
-        log(f"[bench] step {i}: {time.time() - t0:.2f}s cumulative, {len(picks)} picks")
+        tm = pk.timing()
+        p1 += tm.corr_pass1_ms
+        p2 += tm.corr_pass2_ms
+        n1 += tm.corr_pass1_launches
+        n2 += tm.corr_pass2_launches
+        nmaps += tm.corr_sampled_maps
+        log(f"[bench] step {i}: {time.time() - t0:.2f}s cumulative, {len(picks)} picks, "
+            f"prep {tm.micrograph_prep_ms:.1f} ms, pick {tm.pick_total_ms:.1f} ms")
     hip.sync()
     dt = time.time() - t0
     sec_per_step = dt / args.steps
     sec_per_mic = sec_per_step * (72 / npsi)
     mph = 3600.0 / sec_per_mic
     info = pk.info()
-    T = info.tiles_x * info.tiles_y
+    T = info.n_tiles
     N = info.n_views
-
-    # roofline of the dominant correlation kernel, timed with hipEvents on the
-    # library's stream
-    p1, p2, mpl = pk.time_kernels(0, reps=5)
     L = info.l
     H = L // 2 + 1
-    bytes_p1 = 16.0 * L * H * mpl   # template spectrum read + intermediate write
-    bytes_p2 = 8.0 * L * H * mpl    # intermediate read
-    ach1 = bytes_p1 / (p1 * 1e-3) / 1e9
-    ach2 = bytes_p2 / (p2 * 1e-3) / 1e9
-    dom = ("corr_pass1", p1, bytes_p1, ach1) if p1 >= p2 else ("corr_pass2", p2, bytes_p2, ach2)
-    log(f"[bench] pass1 {p1:.3f} ms ({ach1:.0f} GB/s alg), pass2 {p2:.3f} ms ({ach2:.0f} GB/s alg), "
-        f"{mpl} maps/launch")
     pk.close()
+
+    # roofline of the correlation kernels: every launch of the first psi of
+    # each timed step bracketed by hipEvents on its own lane stream
+    mpl = nmaps / max(n1, 1)                     # CC maps per launch
+    t1, t2 = p1 / max(n1, 1), p2 / max(n2, 1)    # ms per launch
+    bytes_p1 = 16.0 * L * H * mpl   # template half spectrum read + intermediate write
+    bytes_p2 = 8.0 * L * H * mpl    # intermediate read
+    ach1 = bytes_p1 / (t1 * 1e-3) / 1e9
+    ach2 = bytes_p2 / (t2 * 1e-3) / 1e9
+    dom = ("k_corr_pass1", t1, bytes_p1, ach1) if t1 >= t2 else ("k_corr_pass2", t2, bytes_p2, ach2)
+    log(f"[bench] pass1 {t1:.4f} ms/launch ({ach1:.0f} GB/s alg), pass2 {t2:.4f} ms/launch "
+        f"({ach2:.0f} GB/s alg), {mpl:.1f} maps/launch, {n1} launches sampled")
+
+    plain = None
+    if not args.no_plain:
+        plain = plain_rate(templates, d_img, nx, ny, defocus, args.plain_psi)
+        log(f"[bench] plain: {plain['sample']}: {plain['value']:.2f} micrographs/hour")
     hip.free(d_img)
 
     cpu = None
@@ -201,13 +264,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain full-size: 5760x4092 micrograph, 1649 views x 72 psi, "
+        "config": {"workload": "full-size: one 5760x4092 micrograph, 1649 views x 72 psi, "
                                "180px templates, window 720 overlap 180 (88 tiles)",
-                   "views": N, "psi": 72, "tiles": T, "window": L,
-                   "psi_timed": npsi, "maps_per_micrograph": N * 72 * T},
+                   "views": N, "psi": 72, "tiles": T, "window": L, "psi_timed": npsi,
+                   "maps_per_micrograph": N * 72 * T, "threshold": args.threshold,
+                   "mode": "fused" + ("+graphs" if args.graphs else "")},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[3], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom[3] / HBM_PEAK_GBS, "traffic": None,
-                     "launch_ms": dom[1], "maps_per_launch": mpl},
+                     "unit": "GB/s", "frac": dom[3] / HBM_PEAK_GBS,
+                     "traffic": (TRAFFIC_PER_MAP_720[dom[0]] * mpl) if L == 720 else None,
+                     "launch_ms": dom[1], "maps_per_launch": mpl,
+                     "other": {"kernel": "k_corr_pass2" if dom[0] == "k_corr_pass1" else
+                               "k_corr_pass1",
+                               "achieved": ach2 if dom[0] == "k_corr_pass1" else ach1,
+                               "launch_ms": t2 if dom[0] == "k_corr_pass1" else t1}},
+        "plain": plain,
+        "speedup_vs_plain": (mph / plain["value"]) if plain else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

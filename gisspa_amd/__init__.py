@@ -48,10 +48,9 @@ class GsInfo(ctypes.Structure):
 
 class GsTiming(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("template_prep_ms", "micrograph_prep_ms", "rotate_ms", "corr_pass1_ms",
-                 "corr_pass2_ms", "finalize_ms", "pick_total_ms")] + \
+                ("pick_total_ms", "micrograph_prep_ms", "corr_pass1_ms", "corr_pass2_ms")] + \
                [(n, ctypes.c_longlong) for n in
-                ("corr_maps", "corr_pass1_launches", "corr_pass2_launches")]
+                ("corr_pass1_launches", "corr_pass2_launches", "corr_sampled_maps", "corr_maps")]
 
 
 MAPSTAT_DTYPE = np.dtype([("score", "<f4"), ("pos", "<i4"), ("peak", "<f4"), ("sd", "<f4"),
@@ -108,6 +107,7 @@ def load() -> ctypes.CDLL:
     lib.gs_get_info.argtypes = [vp, ctypes.POINTER(GsInfo)]
     lib.gs_get_timing.argtypes = [vp, ctypes.POINTER(GsTiming)]
     lib.gs_set_batch.argtypes = [vp, ctypes.c_int]
+    lib.gs_set_profiling.argtypes = [vp, ctypes.c_int]
     lib.gs_time_kernels.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
     lib.gs_destroy.argtypes = [vp]
     _lib = lib
@@ -245,6 +245,9 @@ class Picker:
         t = GsTiming()
         _check(self.lib.gs_get_timing(self.ctx, ctypes.byref(t)), "gs_get_timing")
         return t
+
+    def set_profiling(self, enable: bool) -> None:
+        _check(self.lib.gs_set_profiling(self.ctx, int(bool(enable))), "gs_set_profiling")
 
     def set_batch(self, n: int) -> None:
         _check(self.lib.gs_set_batch(self.ctx, n), "gs_set_batch")

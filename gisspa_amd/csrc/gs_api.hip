@@ -105,6 +105,10 @@ struct Ctx {
   // per-psi graphs of the fused correlation (gs_set_graphs)
   std::vector<hipGraphExec_t> graphs;
   std::vector<const void*> graph_key;  // buffers the graphs were captured with
+  // launch sampling (gs_set_profiling): three events per batch of one psi
+  bool profiling = false, profile_now = false;
+  std::vector<hipEvent_t> prof_ev;
+  int prof_batches = 0;
   std::vector<gs_particle> picks;  // of the last gs_pick
   int stats_psi_begin = 0, stats_npsi = 0;
   gs_timing timing{};
@@ -510,17 +514,29 @@ int correlate_psi(Ctx* c, MapStat* out) {
   // the lanes start after the rotated templates of this psi are ready
   HIPCHK(hipEventRecord(c->ev_ready, c->stream));
   for (int i = 0; i < Ctx::NLANE; ++i) HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
-  int lane = 0;
-  for (int m0 = 0; m0 < nm; m0 += nb, lane = (lane + 1) % Ctx::NLANE) {
+  int lane = 0, ib = 0;
+  if (c->profile_now) {
+    const size_t need = 3 * (size_t)((nm + nb - 1) / nb);
+    while (c->prof_ev.size() < need) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->prof_ev.push_back(e);
+    }
+  }
+  for (int m0 = 0; m0 < nm; m0 += nb, lane = (lane + 1) % Ctx::NLANE, ++ib) {
     const int cnt = std::min(nb, nm - m0);
     hipStream_t ls = c->lane_stream[lane];
+    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * ib], ls));
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>() + m0,
                       cnt, sgn, c->G[lane].as<float2>(),
                       c->psum.as<float2>() + (size_t)m0 * corr_ncb(L), c->twL.as<float2>(), ls);
+    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * ib + 1], ls));
     launch_corr_pass2(L, c->G[lane].as<float2>(), cnt, lo, hi,
                       c->part.as<CorrPartial>() + (size_t)m0 * corr_nparts(L),
                       c->twL.as<float2>(), c->twM.as<float2>(), ls);
+    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * ib + 2], ls));
   }
+  if (c->profile_now) c->prof_batches = ib;
   // the context stream (finalisation, next psi's rotation, selection) waits
   // for both lanes
   for (int i = 0; i < Ctx::NLANE; ++i) {
@@ -536,7 +552,7 @@ int correlate_psi(Ctx* c, MapStat* out) {
 // captured graph when graphs are on.  The graph of a psi is keyed on every
 // buffer it touches; any reallocation drops all graphs.
 int fused_psi(Ctx* c, int ip, MapStat* out) {
-  if (!c->use_graphs) {
+  if (!c->use_graphs || c->profile_now) {
     CHK(rotate_templates(c, ip));
     return correlate_psi(c, out);
   }
@@ -815,6 +831,7 @@ int gs_destroy(void* ctx) {
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   drop_graphs(c);
+  for (auto e : c->prof_ev) (void)hipEventDestroy(e);
   for (int i = 0; i < Ctx::NLANE; ++i) {
     if (c->lane_stream[i]) (void)hipStreamDestroy(c->lane_stream[i]);
     if (c->ev_lane[i]) (void)hipEventDestroy(c->ev_lane[i]);
@@ -850,7 +867,19 @@ static int set_mics(Ctx* c, int n, const float* const* images, const int* nx, co
   if (c->mode == GS_MODE_PLAIN && c->p.phase_flip != 0) return GS_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipDeviceSynchronize());  // graphs and lanes may still read the pool
-  return set_batch(c, n, images, nx, ny, ctfs, dev);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  const int r = set_batch(c, n, images, nx, ny, ctfs, dev);
+  float ms = 0;
+  if (r == GS_OK && hipEventRecord(e1, c->stream) == hipSuccess &&
+      hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+    c->timing.micrograph_prep_ms = ms;
+  (void)hipGetLastError();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return r;
 }
 
 int gs_set_micrograph(void* ctx, const float* host, int nx, int ny, const gs_ctf* ctf) {
@@ -873,6 +902,13 @@ int gs_set_mode(void* ctx, int mode) {
     c->mode = mode;
     c->have_mic = false;  // the two modes keep different tile spectra
   }
+  return GS_OK;
+}
+
+int gs_set_profiling(void* ctx, int enable) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return GS_ERR_ARG;
+  c->profiling = enable != 0;
   return GS_OK;
 }
 
@@ -920,16 +956,20 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   c->stats_psi_begin = psi_begin;
   c->stats_npsi = npsi;
   c->timing.corr_maps = npsi * per_psi;
-  c->timing.corr_pass1_launches = 0;
-  c->timing.corr_pass2_launches = 0;
+  c->timing.corr_pass1_ms = c->timing.corr_pass2_ms = 0;
+  c->timing.corr_pass1_launches = c->timing.corr_pass2_launches = 0;
+  c->timing.corr_sampled_maps = 0;
+  c->prof_batches = 0;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c->stream));
   for (int ip = psi_begin; ip < psi_end; ++ip) {
     MapStat* out = c->stats.as<MapStat>() + (size_t)(ip - psi_begin) * per_psi;
-    if (c->mode == GS_MODE_PLAIN) CHK(plain_psi(c, ip, out));
-    else CHK(fused_psi(c, ip, out));
+    c->profile_now = c->profiling && ip == psi_begin && c->mode == GS_MODE_FUSED;
+    const int r = (c->mode == GS_MODE_PLAIN) ? plain_psi(c, ip, out) : fused_psi(c, ip, out);
+    c->profile_now = false;
+    CHK(r);
   }
   // device-side selection, then canonical order on the host
   const long long nmaps = npsi * per_psi;
@@ -951,6 +991,15 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   c->timing.pick_total_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  for (int ib = 0; ib < c->prof_batches; ++ib) {
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, c->prof_ev[3 * ib], c->prof_ev[3 * ib + 1]));
+    HIPCHK(hipEventElapsedTime(&b, c->prof_ev[3 * ib + 1], c->prof_ev[3 * ib + 2]));
+    c->timing.corr_pass1_ms += a;
+    c->timing.corr_pass2_ms += b;
+  }
+  c->timing.corr_pass1_launches = c->timing.corr_pass2_launches = c->prof_batches;
+  if (c->prof_batches) c->timing.corr_sampled_maps = per_psi;
   if (count > capsel) return GS_ERR_DEVICE;  // cannot happen: one record per map at most
   std::vector<SelRec> recs(count);
   if (count)

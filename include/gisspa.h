@@ -111,18 +111,18 @@ typedef struct gs_info {
   int n_tiles;     /* tiles of all of them */
 } gs_info;
 
-/* Per-stage device time of the last calls, milliseconds (hipEvent). */
+/* Device time of the last calls, milliseconds (hipEvent). */
 typedef struct gs_timing {
-  double template_prep_ms;
-  double micrograph_prep_ms;
-  double rotate_ms;          /* per-psi template rotation + FFT, summed */
-  double corr_pass1_ms;      /* product + column inverse FFT, summed */
-  double corr_pass2_ms;      /* row C2R + reduction, summed */
-  double finalize_ms;
-  double pick_total_ms;      /* whole gs_pick, device time */
-  long long corr_maps;       /* CC maps evaluated by the last gs_pick */
+  double pick_total_ms;        /* whole last gs_pick */
+  double micrograph_prep_ms;   /* last gs_set_micrograph(s): upload, prep, tile FFTs */
+  /* With gs_set_profiling on, every correlation launch of the first psi of
+   * a gs_pick is bracketed by events on the stream it runs on: */
+  double corr_pass1_ms;        /* summed duration of the sampled pass-1 launches */
+  double corr_pass2_ms;        /* summed duration of the sampled pass-2 launches */
   long long corr_pass1_launches;
   long long corr_pass2_launches;
+  long long corr_sampled_maps; /* CC maps those launches covered (each pass) */
+  long long corr_maps;         /* CC maps evaluated by the last gs_pick */
 } gs_timing;
 
 /* Fill p with the reference's defaults (DataReader.h:16-42); required keys
@@ -195,6 +195,8 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
 
 int gs_get_info(void* ctx, gs_info* info);
 int gs_get_timing(void* ctx, gs_timing* t);
+/* Sample per-launch correlation times in gs_pick (see gs_timing). */
+int gs_set_profiling(void* ctx, int enable);
 /* Number of CC-map evaluations one gs_pick launch batch holds (tuning). */
 int gs_set_batch(void* ctx, int maps_per_batch);
 int gs_destroy(void* ctx);
