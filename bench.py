@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per CC map at window 720 from rocprofv3 FETCH_SIZE (x2.0) and
 # WRITE_SIZE (x1.0) counters, scaled by a 1 GiB copy kernel's readings
 # (profiles/corr_720_full_workload.md)
-TRAFFIC_PER_MAP_720 = {"k_corr_pass1": 4.257e6, "k_corr_pass2": 2.251e6}
+TRAFFIC_PER_MAP_720 = None
 
 
 def log(*a):
@@ -153,6 +153,10 @@ def main():
                     help="exploration/profiling only: psi steps per micrograph (0 = all 72)")
     ap.add_argument("--batch", type=int, default=0, help="CC maps per launch (0 = library default)")
     ap.add_argument("--graphs", action="store_true", help="replay per-psi HIP graphs")
+    ap.add_argument("--tile-block", type=int, default=0, help="tiles per block (0 = default)")
+    ap.add_argument("--pipeline", type=int, default=0, help="0: one launch per psi, 1: per batch")
+    ap.add_argument("--lag", type=int, default=0)
+    ap.add_argument("--ring", type=int, default=0)
     ap.add_argument("--threshold", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -186,6 +190,9 @@ def main():
     pk = gisspa_amd.Picker(0, **FULL_PARAMS)
     if args.batch:
         pk.set_batch(args.batch)
+    if args.tile_block:
+        pk.set_tile_block(args.tile_block)
+    pk.set_pipeline(args.pipeline, args.lag, args.ring)
     pk.set_graphs(args.graphs)
     pk.set_profiling(True)
     t = time.time()
@@ -203,15 +210,13 @@ def main():
         log(f"[bench] warmup {i}: {time.time() - t:.2f}s, {len(picks)} picks")
     hip.sync()
     t0 = time.time()
-    p1 = p2 = 0.0
-    n1 = n2 = nmaps = 0
+    tms = 0.0
+    nl = nmaps = 0
     for i in range(args.steps):
         picks = step()
         tm = pk.timing()
-        p1 += tm.corr_pass1_ms
-        p2 += tm.corr_pass2_ms
-        n1 += tm.corr_pass1_launches
-        n2 += tm.corr_pass2_launches
+        tms += tm.corr_ms
+        nl += tm.corr_launches
         nmaps += tm.corr_sampled_maps
         log(f"[bench] step {i}: {time.time() - t0:.2f}s cumulative, {len(picks)} picks, "
             f"prep {tm.micrograph_prep_ms:.1f} ms, pick {tm.pick_total_ms:.1f} ms")
@@ -227,17 +232,17 @@ def main():
     H = L // 2 + 1
     pk.close()
 
-    # roofline of the correlation kernels: every launch of the first psi of
-    # each timed step bracketed by hipEvents on its own lane stream
-    mpl = nmaps / max(n1, 1)                     # CC maps per launch
-    t1, t2 = p1 / max(n1, 1), p2 / max(n2, 1)    # ms per launch
-    bytes_p1 = 16.0 * L * H * mpl   # template half spectrum read + intermediate write
-    bytes_p2 = 8.0 * L * H * mpl    # intermediate read
-    ach1 = bytes_p1 / (t1 * 1e-3) / 1e9
-    ach2 = bytes_p2 / (t2 * 1e-3) / 1e9
-    dom = ("k_corr_pass1", t1, bytes_p1, ach1) if t1 >= t2 else ("k_corr_pass2", t2, bytes_p2, ach2)
-    log(f"[bench] pass1 {t1:.4f} ms/launch ({ach1:.0f} GB/s alg), pass2 {t2:.4f} ms/launch "
-        f"({ach2:.0f} GB/s alg), {mpl:.1f} maps/launch, {n1} launches sampled")
+    # roofline of the correlation kernel (k_corr_fused: pass 1 of one batch +
+    # pass 2 of the previous one per launch): every launch of the first psi
+    # of each timed step bracketed by hipEvents on the library's stream.
+    # Algorithmic bytes per CC map: 24 L H (template half spectrum read 8 L H,
+    # intermediate written and read back 16 L H).
+    t_launch = tms / max(nl, 1)                # ms per launch
+    mpl = nmaps / max(nl, 1)                   # CC maps (both passes) per launch
+    bytes_launch = 24.0 * L * H * mpl
+    ach = bytes_launch / (t_launch * 1e-3) / 1e9
+    log(f"[bench] k_corr_fused {t_launch:.4f} ms/launch, {mpl:.1f} maps/launch, "
+        f"{ach:.0f} GB/s algorithmic, {nl} launches sampled")
 
     plain = None
     if not args.no_plain:
@@ -269,14 +274,12 @@ def main():
                    "views": N, "psi": 72, "tiles": T, "window": L, "psi_timed": npsi,
                    "maps_per_micrograph": N * 72 * T, "threshold": args.threshold,
                    "mode": "fused" + ("+graphs" if args.graphs else "")},
-        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[3], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom[3] / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_PER_MAP_720[dom[0]] * mpl) if L == 720 else None,
-                     "launch_ms": dom[1], "maps_per_launch": mpl,
-                     "other": {"kernel": "k_corr_pass2" if dom[0] == "k_corr_pass1" else
-                               "k_corr_pass1",
-                               "achieved": ach2 if dom[0] == "k_corr_pass1" else ach1,
-                               "launch_ms": t2 if dom[0] == "k_corr_pass1" else t1}},
+        "roofline": {"bound": "hbm", "kernel": "k_corr_fused", "achieved": ach,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": (TRAFFIC_PER_MAP_720 * mpl) if (L == 720 and TRAFFIC_PER_MAP_720)
+                     else None,
+                     "launch_ms": t_launch, "maps_per_launch": mpl,
+                     "bytes_per_map": 24.0 * L * H},
         "plain": plain,
         "speedup_vs_plain": (mph / plain["value"]) if plain else None,
         "cpu_baseline": cpu,

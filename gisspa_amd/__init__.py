@@ -47,10 +47,8 @@ class GsInfo(ctypes.Structure):
 
 
 class GsTiming(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_double) for n in
-                ("pick_total_ms", "micrograph_prep_ms", "corr_pass1_ms", "corr_pass2_ms")] + \
-               [(n, ctypes.c_longlong) for n in
-                ("corr_pass1_launches", "corr_pass2_launches", "corr_sampled_maps", "corr_maps")]
+    _fields_ = [(n, ctypes.c_double) for n in ("pick_total_ms", "micrograph_prep_ms", "corr_ms")] + \
+               [(n, ctypes.c_longlong) for n in ("corr_launches", "corr_sampled_maps", "corr_maps")]
 
 
 MAPSTAT_DTYPE = np.dtype([("score", "<f4"), ("pos", "<i4"), ("peak", "<f4"), ("sd", "<f4"),
@@ -61,6 +59,8 @@ PARTICLE_DTYPE = np.dtype([("psi_index", "<i4"), ("view", "<i4"), ("tile", "<i4"
 
 GS_MODE_FUSED = 0
 GS_MODE_PLAIN = 1
+GS_PIPE_STREAM = 0
+GS_PIPE_LAUNCH = 1
 
 _lib = None
 
@@ -108,6 +108,8 @@ def load() -> ctypes.CDLL:
     lib.gs_get_timing.argtypes = [vp, ctypes.POINTER(GsTiming)]
     lib.gs_set_batch.argtypes = [vp, ctypes.c_int]
     lib.gs_set_profiling.argtypes = [vp, ctypes.c_int]
+    lib.gs_set_tile_block.argtypes = [vp, ctypes.c_int]
+    lib.gs_set_pipeline.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.gs_time_kernels.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
     lib.gs_destroy.argtypes = [vp]
     _lib = lib
@@ -248,6 +250,12 @@ class Picker:
 
     def set_profiling(self, enable: bool) -> None:
         _check(self.lib.gs_set_profiling(self.ctx, int(bool(enable))), "gs_set_profiling")
+
+    def set_pipeline(self, mode: int, lag: int = 0, ring: int = 0) -> None:
+        _check(self.lib.gs_set_pipeline(self.ctx, mode, lag, ring), "gs_set_pipeline")
+
+    def set_tile_block(self, n: int) -> None:
+        _check(self.lib.gs_set_tile_block(self.ctx, n), "gs_set_tile_block")
 
     def set_batch(self, n: int) -> None:
         _check(self.lib.gs_set_batch(self.ctx, n), "gs_set_batch")

@@ -88,17 +88,21 @@ struct Ctx {
   DevBuf img, img_part, img_stats, img_mean, offx_d, offy_d, zero_mean;
   DevBuf cimg, ctmp, tile_part, tile_scale;
   std::vector<std::pair<long long, std::unique_ptr<FlipCache>>> flips;
-  // correlation: two lanes (stream + intermediate) so that pass 1 of one
-  // batch overlaps pass 2 of the previous one
+  // correlation: two intermediates, so that one launch runs pass 1 of a
+  // batch into one while pass 2 of the previous batch reads the other
   static constexpr int NLANE = 2;
   int batch = 0;
-  hipStream_t lane_stream[NLANE] = {nullptr, nullptr};
-  hipEvent_t ev_ready = nullptr, ev_lane[NLANE] = {nullptr, nullptr};
   DevBuf G[NLANE];
+  // GS_PIPE_STREAM: one launch per psi, pass 2 of map m - lag beside pass 1
+  // of map m, intermediates in a ring of `ring` maps
+  int pipeline = GS_PIPE_STREAM, lag = 16, ring = 48;
+  DevBuf Gring, counters, err;
   DevBuf part;  // row-block partials of every map of one psi
   DevBuf psum;  // column-block Parseval sums of every map of one psi
   DevBuf maps, stats, sel, sel_count;
-  int maps_T = -1, maps_N = -1;
+  DevBuf maps_blk;
+  int tile_block = 1 << 30;  // one block: view-major (each template read once per psi)
+  int maps_T = -1, maps_N = -1, maps_tb = -1;
   DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_z, n1_rec;  // norm_type 1
   // plain mode buffers
   DevBuf pl_tspec, pl_tiles, pl_ccg, pl_part, pl_rotpart, pl_mean, pl_var;
@@ -390,6 +394,8 @@ int prep_micrograph(Ctx* c, int mi, const float* src, bool src_is_device) {
     launch_tile_ampnorm(pool, L, ms.ntiles, c->half_count, c->tile_part.as<double>(),
                         c->tile_scale.as<float>(), s);
   }
+  // the correlation's (-1)^(kx+ky) factor rides on the tile spectra
+  launch_checker_sign(L, pool, ms.ntiles, s);
   if (c->mode == GS_MODE_PLAIN) {
     // plain mode keeps full complex row-major tile spectra (fft_on_img)
     const long L2 = (long)L * L;
@@ -481,70 +487,103 @@ int ensure_corr_buffers(Ctx* c) {
   if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (128l << 20) / per_map));
   const int nb = std::min(c->batch, c->T * c->N);
   const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
-  for (int i = 0; i < Ctx::NLANE; ++i) CHK(c->G[i].ensure((size_t)nb * per_map));
+  // G[0] serves norm_type 1 and the kernel timing in every schedule; G[1]
+  // only the launch-per-batch schedule
+  CHK(c->G[0].ensure((size_t)nb * per_map));
+  if (c->pipeline == GS_PIPE_STREAM) {
+    CHK(c->Gring.ensure((size_t)c->ring * per_map));
+    CHK(c->counters.ensure((size_t)2 * c->T * c->N * sizeof(int)));
+  } else {
+    CHK(c->G[1].ensure((size_t)nb * per_map));
+  }
+  CHK(c->err.ensure(sizeof(int)));
   CHK(c->part.ensure((size_t)c->T * c->N * nrb * sizeof(CorrPartial)));
   CHK(c->psum.ensure((size_t)c->T * c->N * corr_ncb(L) * sizeof(float2)));
-  // processing order: view-major, so consecutive maps share one template
-  // spectrum (kept in cache across all tiles) while the tile spectra stay
-  // resident in the last-level cache; results are stored in the canonical
-  // [tile][view] order by the finalisation
+  // Processing orders; results land in the canonical [tile][view] order
+  // through the MapRef of each map.
+  //  * maps (norm_type 1): view-major, so a batch of at most T maps never
+  //    holds one tile twice;
+  //  * maps_blk (norm_type 0): blocks of tile_block tiles, all views of a
+  //    block before the next block, tiles fastest.  The default single block
+  //    is view-major: the template spectra (N half spectra per psi, far
+  //    beyond any cache) are read once per psi and stay in L2 across the
+  //    tiles, while the tile spectra (T half spectra) are re-read from the
+  //    last-level cache.  Smaller blocks trade that for tile reuse.
   const size_t nm = (size_t)c->T * c->N;
-  if (c->maps_T != c->T || c->maps_N != c->N) {
-    std::vector<MapRef> mr(nm);
+  if (c->maps_T != c->T || c->maps_N != c->N || c->maps_tb != c->tile_block) {
+    std::vector<MapRef> mr(nm), mb;
+    mb.reserve(nm);
     for (int j = 0; j < c->N; ++j)
       for (int t = 0; t < c->T; ++t) mr[(size_t)j * c->T + t] = MapRef{t, j};
+    const int tb = std::max(1, c->tile_block);
+    for (int t0 = 0; t0 < c->T; t0 += tb)
+      for (int j = 0; j < c->N; ++j)
+        for (int t = t0; t < std::min(c->T, t0 + tb); ++t) mb.push_back(MapRef{t, j});
     CHK(c->maps.ensure(nm * sizeof(MapRef)));
+    CHK(c->maps_blk.ensure(nm * sizeof(MapRef)));
     HIPCHK(hipMemcpyAsync(c->maps.p, mr.data(), nm * sizeof(MapRef), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));  // mr is released on return
+    HIPCHK(hipMemcpyAsync(c->maps_blk.p, mb.data(), nm * sizeof(MapRef), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));  // mr, mb are released on return
     c->maps_T = c->T;
     c->maps_N = c->N;
+    c->maps_tb = c->tile_block;
   }
   return GS_OK;
 }
 
 // Correlate all maps of one psi (the templates of this psi are in tspec):
-// batches alternate between the two lanes; stats[tile][view] <- scores.
+// launch k runs pass 1 of batch k and pass 2 of batch k - 1 on the context
+// stream (the stream order completes each intermediate before it is read);
+// stats[tile][view] <- scores.
 int correlate_psi(Ctx* c, MapStat* out) {
+  if (!c->psum.p || !c->part.p || !c->maps_blk.p || !c->err.p) return GS_ERR_STATE;
+  if (c->pipeline == GS_PIPE_STREAM ? (!c->Gring.p || !c->counters.p) : (!c->G[0].p || !c->G[1].p))
+    return GS_ERR_STATE;
   const int L = c->L;
   const int nm = c->T * c->N;
   const int nb = std::min(c->batch, nm);
-  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;  // compute_corner_CCG `of` rule
+  const int nbatch = (nm + nb - 1) / nb;
   const float lo = c->p.diameter / 4.0f;
   const float hi = (float)L - c->p.diameter / 4.0f;
-  // the lanes start after the rotated templates of this psi are ready
-  HIPCHK(hipEventRecord(c->ev_ready, c->stream));
-  for (int i = 0; i < Ctx::NLANE; ++i) HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
-  int lane = 0, ib = 0;
-  if (c->profile_now) {
-    const size_t need = 3 * (size_t)((nm + nb - 1) / nb);
-    while (c->prof_ev.size() < need) {
-      hipEvent_t e;
-      HIPCHK(hipEventCreate(&e));
-      c->prof_ev.push_back(e);
+  const int ncb = corr_ncb(L), np = corr_nparts(L);
+  hipStream_t s = c->stream;
+  if (c->pipeline == GS_PIPE_STREAM) {
+    if (c->profile_now) {
+      while (c->prof_ev.size() < 2) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->prof_ev.push_back(e);
+      }
+      HIPCHK(hipEventRecord(c->prof_ev[0], s));
     }
+    launch_corr_stream(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps_blk.as<MapRef>(),
+                       nm, c->lag, c->ring, c->Gring.as<float2>(), c->psum.as<float2>(),
+                       c->counters.as<int>(), lo, hi, c->part.as<CorrPartial>(),
+                       c->err.as<int>(), c->twL.as<float2>(), c->twM.as<float2>(), s);
+    if (c->profile_now) {
+      HIPCHK(hipEventRecord(c->prof_ev[1], s));
+      c->prof_batches = 1;
+    }
+    launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
+                         c->maps_blk.as<MapRef>(), c->N, out, s);
+    return GS_OK;
   }
-  for (int m0 = 0; m0 < nm; m0 += nb, lane = (lane + 1) % Ctx::NLANE, ++ib) {
-    const int cnt = std::min(nb, nm - m0);
-    hipStream_t ls = c->lane_stream[lane];
-    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * ib], ls));
-    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>() + m0,
-                      cnt, sgn, c->G[lane].as<float2>(),
-                      c->psum.as<float2>() + (size_t)m0 * corr_ncb(L), c->twL.as<float2>(), ls);
-    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * ib + 1], ls));
-    launch_corr_pass2(L, c->G[lane].as<float2>(), cnt, lo, hi,
-                      c->part.as<CorrPartial>() + (size_t)m0 * corr_nparts(L),
-                      c->twL.as<float2>(), c->twM.as<float2>(), ls);
-    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * ib + 2], ls));
+  for (int k = 0; k <= nbatch; ++k) {
+    const int m1 = k * nb, cnt1 = k < nbatch ? std::min(nb, nm - m1) : 0;
+    const int m2 = (k - 1) * nb, cnt2 = k >= 1 ? std::min(nb, nm - m2) : 0;
+    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[2 * k], s));
+    launch_corr_fused(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
+                      c->maps_blk.as<MapRef>() + (cnt1 ? m1 : 0), cnt1, c->G[k % 2].as<float2>(),
+                      c->psum.as<float2>() + (size_t)(cnt1 ? m1 : 0) * ncb,
+                      c->G[(k + 1) % 2].as<float2>(), cnt2, lo, hi,
+                      c->part.as<CorrPartial>() + (size_t)(cnt2 ? m2 : 0) * np,
+                      c->twL.as<float2>(), c->twM.as<float2>(), s);
+    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[2 * k + 1], s));
   }
-  if (c->profile_now) c->prof_batches = ib;
-  // the context stream (finalisation, next psi's rotation, selection) waits
-  // for both lanes
-  for (int i = 0; i < Ctx::NLANE; ++i) {
-    HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
-    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_lane[i], 0));
-  }
+  if (c->profile_now) c->prof_batches = nbatch + 1;
   launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
-                       c->maps.as<MapRef>(), c->N, out, c->stream);
+                       c->maps_blk.as<MapRef>(), c->N, out, s);
   return GS_OK;
 }
 
@@ -557,7 +596,10 @@ int fused_psi(Ctx* c, int ip, MapStat* out) {
     return correlate_psi(c, out);
   }
   const std::vector<const void*> key = {c->tiles.p, c->tspec.p, c->treal.p, c->G[0].p, c->G[1].p,
-                                        c->part.p, c->psum.p, c->maps.p, c->rot_part.p,
+                                        c->Gring.p, c->counters.p, c->err.p,
+                                        (const void*)(long)c->pipeline, (const void*)(long)c->lag,
+                                        (const void*)(long)c->ring,
+                                        c->part.p, c->psum.p, c->maps_blk.p, c->rot_part.p,
                                         c->bscale.p, out, (const void*)(long)c->T,
                                         (const void*)(long)c->batch};
   const int P = num_psi(c->p.phi_step);
@@ -599,7 +641,6 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   const int P = num_psi(c->p.phi_step);
   const int nm = c->T * c->N;
   const int nb = std::min({c->batch, nm, c->T});
-  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;
   const size_t npix = (size_t)c->T * L2;
   CHK(c->n1_s1.ensure(npix * sizeof(double)));
   CHK(c->n1_s2.ensure(npix * sizeof(double)));
@@ -610,11 +651,12 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   CHK(c->n1_rec.ensure((size_t)cap * sizeof(N1Rec)));
   CHK(c->sel_count.ensure(sizeof(int)));
   hipStream_t s = c->stream;
+  if (!c->G[0].p || !c->maps.p || !c->psum.p) return GS_ERR_STATE;
   HIPCHK(hipMemsetAsync(c->n1_s1.p, 0, npix * sizeof(double), s));
   HIPCHK(hipMemsetAsync(c->n1_s2.p, 0, npix * sizeof(double), s));
   const MapRef* maps = c->maps.as<MapRef>();
   auto run_pass1 = [&](int m0, int cnt) {
-    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), maps + m0, cnt, sgn,
+    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), maps + m0, cnt,
                       c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(), s);
   };
   for (int ip = 0; ip < P; ++ip) {
@@ -811,11 +853,7 @@ int gs_create(const gs_params* p, int device, void** out) {
   c->p = *p;
   c->device = device;
   c->L_config = p->window_size;
-  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-  for (int i = 0; ok && i < Ctx::NLANE; ++i)
-    ok = hipStreamCreateWithFlags(&c->lane_stream[i], hipStreamNonBlocking) == hipSuccess &&
-         hipEventCreateWithFlags(&c->ev_lane[i], hipEventDisableTiming) == hipSuccess;
-  ok = ok && hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming) == hipSuccess;
+  const bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
     delete c;
@@ -832,11 +870,6 @@ int gs_destroy(void* ctx) {
   (void)hipDeviceSynchronize();
   drop_graphs(c);
   for (auto e : c->prof_ev) (void)hipEventDestroy(e);
-  for (int i = 0; i < Ctx::NLANE; ++i) {
-    if (c->lane_stream[i]) (void)hipStreamDestroy(c->lane_stream[i]);
-    if (c->ev_lane[i]) (void)hipEventDestroy(c->ev_lane[i]);
-  }
-  if (c->ev_ready) (void)hipEventDestroy(c->ev_ready);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return GS_OK;
@@ -922,6 +955,32 @@ int gs_set_graphs(void* ctx, int enable) {
   return GS_OK;
 }
 
+int gs_set_pipeline(void* ctx, int mode, int lag, int ring) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || (mode != GS_PIPE_STREAM && mode != GS_PIPE_LAUNCH)) return GS_ERR_ARG;
+  if (lag <= 0) lag = 16;
+  if (ring <= 0) ring = 3 * lag;
+  if (ring <= lag) return GS_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
+  c->pipeline = mode;
+  c->lag = lag;
+  c->ring = ring;
+  drop_graphs(c);
+  c->Gring.release();
+  return GS_OK;
+}
+
+int gs_set_tile_block(void* ctx, int tiles) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || tiles < 1) return GS_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipDeviceSynchronize());
+  c->tile_block = tiles;
+  drop_graphs(c);
+  return GS_OK;
+}
+
 int gs_set_batch(void* ctx, int maps_per_batch) {
   Ctx* c = static_cast<Ctx*>(ctx);
   if (!c || maps_per_batch < 0) return GS_ERR_ARG;
@@ -956,14 +1015,15 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   c->stats_psi_begin = psi_begin;
   c->stats_npsi = npsi;
   c->timing.corr_maps = npsi * per_psi;
-  c->timing.corr_pass1_ms = c->timing.corr_pass2_ms = 0;
-  c->timing.corr_pass1_launches = c->timing.corr_pass2_launches = 0;
+  c->timing.corr_ms = 0;
+  c->timing.corr_launches = 0;
   c->timing.corr_sampled_maps = 0;
   c->prof_batches = 0;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c->stream));
+  launch_fill_int(c->err.as<int>(), 1, 0, c->stream);
   for (int ip = psi_begin; ip < psi_end; ++ip) {
     MapStat* out = c->stats.as<MapStat>() + (size_t)(ip - psi_begin) * per_psi;
     c->profile_now = c->profiling && ip == psi_begin && c->mode == GS_MODE_FUSED;
@@ -982,23 +1042,23 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
                 c->p.threshold, c->p.diameter, c->sel_count.as<int>(), c->sel.as<SelRec>(), capsel,
                 c->stream);
   HIPCHK(hipEventRecord(e1, c->stream));
-  int count = 0;
+  int count = 0, err = 0;
   HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&err, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipGetLastError());
+  if (err) return GS_ERR_DEVICE;  // an in-kernel dependency wait ran out
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   c->timing.pick_total_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   for (int ib = 0; ib < c->prof_batches; ++ib) {
-    float a = 0, b = 0;
-    HIPCHK(hipEventElapsedTime(&a, c->prof_ev[3 * ib], c->prof_ev[3 * ib + 1]));
-    HIPCHK(hipEventElapsedTime(&b, c->prof_ev[3 * ib + 1], c->prof_ev[3 * ib + 2]));
-    c->timing.corr_pass1_ms += a;
-    c->timing.corr_pass2_ms += b;
+    float a = 0;
+    HIPCHK(hipEventElapsedTime(&a, c->prof_ev[2 * ib], c->prof_ev[2 * ib + 1]));
+    c->timing.corr_ms += a;
   }
-  c->timing.corr_pass1_launches = c->timing.corr_pass2_launches = c->prof_batches;
+  c->timing.corr_launches = c->prof_batches;
   if (c->prof_batches) c->timing.corr_sampled_maps = per_psi;
   if (count > capsel) return GS_ERR_DEVICE;  // cannot happen: one record per map at most
   std::vector<SelRec> recs(count);
@@ -1073,6 +1133,16 @@ int gs_get_tile_spectra(void* ctx, float* out, size_t cap) {
   if (cap < n) return GS_ERR_CAPACITY;
   HIPCHK(hipSetDevice(c->device));
   CHK(copy_out(c, out, c->tiles.p, n * sizeof(float)));
+  // undo the correlation sign the pool carries: plain tile spectra / L^2
+  const int L = c->L, H = c->geo.H;
+  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;
+  for (size_t i = 0; i < n / 2; ++i) {
+    const int ky = (int)(i % L), kx = (int)((i / L) % H);
+    if (((kx + ky) & 1) ? (sgn > 0) : (sgn < 0)) {
+      out[2 * i] = -out[2 * i];
+      out[2 * i + 1] = -out[2 * i + 1];
+    }
+  }
   return GS_OK;
 }
 
@@ -1097,15 +1167,14 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
   CHK(rotate_templates(c, psi_index));
   const int L = c->L;
   const int nb = std::min(c->batch, c->T * c->N);
-  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;
   const float lo = c->p.diameter / 4.0f, hi = (float)L - c->p.diameter / 4.0f;
   hipEvent_t e[3];
   for (auto& x : e) HIPCHK(hipEventCreate(&x));
   HIPCHK(hipEventRecord(e[0], c->stream));
   CHK(c->stats.ensure((size_t)nb * sizeof(MapStat)));
   for (int i = 0; i < reps; ++i)
-    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps.as<MapRef>(), nb,
-                      sgn, c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(),
+    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps_blk.as<MapRef>(), nb,
+                      c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(),
                       c->stream);
   HIPCHK(hipEventRecord(e[1], c->stream));
   for (int i = 0; i < reps; ++i)

@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace gs {
 
 // ---------------------------------------------------------------------------
@@ -175,11 +177,29 @@ struct DirectOut {
   }
 };
 
-template <int R, int DIR, bool PRIME>
+// largest power of R's smallest prime that divides R
+constexpr int prime_power_part(int r) {
+  const int p = smallest_factor(r);
+  int a = 1;
+  while (r % (a * p) == 0) a *= p;
+  return a;
+}
+constexpr int mod_inverse(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 1;  // m == 1
+}
+// how an R-point DFT is built: 0 direct (prime), 1 Cooley-Tukey over a prime
+// power, 2 Good-Thomas prime-factor split into coprime parts
+constexpr int dft_kind(int r) {
+  return smallest_factor(r) == r ? 0 : (prime_power_part(r) == r ? 1 : 2);
+}
+
+template <int R, int DIR, int KIND>
 struct DftImpl;
 
 template <int R, int DIR>
-struct DftImpl<R, DIR, true> {
+struct DftImpl<R, DIR, 0> {
   __device__ __forceinline__ static void run(float2* v) {
     float2 o[R];
     DirectOut<R, DIR, 0>::run(v, o);
@@ -211,7 +231,7 @@ struct CompStage1 {
 };
 
 template <int R, int DIR>
-struct DftImpl<R, DIR, false> {
+struct DftImpl<R, DIR, 1> {
   static constexpr int A = smallest_factor(R);
   static constexpr int B = R / A;
   __device__ __forceinline__ static void run(float2* v) {
@@ -229,11 +249,47 @@ struct DftImpl<R, DIR, false> {
   }
 };
 
+// Good-Thomas split R = A B, gcd(A, B) = 1: with input index
+// n = (B n1 + A n2) mod R and output index k = (B b' k1 + A a' k2) mod R
+// (b' = B^-1 mod A, a' = A^-1 mod B) the DFT factors into A-point DFTs over n1
+// and B-point DFTs over n2 with no twiddle factors in between.
+template <int R, int DIR, int A, int B, int N2>
+struct PfaStage1 {
+  __device__ __forceinline__ static void run(const float2* v, float2 (*y)[A]) {
+    float2 t[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) t[n1] = v[(B * n1 + A * N2) % R];
+    Dft<A, DIR>::run(t);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) y[N2][k1] = t[k1];
+    if constexpr (N2 + 1 < B) PfaStage1<R, DIR, A, B, N2 + 1>::run(v, y);
+  }
+};
+
+template <int R, int DIR>
+struct DftImpl<R, DIR, 2> {
+  static constexpr int A = prime_power_part(R);
+  static constexpr int B = R / A;
+  static constexpr int BI = mod_inverse(B % A, A);
+  static constexpr int AI = mod_inverse(A % B, B);
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 y[B][A];
+    PfaStage1<R, DIR, A, B, 0>::run(v, y);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      float2 t[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) t[n2] = y[n2][k1];
+      Dft<B, DIR>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) v[(B * BI * k1 + A * AI * k2) % R] = t[k2];
+    }
+  }
+};
+
 template <int R, int DIR>
 struct Dft {
-  __device__ __forceinline__ static void run(float2* v) {
-    DftImpl<R, DIR, smallest_factor(R) == R>::run(v);
-  }
+  __device__ __forceinline__ static void run(float2* v) { DftImpl<R, DIR, dft_kind(R)>::run(v); }
 };
 
 // ---------------------------------------------------------------------------
@@ -496,8 +552,15 @@ __device__ __forceinline__ void fft_exec_pass(float2* lds, const float2* tw, Loa
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if constexpr (LAST) store(b, base + r * NS, v[p][r]);
-        else lds[b * STRIDE + pidx(base + r * NS)] = v[p][r];
+        if constexpr (LAST) {
+          // a 5-argument store also receives the round p and output slot r
+          if constexpr (std::is_invocable_v<StoreF&, int, int, float2, int, int>)
+            store(b, base + r * NS, v[p][r], p, r);
+          else
+            store(b, base + r * NS, v[p][r]);
+        } else {
+          lds[b * STRIDE + pidx(base + r * NS)] = v[p][r];
+        }
       }
     }
   }

@@ -28,8 +28,77 @@ struct Geo {
   static constexpr int HP = (H + 15) / 16 * 16;
   static constexpr int RB = clampi(pow2floor(8192 / M), 8, 32);
   static constexpr int CB = clampi(pow2floor(8192 / L), 8, 32);
-  static_assert(L % 2 == 0, "even tile size");
+  // correlation column blocks pair column k with column M - k: block cb holds
+  // the pair units p = cb * CB/2 + u (u < CB/2) in slots u (column p) and
+  // CB-1-u (column M - p), p = 0 .. M/2
+  static constexpr int NCB = (M / 2 + 1 + CB / 2 - 1) / (CB / 2);
+  static_assert(L % 4 == 0, "tile size divisible by 4");
 };
+
+// column of slot c of paired block cb
+template <int L>
+__device__ __forceinline__ int pair_col(int cb, int c) {
+  constexpr int M = L / 2, CB = Geo<L>::CB;
+  const int u = c < CB / 2 ? c : CB - 1 - c;
+  const int p = cb * (CB / 2) + u;
+  return c < CB / 2 ? p : M - p;
+}
+// slot offset of column k (0 <= k < M) in its paired block: (cb * L + y) * CB + c
+template <int L>
+__device__ __forceinline__ int pair_slot(int k, int y) {
+  constexpr int M = L / 2, CB = Geo<L>::CB;
+  const int p = min(k, M - k);
+  const int u = p % (CB / 2), cb = p / (CB / 2);
+  const int c = (k <= M / 2) ? u : CB - 1 - u;
+  return (cb * L + y) * CB + c;
+}
+
+// streaming accesses of the correlation intermediate: written once by pass 1,
+// read once by pass 2, so they should not displace the spectra in L2
+typedef float nt_float2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(float2* p, float2 v) {
+  nt_float2 t;
+  t.x = v.x;
+  t.y = v.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<nt_float2*>(p));
+}
+__device__ __forceinline__ float2 load_nt(const float2* p) {
+  const nt_float2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_float2*>(p));
+  return make_float2(t.x, t.y);
+}
+
+// device-coherent accesses (agent-scope relaxed atomics: stores written
+// through to the coherence point, loads that do not hit stale L2 lines), for
+// an intermediate that a workgroup on another XCD consumes within the same
+// launch
+__device__ __forceinline__ void store_coh(float2* p, float2 v) {
+  const unsigned long long u =
+      ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float2 load_coh(const float2* p) {
+  const unsigned long long u = __hip_atomic_load(
+      reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
+}
+
+// value of lane (lane ^ (CB - 1)): the partner slot of a paired block when
+// CB consecutive lanes hold the CB slots of one row
+template <int CB>
+__device__ __forceinline__ float xchg_f(float v) {
+  if constexpr (CB == 8) {  // row_half_mirror
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
+  } else if constexpr (CB == 16) {  // row_mirror
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));
+  } else {
+    return __shfl_xor(v, CB - 1, 64);
+  }
+}
+template <int CB>
+__device__ __forceinline__ float2 xchg(float2 v) {
+  return make_float2(xchg_f<CB>(v.x), xchg_f<CB>(v.y));
+}
 
 // ---------------------------------------------------------------------------
 // wave / block reductions (wave64)
@@ -330,71 +399,110 @@ __global__ __launch_bounds__(WG) void k_rows_c2r(const float2* __restrict__ Gin,
 // main.cu:848, get_peak_and_SUM GPU_func.cu:667-708, host loop main.cu:856-881)
 // ---------------------------------------------------------------------------
 template <int L>
-__global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ tiles,
-                                                   const float2* __restrict__ tspec,
-                                                   const MapRef* __restrict__ maps, float sgn,
-                                                   float2* __restrict__ Gout,
-                                                   float2* __restrict__ psum,
-                                                   const float2* __restrict__ twL) {
+constexpr int corr_lds1() { return Geo<L>::CB * exec_stride<L, Geo<L>::CB, WG, MAP_B>(); }
+template <int L>
+constexpr int corr_lds2() { return Geo<L>::RB * exec_stride<L / 2, Geo<L>::RB, WG, MAP_J>(); }
+
+// Pass 1 of map m, column block cb: P = tile * conj(template) on the paired
+// columns, inverse column transforms, C2R pre-processing, blocked store.
+template <int L, bool COH = false>
+__device__ __forceinline__ void corr_pass1_body(float2* lds, float* red_q, int cb, int m, int gm,
+                                                const float2* __restrict__ tiles,
+                                                const float2* __restrict__ tspec,
+                                                const MapRef* __restrict__ maps,
+                                                float2* __restrict__ Gout,
+                                                float2* __restrict__ psum,
+                                                const float2* __restrict__ twL) {
   using G = Geo<L>;
-  constexpr int H = G::H, HP = G::HP, CB = G::CB;
-  __shared__ float2 lds[CB * exec_stride<L, CB, WG, MAP_B>()];
-  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
-  // w % 8 picks the XCD.  Each XCD owns the column blocks cb = xcd + 8 i of
-  // every map, which keeps its slice of the tile spectrum resident in its L2
-  // while the template spectra stream through.
-  constexpr int NCB = (H + CB - 1) / CB;
-  constexpr int NCB8 = (NCB + 7) / 8;
-  const int w = blockIdx.x;
-  const int xcd = w & 7, k = w >> 3;
-  const int cb = xcd + 8 * (k % NCB8);
-  const int m = k / NCB8;
-  if (cb >= NCB) return;
+  constexpr int M = G::M, CB = G::CB, NCB = G::NCB;
   const MapRef mr = maps[m];
-  const int kx0 = cb * CB;
-  const int ncol = min(CB, H - kx0);
-  const float2* tw = twL;  // twiddles through the L1/L2 caches, LDS kept for data
-  const float2* ib = tiles + ((long)mr.tile * H + kx0) * L;
-  const float2* tb = tspec + ((long)mr.view * H + kx0) * L;
-  // blocked intermediate G[m][cb][y][c] (c < CB): every workgroup writes one
-  // contiguous block, rows of CB columns at a time
-  float2* gb = Gout + ((long)m * NCB + cb) * L * CB;
-  // P = tile * conj(template) * (-1)^(kx+ky) * sgn, read straight into the
-  // first radix pass
-  // branch-free so the compiler issues all loads of a butterfly back to back
+  const float2* ib = tiles + (long)mr.tile * (M + 1) * L;
+  const float2* tb = tspec + (long)mr.view * (M + 1) * L;
+  // blocked intermediate G[m][cb][y][c]: every workgroup writes one
+  // contiguous block, rows of CB slots at a time
+  float2* gb = Gout + ((long)gm * NCB + cb) * L * CB;
+  // P = tile * conj(template), read straight into the first radix pass (the
+  // tile spectra carry the factor (-1)^(kx+ky) sgn of compute_corner_CCG,
+  // applied once per micrograph).  Slots past the last pair unit read column
+  // M and feed only unused outputs.
   //
   // The map's sums follow from Parseval on the way in: sum_u c(u)^2 =
   // L^2 sum_k |P(k)|^2 over the Hermitian-completed spectrum (columns
-  // 0 < kx < L/2 count twice) and sum_u c(u) = L^2 P(0,0).
-  float sumsq = 0.0f, dc = 0.0f;
+  // 0 < kx < L/2 count twice, a column in two slots or past M/2 not at all)
+  // and sum_u c(u) = L^2 P(0,0).
+  float sumsq = 0.0f;
   auto load = [&](int c, int y) -> float2 {
-    const bool ok = c < ncol;
-    const int cc = ok ? c : 0;
-    const float2 v = cmulc(ib[cc * L + y], tb[cc * L + y]);
-    const float sg = ok ? (((kx0 + c + y) & 1) ? -sgn : sgn) : 0.0f;
-    const float2 p = cscale(v, sg);
-    const int kx = kx0 + c;
-    const float wgt = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
-    sumsq += wgt * (p.x * p.x + p.y * p.y);
-    if (kx == 0 && y == 0) dc = p.x;
-    return p;
+    const int u = c < CB / 2 ? c : CB - 1 - c;
+    const int p = cb * (CB / 2) + u;
+    const int kx = min(c < CB / 2 ? p : M - p, M);
+    const float wgt = (p > M / 2 || (c >= CB / 2 && p == M / 2)) ? 0.0f
+                      : ((kx == 0 || kx == M) ? 1.0f : 2.0f);
+    const float2 v = cmulc(ib[kx * L + y], tb[kx * L + y]);
+    sumsq = fmaf(wgt, fmaf(v.x, v.x, v.y * v.y), sumsq);
+    return v;
   };
-  // last pass: columns fastest across lanes, consecutive rows -> contiguous
+  // Last pass: the CB slots of one row y sit in CB consecutive lanes, so slot
+  // c finds its partner column M - k in lane ^ (CB - 1).  Each lane then
+  // writes the C2R pre-processed value
+  //   Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) e^{+2 pi i k / L}
+  // of its own column k < M (the row transform in pass 2 is then a plain
+  // M-point complex FFT of Z').
+  // Every slot is written: the slots pass 2 never reads (column M, the
+  // second copy of column M/2, units past M/2) only hold unused values.
+  const int cl = threadIdx.x % CB;  // the slot this lane stores (MAP_B)
+  float2 wk = twL[pair_col<L>(cb, cl)];  // column index <= M < L
+  wk.y = -wk.y;
   auto store = [&](int c, int y, float2 v) {
-    if (c < ncol) gb[y * CB + c] = v;
+    const float2 pv = xchg<CB>(v);
+    if constexpr (COH) store_coh(gb + y * CB + c, c2r_pre(v, pv, wk));
+    else store_nt(gb + y * CB + c, c2r_pre(v, pv, wk));
   };
-  fft_exec<L, CB, WG, +1, MAP_J, MAP_B>(lds, tw, load, store);
-  __shared__ float red_q[WG / 64], red_d[WG / 64];
+  fft_exec<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
   sumsq = wave_sum_f(sumsq);
-  dc = wave_sum_f(dc);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) { red_q[wv] = sumsq; red_d[wv] = dc; }
+  if (lane == 0) red_q[wv] = sumsq;
   __syncthreads();
   if (threadIdx.x == 0) {
-    float q = 0.0f, d = 0.0f;
-    for (int i = 0; i < WG / 64; ++i) { q += red_q[i]; d += red_d[i]; }
+    float q = 0.0f;
+    for (int i = 0; i < WG / 64; ++i) q += red_q[i];
+    const float d = cb == 0 ? cmulc(ib[0], tb[0]).x : 0.0f;  // P(0, 0)
     psum[(long)m * NCB + cb] = make_float2(q, d);
   }
+}
+
+// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so item
+// w % 8 runs on XCD w % 8.  Each XCD owns the column blocks cb = xcd + 8 i of
+// every map, so an XCD's L2 sees one slice of each spectrum.
+template <int L>
+__device__ __forceinline__ bool corr_pass1_item(int w, int nmaps, int* cb, int* m) {
+  constexpr int NCB = Geo<L>::NCB, NCB8 = (NCB + 7) / 8;
+  const int xcd = w & 7, k8 = w >> 3;
+  *cb = xcd + 8 * (k8 % NCB8);
+  *m = k8 / NCB8;
+  return *cb < NCB && *m < nmaps;
+}
+
+template <int L>
+__global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ tiles,
+                                                   const float2* __restrict__ tspec,
+                                                   const MapRef* __restrict__ maps, int nmaps,
+                                                   float2* __restrict__ Gout,
+                                                   float2* __restrict__ psum,
+                                                   const float2* __restrict__ twL) {
+  __shared__ float2 lds[corr_lds1<L>()];
+  __shared__ float red_q[WG / 64];
+  int cb, m;
+  if (!corr_pass1_item<L>(blockIdx.x, nmaps, &cb, &m)) return;
+  corr_pass1_body<L>(lds, red_q, cb, m, m, tiles, tspec, maps, Gout, psum, twL);
+}
+
+// (-1)^(kx+ky) sgn on half spectra [t][kx][ky], in place
+template <int L>
+__global__ void k_checker_sign(float2* __restrict__ spec, long n, float sgn) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ky = (int)(i % L), kx = (int)((i / L) % (L / 2 + 1));
+  spec[i] = cscale(spec[i], ((kx + ky) & 1) ? -sgn : sgn);
 }
 
 // per-map finalisation (host loop of main.cu:856-881 in FP64): combine the
@@ -424,54 +532,62 @@ __device__ void corr_finalize_map(const CorrPartial* p, int nparts, const float2
   *out = st;
 }
 
-template <int L>
-__global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gin, int blo,
-                                                   int bhi, CorrPartial* __restrict__ part,
-                                                   const float2* __restrict__ twL,
-                                                   const float2* __restrict__ twM) {
+// Pass 2 of map m, row block rb: row transforms of Z' and the border-
+// excluded maximum with its position.
+template <int L, bool COH = false>
+__device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* red_p, int m,
+                                                int gm, int rb, const float2* __restrict__ Gin,
+                                                int blo,
+                                                int bhi, CorrPartial* __restrict__ part,
+                                                int nparts, const float2* __restrict__ twM) {
   using G = Geo<L>;
-  constexpr int M = G::M, H = G::H, RB = G::RB, CB = G::CB;
-  constexpr int NCB = (H + CB - 1) / CB;
-  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
-  __shared__ float red_v[WG / 64];
-  __shared__ int red_p[WG / 64];
-  const int m = blockIdx.y;
-  const int y0 = blockIdx.x * RB;
-  const float2* tw = twM;
-  const float2* gb = Gin + (long)m * NCB * L * CB;
-  // C2R pre-processing fused into the first radix pass:
-  // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) e^{+2 pi i k / L}
-  // with X[k] of row y at G[m][k / CB][y][k % CB]; branch-free (rows past L
-  // only exist when RB does not divide L)
-  auto at = [&](int y, int k) { return gb[((k / CB) * L + y) * CB + (k % CB)]; };
+  constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
+  const int y0 = rb * RB;
+  const float2* gb = Gin + (long)gm * NCB * L * CB;
+  // Z'[k] of row y, already pre-processed by pass 1 (rows past L only exist
+  // when RB does not divide L; they are read clamped and never stored)
   auto load = [&](int r, int k) -> float2 {
-    const int y0r = y0 + r;
-    const bool ok = y0r < L;
-    const int y = ok ? y0r : 0;
-    float2 w = twL[k];
-    w.y = -w.y;
-    const float2 z = c2r_pre(at(y, k), at(y, M - k), w);
-    return ok ? z : make_float2(0.f, 0.f);
+    const float2* q = gb + pair_slot<L>(k, min(y0 + r, L - 1));
+    if constexpr (COH) return load_coh(q);
+    else return load_nt(q);
   };
   // z'[n] = (c[y][2n], c[y][2n+1]).  Border pixels (x or y outside
-  // [blo, bhi], the integer form of get_peak_and_SUM's d/4 test) count as 0;
-  // the max keeps the lowest index among equal values.  The common case is a
-  // value below the running max: one compare per pixel.
-  float vmax = 0.0f;
-  int pos = INT_MAX;
+  // [blo, bhi], the integer form of get_peak_and_SUM's d/4 test) count as 0.
+  // Each thread visits its pixels in increasing linear index (round, then
+  // output slot, then x), so a strict '>' keeps the lowest index among equal
+  // values; the running maximum records a compile-time code of the winning
+  // (round, slot, half) and the position is decoded once at the end.  The
+  // border tests are uniform: rows per workgroup, columns per output slot.
+  constexpr Plan PL = tuned_plan(M, RB, WG, MAP_J);
+  constexpr int RL = PL.r[PL.count - 1];  // radix of the last pass
+  constexpr int NSL = M / RL;             // its butterflies per transform
   const unsigned brange = (unsigned)(bhi - blo);
-  auto store = [&](int r, int n, float2 z) {
-    const int y = y0 + r;
-    if ((unsigned)(y - blo) > brange) return;
-    const int x0 = 2 * n;
-    const float a = ((unsigned)(x0 - blo) <= brange) ? z.x : 0.0f;
-    const float b = ((unsigned)(x0 + 1 - blo) <= brange) ? z.y : 0.0f;
-    if (a >= vmax || b >= vmax) {
-      if (a > 0.0f) vmax_combine(vmax, pos, a, y * L + x0);
-      if (b > 0.0f) vmax_combine(vmax, pos, b, y * L + x0 + 1);
+  const bool rows_in = y0 >= blo && y0 + RB - 1 <= bhi;
+  float vmax = 0.0f;
+  int vcode = -1;
+  auto store = [&](int r, int n, float2 z, int pr, int slot) {
+    float a = z.x, b = z.y;
+    if (!rows_in) {
+      if ((unsigned)(y0 + r - blo) > brange) { a = 0.0f; b = 0.0f; }
     }
+    const int xlo = 2 * slot * NSL, xhi = 2 * (slot + 1) * NSL - 1;
+    if (xlo < blo || xhi > bhi) {
+      const int x0 = 2 * n;
+      a = ((unsigned)(x0 - blo) <= brange) ? a : 0.0f;
+      b = ((unsigned)(x0 + 1 - blo) <= brange) ? b : 0.0f;
+    }
+    const int code = (pr * RL + slot) * 2;
+    if (a > vmax) { vmax = a; vcode = code; }
+    if (b > vmax) { vmax = b; vcode = code + 1; }
   };
-  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, tw, load, store);
+  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+  int pos = INT_MAX;
+  if (vcode >= 0) {
+    const int pr = vcode / (2 * RL), slot = (vcode >> 1) % RL, half = vcode & 1;
+    const int g = threadIdx.x + pr * WG;
+    const int row = g / NSL, j = g % NSL;
+    pos = (y0 + row) * L + 2 * (j + slot * NSL) + half;
+  }
   wave_max_idx(vmax, pos);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) { red_v[w] = vmax; red_p[w] = pos; }
@@ -482,8 +598,139 @@ __global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gi
     for (int i = 1; i < WG / 64; ++i) vmax_combine(v, p, red_v[i], red_p[i]);
     CorrPartial cp;
     cp.vmax = v; cp.pos = p; cp.s1 = 0.0f; cp.s2 = 0.0f;
-    part[(long)m * gridDim.x + blockIdx.x] = cp;
+    part[(long)m * nparts + rb] = cp;
   }
+}
+
+template <int L>
+__global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gin, int blo,
+                                                   int bhi, CorrPartial* __restrict__ part,
+                                                   const float2* __restrict__ twM) {
+  __shared__ float2 lds[corr_lds2<L>()];
+  __shared__ float red_v[WG / 64];
+  __shared__ int red_p[WG / 64];
+  corr_pass2_body<L>(lds, red_v, red_p, blockIdx.y, blockIdx.y, blockIdx.x, Gin, blo, bhi, part,
+                     gridDim.x, twM);
+}
+
+// Software-pipelined correlation: one launch runs pass 1 of batch k and
+// pass 2 of batch k-1 (whose intermediate the previous launch completed).
+// Groups of 8 consecutive workgroups (one per XCD) alternate between the two
+// kinds, so every CU mixes the load-heavy pass 1 with the transform-heavy
+// pass 2; groups past one kind's work go to the other kind.
+template <int L>
+__global__ __launch_bounds__(WG) void k_corr_fused(
+    const float2* __restrict__ tiles, const float2* __restrict__ tspec,
+    const MapRef* __restrict__ maps1, int nmaps1, float2* __restrict__ G1,
+    float2* __restrict__ psum1, int ng1, const float2* __restrict__ G2, int nmaps2, int ng2,
+    int blo, int bhi, CorrPartial* __restrict__ part2, int nparts,
+    const float2* __restrict__ twL, const float2* __restrict__ twM) {
+  constexpr int LDS = corr_lds1<L>() > corr_lds2<L>() ? corr_lds1<L>() : corr_lds2<L>();
+  __shared__ float2 lds[LDS];
+  __shared__ float red_f[WG / 64];
+  __shared__ int red_i[WG / 64];
+  const int g = blockIdx.x >> 3, x = blockIdx.x & 7;
+  // the first 2 min(ng1, ng2) groups alternate, the rest belong to the kind
+  // with more groups
+  const int mn = min(ng1, ng2);
+  int kind, gi;
+  if (g < 2 * mn) {
+    kind = g & 1;
+    gi = g >> 1;
+  } else {
+    kind = ng1 > ng2 ? 0 : 1;
+    gi = g - mn;
+  }
+  const int w = gi * 8 + x;
+  if (kind == 0) {
+    int cb, m;
+    if (!corr_pass1_item<L>(w, nmaps1, &cb, &m)) return;
+    corr_pass1_body<L>(lds, red_f, cb, m, m, tiles, tspec, maps1, G1, psum1, twL);
+  } else {
+    const int m = w / nparts, rb = w % nparts;
+    if (m >= nmaps2) return;
+    corr_pass2_body<L>(lds, red_f, red_i, m, m, rb, G2, blo, bhi, part2, nparts, twM);
+  }
+}
+
+// Bounded wait of thread 0 until *ctr >= target (agent-scope loads).  A wait
+// that runs out (about 2^18 polls) sets *err and gives up, and once *err is
+// set every later wait returns at once, so the launch always drains; the host
+// then reports the error instead of results.
+__device__ __forceinline__ void wait_count(const int* ctr, int target, int* err) {
+  if (threadIdx.x == 0) {
+    int it = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++it > (1 << 18) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+}
+// every thread's stores complete (vmcnt(0)), then thread 0 counts them
+__device__ __forceinline__ void signal_count(int* ctr) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores done
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Whole-psi correlation in one launch.  Step s holds pass 1 of map s and
+// pass 2 of map s - lag as groups of 8 workgroups (one per XCD), the two kinds
+// alternating.  The intermediate lives in a ring of `ring` map slots:
+//  * pass 2 of map m waits until the NCB pass-1 blocks of m have signalled
+//    ready[m];
+//  * pass 1 of map m (slot m % ring) waits until the NP pass-2 blocks of map
+//    m - ring have signalled done[m - ring].
+// Every wait is on workgroups of lower index (lag < ring), which in-order
+// dispatch has already placed on the GPU, so the waits always end; they are
+// bounded anyway (err).  Intermediate accesses are device-coherent because
+// producer and consumer may sit on different XCDs.
+template <int L>
+__global__ __launch_bounds__(WG) void k_corr_stream(
+    const float2* __restrict__ tiles, const float2* __restrict__ tspec,
+    const MapRef* __restrict__ maps, int nmaps, int lag, int ring, float2* __restrict__ Gring,
+    float2* __restrict__ psum, int* __restrict__ ready, int* __restrict__ done, int blo, int bhi,
+    CorrPartial* __restrict__ part, int* __restrict__ err, const float2* __restrict__ twL,
+    const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int NCB = G::NCB, NCB8 = (NCB + 7) / 8;
+  constexpr int NP = (L + G::RB - 1) / G::RB;
+  constexpr int G1 = NCB8, G2 = (NP + 7) / 8, GS = G1 + G2, MN = G1 < G2 ? G1 : G2;
+  constexpr int LDS = corr_lds1<L>() > corr_lds2<L>() ? corr_lds1<L>() : corr_lds2<L>();
+  __shared__ float2 lds[LDS];
+  __shared__ float red_f[WG / 64];
+  __shared__ int red_i[WG / 64];
+  const int grp = blockIdx.x >> 3, x = blockIdx.x & 7;
+  const int step = grp / GS, gl = grp % GS;
+  int kind, gi;
+  if (gl < 2 * MN) {
+    kind = gl & 1;
+    gi = gl >> 1;
+  } else {
+    kind = G1 > G2 ? 0 : 1;
+    gi = gl - MN;
+  }
+  if (kind == 0) {
+    const int m = step, cb = x + 8 * gi;
+    if (m >= nmaps || cb >= NCB) return;
+    if (m >= ring) wait_count(done + (m - ring), NP, err);
+    corr_pass1_body<L, true>(lds, red_f, cb, m, m % ring, tiles, tspec, maps, Gring, psum, twL);
+    signal_count(ready + m);
+  } else {
+    const int m = step - lag, rb = gi * 8 + x;
+    if (m < 0 || m >= nmaps || rb >= NP) return;
+    wait_count(ready + m, NCB, err);
+    corr_pass2_body<L, true>(lds, red_f, red_i, m, m % ring, rb, Gring, blo, bhi, part, NP, twM);
+    signal_count(done + m);
+  }
+}
+
+__global__ void k_fill_int(int* __restrict__ p, long n, int v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // one thread per map: combine the row-block partials of every map of a psi;
@@ -520,24 +767,15 @@ __global__ __launch_bounds__(WG) void k_norm1_rows(const float2* __restrict__ Gi
                                                    const float2* __restrict__ twL,
                                                    const float2* __restrict__ twM) {
   using G = Geo<L>;
-  constexpr int M = G::M, H = G::H, RB = G::RB, CB = G::CB;
-  constexpr int NCB = (H + CB - 1) / CB;
+  constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
   constexpr long L2 = (long)L * L;
   __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
   const int m = blockIdx.y;
   const int y0 = blockIdx.x * RB;
   const int t = maps[m].tile;
   const float2* gb = Gin + (long)m * NCB * L * CB;
-  auto at = [&](int y, int k) { return gb[((k / CB) * L + y) * CB + (k % CB)]; };
-  auto load = [&](int r, int k) -> float2 {
-    const int y0r = y0 + r;
-    const bool ok = y0r < L;
-    const int y = ok ? y0r : 0;
-    float2 w = twL[k];
-    w.y = -w.y;
-    const float2 z = c2r_pre(at(y, k), at(y, M - k), w);
-    return ok ? z : make_float2(0.f, 0.f);
-  };
+  // Z'[k] of row y, pre-processed by pass 1
+  auto load = [&](int r, int k) -> float2 { return gb[pair_slot<L>(k, min(y0 + r, L - 1))]; };
   const float inv = 1.0f / (float)L2;
   auto store = [&](int r, int n, float2 z) {
     const int y = y0 + r;
@@ -852,7 +1090,7 @@ Geometry geometry_of(int L) {
   case n:                                                                              \
     g.L = n; g.M = Geo<n>::M; g.H = Geo<n>::H; g.HP = Geo<n>::HP; g.RB = Geo<n>::RB;   \
     g.CB = Geo<n>::CB;                                                                 \
-    g.gmap = (long)((Geo<n>::H + Geo<n>::CB - 1) / Geo<n>::CB) * Geo<n>::CB * n;       \
+    g.gmap = (long)Geo<n>::NCB * Geo<n>::CB * n;                                       \
     break;
     GS_SIZES(CASE)
 #undef CASE
@@ -967,38 +1205,103 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
 
 template <int L>
 static void corr_pass1_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
-                         float sgn, float2* G, float2* psum, const float2* twL, hipStream_t s) {
-  constexpr int NCB = (Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB;
-  constexpr int NCB8 = (NCB + 7) / 8;
+                         float2* G, float2* psum, const float2* twL, hipStream_t s) {
+  constexpr int NCB8 = (Geo<L>::NCB + 7) / 8;
   dim3 grid(8 * NCB8 * nmaps);
-  hipLaunchKernelGGL((k_corr_pass1<L>), grid, dim3(WG), 0, s, tiles, tspec, maps, sgn, G, psum,
+  hipLaunchKernelGGL((k_corr_pass1<L>), grid, dim3(WG), 0, s, tiles, tspec, maps, nmaps, G, psum,
                      twL);
 }
+
+template <int L>
+static void corr_fused_t(const float2* tiles, const float2* tspec, const MapRef* maps1, int nmaps1,
+                         float2* G1, float2* psum1, const float2* G2, int nmaps2, int blo,
+                         int bhi, CorrPartial* part2, const float2* twL, const float2* twM,
+                         hipStream_t s) {
+  constexpr int NCB8 = (Geo<L>::NCB + 7) / 8;
+  constexpr int NP = (L + Geo<L>::RB - 1) / Geo<L>::RB;
+  const int ng1 = NCB8 * nmaps1;              // groups of 8 pass-1 items
+  const int ng2 = (NP * nmaps2 + 7) / 8;      // groups of 8 pass-2 items
+  const int groups = ng1 + ng2;
+  if (groups == 0) return;
+  hipLaunchKernelGGL((k_corr_fused<L>), dim3(8 * groups), dim3(WG), 0, s, tiles, tspec, maps1,
+                     nmaps1, G1, psum1, ng1, G2, nmaps2, ng2, blo, bhi, part2, NP, twL, twM);
+}
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float sgn, float2* G, float2* psum, const float2* twL,
-                       hipStream_t s) {
-  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, sgn, G, psum, twL, s);
+                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s) {
+  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, G, psum, twL, s);
+}
+
+void launch_checker_sign(int L, float2* spec, int ntiles, hipStream_t s) {
+  const long n = (long)ntiles * (L / 2 + 1) * L;
+  const float sgn = ((L / 2) % 2 == 1) ? -1.0f : 1.0f;  // compute_corner_CCG's `of` rule
+  const unsigned nb = (unsigned)((n + 255) / 256);
+#define CS(n_) case n_: hipLaunchKernelGGL((k_checker_sign<n_>), dim3(nb), dim3(256), 0, s, spec, n, sgn); break;
+  switch (L) { GS_SIZES(CS) default: break; }
+#undef CS
 }
 
 template <int L>
 static void corr_pass2_t(const float2* G, int nmaps, int blo, int bhi, CorrPartial* part,
                          const float2* twL, const float2* twM, hipStream_t s) {
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
-  hipLaunchKernelGGL((k_corr_pass2<L>), grid, dim3(WG), 0, s, G, blo, bhi, part, twL, twM);
+  hipLaunchKernelGGL((k_corr_pass2<L>), grid, dim3(WG), 0, s, G, blo, bhi, part, twM);
+}
+template <int L>
+static void corr_stream_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
+                          int lag, int ring, float2* Gring, float2* psum, int* ready, int* done,
+                          int blo, int bhi, CorrPartial* part, int* err, const float2* twL,
+                          const float2* twM, hipStream_t s) {
+  using G = Geo<L>;
+  constexpr int NP = (L + G::RB - 1) / G::RB;
+  constexpr int GS = (G::NCB + 7) / 8 + (NP + 7) / 8;
+  const long nctr = 2l * nmaps;
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, s, ready,
+                     nctr, 0);
+  const long nwg = 8l * GS * (nmaps + lag);
+  hipLaunchKernelGGL((k_corr_stream<L>), dim3((unsigned)nwg), dim3(WG), 0, s, tiles, tspec, maps,
+                     nmaps, lag, ring, Gring, psum, ready, done, blo, bhi, part, err, twL, twM);
+}
+
+// integer form of get_peak_and_SUM's exclusion: x < lo || x > hi
+static void border_range(int L, float lo, float hi, int* blo, int* bhi) {
+  *blo = (int)ceilf(lo);
+  *bhi = (int)floorf(hi);
+  if (*bhi < *blo) { *blo = L + 1; *bhi = L + 1; }  // nothing valid
 }
 void launch_corr_pass2(int L, const float2* G, int nmaps, float lo, float hi, CorrPartial* part,
                        const float2* twL, const float2* twM, hipStream_t s) {
-  // integer form of get_peak_and_SUM's exclusion: x < lo || x > hi
-  int blo = (int)ceilf(lo), bhi = (int)floorf(hi);
-  if (bhi < blo) { blo = L + 1; bhi = L + 1; }  // nothing valid
+  int blo, bhi;
+  border_range(L, lo, hi, &blo, &bhi);
   DISPATCH(L, corr_pass2_t, G, nmaps, blo, bhi, part, twL, twM, s);
+}
+void launch_fill_int(int* p, long n, int v, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+void launch_corr_stream(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
+                        int nmaps, int lag, int ring, float2* Gring, float2* psum, int* counters,
+                        float lo, float hi, CorrPartial* part, int* err, const float2* twL,
+                        const float2* twM, hipStream_t s) {
+  int blo, bhi;
+  border_range(L, lo, hi, &blo, &bhi);
+  DISPATCH(L, corr_stream_t, tiles, tspec, maps, nmaps, lag, ring, Gring, psum, counters,
+           counters + nmaps, blo, bhi, part, err, twL, twM, s);
+}
+void launch_corr_fused(int L, const float2* tiles, const float2* tspec, const MapRef* maps1,
+                       int nmaps1, float2* G1, float2* psum1, const float2* G2, int nmaps2,
+                       float lo, float hi, CorrPartial* part2, const float2* twL,
+                       const float2* twM, hipStream_t s) {
+  int blo, bhi;
+  border_range(L, lo, hi, &blo, &bhi);
+  DISPATCH(L, corr_fused_t, tiles, tspec, maps1, nmaps1, G1, psum1, G2, nmaps2, blo, bhi, part2,
+           twL, twM, s);
 }
 
 int corr_nparts(int L) { return (L + geometry_of(L).RB - 1) / geometry_of(L).RB; }
 
 int corr_ncb(int L) {
   const Geometry g = geometry_of(L);
-  return (g.H + g.CB - 1) / g.CB;
+  return (int)(g.gmap / ((long)g.CB * g.L));
 }
 
 void launch_corr_finalize(int L, const CorrPartial* part, const float2* psum, int nmaps,

@@ -92,8 +92,23 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
 // (Geometry::gmap elements per map).
 // psum[m * corr_ncb(L) + cb] = (sum of weighted |P|^2, P(0,0).re) of the block
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float sgn, float2* G, float2* psum, const float2* twL,
-                       hipStream_t s);
+                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s);
+// one launch: pass 1 of maps1 (into G1/psum1) and pass 2 of the maps whose
+// intermediate G2 a previous launch wrote (into part2); either count may be 0
+void launch_corr_fused(int L, const float2* tiles, const float2* tspec, const MapRef* maps1,
+                       int nmaps1, float2* G1, float2* psum1, const float2* G2, int nmaps2,
+                       float lo, float hi, CorrPartial* part2, const float2* twL,
+                       const float2* twM, hipStream_t s);
+void launch_fill_int(int* p, long n, int v, hipStream_t s);
+// one launch for all maps of a psi: pass 1 and pass 2 with in-kernel
+// dependencies through a ring of `ring` intermediates (lag < ring);
+// counters: 2 nmaps ints (zeroed by the launcher); *err set if a wait ran out
+void launch_corr_stream(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
+                        int nmaps, int lag, int ring, float2* Gring, float2* psum, int* counters,
+                        float lo, float hi, CorrPartial* part, int* err, const float2* twL,
+                        const float2* twM, hipStream_t s);
+// multiply tile half spectra by (-1)^(kx+ky) sgn (compute_corner_CCG's factor)
+void launch_checker_sign(int L, float2* spec, int ntiles, hipStream_t s);
 int corr_ncb(int L);
 // Correlation pass 2: row C2R of G and the border-excluded max / sums, one
 // CorrPartial per (map, row block): part[m * corr_nparts(L) + block].

@@ -116,12 +116,12 @@ typedef struct gs_timing {
   double pick_total_ms;        /* whole last gs_pick */
   double micrograph_prep_ms;   /* last gs_set_micrograph(s): upload, prep, tile FFTs */
   /* With gs_set_profiling on, every correlation launch of the first psi of
-   * a gs_pick is bracketed by events on the stream it runs on: */
-  double corr_pass1_ms;        /* summed duration of the sampled pass-1 launches */
-  double corr_pass2_ms;        /* summed duration of the sampled pass-2 launches */
-  long long corr_pass1_launches;
-  long long corr_pass2_launches;
-  long long corr_sampled_maps; /* CC maps those launches covered (each pass) */
+   * a gs_pick is bracketed by events on the context stream.  Each launch
+   * runs pass 1 of one batch of CC maps and pass 2 of the previous batch;
+   * the first launch of a psi has no pass 2 and the last no pass 1. */
+  double corr_ms;              /* summed duration of the sampled launches */
+  long long corr_launches;
+  long long corr_sampled_maps; /* CC maps those launches covered (both passes) */
   long long corr_maps;         /* CC maps evaluated by the last gs_pick */
 } gs_timing;
 
@@ -199,6 +199,17 @@ int gs_get_timing(void* ctx, gs_timing* t);
 int gs_set_profiling(void* ctx, int enable);
 /* Number of CC-map evaluations one gs_pick launch batch holds (tuning). */
 int gs_set_batch(void* ctx, int maps_per_batch);
+/* Correlation schedule (norm_type 0, fused mode):
+ *   GS_PIPE_STREAM: one launch per psi; pass 2 of map m - lag runs beside
+ *     pass 1 of map m, the intermediates in a ring of `ring` maps (default);
+ *   GS_PIPE_LAUNCH: one launch per batch (gs_set_batch), pass 1 of batch k
+ *     beside pass 2 of batch k - 1.
+ * lag, ring <= 0 select the defaults (16, 3 lag); ring must exceed lag. */
+enum { GS_PIPE_STREAM = 0, GS_PIPE_LAUNCH = 1 };
+int gs_set_pipeline(void* ctx, int mode, int lag, int ring);
+/* Tiles per block of the norm_type 0 processing order (tuning; the default,
+ * all tiles in one block, is view-major). */
+int gs_set_tile_block(void* ctx, int tiles);
 int gs_destroy(void* ctx);
 const char* gs_strerror(int code);
 
