@@ -154,7 +154,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="CC maps per launch (0 = library default)")
     ap.add_argument("--graphs", action="store_true", help="replay per-psi HIP graphs")
     ap.add_argument("--tile-block", type=int, default=0, help="tiles per block (0 = default)")
-    ap.add_argument("--pipeline", type=int, default=0, help="0: one launch per psi, 1: per batch")
+    ap.add_argument("--pipeline", type=int, default=2,
+                    help="0: one launch per psi, 1: one launch per batch, 2: two streams")
     ap.add_argument("--lag", type=int, default=0)
     ap.add_argument("--ring", type=int, default=0)
     ap.add_argument("--threshold", type=float, default=15.0)
@@ -210,12 +211,14 @@ def main():
         log(f"[bench] warmup {i}: {time.time() - t:.2f}s, {len(picks)} picks")
     hip.sync()
     t0 = time.time()
-    tms = 0.0
+    tms = tms2 = tpsi = 0.0
     nl = nmaps = 0
     for i in range(args.steps):
         picks = step()
         tm = pk.timing()
         tms += tm.corr_ms
+        tms2 += tm.corr_pass2_ms
+        tpsi += tm.corr_psi_ms
         nl += tm.corr_launches
         nmaps += tm.corr_sampled_maps
         log(f"[bench] step {i}: {time.time() - t0:.2f}s cumulative, {len(picks)} picks, "
@@ -232,17 +235,31 @@ def main():
     H = L // 2 + 1
     pk.close()
 
-    # roofline of the correlation kernel (k_corr_fused: pass 1 of one batch +
-    # pass 2 of the previous one per launch): every launch of the first psi
-    # of each timed step bracketed by hipEvents on the library's stream.
-    # Algorithmic bytes per CC map: 24 L H (template half spectrum read 8 L H,
-    # intermediate written and read back 16 L H).
-    t_launch = tms / max(nl, 1)                # ms per launch
-    mpl = nmaps / max(nl, 1)                   # CC maps (both passes) per launch
-    bytes_launch = 24.0 * L * H * mpl
-    ach = bytes_launch / (t_launch * 1e-3) / 1e9
-    log(f"[bench] k_corr_fused {t_launch:.4f} ms/launch, {mpl:.1f} maps/launch, "
-        f"{ach:.0f} GB/s algorithmic, {nl} launches sampled")
+    # Roofline of the correlation kernel, from hipEvents bracketing every
+    # correlation launch of the first psi of each timed step on the stream it
+    # runs on.  Algorithmic bytes per CC map: 24 L H = pass 1 (template half
+    # spectrum read 8 L H + intermediate written 8 L H) + pass 2 (intermediate
+    # read 8 L H).
+    mpl = nmaps / max(nl, 1)                   # CC maps per launch
+    if args.pipeline == 2:                     # separate pass-1 / pass-2 launches
+        t1, t2 = tms / max(nl, 1), tms2 / max(nl, 1)
+        a1 = 16.0 * L * H * mpl / (t1 * 1e-3) / 1e9
+        a2 = 8.0 * L * H * mpl / (t2 * 1e-3) / 1e9
+        kern, t_launch, ach, bpm = (("k_corr_pass1", t1, a1, 16.0 * L * H) if t1 >= t2 else
+                                    ("k_corr_pass2", t2, a2, 8.0 * L * H))
+        log(f"[bench] k_corr_pass1 {t1:.4f} ms/launch ({a1:.0f} GB/s), k_corr_pass2 {t2:.4f} "
+            f"ms/launch ({a2:.0f} GB/s), {mpl:.1f} maps/launch (two streams overlap)")
+    else:
+        kern, t_launch, bpm = ("k_corr_stream" if args.pipeline == 0 else "k_corr_fused",
+                               tms / max(nl, 1), 24.0 * L * H)
+        ach = bpm * mpl / (t_launch * 1e-3) / 1e9
+        log(f"[bench] {kern} {t_launch:.4f} ms/launch, {mpl:.1f} maps/launch, "
+            f"{ach:.0f} GB/s algorithmic, {nl} launches sampled")
+    # both passes of the sampled psi against its correlation wall time
+    pair = (24.0 * L * H * nmaps / (tpsi * 1e-3) / 1e9) if tpsi > 0 else None
+    if pair:
+        log(f"[bench] psi correlation: {tpsi / max(args.steps, 1):.1f} ms, {pair:.0f} GB/s "
+            f"algorithmic ({pair / HBM_PEAK_GBS:.3f} of HBM peak)")
 
     plain = None
     if not args.no_plain:
@@ -274,12 +291,14 @@ def main():
                    "views": N, "psi": 72, "tiles": T, "window": L, "psi_timed": npsi,
                    "maps_per_micrograph": N * 72 * T, "threshold": args.threshold,
                    "mode": "fused" + ("+graphs" if args.graphs else "")},
-        "roofline": {"bound": "hbm", "kernel": "k_corr_fused", "achieved": ach,
+        "roofline": {"bound": "hbm", "kernel": kern, "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_PER_MAP_720 * mpl) if (L == 720 and TRAFFIC_PER_MAP_720)
+                     "traffic": (TRAFFIC_PER_MAP_720[kern] * mpl)
+                     if (L == 720 and TRAFFIC_PER_MAP_720 and kern in TRAFFIC_PER_MAP_720)
                      else None,
-                     "launch_ms": t_launch, "maps_per_launch": mpl,
-                     "bytes_per_map": 24.0 * L * H},
+                     "launch_ms": t_launch, "maps_per_launch": mpl, "bytes_per_map": bpm,
+                     "psi_correlation_achieved": pair,
+                     "psi_correlation_frac": (pair / HBM_PEAK_GBS) if pair else None},
         "plain": plain,
         "speedup_vs_plain": (mph / plain["value"]) if plain else None,
         "cpu_baseline": cpu,

@@ -47,7 +47,8 @@ class GsInfo(ctypes.Structure):
 
 
 class GsTiming(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_double) for n in ("pick_total_ms", "micrograph_prep_ms", "corr_ms")] + \
+    _fields_ = [(n, ctypes.c_double) for n in ("pick_total_ms", "micrograph_prep_ms", "corr_ms",
+                                               "corr_pass2_ms", "corr_psi_ms")] + \
                [(n, ctypes.c_longlong) for n in ("corr_launches", "corr_sampled_maps", "corr_maps")]
 
 
@@ -61,6 +62,7 @@ GS_MODE_FUSED = 0
 GS_MODE_PLAIN = 1
 GS_PIPE_STREAM = 0
 GS_PIPE_LAUNCH = 1
+GS_PIPE_LANES = 2
 
 _lib = None
 

@@ -93,10 +93,15 @@ struct Ctx {
   static constexpr int NLANE = 2;
   int batch = 0;
   DevBuf G[NLANE];
-  // GS_PIPE_STREAM: one launch per psi, pass 2 of map m - lag beside pass 1
-  // of map m, intermediates in a ring of `ring` maps
-  int pipeline = GS_PIPE_STREAM, lag = 16, ring = 48;
+  // GS_PIPE_STREAM: one launch per psi; each XCD runs pass 2 of its map
+  // `lag` rounds back beside pass 1 of its current map, intermediates in a
+  // ring of `ring` slots per XCD
+  int pipeline = GS_PIPE_LANES, lag = 2, ring = 4;  // lag, ring: rounds of 8 maps
   DevBuf Gring, counters, err;
+  // GS_PIPE_LANES: two streams, each running pass 1 then pass 2 of every
+  // other batch
+  hipStream_t lane_stream[NLANE] = {nullptr, nullptr};
+  hipEvent_t ev_ready = nullptr, ev_lane[NLANE] = {nullptr, nullptr};
   DevBuf part;  // row-block partials of every map of one psi
   DevBuf psum;  // column-block Parseval sums of every map of one psi
   DevBuf maps, stats, sel, sel_count;
@@ -484,14 +489,15 @@ void drop_graphs(Ctx* c) {
 int ensure_corr_buffers(Ctx* c) {
   const int L = c->L;
   const long per_map = c->geo.gmap * (long)sizeof(float2);
-  if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (128l << 20) / per_map));
+  // default batch: about 384 MB of intermediate per stream (189 maps at L = 720)
+  if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (384l << 20) / per_map));
   const int nb = std::min(c->batch, c->T * c->N);
   const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
   // G[0] serves norm_type 1 and the kernel timing in every schedule; G[1]
   // only the launch-per-batch schedule
   CHK(c->G[0].ensure((size_t)nb * per_map));
   if (c->pipeline == GS_PIPE_STREAM) {
-    CHK(c->Gring.ensure((size_t)c->ring * per_map));
+    CHK(c->Gring.ensure((size_t)8 * c->ring * per_map));
     CHK(c->counters.ensure((size_t)2 * c->T * c->N * sizeof(int)));
   } else {
     CHK(c->G[1].ensure((size_t)nb * per_map));
@@ -539,7 +545,7 @@ int ensure_corr_buffers(Ctx* c) {
 int correlate_psi(Ctx* c, MapStat* out) {
   if (!c->psum.p || !c->part.p || !c->maps_blk.p || !c->err.p) return GS_ERR_STATE;
   if (c->pipeline == GS_PIPE_STREAM ? (!c->Gring.p || !c->counters.p) : (!c->G[0].p || !c->G[1].p))
-    return GS_ERR_STATE;
+    return GS_ERR_STATE;  // ensure_corr_buffers allocates what the schedule uses
   const int L = c->L;
   const int nm = c->T * c->N;
   const int nb = std::min(c->batch, nm);
@@ -568,6 +574,48 @@ int correlate_psi(Ctx* c, MapStat* out) {
     launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
                          c->maps_blk.as<MapRef>(), c->N, out, s);
     return GS_OK;
+  }
+  if (c->pipeline == GS_PIPE_LANES) {
+    if (c->profile_now) {
+      while (c->prof_ev.size() < 3 * (size_t)nbatch + 2) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->prof_ev.push_back(e);
+      }
+      HIPCHK(hipEventRecord(c->prof_ev[3 * nbatch], s));
+    }
+    // the lanes start after the rotated templates of this psi are ready
+    HIPCHK(hipEventRecord(c->ev_ready, s));
+    for (int i = 0; i < Ctx::NLANE; ++i) HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
+    for (int k = 0; k < nbatch; ++k) {
+      const int m0 = k * nb, cnt = std::min(nb, nm - m0);
+      hipStream_t ls = c->lane_stream[k % 2];
+      if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k], ls));
+      launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
+                        c->maps_blk.as<MapRef>() + m0, cnt, c->G[k % 2].as<float2>(),
+                        c->psum.as<float2>() + (size_t)m0 * ncb, c->twL.as<float2>(), ls);
+      if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k + 1], ls));
+      launch_corr_pass2(L, c->G[k % 2].as<float2>(), cnt, lo, hi,
+                        c->part.as<CorrPartial>() + (size_t)m0 * np, c->twL.as<float2>(),
+                        c->twM.as<float2>(), ls);
+      if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k + 2], ls));
+    }
+    if (c->profile_now) c->prof_batches = nbatch;
+    for (int i = 0; i < Ctx::NLANE; ++i) {
+      HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
+      HIPCHK(hipStreamWaitEvent(s, c->ev_lane[i], 0));
+    }
+    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * nbatch + 1], s));
+    launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
+                         c->maps_blk.as<MapRef>(), c->N, out, s);
+    return GS_OK;
+  }
+  if (c->profile_now) {
+    while (c->prof_ev.size() < 2 * (size_t)(nbatch + 1)) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->prof_ev.push_back(e);
+    }
   }
   for (int k = 0; k <= nbatch; ++k) {
     const int m1 = k * nb, cnt1 = k < nbatch ? std::min(nb, nm - m1) : 0;
@@ -853,7 +901,11 @@ int gs_create(const gs_params* p, int device, void** out) {
   c->p = *p;
   c->device = device;
   c->L_config = p->window_size;
-  const bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int i = 0; ok && i < Ctx::NLANE; ++i)
+    ok = hipStreamCreateWithFlags(&c->lane_stream[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&c->ev_lane[i], hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
     delete c;
@@ -870,6 +922,11 @@ int gs_destroy(void* ctx) {
   (void)hipDeviceSynchronize();
   drop_graphs(c);
   for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+  for (int i = 0; i < Ctx::NLANE; ++i) {
+    if (c->lane_stream[i]) (void)hipStreamDestroy(c->lane_stream[i]);
+    if (c->ev_lane[i]) (void)hipEventDestroy(c->ev_lane[i]);
+  }
+  if (c->ev_ready) (void)hipEventDestroy(c->ev_ready);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return GS_OK;
@@ -957,9 +1014,9 @@ int gs_set_graphs(void* ctx, int enable) {
 
 int gs_set_pipeline(void* ctx, int mode, int lag, int ring) {
   Ctx* c = static_cast<Ctx*>(ctx);
-  if (!c || (mode != GS_PIPE_STREAM && mode != GS_PIPE_LAUNCH)) return GS_ERR_ARG;
-  if (lag <= 0) lag = 16;
-  if (ring <= 0) ring = 3 * lag;
+  if (!c || mode < GS_PIPE_STREAM || mode > GS_PIPE_LANES) return GS_ERR_ARG;
+  if (lag <= 0) lag = 2;
+  if (ring <= 0) ring = 2 * lag;
   if (ring <= lag) return GS_ERR_ARG;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipDeviceSynchronize());
@@ -1016,6 +1073,7 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   c->stats_npsi = npsi;
   c->timing.corr_maps = npsi * per_psi;
   c->timing.corr_ms = 0;
+  c->timing.corr_pass2_ms = 0;
   c->timing.corr_launches = 0;
   c->timing.corr_sampled_maps = 0;
   c->prof_batches = 0;
@@ -1053,12 +1111,25 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   c->timing.pick_total_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  const int evs = c->pipeline == GS_PIPE_LANES ? 3 : 2;
   for (int ib = 0; ib < c->prof_batches; ++ib) {
-    float a = 0;
-    HIPCHK(hipEventElapsedTime(&a, c->prof_ev[2 * ib], c->prof_ev[2 * ib + 1]));
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, c->prof_ev[evs * ib], c->prof_ev[evs * ib + 1]));
     c->timing.corr_ms += a;
+    if (evs == 3) {
+      HIPCHK(hipEventElapsedTime(&b, c->prof_ev[evs * ib + 1], c->prof_ev[evs * ib + 2]));
+      c->timing.corr_pass2_ms += b;
+    }
   }
   c->timing.corr_launches = c->prof_batches;
+  if (c->prof_batches && evs == 3) {
+    float w = 0;
+    HIPCHK(hipEventElapsedTime(&w, c->prof_ev[3 * c->prof_batches],
+                               c->prof_ev[3 * c->prof_batches + 1]));
+    c->timing.corr_psi_ms = w;
+  } else {
+    c->timing.corr_psi_ms = c->timing.corr_ms;
+  }
   if (c->prof_batches) c->timing.corr_sampled_maps = per_psi;
   if (count > capsel) return GS_ERR_DEVICE;  // cannot happen: one record per map at most
   std::vector<SelRec> recs(count);

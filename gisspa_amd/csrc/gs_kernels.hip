@@ -677,17 +677,20 @@ __device__ __forceinline__ void signal_count(int* ctr) {
   if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Whole-psi correlation in one launch.  Step s holds pass 1 of map s and
-// pass 2 of map s - lag as groups of 8 workgroups (one per XCD), the two kinds
-// alternating.  The intermediate lives in a ring of `ring` map slots:
-//  * pass 2 of map m waits until the NCB pass-1 blocks of m have signalled
+// Whole-psi correlation in one launch.  Map m belongs to XCD m % 8: all its
+// pass-1 and pass-2 workgroups have blockIdx = 8 i + m % 8, which the
+// round-robin dispatch places on that XCD, so the XCDs never wait on each
+// other.  XCD x walks rounds r = 0, 1, ...; round r holds pass 1 of map
+// 8 r + x and pass 2 of map 8 (r - lag) + x, the two kinds alternating.  The
+// intermediate lives in ring slots ((m / 8) % ring) * 8 + m % 8:
+//  * pass 2 of map m waits until the NCB pass-1 blocks of m signalled
 //    ready[m];
-//  * pass 1 of map m (slot m % ring) waits until the NP pass-2 blocks of map
-//    m - ring have signalled done[m - ring].
-// Every wait is on workgroups of lower index (lag < ring), which in-order
-// dispatch has already placed on the GPU, so the waits always end; they are
-// bounded anyway (err).  Intermediate accesses are device-coherent because
-// producer and consumer may sit on different XCDs.
+//  * pass 1 of map m waits until the NP pass-2 blocks of map m - 8 ring
+//    signalled done[m - 8 ring].
+// Every wait is on workgroups of lower index on the same XCD (lag < ring),
+// which in-order dispatch has already placed, so the waits end; they are
+// bounded anyway (err).  Intermediate accesses are device-coherent, which
+// does not depend on that placement.
 template <int L>
 __global__ __launch_bounds__(WG) void k_corr_stream(
     const float2* __restrict__ tiles, const float2* __restrict__ tspec,
@@ -698,32 +701,35 @@ __global__ __launch_bounds__(WG) void k_corr_stream(
   using G = Geo<L>;
   constexpr int NCB = G::NCB, NCB8 = (NCB + 7) / 8;
   constexpr int NP = (L + G::RB - 1) / G::RB;
-  constexpr int G1 = NCB8, G2 = (NP + 7) / 8, GS = G1 + G2, MN = G1 < G2 ? G1 : G2;
+  constexpr int S1 = 8 * NCB8, S2 = 8 * ((NP + 7) / 8);  // slots per round of each kind
+  constexpr int SR = S1 + S2, MN = S1 < S2 ? S1 : S2;
   constexpr int LDS = corr_lds1<L>() > corr_lds2<L>() ? corr_lds1<L>() : corr_lds2<L>();
   __shared__ float2 lds[LDS];
   __shared__ float red_f[WG / 64];
   __shared__ int red_i[WG / 64];
-  const int grp = blockIdx.x >> 3, x = blockIdx.x & 7;
-  const int step = grp / GS, gl = grp % GS;
-  int kind, gi;
-  if (gl < 2 * MN) {
-    kind = gl & 1;
-    gi = gl >> 1;
+  const int x = blockIdx.x & 7, i = blockIdx.x >> 3;
+  const int r = i / SR, li = i % SR;
+  int kind, k;
+  if (li < 2 * MN) {
+    kind = li & 1;
+    k = li >> 1;
   } else {
-    kind = G1 > G2 ? 0 : 1;
-    gi = gl - MN;
+    kind = S1 > S2 ? 0 : 1;
+    k = li - MN;
   }
   if (kind == 0) {
-    const int m = step, cb = x + 8 * gi;
+    const int m = 8 * r + x, cb = k;
     if (m >= nmaps || cb >= NCB) return;
-    if (m >= ring) wait_count(done + (m - ring), NP, err);
-    corr_pass1_body<L, true>(lds, red_f, cb, m, m % ring, tiles, tspec, maps, Gring, psum, twL);
+    if (m >= 8 * ring) wait_count(done + (m - 8 * ring), NP, err);
+    corr_pass1_body<L, true>(lds, red_f, cb, m, (r % ring) * 8 + x, tiles, tspec, maps, Gring,
+                             psum, twL);
     signal_count(ready + m);
   } else {
-    const int m = step - lag, rb = gi * 8 + x;
-    if (m < 0 || m >= nmaps || rb >= NP) return;
+    const int m = 8 * (r - lag) + x, rb = k;
+    if (r < lag || m >= nmaps || rb >= NP) return;
     wait_count(ready + m, NCB, err);
-    corr_pass2_body<L, true>(lds, red_f, red_i, m, m % ring, rb, Gring, blo, bhi, part, NP, twM);
+    corr_pass2_body<L, true>(lds, red_f, red_i, m, ((r - lag) % ring) * 8 + x, rb, Gring, blo,
+                             bhi, part, NP, twM);
     signal_count(done + m);
   }
 }
@@ -1253,11 +1259,12 @@ static void corr_stream_t(const float2* tiles, const float2* tspec, const MapRef
                           const float2* twM, hipStream_t s) {
   using G = Geo<L>;
   constexpr int NP = (L + G::RB - 1) / G::RB;
-  constexpr int GS = (G::NCB + 7) / 8 + (NP + 7) / 8;
+  constexpr int SR = 8 * ((G::NCB + 7) / 8) + 8 * ((NP + 7) / 8);
   const long nctr = 2l * nmaps;
   hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((nctr + 255) / 256)), dim3(256), 0, s, ready,
                      nctr, 0);
-  const long nwg = 8l * GS * (nmaps + lag);
+  const long rounds = (nmaps + 7) / 8 + lag;
+  const long nwg = 8l * SR * rounds;
   hipLaunchKernelGGL((k_corr_stream<L>), dim3((unsigned)nwg), dim3(WG), 0, s, tiles, tspec, maps,
                      nmaps, lag, ring, Gring, psum, ready, done, blo, bhi, part, err, twL, twM);
 }

@@ -116,10 +116,13 @@ typedef struct gs_timing {
   double pick_total_ms;        /* whole last gs_pick */
   double micrograph_prep_ms;   /* last gs_set_micrograph(s): upload, prep, tile FFTs */
   /* With gs_set_profiling on, every correlation launch of the first psi of
-   * a gs_pick is bracketed by events on the context stream.  Each launch
-   * runs pass 1 of one batch of CC maps and pass 2 of the previous batch;
-   * the first launch of a psi has no pass 2 and the last no pass 1. */
-  double corr_ms;              /* summed duration of the sampled launches */
+   * a gs_pick is bracketed by events on the stream it runs on (see
+   * gs_set_pipeline for what one launch holds). */
+  double corr_ms;              /* summed duration of the sampled launches (pass-1
+                                  launches under GS_PIPE_LANES) */
+  double corr_pass2_ms;        /* GS_PIPE_LANES: summed pass-2 launch durations */
+  double corr_psi_ms;          /* the sampled psi's whole correlation, first launch
+                                  start to last launch end */
   long long corr_launches;
   long long corr_sampled_maps; /* CC maps those launches covered (both passes) */
   long long corr_maps;         /* CC maps evaluated by the last gs_pick */
@@ -143,7 +146,8 @@ int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const 
 
 /* Upload and prepare a micrograph (replaces init_d_image,
  * split_normalize_image and fft_on_img, main.cu:523-655, 758-769).
- * host: ny rows of nx floats.  The first call fixes nx, ny. */
+ * host: ny rows of nx floats (any size; the reference requires every
+ * micrograph to have the first one's size, main.cu:1108-1109). */
 int gs_set_micrograph(void* ctx, const float* host, int nx, int ny, const gs_ctf* c);
 /* Same, with the image already in device memory (left unmodified). */
 int gs_set_micrograph_device(void* ctx, const float* dev, int nx, int ny, const gs_ctf* c);
@@ -200,12 +204,17 @@ int gs_set_profiling(void* ctx, int enable);
 /* Number of CC-map evaluations one gs_pick launch batch holds (tuning). */
 int gs_set_batch(void* ctx, int maps_per_batch);
 /* Correlation schedule (norm_type 0, fused mode):
- *   GS_PIPE_STREAM: one launch per psi; pass 2 of map m - lag runs beside
- *     pass 1 of map m, the intermediates in a ring of `ring` maps (default);
+ *   GS_PIPE_STREAM: one launch per psi; map m runs on XCD m % 8, which
+ *     processes its maps in rounds, pass 2 of the map `lag` rounds back
+ *     beside pass 1 of the current one, the intermediates in a ring of `ring`
+ *     slots per XCD;
  *   GS_PIPE_LAUNCH: one launch per batch (gs_set_batch), pass 1 of batch k
- *     beside pass 2 of batch k - 1.
- * lag, ring <= 0 select the defaults (16, 3 lag); ring must exceed lag. */
-enum { GS_PIPE_STREAM = 0, GS_PIPE_LAUNCH = 1 };
+ *     beside pass 2 of batch k - 1;
+ *   GS_PIPE_LANES: separate pass-1 and pass-2 launches per batch on two
+ *     streams, alternate batches per stream (default).
+ * lag, ring <= 0 select the defaults (2, 2 lag); ring must exceed lag.
+ * All schedules give bit-identical results. */
+enum { GS_PIPE_STREAM = 0, GS_PIPE_LAUNCH = 1, GS_PIPE_LANES = 2 };
 int gs_set_pipeline(void* ctx, int mode, int lag, int ring);
 /* Tiles per block of the norm_type 0 processing order (tuning; the default,
  * all tiles in one block, is view-major). */
