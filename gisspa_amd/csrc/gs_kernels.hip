@@ -53,18 +53,14 @@ __device__ __forceinline__ int pair_slot(int k, int y) {
   return (cb * L + y) * CB + c;
 }
 
-// streaming accesses of the correlation intermediate: written once by pass 1,
-// read once by pass 2, so they should not displace the spectra in L2
+// streaming stores of the correlation intermediate (written once by pass 1,
+// read once by pass 2), so they do not displace the spectra in L2
 typedef float nt_float2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void store_nt(float2* p, float2 v) {
   nt_float2 t;
   t.x = v.x;
   t.y = v.y;
   __builtin_nontemporal_store(t, reinterpret_cast<nt_float2*>(p));
-}
-__device__ __forceinline__ float2 load_nt(const float2* p) {
-  const nt_float2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_float2*>(p));
-  return make_float2(t.x, t.y);
 }
 
 // device-coherent accesses (agent-scope relaxed atomics: stores written
@@ -549,7 +545,9 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
   auto load = [&](int r, int k) -> float2 {
     const float2* q = gb + pair_slot<L>(k, min(y0 + r, L - 1));
     if constexpr (COH) return load_coh(q);
-    else return load_nt(q);
+    // plain loads: the two halves of a 64-byte slot row are read by
+    // different loads, and a cached line serves both
+    else return *q;
   };
   // z'[n] = (c[y][2n], c[y][2n+1]).  Border pixels (x or y outside
   // [blo, bhi], the integer form of get_peak_and_SUM's d/4 test) count as 0.
