@@ -40,6 +40,15 @@ class GsParticle(ctypes.Structure):
                 ("score", ctypes.c_float), ("mic", ctypes.c_int)]
 
 
+class GsPeak(ctypes.Structure):
+    _fields_ = [("mic", ctypes.c_int), ("tile", ctypes.c_int), ("view", ctypes.c_int)] + \
+               [(n, ctypes.c_float) for n in ("cx", "cy", "psi", "dx", "dy", "defocus")]
+
+
+class GsGrad(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("score", "d_psi", "d_dx", "d_dy", "d_defocus")]
+
+
 class GsInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in
                 ("l", "n_views", "n_psi", "nx", "ny", "tiles_x", "tiles_y", "box_x", "box_y",
@@ -57,6 +66,32 @@ MAPSTAT_DTYPE = np.dtype([("score", "<f4"), ("pos", "<i4"), ("peak", "<f4"), ("s
 PARTICLE_DTYPE = np.dtype([("psi_index", "<i4"), ("view", "<i4"), ("tile", "<i4"),
                            ("psi", "<f4"), ("cx", "<f4"), ("cy", "<f4"), ("score", "<f4"),
                            ("mic", "<i4")])
+
+PEAK_DTYPE = np.dtype([("mic", "<i4"), ("tile", "<i4"), ("view", "<i4"), ("cx", "<f4"),
+                       ("cy", "<f4"), ("psi", "<f4"), ("dx", "<f4"), ("dy", "<f4"),
+                       ("defocus", "<f4")])
+GRAD_DTYPE = np.dtype([("score", "<f4"), ("d_psi", "<f4"), ("d_dx", "<f4"), ("d_dy", "<f4"),
+                       ("d_defocus", "<f4")])
+
+
+def psi_value(phi_step: float, index: int) -> float:
+    """euler3 of in-plane step `index`: the reference's float accumulator
+    (main.cu:1230)."""
+    e = np.float32(0.0)
+    for _ in range(index):
+        e = np.float32(e + np.float32(phi_step))
+    return float(e)
+
+
+def peaks_from_picks(picks: np.ndarray, phi_step: float, defocus: float) -> np.ndarray:
+    """Refinement candidates (PEAK_DTYPE) at the picked grid parameters."""
+    out = np.zeros(len(picks), dtype=PEAK_DTYPE)
+    for n in ("mic", "tile", "view", "cx", "cy"):
+        out[n] = picks[n]
+    out["psi"] = [psi_value(phi_step, int(i)) for i in picks["psi_index"]]
+    out["defocus"] = defocus
+    return out
+
 
 GS_MODE_FUSED = 0
 GS_MODE_PLAIN = 1
@@ -112,6 +147,8 @@ def load() -> ctypes.CDLL:
     lib.gs_set_profiling.argtypes = [vp, ctypes.c_int]
     lib.gs_set_tile_block.argtypes = [vp, ctypes.c_int]
     lib.gs_set_pipeline.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.gs_adjoint.argtypes = [vp, vp, ctypes.c_int, vp]
+    lib.gs_refine.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
     lib.gs_time_kernels.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
     lib.gs_destroy.argtypes = [vp]
     _lib = lib
@@ -252,6 +289,22 @@ class Picker:
 
     def set_profiling(self, enable: bool) -> None:
         _check(self.lib.gs_set_profiling(self.ctx, int(bool(enable))), "gs_set_profiling")
+
+    def adjoint(self, peaks: np.ndarray) -> np.ndarray:
+        """Refinement objective and gradient (GRAD_DTYPE) of PEAK_DTYPE candidates."""
+        pk = np.ascontiguousarray(peaks, dtype=PEAK_DTYPE)
+        out = np.zeros(len(pk), dtype=GRAD_DTYPE)
+        _check(self.lib.gs_adjoint(self.ctx, pk.ctypes.data, len(pk), out.ctypes.data),
+               "gs_adjoint")
+        return out
+
+    def refine(self, peaks: np.ndarray, iters: int = 20):
+        """Refined candidates and their final (objective, gradient)."""
+        pk = np.array(peaks, dtype=PEAK_DTYPE, copy=True)
+        out = np.zeros(len(pk), dtype=GRAD_DTYPE)
+        _check(self.lib.gs_refine(self.ctx, pk.ctypes.data, len(pk), iters, out.ctypes.data),
+               "gs_refine")
+        return pk, out
 
     def set_pipeline(self, mode: int, lag: int = 0, ring: int = 0) -> None:
         _check(self.lib.gs_set_pipeline(self.ctx, mode, lag, ring), "gs_set_pipeline")

@@ -79,6 +79,15 @@ struct Ctx {
   int N = 0, bx = 0, by = 0;
   gs_ctf tctf{};
   DevBuf treal, tspec, rot_part, bscale;
+  DevBuf tbox;                            // the template boxes as uploaded
+  DevBuf bin_start, bin_pix, bin_count;   // radial bins of the half plane
+  int bins_L = 0;
+  DevBuf twhite;                          // whitened spectra (refinement)
+  bool white_ok = false;
+  // refinement batch buffers
+  int ref_cap = 0;
+  DevBuf r_cand, r_cs, r_A, r_gtmp, r_Tun, r_spec, r_rot, r_bscale, r_pre, r_gRn, r_gTb, r_part,
+      r_sc, r_out;
   double half_count = 0;  // pixels of the x <= l/2 half plane inside radius l/2
   // micrograph batch: tile pool of every micrograph, tile descriptors
   std::vector<MicSlot> mics;
@@ -211,56 +220,85 @@ WeightArgs weight_args(const Ctx* c, const gs_ctf& ctf) {
   return w;
 }
 
+// Template half spectra [N][kx][ky] from the device copy of the boxes
+// (readAndPaddingTemplate, edgeNormalize, handleTemplate: main.cu:182-474):
+// pad, sigma-normalise, FFT / L^2, whiten by the radial amplitude, and with
+// weight != 0 the low pass + |CTF| weighting and the amplitude normaliser.
+int template_spectra(Ctx* c, float2* spec, int weight) {
+  const int L = c->L, H = c->geo.H;
+  const int n = c->N, bx = c->bx, by = c->by;
+  hipStream_t s = c->stream;
+  const long L2 = (long)L * L;
+  DevBuf sums, scale, ra, wpart, inv_m;
+  CHK(sums.ensure((size_t)n * 2 * sizeof(double)));
+  CHK(scale.ensure((size_t)n * sizeof(float)));
+  launch_box_sums(c->tbox.as<float>(), bx * by, n, sums.as<double>(), s);
+  launch_sigma_scale(sums.as<double>(), n, L, scale.as<float>(), s);
+  launch_rows_r2c_pad(L, c->tbox.as<float>(), bx, by, scale.as<float>(), n, spec,
+                      c->twL.as<float2>(), c->twM.as<float2>(), s);
+  launch_cols_fwd(L, spec, n, (float)(1.0 / (double)L2), nullptr, 0, c->twL.as<float2>(), s);
+  // radial bins of the half plane x <= l/2 (SQRSum_by_circle, GPU_func.cu:98-128)
+  const int NBIN = L / 2;
+  if (c->bins_L != L) {
+    std::vector<std::vector<int>> bins(NBIN);
+    for (int kx = 0; kx <= L / 2; ++kx)
+      for (int ky = 0; ky < L; ++ky) {
+        const double dy = std::min(ky, L - ky), dx = std::min(kx, L - kx);
+        const int r = (int)std::floor(std::hypot(dy, dx) + 0.5) - 1;
+        if (r >= 0 && r < NBIN) bins[r].push_back(kx * L + ky);
+      }
+    std::vector<int> bstart(NBIN + 1, 0), bpix;
+    std::vector<float> rcount(NBIN);
+    double count_total = 0;
+    for (int r = 0; r < NBIN; ++r) {
+      bstart[r + 1] = bstart[r] + (int)bins[r].size();
+      bpix.insert(bpix.end(), bins[r].begin(), bins[r].end());
+      rcount[r] = (float)bins[r].size();
+      count_total += bins[r].size();
+    }
+    CHK(c->bin_start.ensure(bstart.size() * sizeof(int)));
+    CHK(c->bin_pix.ensure(std::max<size_t>(1, bpix.size()) * sizeof(int)));
+    CHK(c->bin_count.ensure(NBIN * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(c->bin_start.p, bstart.data(), bstart.size() * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->bin_pix.p, bpix.data(), bpix.size() * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->bin_count.p, rcount.data(), NBIN * sizeof(float),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));  // host tables are released on return
+    c->half_count = count_total;
+    c->bins_L = L;
+  }
+  CHK(ra.ensure((size_t)n * NBIN * sizeof(double)));
+  launch_radial_power(L, spec, n, c->bin_start.as<int>(), c->bin_pix.as<int>(), ra.as<double>(),
+                      s);
+  const long tot = (long)H * L;
+  const int nwp = (int)((tot + 255) / 256);
+  CHK(wpart.ensure((size_t)n * nwp * sizeof(double)));
+  CHK(inv_m.ensure((size_t)n * sizeof(float)));
+  launch_tmpl_weight(L, spec, n, ra.as<double>(), c->bin_count.as<float>(), weight_args(c, c->tctf),
+                     wpart.as<double>(), weight, s);
+  if (weight)
+    launch_tmpl_ampnorm(L, spec, n, wpart.as<double>(), c->half_count, inv_m.as<float>(), s);
+  HIPCHK(hipStreamSynchronize(s));  // the scratch buffers are released on return
+  HIPCHK(hipGetLastError());
+  return GS_OK;
+}
+
 int prep_templates(Ctx* c, const float* host, int n, int bx, int by) {
   const int L = c->L, H = c->geo.H, HP = c->geo.HP;
   hipStream_t s = c->stream;
   const long L2 = (long)L * L;
-  DevBuf box, sums, scale, ra, rbins, wpart, binstart, binpix, gtmp;
-  CHK(box.ensure((size_t)n * bx * by * sizeof(float)));
-  HIPCHK(hipMemcpyAsync(box.p, host, (size_t)n * bx * by * sizeof(float), hipMemcpyHostToDevice, s));
-  CHK(sums.ensure((size_t)n * 2 * sizeof(double)));
-  CHK(scale.ensure((size_t)n * sizeof(float)));
-  launch_box_sums(box.as<float>(), bx * by, n, sums.as<double>(), s);
-  launch_sigma_scale(sums.as<double>(), n, L, scale.as<float>(), s);
+  // the boxes stay on the device (the refinement rebuilds the whitened spectra)
+  CHK(c->tbox.ensure((size_t)n * bx * by * sizeof(float)));
+  HIPCHK(hipMemcpyAsync(c->tbox.p, host, (size_t)n * bx * by * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  c->white_ok = false;
   CHK(c->tspec.ensure((size_t)n * H * L * sizeof(float2)));
   float2* spec = c->tspec.as<float2>();
-  launch_rows_r2c_pad(L, box.as<float>(), bx, by, scale.as<float>(), n, spec, c->twL.as<float2>(),
-                      c->twM.as<float2>(), s);
-  launch_cols_fwd(L, spec, n, (float)(1.0 / (double)L2), nullptr, 0, c->twL.as<float2>(), s);
-  // radial bins of the half plane x <= l/2 (SQRSum_by_circle, GPU_func.cu:98-128)
-  const int NBIN = L / 2;
-  std::vector<std::vector<int>> bins(NBIN);
-  for (int kx = 0; kx <= L / 2; ++kx)
-    for (int ky = 0; ky < L; ++ky) {
-      const double dy = std::min(ky, L - ky), dx = std::min(kx, L - kx);
-      const int r = (int)std::floor(std::hypot(dy, dx) + 0.5) - 1;
-      if (r >= 0 && r < NBIN) bins[r].push_back(kx * L + ky);
-    }
-  std::vector<int> bstart(NBIN + 1, 0), bpix;
-  std::vector<float> rcount(NBIN);
-  double count_total = 0;
-  for (int r = 0; r < NBIN; ++r) {
-    bstart[r + 1] = bstart[r] + (int)bins[r].size();
-    bpix.insert(bpix.end(), bins[r].begin(), bins[r].end());
-    rcount[r] = (float)bins[r].size();
-    count_total += bins[r].size();
-  }
-  CHK(binstart.ensure(bstart.size() * sizeof(int)));
-  CHK(binpix.ensure(std::max<size_t>(1, bpix.size()) * sizeof(int)));
-  CHK(rbins.ensure(NBIN * sizeof(float)));
-  HIPCHK(hipMemcpyAsync(binstart.p, bstart.data(), bstart.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(binpix.p, bpix.data(), bpix.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(rbins.p, rcount.data(), NBIN * sizeof(float), hipMemcpyHostToDevice, s));
-  CHK(ra.ensure((size_t)n * NBIN * sizeof(double)));
-  launch_radial_power(L, spec, n, binstart.as<int>(), binpix.as<int>(), ra.as<double>(), s);
-  const long tot = (long)H * L;
-  const int nwp = (int)((tot + 255) / 256);
-  CHK(wpart.ensure((size_t)n * nwp * sizeof(double)));
-  launch_tmpl_weight(L, spec, n, ra.as<double>(), rbins.as<float>(), weight_args(c, c->tctf),
-                     wpart.as<double>(), s);
-  launch_tmpl_ampnorm(L, spec, n, wpart.as<double>(), count_total, s);
-  c->half_count = count_total;
+  CHK(template_spectra(c, spec, 1));
   // inverse 2-D transform to the real-space templates, in chunks
+  DevBuf gtmp;
   CHK(c->treal.ensure((size_t)n * L2 * sizeof(float)));
   const int chunk = std::max(1, std::min(n, (int)((512l << 20) / ((long)L * HP * 8))));
   CHK(gtmp.ensure((size_t)chunk * L * HP * sizeof(float2)));
@@ -272,6 +310,15 @@ int prep_templates(Ctx* c, const float* host, int n, int bx, int by) {
   }
   HIPCHK(hipStreamSynchronize(s));
   HIPCHK(hipGetLastError());
+  return GS_OK;
+}
+
+// whitened template half spectra for the refinement, built on first use
+int ensure_white(Ctx* c) {
+  if (c->white_ok) return GS_OK;
+  CHK(c->twhite.ensure((size_t)c->N * c->geo.H * c->L * sizeof(float2)));
+  CHK(template_spectra(c, c->twhite.as<float2>(), 0));
+  c->white_ok = true;
   return GS_OK;
 }
 
@@ -846,6 +893,98 @@ int plain_psi(Ctx* c, int ip, MapStat* out) {
   return GS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// refinement
+// ---------------------------------------------------------------------------
+constexpr int kRefBatch = 64;  // candidates per forward/backward launch sequence
+
+int ref_ensure(Ctx* c, int nb) {
+  if (c->ref_cap >= nb) return GS_OK;
+  const size_t L = c->L, H = c->geo.H, HP = c->geo.HP;
+  const size_t nrb = (L + c->geo.RB - 1) / c->geo.RB;
+  CHK(c->r_cand.ensure(nb * sizeof(RefCand)));
+  CHK(c->r_cs.ensure(nb * sizeof(float2)));
+  CHK(c->r_A.ensure(nb * H * L * sizeof(float2)));
+  CHK(c->r_gtmp.ensure(nb * L * HP * sizeof(float2)));
+  CHK(c->r_Tun.ensure(nb * L * L * sizeof(float)));
+  CHK(c->r_spec.ensure(nb * H * L * sizeof(float2)));
+  CHK(c->r_rot.ensure(nb * nrb * 2 * sizeof(double)));
+  CHK(c->r_bscale.ensure(nb * sizeof(float)));
+  CHK(c->r_pre.ensure(nb * sizeof(float)));
+  CHK(c->r_gRn.ensure(nb * L * L * sizeof(float)));
+  CHK(c->r_gTb.ensure(nb * L * L * sizeof(float)));
+  CHK(c->r_part.ensure(nb * (size_t)ref_parts() * 4 * sizeof(double)));
+  CHK(c->r_sc.ensure(nb * sizeof(RefScal)));
+  CHK(c->r_out.ensure(nb * 5 * sizeof(float)));
+  c->ref_cap = nb;
+  return GS_OK;
+}
+
+// candidates of the current batch in kernel form (checks every index)
+int ref_cands(const Ctx* c, const gs_peak* pk, int n, std::vector<RefCand>& rc,
+              std::vector<float2>& cs) {
+  rc.resize(n);
+  cs.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const gs_peak& p = pk[i];
+    if (p.mic < 0 || p.mic >= (int)c->mics.size() || p.view < 0 || p.view >= c->N) return GS_ERR_ARG;
+    const MicSlot& ms = c->mics[p.mic];
+    if (p.tile < 0 || p.tile >= ms.ntiles) return GS_ERR_ARG;
+    const int bxn = (int)ms.offx.size();
+    const float x0 = p.cx - (float)ms.offx[p.tile % bxn], y0 = p.cy - (float)ms.offy[p.tile / bxn];
+    if (!(x0 >= 0 && x0 < c->L && y0 >= 0 && y0 < c->L)) return GS_ERR_ARG;
+    if (!std::isfinite(p.psi) || !std::isfinite(p.dx) || !std::isfinite(p.dy) ||
+        !std::isfinite(p.defocus))
+      return GS_ERR_ARG;
+    rc[i] = RefCand{ms.tile0 + p.tile, p.view, x0, y0, p.psi, p.dx, p.dy, p.defocus};
+    const double a = (double)p.psi * kRefPiD / 180;  // rotate_subIMG's angle (GPU_func.cu:498-499)
+    cs[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return GS_OK;
+}
+
+// objective and gradient of n candidates: out[5 i ..] = score, d psi, d dx,
+// d dy, d defocus
+int ref_eval(Ctx* c, const RefCand* rc, const float2* cs, int n, float* out) {
+  CHK(ensure_white(c));
+  hipStream_t s = c->stream;
+  const int nb0 = std::min(n, kRefBatch);
+  CHK(ref_ensure(c, nb0));
+  for (int b0 = 0; b0 < n; b0 += kRefBatch) {
+    const int nb = std::min(kRefBatch, n - b0);
+    HIPCHK(hipMemcpyAsync(c->r_cand.p, rc + b0, nb * sizeof(RefCand), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->r_cs.p, cs + b0, nb * sizeof(float2), hipMemcpyHostToDevice, s));
+    RefArgs a{};
+    a.nb = nb;
+    a.white = c->twhite.as<float2>();
+    a.tiles = c->tiles.as<float2>();
+    a.cand = c->r_cand.as<RefCand>();
+    a.w = weight_args(c, c->tctf);
+    a.count = c->half_count;
+    a.dmask = 1.5f * c->p.diameter;
+    a.cs = c->r_cs.as<float2>();
+    a.A = c->r_A.as<float2>();
+    a.gtmp = c->r_gtmp.as<float2>();
+    a.Tun = c->r_Tun.as<float>();
+    a.spec = c->r_spec.as<float2>();
+    a.rot_part = c->r_rot.as<double>();
+    a.bscale = c->r_bscale.as<float>();
+    a.pre = c->r_pre.as<float>();
+    a.gRn = c->r_gRn.as<float>();
+    a.gTb = c->r_gTb.as<float>();
+    a.part = c->r_part.as<double>();
+    a.sc = c->r_sc.as<RefScal>();
+    a.out = c->r_out.as<float>();
+    a.twL = c->twL.as<float2>();
+    a.twM = c->twM.as<float2>();
+    launch_ref_forward(c->L, a, s);
+    launch_ref_backward(c->L, a, s);
+    CHK(copy_out(c, out + 5 * (size_t)b0, c->r_out.p, nb * 5 * sizeof(float)));  // synchronises
+  }
+  HIPCHK(hipGetLastError());
+  return GS_OK;
+}
+
 // Map the pool tile index t to (micrograph, tile within it).
 void tile_owner(const Ctx* c, int t, int* mic, int* local) {
   int lo = 0, hi = (int)c->mics.size() - 1;
@@ -944,6 +1083,7 @@ int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const 
   }
   CHK(set_window(c, L));
   drop_graphs(c);
+  c->ref_cap = 0;  // refinement buffers follow the window
   c->N = n; c->bx = bx; c->by = by; c->tctf = *tctf;
   c->have_mic = false;  // tiles depend on the window
   return prep_templates(c, host, n, bx, by);
@@ -1261,6 +1401,81 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
   out_ms[0] = a / reps;
   out_ms[1] = b / reps;
   *maps_per_launch = nb;
+  return GS_OK;
+}
+
+int gs_adjoint(void* ctx, const gs_peak* peaks, int n, gs_grad* out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || n < 0 || (n > 0 && (!peaks || !out))) return GS_ERR_ARG;
+  if (c->N == 0 || !c->have_mic) return GS_ERR_STATE;
+  if (c->mode != GS_MODE_FUSED) return GS_ERR_UNSUPPORTED;  // needs the half-spectrum tile pool
+  if (n == 0) return GS_OK;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<RefCand> rc;
+  std::vector<float2> cs;
+  CHK(ref_cands(c, peaks, n, rc, cs));
+  std::vector<float> r((size_t)n * 5);
+  CHK(ref_eval(c, rc.data(), cs.data(), n, r.data()));
+  for (int i = 0; i < n; ++i)
+    out[i] = gs_grad{r[5 * i], r[5 * i + 1], r[5 * i + 2], r[5 * i + 3], r[5 * i + 4]};
+  return GS_OK;
+}
+
+int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || n < 0 || iters < 0 || (n > 0 && !peaks)) return GS_ERR_ARG;
+  if (c->N == 0 || !c->have_mic) return GS_ERR_STATE;
+  if (c->mode != GS_MODE_FUSED) return GS_ERR_UNSUPPORTED;
+  if (n == 0) return GS_OK;
+  HIPCHK(hipSetDevice(c->device));
+  // scaled steepest ascent: a step moves `step` in units of (1 degree,
+  // 1 pixel, 1 pixel, 0.1 um) along the scaled gradient; accepted steps grow
+  // the step by 1.25, rejected ones halve it
+  const double scl[4] = {1.0, 1.0, 1.0, 0.1};
+  const double lim[4] = {c->p.phi_step, 1.5, 1.5, 0.5};
+  std::vector<gs_peak> cur(peaks, peaks + n), start(peaks, peaks + n), trial(n);
+  std::vector<RefCand> rc;
+  std::vector<float2> cs;
+  std::vector<float> fc((size_t)n * 5), ft((size_t)n * 5);
+  CHK(ref_cands(c, cur.data(), n, rc, cs));
+  CHK(ref_eval(c, rc.data(), cs.data(), n, fc.data()));
+  std::vector<double> step(n, 0.5);
+  for (int it = 0; it < iters; ++it) {
+    for (int i = 0; i < n; ++i) {
+      const float* g = &fc[5 * (size_t)i + 1];
+      double nrm = 0;
+      for (int k = 0; k < 4; ++k) nrm += (scl[k] * g[k]) * (scl[k] * g[k]);
+      nrm = std::sqrt(nrm);
+      double th[4] = {cur[i].psi, cur[i].dx, cur[i].dy, cur[i].defocus};
+      const double t0[4] = {start[i].psi, start[i].dx, start[i].dy, start[i].defocus};
+      for (int k = 0; k < 4; ++k) {
+        if (nrm > 0) th[k] += step[i] * scl[k] * scl[k] * g[k] / nrm;
+        th[k] = std::min(std::max(th[k], t0[k] - lim[k]), t0[k] + lim[k]);
+      }
+      trial[i] = cur[i];
+      trial[i].psi = (float)th[0];
+      trial[i].dx = (float)th[1];
+      trial[i].dy = (float)th[2];
+      trial[i].defocus = (float)th[3];
+    }
+    CHK(ref_cands(c, trial.data(), n, rc, cs));
+    CHK(ref_eval(c, rc.data(), cs.data(), n, ft.data()));
+    for (int i = 0; i < n; ++i) {
+      if (ft[5 * (size_t)i] > fc[5 * (size_t)i]) {
+        cur[i] = trial[i];
+        std::copy(&ft[5 * (size_t)i], &ft[5 * (size_t)i] + 5, &fc[5 * (size_t)i]);
+        step[i] *= 1.25;
+      } else {
+        step[i] *= 0.5;
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    peaks[i] = cur[i];
+    if (out)
+      out[i] = gs_grad{fc[5 * (size_t)i], fc[5 * (size_t)i + 1], fc[5 * (size_t)i + 2],
+                       fc[5 * (size_t)i + 3], fc[5 * (size_t)i + 4]};
+  }
   return GS_OK;
 }
 

@@ -1,0 +1,158 @@
+// gs_device.hpp -- device-side helpers shared by the HIP kernel files of
+// the picker (geometry of a tile size, wave64 reductions, the CTF).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+
+#include "gs_fft.hpp"
+#include "gs_kernels.hpp"
+
+namespace gs {
+
+constexpr int WG = 256;
+constexpr float kRefPi = 3.141592654f;  // the reference's PI (GPU_func.cuh:29)
+
+constexpr int pow2floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <int L>
+struct Geo {
+  static constexpr int M = L / 2;
+  static constexpr int H = M + 1;
+  static constexpr int HP = (H + 15) / 16 * 16;
+  static constexpr int RB = clampi(pow2floor(8192 / M), 8, 32);
+  static constexpr int CB = clampi(pow2floor(8192 / L), 8, 32);
+  // correlation column blocks pair column k with column M - k: block cb holds
+  // the pair units p = cb * CB/2 + u (u < CB/2) in slots u (column p) and
+  // CB-1-u (column M - p), p = 0 .. M/2
+  static constexpr int NCB = (M / 2 + 1 + CB / 2 - 1) / (CB / 2);
+  static_assert(L % 4 == 0, "tile size divisible by 4");
+};
+
+// column of slot c of paired block cb
+template <int L>
+__device__ __forceinline__ int pair_col(int cb, int c) {
+  constexpr int M = L / 2, CB = Geo<L>::CB;
+  const int u = c < CB / 2 ? c : CB - 1 - c;
+  const int p = cb * (CB / 2) + u;
+  return c < CB / 2 ? p : M - p;
+}
+// slot offset of column k (0 <= k < M) in its paired block: (cb * L + y) * CB + c
+template <int L>
+__device__ __forceinline__ int pair_slot(int k, int y) {
+  constexpr int M = L / 2, CB = Geo<L>::CB;
+  const int p = min(k, M - k);
+  const int u = p % (CB / 2), cb = p / (CB / 2);
+  const int c = (k <= M / 2) ? u : CB - 1 - u;
+  return (cb * L + y) * CB + c;
+}
+
+// streaming stores of the correlation intermediate (written once by pass 1,
+// read once by pass 2), so they do not displace the spectra in L2
+typedef float nt_float2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(float2* p, float2 v) {
+  nt_float2 t;
+  t.x = v.x;
+  t.y = v.y;
+  __builtin_nontemporal_store(t, reinterpret_cast<nt_float2*>(p));
+}
+
+// device-coherent accesses (agent-scope relaxed atomics: stores written
+// through to the coherence point, loads that do not hit stale L2 lines), for
+// an intermediate that a workgroup on another XCD consumes within the same
+// launch
+__device__ __forceinline__ void store_coh(float2* p, float2 v) {
+  const unsigned long long u =
+      ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float2 load_coh(const float2* p) {
+  const unsigned long long u = __hip_atomic_load(
+      reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
+}
+
+// value of lane (lane ^ (CB - 1)): the partner slot of a paired block when
+// CB consecutive lanes hold the CB slots of one row
+template <int CB>
+__device__ __forceinline__ float xchg_f(float v) {
+  if constexpr (CB == 8) {  // row_half_mirror
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
+  } else if constexpr (CB == 16) {  // row_mirror
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));
+  } else {
+    return __shfl_xor(v, CB - 1, 64);
+  }
+}
+template <int CB>
+__device__ __forceinline__ float2 xchg(float2 v) {
+  return make_float2(xchg_f<CB>(v.x), xchg_f<CB>(v.y));
+}
+
+// ---------------------------------------------------------------------------
+// wave / block reductions (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// (value, index) max with lowest index on ties
+__device__ __forceinline__ void vmax_combine(float& v, int& p, float v2, int p2) {
+  if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
+}
+__device__ __forceinline__ void wave_max_idx(float& v, int& p) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float v2 = __shfl_xor(v, o, 64);
+    int p2 = __shfl_xor(p, o, 64);
+    vmax_combine(v, p, v2, p2);
+  }
+}
+
+// block-wide FP64 sum of two values; result valid in thread 0
+__device__ __forceinline__ void block_sum2_d(double& a, double& b, double* scratch) {
+  a = wave_sum_d(a);
+  b = wave_sum_d(b);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) { scratch[2 * w] = a; scratch[2 * w + 1] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0, sb = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) { sa += scratch[2 * i]; sb += scratch[2 * i + 1]; }
+    a = sa; b = sb;
+  }
+}
+
+
+__device__ __forceinline__ float ctf_ast_f(int x1, int y1, int nx, int ny, const WeightArgs& w, int mode) {
+  // CTF_AST, GPU_func.cu:292-315
+  y1 = x1 > nx / 2 ? ny - y1 : y1;
+  x1 = min(x1, nx - x1);
+  const float dx = (float)min(x1, nx - x1);
+  const float dy = (float)(y1 - ny / 2);
+  const float ss = (dx * dx / (float)(nx * nx) + dy * dy / (float)(ny * ny)) / (w.apix * w.apix);
+  const float ag = atan2f((float)(y1 - ny / 2), (float)x1);
+  const float df_ast = 0.5f * (w.dfu + w.dfv + 2 * w.dfdiff * cosf(2 * (w.dfang * kRefPi / 180 - ag)));
+  const float gamma = -2 * kRefPi * (w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * ss * ss +
+                                     df_ast * 5000.0f * w.lambda * ss);
+  const float v = sqrtf(1.0f - w.ampconst * w.ampconst) * sinf(gamma) + w.ampconst * cosf(gamma);
+  if (mode == 0) return v > 0 ? 1.0f : -1.0f;
+  if (mode == 2) return fabsf(v);
+  return v;
+}
+
+}  // namespace gs

@@ -6,136 +6,11 @@
 #include <climits>
 #include <cmath>
 
+#include "gs_device.hpp"
 #include "gs_fft.hpp"
 #include "gs_kernels.hpp"
 
 namespace gs {
-
-constexpr int WG = 256;
-constexpr float kRefPi = 3.141592654f;  // the reference's PI (GPU_func.cuh:29)
-
-constexpr int pow2floor(int v) {
-  int p = 1;
-  while (p * 2 <= v) p *= 2;
-  return p;
-}
-constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-template <int L>
-struct Geo {
-  static constexpr int M = L / 2;
-  static constexpr int H = M + 1;
-  static constexpr int HP = (H + 15) / 16 * 16;
-  static constexpr int RB = clampi(pow2floor(8192 / M), 8, 32);
-  static constexpr int CB = clampi(pow2floor(8192 / L), 8, 32);
-  // correlation column blocks pair column k with column M - k: block cb holds
-  // the pair units p = cb * CB/2 + u (u < CB/2) in slots u (column p) and
-  // CB-1-u (column M - p), p = 0 .. M/2
-  static constexpr int NCB = (M / 2 + 1 + CB / 2 - 1) / (CB / 2);
-  static_assert(L % 4 == 0, "tile size divisible by 4");
-};
-
-// column of slot c of paired block cb
-template <int L>
-__device__ __forceinline__ int pair_col(int cb, int c) {
-  constexpr int M = L / 2, CB = Geo<L>::CB;
-  const int u = c < CB / 2 ? c : CB - 1 - c;
-  const int p = cb * (CB / 2) + u;
-  return c < CB / 2 ? p : M - p;
-}
-// slot offset of column k (0 <= k < M) in its paired block: (cb * L + y) * CB + c
-template <int L>
-__device__ __forceinline__ int pair_slot(int k, int y) {
-  constexpr int M = L / 2, CB = Geo<L>::CB;
-  const int p = min(k, M - k);
-  const int u = p % (CB / 2), cb = p / (CB / 2);
-  const int c = (k <= M / 2) ? u : CB - 1 - u;
-  return (cb * L + y) * CB + c;
-}
-
-// streaming stores of the correlation intermediate (written once by pass 1,
-// read once by pass 2), so they do not displace the spectra in L2
-typedef float nt_float2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void store_nt(float2* p, float2 v) {
-  nt_float2 t;
-  t.x = v.x;
-  t.y = v.y;
-  __builtin_nontemporal_store(t, reinterpret_cast<nt_float2*>(p));
-}
-
-// device-coherent accesses (agent-scope relaxed atomics: stores written
-// through to the coherence point, loads that do not hit stale L2 lines), for
-// an intermediate that a workgroup on another XCD consumes within the same
-// launch
-__device__ __forceinline__ void store_coh(float2* p, float2 v) {
-  const unsigned long long u =
-      ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float2 load_coh(const float2* p) {
-  const unsigned long long u = __hip_atomic_load(
-      reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
-}
-
-// value of lane (lane ^ (CB - 1)): the partner slot of a paired block when
-// CB consecutive lanes hold the CB slots of one row
-template <int CB>
-__device__ __forceinline__ float xchg_f(float v) {
-  if constexpr (CB == 8) {  // row_half_mirror
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));
-  } else if constexpr (CB == 16) {  // row_mirror
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));
-  } else {
-    return __shfl_xor(v, CB - 1, 64);
-  }
-}
-template <int CB>
-__device__ __forceinline__ float2 xchg(float2 v) {
-  return make_float2(xchg_f<CB>(v.x), xchg_f<CB>(v.y));
-}
-
-// ---------------------------------------------------------------------------
-// wave / block reductions (wave64)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-// (value, index) max with lowest index on ties
-__device__ __forceinline__ void vmax_combine(float& v, int& p, float v2, int p2) {
-  if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
-}
-__device__ __forceinline__ void wave_max_idx(float& v, int& p) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    float v2 = __shfl_xor(v, o, 64);
-    int p2 = __shfl_xor(p, o, 64);
-    vmax_combine(v, p, v2, p2);
-  }
-}
-
-// block-wide FP64 sum of two values; result valid in thread 0
-__device__ __forceinline__ void block_sum2_d(double& a, double& b, double* scratch) {
-  a = wave_sum_d(a);
-  b = wave_sum_d(b);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __syncthreads();
-  if (lane == 0) { scratch[2 * w] = a; scratch[2 * w + 1] = b; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double sa = 0, sb = 0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) { sa += scratch[2 * i]; sb += scratch[2 * i + 1]; }
-    a = sa; b = sb;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // real-FFT pre/post processing
@@ -159,7 +34,7 @@ __device__ __forceinline__ float2 c2r_pre(float2 xk, float2 xmk, float2 wk_conj)
 // ---------------------------------------------------------------------------
 // Row R2C: rows y0..y0+RB-1 of image b -> columns [b][kx][y] (transposed)
 // ---------------------------------------------------------------------------
-enum { SRC_PAD = 0, SRC_TILE = 1, SRC_ROT = 2 };
+enum { SRC_PAD = 0, SRC_TILE = 1, SRC_ROT = 2, SRC_ROTB = 3, SRC_IMG = 4 };
 
 struct RowSrc {
   // SRC_PAD
@@ -173,10 +48,15 @@ struct RowSrc {
   const int* offy;
   int bxn;
   const double* mean;
-  // SRC_ROT
+  // SRC_ROT (one angle for every b), SRC_ROTB (angle cs_b[b] = (cos, sin)
+  // and a factor pre_b[b] applied to the interpolated value, per b)
   const float* treal;
   float cose, sine, dmask;
   double* partials;
+  const float2* cs_b;
+  const float* pre_b;
+  // SRC_IMG: real images [b][L][L]
+  const float* imgs;
 };
 
 template <int L, int SRC>
@@ -191,8 +71,12 @@ __device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x,
     const int gx = s.offx[ix] + x, gy = s.offy[iy] + y;
     if (gx >= s.nx || gy >= s.ny) return 0.0f;
     return (float)((double)s.img[(long)gy * s.nx + gx] - (double)aux);
+  } else if constexpr (SRC == SRC_IMG) {
+    return s.imgs[((long)b * L + y) * L + x];
   } else {
     // rotate_subIMG (GPU_func.cu:496-560) then apply_mask (GPU_func.cu:229-247)
+    const float cose = SRC == SRC_ROTB ? s.cs_b[b].x : s.cose;
+    const float sine = SRC == SRC_ROTB ? s.cs_b[b].y : s.sine;
     const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
     const float r = hypotf(xf, yf);
     const float dm2 = s.dmask * 0.5f;  // d_m = 1.5 * diameter, radius d_m / 2
@@ -201,8 +85,8 @@ __device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x,
     // the reference's float expression, rounded step by step (no FMA), so the
     // out-of-window test and the interpolation cell are those of
     // fl(fl(fl(c x) + fl(s y)) + L/2)
-    const float x2 = __fadd_rn(__fadd_rn(__fmul_rn(s.cose, xf), __fmul_rn(s.sine, yf)), (float)(L / 2));
-    const float y2 = __fadd_rn(__fadd_rn(__fmul_rn(-s.sine, xf), __fmul_rn(s.cose, yf)), (float)(L / 2));
+    const float x2 = __fadd_rn(__fadd_rn(__fmul_rn(cose, xf), __fmul_rn(sine, yf)), (float)(L / 2));
+    const float y2 = __fadd_rn(__fadd_rn(__fmul_rn(-sine, xf), __fmul_rn(cose, yf)), (float)(L / 2));
     float v = 0.0f;
     if (!(x2 < 0 || x2 > L - 1.0f || y2 < 0 || y2 > L - 1.0f)) {
       const int ii = (int)floorf(x2), jj = (int)floorf(y2);
@@ -215,6 +99,7 @@ __device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x,
       const float p0 = src[k0] * tt * uu, p1 = src[k1] * t * uu;
       const float p3 = src[k3] * tt * u, p2 = src[k2] * t * u;
       v = p0 + p1 + p2 + p3;
+      if constexpr (SRC == SRC_ROTB) v *= s.pre_b[b];
     }
     if (r >= dm2) v *= 0.5f * cosf(kRefPi * (r - dm2) / (2 * ehw)) + 0.5f;
     return v;
@@ -259,13 +144,13 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(RowSrc s, float2* __restrict__ 
         v1 = row_value<L, SRC>(s, b, y, 2 * m + 1, aux);
       }
     }
-    if constexpr (SRC == SRC_ROT) {
+    if constexpr (SRC == SRC_ROT || SRC == SRC_ROTB) {
       acc1 += (double)v0 + (double)v1;
       acc2 += (double)v0 * v0 + (double)v1 * v1;
     }
     lds[r * M + m] = make_float2(v0, v1);
   }
-  if constexpr (SRC == SRC_ROT) {
+  if constexpr (SRC == SRC_ROT || SRC == SRC_ROTB) {
     block_sum2_d(acc1, acc2, red);
     if (threadIdx.x == 0) {
       s.partials[2 * ((long)b * gridDim.x + blockIdx.x)] = acc1;
@@ -922,22 +807,6 @@ __global__ __launch_bounds__(WG) void k_radial_power(const float2* __restrict__ 
   }
 }
 
-__device__ float ctf_ast_f(int x1, int y1, int nx, int ny, const WeightArgs& w, int mode) {
-  // CTF_AST, GPU_func.cu:292-315
-  y1 = x1 > nx / 2 ? ny - y1 : y1;
-  x1 = min(x1, nx - x1);
-  const float dx = (float)min(x1, nx - x1);
-  const float dy = (float)(y1 - ny / 2);
-  const float ss = (dx * dx / (float)(nx * nx) + dy * dy / (float)(ny * ny)) / (w.apix * w.apix);
-  const float ag = atan2f((float)(y1 - ny / 2), (float)x1);
-  const float df_ast = 0.5f * (w.dfu + w.dfv + 2 * w.dfdiff * cosf(2 * (w.dfang * kRefPi / 180 - ag)));
-  const float gamma = -2 * kRefPi * (w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * ss * ss +
-                                     df_ast * 5000.0f * w.lambda * ss);
-  const float v = sqrtf(1.0f - w.ampconst * w.ampconst) * sinf(gamma) + w.ampconst * cosf(gamma);
-  if (mode == 0) return v > 0 ? 1.0f : -1.0f;
-  if (mode == 2) return fabsf(v);
-  return v;
-}
 
 // whiten_Tmp (GPU_func.cu:130-152) + apply_weighting_function (249-290) +
 // compute_area_sum_ofSQR partials (318-362); spectrum pixel (kx, ky) with
@@ -946,7 +815,7 @@ template <int L>
 __global__ __launch_bounds__(WG) void k_tmpl_weight(float2* __restrict__ spec,
                                                     const double* __restrict__ ra,
                                                     const float* __restrict__ rbins, WeightArgs w,
-                                                    double* __restrict__ partials) {
+                                                    double* __restrict__ partials, int weight) {
   using G = Geo<L>;
   constexpr int H = G::H, NB = L / 2;
   __shared__ double red[2 * (WG / 64)];
@@ -963,11 +832,13 @@ __global__ __launch_bounds__(WG) void k_tmpl_weight(float2* __restrict__ spec,
       const double fb = ra[(long)b * NB + rbin] / (double)rbins[rbin];
       v = cscale(v, (float)(1.0 / sqrt(fb)));
     }
-    // low pass
+    // low pass and weighting (weight == 0: the whitened spectrum only, the
+    // refinement's starting point; every thread still reaches the reduction)
     const int r_round = rbin;
     const float hi = L * w.apix / w.highres, lo = L * w.apix / w.lowres;
     float f = 1.0f;
-    if (r_round < hi && r_round >= 0) {
+    if (!weight) {
+    } else if (r_round < hi && r_round >= 0) {
     } else if (r_round >= hi && r_round < hi + 8) {
       f = 0.5f * cosf(kRefPi * (r_round - hi) / (2 * 8)) + 0.5f;
     } else if (r_round >= (lo - 8) && r_round < lo && r_round >= 0) {
@@ -976,7 +847,7 @@ __global__ __launch_bounds__(WG) void k_tmpl_weight(float2* __restrict__ spec,
       f = 0.0f;
     }
     const float ss = rr * w.ds;
-    if (r_round < L / 2 && r_round >= 0) {
+    if (weight && r_round < L / 2 && r_round >= 0) {
       const float vv = ctf_ast_f(kx, (ky + L / 2) % L, L, L, w, 2);
       const float signal = expf(w.bf * ss * ss + w.bf2 * ss + w.bf3) / (w.kk + 1);
       const float ncurve = expf(w.a * ss * ss + w.b * ss + w.b2) / signal;
@@ -990,23 +861,25 @@ __global__ __launch_bounds__(WG) void k_tmpl_weight(float2* __restrict__ spec,
   if (threadIdx.x == 0) partials[(long)b * gridDim.x + blockIdx.x] = acc;
 }
 
+// amplitude normaliser m = sqrt(S) / count per template (main.cu:444-452)
+__global__ void k_tmpl_ampscale(const double* __restrict__ partials, int nparts, int nb,
+                                double count, float* __restrict__ inv_m) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double S = 0;
+  for (int i = 0; i < nparts; ++i) S += partials[(long)b * nparts + i];
+  const double m = sqrt(S / (count * count));
+  inv_m[b] = (m != 0) ? (float)(1.0 / m) : 1.0f;
+}
+
 template <int L>
 __global__ __launch_bounds__(WG) void k_tmpl_ampnorm(float2* __restrict__ spec,
-                                                     const double* __restrict__ partials,
-                                                     int nparts, double count) {
+                                                     const float* __restrict__ inv_m) {
   using G = Geo<L>;
   constexpr int H = G::H;
-  __shared__ float inv_m;
   const int b = blockIdx.y;
-  if (threadIdx.x == 0) {
-    double S = 0;
-    for (int i = 0; i < nparts; ++i) S += partials[(long)b * nparts + i];
-    const double m = sqrt(S / (count * count));
-    inv_m = (m != 0) ? (float)(1.0 / m) : 1.0f;
-  }
-  __syncthreads();
   const long i = (long)blockIdx.x * WG + threadIdx.x;
-  if (i < (long)H * L) spec[(long)b * H * L + i] = cscale(spec[(long)b * H * L + i], inv_m);
+  if (i < (long)H * L) spec[(long)b * H * L + i] = cscale(spec[(long)b * H * L + i], inv_m[b]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1167,6 +1040,34 @@ void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float si
                          float2* out, double* partials, const float2* twL, const float2* twM,
                          hipStream_t s) {
   DISPATCH(L, rows_r2c_rot_t, treal, nb, cose, sine, dmask, out, partials, twL, twM, s);
+}
+
+template <int L>
+static void rows_r2c_rotb_t(const float* treal, int nb, const float2* cs_b, const float* pre_b,
+                            float dmask, float2* out, double* partials, const float2* twL,
+                            const float2* twM, hipStream_t s) {
+  RowSrc rs{};
+  rs.treal = treal; rs.cs_b = cs_b; rs.pre_b = pre_b; rs.dmask = dmask; rs.partials = partials;
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  hipLaunchKernelGGL((k_rows_r2c<L, SRC_ROTB>), grid, dim3(WG), 0, s, rs, out, twL, twM);
+}
+void launch_rows_r2c_rotb(int L, const float* treal, int nb, const float2* cs_b, const float* pre_b,
+                          float dmask, float2* out, double* partials, const float2* twL,
+                          const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_r2c_rotb_t, treal, nb, cs_b, pre_b, dmask, out, partials, twL, twM, s);
+}
+
+template <int L>
+static void rows_r2c_img_t(const float* imgs, int nb, float2* out, const float2* twL,
+                           const float2* twM, hipStream_t s) {
+  RowSrc rs{};
+  rs.imgs = imgs;
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  hipLaunchKernelGGL((k_rows_r2c<L, SRC_IMG>), grid, dim3(WG), 0, s, rs, out, twL, twM);
+}
+void launch_rows_r2c_img(int L, const float* imgs, int nb, float2* out, const float2* twL,
+                         const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_r2c_img_t, imgs, nb, out, twL, twM, s);
 }
 
 template <int L>
@@ -1335,27 +1236,28 @@ void launch_radial_power(int L, const float2* spec, int nb, const int* bin_start
 
 template <int L>
 static void tmpl_weight_t(float2* spec, int nb, const double* ra, const float* rb,
-                          const WeightArgs& w, double* partials, hipStream_t s) {
+                          const WeightArgs& w, double* partials, int weight, hipStream_t s) {
   const long tot = (long)Geo<L>::H * L;
   dim3 grid((unsigned)((tot + WG - 1) / WG), nb);
-  hipLaunchKernelGGL((k_tmpl_weight<L>), grid, dim3(WG), 0, s, spec, ra, rb, w, partials);
+  hipLaunchKernelGGL((k_tmpl_weight<L>), grid, dim3(WG), 0, s, spec, ra, rb, w, partials, weight);
 }
 void launch_tmpl_weight(int L, float2* spec, int nb, const double* ra, const float* rb,
-                        const WeightArgs& w, double* partials, hipStream_t s) {
-  DISPATCH(L, tmpl_weight_t, spec, nb, ra, rb, w, partials, s);
+                        const WeightArgs& w, double* partials, int weight, hipStream_t s) {
+  DISPATCH(L, tmpl_weight_t, spec, nb, ra, rb, w, partials, weight, s);
 }
 
 template <int L>
 static void tmpl_ampnorm_t(float2* spec, int nb, const double* partials, double count,
-                           hipStream_t s) {
+                           float* inv_m, hipStream_t s) {
   const long tot = (long)Geo<L>::H * L;
   const int nparts = (int)((tot + WG - 1) / WG);
-  dim3 grid(nparts, nb);
-  hipLaunchKernelGGL((k_tmpl_ampnorm<L>), grid, dim3(WG), 0, s, spec, partials, nparts, count);
+  hipLaunchKernelGGL(k_tmpl_ampscale, dim3((nb + 63) / 64), dim3(64), 0, s, partials, nparts, nb,
+                     count, inv_m);
+  hipLaunchKernelGGL((k_tmpl_ampnorm<L>), dim3(nparts, nb), dim3(WG), 0, s, spec, inv_m);
 }
 void launch_tmpl_ampnorm(int L, float2* spec, int nb, const double* partials, double count,
-                         hipStream_t s) {
-  DISPATCH(L, tmpl_ampnorm_t, spec, nb, partials, count, s);
+                         float* inv_m, hipStream_t s) {
+  DISPATCH(L, tmpl_ampnorm_t, spec, nb, partials, count, inv_m, s);
 }
 
 int img_reduce_blocks(long n) {

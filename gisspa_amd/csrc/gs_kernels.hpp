@@ -73,6 +73,14 @@ void launch_rows_r2c_tiles(int L, const float* img, int nx, int ny, const int* o
 void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float sine, float dmask,
                          float2* out, double* partials, const float2* twL, const float2* twM,
                          hipStream_t s);
+// Same with one angle per b (cs_b[b] = (cos, sin)) and the interpolated
+// value multiplied by pre_b[b] before the mask.
+void launch_rows_r2c_rotb(int L, const float* treal, int nb, const float2* cs_b, const float* pre_b,
+                          float dmask, float2* out, double* partials, const float2* twL,
+                          const float2* twM, hipStream_t s);
+// Real rows -> half spectrum of real images imgs[b][L][L].
+void launch_rows_r2c_img(int L, const float* imgs, int nb, float2* out, const float2* twL,
+                         const float2* twM, hipStream_t s);
 // In-place column transform of [b][kx][y]; multiplies by scale (global) or
 // per-b scale from `bscale` (may be null); zero_dc zeroes element (0,0).
 void launch_cols_fwd(int L, float2* spec, int nb, float scale, const float* bscale, int zero_dc,
@@ -87,9 +95,9 @@ void launch_cols_inv_T(int L, const float2* spec, int nb, float2* G, const float
 void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* twL,
                      const float2* twM, hipStream_t s);
 
-// Correlation pass 1: P = tile[t] * conj(tmpl[J]) * (-1)^(kx+ky) * sgn,
-// column inverse, written to the blocked G[m][kx / CB][y][kx % CB]
-// (Geometry::gmap elements per map).
+// Correlation pass 1: P = tile[t] * conj(tmpl[J]) (the tile spectra carry
+// (-1)^(kx+ky) sgn), column inverse and C2R pre-processing, written to the
+// paired-column blocks of G (Geometry::gmap elements per map).
 // psum[m * corr_ncb(L) + cb] = (sum of weighted |P|^2, P(0,0).re) of the block
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
                        int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s);
@@ -155,10 +163,13 @@ struct WeightArgs {
   float a, b, b2, bf, bf2, bf3;
   float dfu, dfv, dfdiff, dfang, lambda, cs, ampconst;
 };
+// whitening + (weight != 0) low pass and |CTF| weighting with the amplitude
+// normaliser's partial sums; weight == 0 leaves the whitened spectrum
 void launch_tmpl_weight(int L, float2* spec, int nb, const double* ra, const float* rb,
-                        const WeightArgs& w, double* partials, hipStream_t s);
+                        const WeightArgs& w, double* partials, int weight, hipStream_t s);
+// divide every template by its amplitude normaliser (inv_m: nb floats scratch)
 void launch_tmpl_ampnorm(int L, float2* spec, int nb, const double* partials, double count,
-                         hipStream_t s);
+                         float* inv_m, hipStream_t s);
 
 // ---- micrograph ----
 void launch_img_sums(const float* img, long n, double* partials, int nblocks, hipStream_t s);
@@ -202,5 +213,42 @@ void launch_plain_ccg(int L, const float2* tile, const float2* tspec, int n, flo
                       float2* ccg, hipStream_t s);
 void launch_plain_peak(int L, const float2* ccg, int n, float lo, float hi, void* part,
                        hipStream_t s);
+
+// ---- refinement objective and its adjoint (gs_refine.hip) ----
+struct RefCand {
+  int ptile, view;   // tile of the pool, view (template)
+  float x0, y0;      // peak position in the tile
+  float psi, dx, dy, df;  // psi degrees (euler3), shift (pixels), defocus (um)
+};
+struct RefScal {
+  double m, dm, mu, sd, c, sigma, score, gx, gy, gmean, grn, dpsi, dmacc, ddf;
+};
+struct RefArgs {
+  int nb;
+  const float2* white;  // [N][H][L] whitened template half spectra
+  const float2* tiles;  // tile pool
+  const RefCand* cand;
+  WeightArgs w;         // template CTF weighting (defocus replaced per candidate)
+  double count;         // cells of the amplitude normaliser
+  float dmask;          // 1.5 diameter
+  const float2* cs;     // (cos, sin) of psi per candidate
+  float2* A;            // [nb][H][L] scratch half spectra
+  float2* gtmp;         // [nb][L][HP]
+  float* Tun;           // [nb][L][L]
+  float2* spec;         // [nb][H][L]
+  double* rot_part;     // [nb][row blocks][2]
+  float* bscale;        // [nb]
+  float* pre;           // [nb]
+  float* gRn;           // [nb][L][L]
+  float* gTb;           // [nb][L][L]
+  double* part;         // [nb][ref_parts()][4]
+  RefScal* sc;          // [nb]
+  float* out;           // [nb][5]: score, d psi, d dx, d dy, d defocus
+  const float2* twL;
+  const float2* twM;
+};
+void launch_ref_forward(int L, const RefArgs& a, hipStream_t s);
+void launch_ref_backward(int L, const RefArgs& a, hipStream_t s);
+int ref_parts();
 
 }  // namespace gs

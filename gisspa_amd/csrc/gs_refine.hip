@@ -1,0 +1,577 @@
+// gs_refine.hip -- the refinement objective of a picked candidate and its
+// hand-written adjoint (no counterpart in the reference, which stops at the
+// grid search of main.cu:1230-1257).
+//
+// For candidate b (tile t, view j, peak (x0, y0) in the tile) and
+// theta = (psi degrees, dx, dy pixels, defocus um):
+//   A      = W(k; defocus) B_j(k)        B_j: whitened template half spectrum
+//   m      = sqrt(sum_half |A|^2) / count (amplitude normaliser, main.cu:444-452)
+//   T      = C2R(A)                       real template before 1/m
+//   Tb(x)  = bilinear(T) at the rotation of x by psi; R = Tb / m
+//   Rm     = mask R, Rn = (Rm - mean) / sd over L^2
+//   That   = FFT(Rn) / L^2 with the DC zeroed (as the picker's rotation path)
+//   P      = I_t conj(That)               (the tile spectra carry (-1)^(kx+ky) sgn)
+//   c      = Re sum_k P(k) e^{2 pi i k'.u / L},  u = (x0 + dx, y0 + dy)
+//   sigma  = sqrt(sum_k |P(k)|^2),  score = c / sigma.
+// The backward pass runs the chain in reverse: d score / d That (per
+// frequency), C2R to d/dRn, the normalisation adjoint, the mask, the
+// bilinear adjoint (a gather over the output pixels whose interpolation cell
+// touches each source pixel, so no atomics) and its psi derivative, R2C to
+// d/dA, and the derivative of the CTF weighting.  All sums are FP64.
+#include "gs_device.hpp"
+
+namespace gs {
+
+constexpr int RPARTS = 64;  // partial blocks per candidate in each reduction
+
+// Template weighting factor of apply_weighting_function (GPU_func.cu:249-290,
+// the same arithmetic as k_tmpl_weight) and its derivative by the defocus
+// (um, positive underfocus: df_ast = -defocus + dfdiff cos(...), so
+// d gamma / d defocus = +2 pi 5000 lambda ss).
+struct WeightD {
+  float f, fd;
+  int rbin;
+};
+template <int L>
+__device__ __forceinline__ WeightD weight_deriv(int kx, int ky, const WeightArgs& w) {
+  WeightD o;
+  const float hy = (float)min(ky, L - ky), hx = (float)min(kx, L - kx);
+  const float rr = hypotf(hy, hx);
+  const int rbin = (int)floorf(rr + 0.5f) - 1;
+  o.rbin = rbin;
+  const float hi = L * w.apix / w.highres, lo = L * w.apix / w.lowres;
+  float flp = 1.0f;
+  if (rbin < hi && rbin >= 0) {
+  } else if (rbin >= hi && rbin < hi + 8) {
+    flp = 0.5f * cosf(kRefPi * (rbin - hi) / (2 * 8)) + 0.5f;
+  } else if (rbin >= (lo - 8) && rbin < lo && rbin >= 0) {
+    flp = 0.5f * cosf(kRefPi * (lo - rbin) / (2 * 8)) + 0.5f;
+  } else {
+    flp = 0.0f;
+  }
+  o.f = flp;
+  o.fd = 0.0f;
+  if (rbin < L / 2 && rbin >= 0) {
+    // CTF_AST (GPU_func.cu:292-315) at (kx, (ky + L/2) % L), value and slope
+    int x1 = kx, y1 = (ky + L / 2) % L;
+    y1 = x1 > L / 2 ? L - y1 : y1;
+    x1 = min(x1, L - x1);
+    const float dx = (float)min(x1, L - x1);
+    const float dy = (float)(y1 - L / 2);
+    const float sc = (dx * dx / (float)(L * L) + dy * dy / (float)(L * L)) / (w.apix * w.apix);
+    const float ag = atan2f((float)(y1 - L / 2), (float)x1);
+    const float df_ast =
+        0.5f * (w.dfu + w.dfv + 2 * w.dfdiff * cosf(2 * (w.dfang * kRefPi / 180 - ag)));
+    const float gamma = -2 * kRefPi * (w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * sc * sc +
+                                       df_ast * 5000.0f * w.lambda * sc);
+    const float sa = sqrtf(1.0f - w.ampconst * w.ampconst);
+    const float v = sa * sinf(gamma) + w.ampconst * cosf(gamma);
+    const float dv = (sa * cosf(gamma) - w.ampconst * sinf(gamma)) * (2 * kRefPi * 5000.0f * w.lambda * sc);
+    const float vv = fabsf(v), dvv = v >= 0 ? dv : -dv;
+    const float ss = rr * w.ds;
+    const float signal = expf(w.bf * ss * ss + w.bf2 * ss + w.bf3) / (w.kk + 1);
+    const float ncurve = expf(w.a * ss * ss + w.b * ss + w.b2) / signal;
+    const float q = ncurve + w.kk * vv * vv;
+    o.f = flp * vv * sqrtf(1 / q);
+    o.fd = flp * ncurve / (q * sqrtf(q)) * dvv;
+  }
+  return o;
+}
+
+__device__ __forceinline__ WeightArgs cand_weight(WeightArgs w, float defocus) {
+  w.dfu = -defocus + w.dfdiff;  // defocus is negated at parse (DataReader.cpp:378)
+  w.dfv = -defocus - w.dfdiff;
+  return w;
+}
+
+// block reduction of N doubles; valid in thread 0
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double* scratch) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = wave_sum_d(v[i]);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) scratch[w * N + i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = 0;
+      for (int k = 0; k < WG / 64; ++k) s += scratch[k * N + i];
+      v[i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_weight(const float2* __restrict__ white,
+                                                   const RefCand* __restrict__ cand, WeightArgs w0,
+                                                   float2* __restrict__ A,
+                                                   double* __restrict__ part) {
+  constexpr int H = L / 2 + 1;
+  __shared__ double red[2 * (WG / 64)];
+  const int b = blockIdx.y;
+  const RefCand cd = cand[b];
+  const WeightArgs w = cand_weight(w0, cd.df);
+  const float2* Bj = white + (long)cd.view * H * L;
+  double acc[2] = {0, 0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < (long)H * L; i += (long)RPARTS * WG) {
+    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
+    const WeightD wd = weight_deriv<L>(kx, ky, w);
+    const float2 bv = Bj[i];
+    A[(long)b * H * L + i] = cscale(bv, wd.f);
+    if (wd.rbin < L / 2 && wd.rbin >= 0) {
+      const double b2 = (double)bv.x * bv.x + (double)bv.y * bv.y;
+      acc[0] += b2 * wd.f * wd.f;
+      acc[1] += b2 * wd.f * wd.fd;
+    }
+  }
+  block_sum_n<2>(acc, red);
+  if (threadIdx.x == 0) {
+    part[((long)b * RPARTS + blockIdx.x) * 2] = acc[0];
+    part[((long)b * RPARTS + blockIdx.x) * 2 + 1] = acc[1];
+  }
+}
+
+// m = sqrt(S) / count and dm / d defocus = (sum |B|^2 f f') / (count sqrt(S))
+__global__ void k_ref_norm_m(const double* __restrict__ part, int nb, double count,
+                             RefScal* __restrict__ sc, float* __restrict__ pre) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double S = 0, SW = 0;
+  for (int i = 0; i < RPARTS; ++i) {
+    S += part[((long)b * RPARTS + i) * 2];
+    SW += part[((long)b * RPARTS + i) * 2 + 1];
+  }
+  const double rs = sqrt(S);
+  const double m = rs / count;
+  sc[b].m = m != 0 ? m : 1.0;
+  sc[b].dm = rs != 0 ? SW / (count * rs) : 0.0;
+  pre[b] = (float)(1.0 / sc[b].m);
+}
+
+// mean and sd of the rotated masked template (the row partials of
+// k_rows_r2c), and the spectrum scale 1/(L^2 sd) of the picker's k_rot_scale
+__global__ void k_ref_rot_stats(const double* __restrict__ partials, int nparts, int nb,
+                                double L2, RefScal* __restrict__ sc, float* __restrict__ bscale) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s1 = 0, s2 = 0;
+  for (int i = 0; i < nparts; ++i) {
+    s1 += partials[2 * ((long)b * nparts + i)];
+    s2 += partials[2 * ((long)b * nparts + i) + 1];
+  }
+  const double mean = s1 / L2, var = s2 / L2 - mean * mean;
+  sc[b].mu = mean;
+  sc[b].sd = var != 0 ? sqrt(var) : 1.0;
+  bscale[b] = (float)(1.0 / (L2 * sc[b].sd));
+}
+
+// e^{2 pi i k' u / L} for the columns (k' = kx) and rows (signed ky)
+template <int L>
+__device__ __forceinline__ void phase_tables(float2* ex, float2* ey, double ux, double uy) {
+  constexpr int H = L / 2 + 1;
+  for (int k = threadIdx.x; k < H + L; k += WG) {
+    const bool col = k < H;
+    const int kk = col ? k : k - H;
+    const double ks = col ? kk : (kk <= L / 2 ? kk : kk - L);
+    const double a = fmod(2.0 * ks * (col ? ux : uy) / L, 2.0);
+    double sn, cs;
+    sincospi(a, &sn, &cs);
+    (col ? ex[kk] : ey[kk]) = make_float2((float)cs, (float)sn);
+  }
+  __syncthreads();
+}
+
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_score(const float2* __restrict__ tiles,
+                                                  const float2* __restrict__ spec,
+                                                  const RefCand* __restrict__ cand,
+                                                  double* __restrict__ part) {
+  constexpr int H = L / 2 + 1;
+  __shared__ float2 ex[H], ey[L];
+  __shared__ double red[4 * (WG / 64)];
+  const int b = blockIdx.y;
+  const RefCand cd = cand[b];
+  phase_tables<L>(ex, ey, (double)cd.x0 + cd.dx, (double)cd.y0 + cd.dy);
+  const float2* I = tiles + (long)cd.ptile * H * L;
+  const float2* T = spec + (long)b * H * L;
+  double acc[4] = {0, 0, 0, 0};  // c, sum w|P|^2, sum w kx Re(i P E), sum w ky' Re(i P E)
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < (long)H * L; i += (long)RPARTS * WG) {
+    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
+    const float2 P = cmulc(I[i], T[i]);
+    const float w = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
+    const float2 pe = cmul(P, cmul(ex[kx], ey[ky]));
+    const float kys = (float)(ky <= L / 2 ? ky : ky - L);
+    acc[0] += (double)(w * pe.x);
+    acc[1] += (double)(w * (P.x * P.x + P.y * P.y));
+    acc[2] += (double)(-w * pe.y * (float)kx);
+    acc[3] += (double)(-w * pe.y * kys);
+  }
+  block_sum_n<4>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) part[((long)b * RPARTS + blockIdx.x) * 4 + k] = acc[k];
+}
+
+__global__ void k_ref_score_fin(const double* __restrict__ part, int nb, int L,
+                                RefScal* __restrict__ sc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double a[4] = {0, 0, 0, 0};
+  for (int i = 0; i < RPARTS; ++i)
+    for (int k = 0; k < 4; ++k) a[k] += part[((long)b * RPARTS + i) * 4 + k];
+  const double sigma = sqrt(a[1]);
+  sc[b].c = a[0];
+  sc[b].sigma = sigma;
+  sc[b].score = sigma > 0 ? a[0] / sigma : 0.0;
+  const double k2pi = 2.0 * M_PI / L;
+  sc[b].gx = sigma > 0 ? k2pi * a[2] / sigma : 0.0;
+  sc[b].gy = sigma > 0 ? k2pi * a[3] / sigma : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// backward
+// ---------------------------------------------------------------------------
+// d score / d That(k) = g I with g = E / sigma - (c / sigma^3) conj(P); the
+// DC is zeroed in the forward pass, so it carries no gradient.  The real
+// gradient image is (1 / L^2) sum_half w Re(G e^{+2 pi i k.x / L}); the C2R
+// computes exactly that once the kx = 0 and kx = L/2 columns are Hermitian in
+// ky, so those two columns are written as (G(ky) + conj G(-ky)) / 2.
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_grad_spec(const float2* __restrict__ tiles,
+                                                      const float2* __restrict__ spec,
+                                                      const RefCand* __restrict__ cand,
+                                                      const RefScal* __restrict__ sc,
+                                                      float2* __restrict__ G) {
+  constexpr int H = L / 2 + 1;
+  __shared__ float2 ex[H], ey[L];
+  const int b = blockIdx.y;
+  const RefCand cd = cand[b];
+  phase_tables<L>(ex, ey, (double)cd.x0 + cd.dx, (double)cd.y0 + cd.dy);
+  const float2* I = tiles + (long)cd.ptile * H * L;
+  const float2* T = spec + (long)b * H * L;
+  const double sg = sc[b].sigma;
+  const float is = sg > 0 ? (float)(1.0 / sg) : 0.0f;
+  const float cs3 = sg > 0 ? (float)(sc[b].c / (sg * sg * sg)) : 0.0f;
+  auto grad = [&](int kx, int ky) -> float2 {
+    if (kx == 0 && ky == 0) return make_float2(0.f, 0.f);
+    const long i = (long)kx * L + ky;
+    const float2 iv = I[i];
+    const float2 P = cmulc(iv, T[i]);
+    const float2 e = cmul(ex[kx], ey[ky]);
+    const float2 g = make_float2(e.x * is - cs3 * P.x, e.y * is + cs3 * P.y);
+    return cmul(g, iv);
+  };
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < (long)H * L; i += (long)RPARTS * WG) {
+    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
+    float2 gv = grad(kx, ky);
+    if (kx == 0 || kx == L / 2) {
+      const float2 gm = grad(kx, (L - ky) % L);
+      gv = make_float2(0.5f * (gv.x + gm.x), 0.5f * (gv.y - gm.y));
+    }
+    G[(long)b * H * L + i] = gv;
+  }
+}
+
+// forward rotation sample of one output pixel (the SRC_ROTB arithmetic of
+// k_rows_r2c) with the bilinear slopes
+struct RotSample {
+  float tb, dtx, dty, mask;  // bilinear value, d/dx2, d/dy2, mask factor
+  float X, Y;                // pixel relative to the centre
+  bool in;                   // inside the window
+};
+template <int L>
+__device__ __forceinline__ RotSample rot_sample(const float* src, int x, int y, float cose,
+                                                float sine, float dmask) {
+  RotSample o;
+  const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
+  o.X = xf;
+  o.Y = yf;
+  const float r = hypotf(xf, yf);
+  const float dm2 = dmask * 0.5f, ehw = 6.0f;
+  o.mask = r > dm2 + 2 * ehw ? 0.0f : (r >= dm2 ? 0.5f * cosf(kRefPi * (r - dm2) / (2 * ehw)) + 0.5f : 1.0f);
+  o.tb = o.dtx = o.dty = 0.0f;
+  o.in = false;
+  if (o.mask == 0.0f) return o;
+  const float x2 = __fadd_rn(__fadd_rn(__fmul_rn(cose, xf), __fmul_rn(sine, yf)), (float)(L / 2));
+  const float y2 = __fadd_rn(__fadd_rn(__fmul_rn(-sine, xf), __fmul_rn(cose, yf)), (float)(L / 2));
+  if (x2 < 0 || x2 > L - 1.0f || y2 < 0 || y2 > L - 1.0f) return o;
+  o.in = true;
+  const int ii = (int)floorf(x2), jj = (int)floorf(y2);
+  long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
+  if (ii == L - 1) { k1--; k2--; }
+  if (jj == L - 1) { k2 -= L; k3 -= L; }
+  const float t = x2 - (float)ii, u = y2 - (float)jj;
+  const float tt = 1.0f - t, uu = 1.0f - u;
+  const float a0 = src[k0], a1 = src[k1], a2 = src[k2], a3 = src[k3];
+  o.tb = a0 * tt * uu + a1 * t * uu + a2 * t * u + a3 * tt * u;
+  o.dtx = (a1 - a0) * uu + (a2 - a3) * u;
+  o.dty = (a3 - a0) * tt + (a2 - a1) * t;
+  return o;
+}
+
+// sums of g = d/dRn and of g Rn over the pixels (normalisation adjoint)
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_norm_red(const float* __restrict__ Tun,
+                                                     const float* __restrict__ gRn,
+                                                     const float2* __restrict__ cs_b,
+                                                     const float* __restrict__ pre,
+                                                     const RefScal* __restrict__ sc, float dmask,
+                                                     double* __restrict__ part) {
+  constexpr long L2 = (long)L * L;
+  __shared__ double red[2 * (WG / 64)];
+  const int b = blockIdx.y;
+  const float2 cs = cs_b[b];
+  const float p = pre[b];
+  const double mu = sc[b].mu, isd = 1.0 / sc[b].sd;
+  const float* src = Tun + b * L2;
+  double acc[2] = {0, 0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
+    const int y = (int)(i / L), x = (int)(i - (long)y * L);
+    const RotSample rs = rot_sample<L>(src, x, y, cs.x, cs.y, dmask);
+    const double rn = ((double)(rs.tb * p * rs.mask) - mu) * isd;
+    const double g = (double)gRn[b * L2 + i] / (double)L2;
+    acc[0] += g;
+    acc[1] += g * rn;
+  }
+  block_sum_n<2>(acc, red);
+  if (threadIdx.x == 0) {
+    part[((long)b * RPARTS + blockIdx.x) * 2] = acc[0];
+    part[((long)b * RPARTS + blockIdx.x) * 2 + 1] = acc[1];
+  }
+}
+
+__global__ void k_ref_norm_fin(const double* __restrict__ part, int nb, double L2,
+                               RefScal* __restrict__ sc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double g = 0, gr = 0;
+  for (int i = 0; i < RPARTS; ++i) {
+    g += part[((long)b * RPARTS + i) * 2];
+    gr += part[((long)b * RPARTS + i) * 2 + 1];
+  }
+  sc[b].gmean = g / L2;
+  sc[b].grn = gr / L2;
+}
+
+// d/dTb per output pixel (written to gTb), and the sums for d/dpsi and d/dm
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_rot_grad(const float* __restrict__ Tun,
+                                                     const float* __restrict__ gRn,
+                                                     const float2* __restrict__ cs_b,
+                                                     const float* __restrict__ pre,
+                                                     const RefScal* __restrict__ sc, float dmask,
+                                                     float* __restrict__ gTb,
+                                                     double* __restrict__ part) {
+  constexpr long L2 = (long)L * L;
+  __shared__ double red[2 * (WG / 64)];
+  const int b = blockIdx.y;
+  const float2 cs = cs_b[b];
+  const float p = pre[b];
+  const double mu = sc[b].mu, isd = 1.0 / sc[b].sd, gm = sc[b].gmean, grn = sc[b].grn;
+  const float* src = Tun + b * L2;
+  const float kdeg = kRefPi / 180.0f;
+  double acc[2] = {0, 0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
+    const int y = (int)(i / L), x = (int)(i - (long)y * L);
+    const RotSample rs = rot_sample<L>(src, x, y, cs.x, cs.y, dmask);
+    float out = 0.0f;
+    if (rs.in) {
+      const double rn = ((double)(rs.tb * p * rs.mask) - mu) * isd;
+      const double g = (double)gRn[b * L2 + i] / (double)L2;
+      const double grm = (g - gm - rn * grn) * isd;  // d/dRm
+      const double gv = grm * rs.mask;                // d/d(Tb p)
+      out = (float)(gv * p);                          // d/dTb
+      const float dx2 = (-cs.y * rs.X + cs.x * rs.Y) * kdeg;
+      const float dy2 = (-cs.x * rs.X - cs.y * rs.Y) * kdeg;
+      acc[0] += (double)out * (double)(rs.dtx * dx2 + rs.dty * dy2);
+      acc[1] += gv * (double)rs.tb;
+    }
+    gTb[b * L2 + i] = out;
+  }
+  block_sum_n<2>(acc, red);
+  if (threadIdx.x == 0) {
+    part[((long)b * RPARTS + blockIdx.x) * 2] = acc[0];
+    part[((long)b * RPARTS + blockIdx.x) * 2 + 1] = acc[1];
+  }
+}
+
+__global__ void k_ref_rot_fin(const double* __restrict__ part, int nb, RefScal* __restrict__ sc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double dp = 0, dpre = 0;
+  for (int i = 0; i < RPARTS; ++i) {
+    dp += part[((long)b * RPARTS + i) * 2];
+    dpre += part[((long)b * RPARTS + i) * 2 + 1];
+  }
+  sc[b].dpsi = dp;
+  // R = Tb / m: d/dm = -(sum d/d(Tb/m) Tb) / m^2
+  sc[b].dmacc = -dpre / (sc[b].m * sc[b].m);
+}
+
+// Bilinear adjoint as a gather: source pixel q collects d/dTb of every output
+// pixel whose interpolation cell (the forward's exact arithmetic, boundary
+// rules included) has q as a corner, times that corner's weight.  The
+// outputs whose cell can touch q lie within 2 pixels of the inverse rotation
+// of q.
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gTb,
+                                                    const float2* __restrict__ cs_b, float dmask,
+                                                    float* __restrict__ gT) {
+  constexpr long L2 = (long)L * L;
+  const int b = blockIdx.y;
+  const float2 cs = cs_b[b];
+  const float* gsrc = gTb + b * L2;
+  const float reach = dmask * 0.5f + 12.0f + 3.0f;
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
+    const int qy = (int)(i / L), qx = (int)(i - (long)qy * L);
+    const float qX = (float)(qx - L / 2), qY = (float)(qy - L / 2);
+    float acc = 0.0f;
+    if (qX * qX + qY * qY <= reach * reach) {
+      // inverse rotation: x - c = R^T (q - c)
+      const float cX = cs.x * qX - cs.y * qY, cY = cs.y * qX + cs.x * qY;
+      const int x0 = (int)floorf(cX) + L / 2 - 2, y0 = (int)floorf(cY) + L / 2 - 2;
+      for (int dy = 0; dy < 5; ++dy) {
+        const int y = y0 + dy;
+        if (y < 0 || y >= L) continue;
+        for (int dx = 0; dx < 5; ++dx) {
+          const int x = x0 + dx;
+          if (x < 0 || x >= L) continue;
+          const float g = gsrc[(long)y * L + x];
+          if (g == 0.0f) continue;
+          const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
+          const float x2 = __fadd_rn(__fadd_rn(__fmul_rn(cs.x, xf), __fmul_rn(cs.y, yf)), (float)(L / 2));
+          const float y2 = __fadd_rn(__fadd_rn(__fmul_rn(-cs.y, xf), __fmul_rn(cs.x, yf)), (float)(L / 2));
+          if (x2 < 0 || x2 > L - 1.0f || y2 < 0 || y2 > L - 1.0f) continue;
+          const int ii = (int)floorf(x2), jj = (int)floorf(y2);
+          long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
+          if (ii == L - 1) { k1--; k2--; }
+          if (jj == L - 1) { k2 -= L; k3 -= L; }
+          const float t = x2 - (float)ii, u = y2 - (float)jj;
+          const float tt = 1.0f - t, uu = 1.0f - u;
+          float wq = 0.0f;
+          if (k0 == i) wq += tt * uu;
+          if (k1 == i) wq += t * uu;
+          if (k2 == i) wq += t * u;
+          if (k3 == i) wq += tt * u;
+          acc += g * wq;
+        }
+      }
+    }
+    gT[b * L2 + i] = acc;
+  }
+}
+
+// d score / d defocus through A = f(defocus) B: sum_half w Re(conj(G_A) f' B)
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_df(const float2* __restrict__ white,
+                                               const float2* __restrict__ GA,
+                                               const RefCand* __restrict__ cand, WeightArgs w0,
+                                               double* __restrict__ part) {
+  constexpr int H = L / 2 + 1;
+  __shared__ double red[1 * (WG / 64)];
+  const int b = blockIdx.y;
+  const RefCand cd = cand[b];
+  const WeightArgs w = cand_weight(w0, cd.df);
+  const float2* Bj = white + (long)cd.view * H * L;
+  double acc[1] = {0};
+  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < (long)H * L; i += (long)RPARTS * WG) {
+    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
+    const WeightD wd = weight_deriv<L>(kx, ky, w);
+    if (wd.fd == 0.0f) continue;
+    const float2 ga = GA[(long)b * H * L + i], bv = Bj[i];
+    const float wh = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
+    acc[0] += (double)(wh * wd.fd * (ga.x * bv.x + ga.y * bv.y));
+  }
+  block_sum_n<1>(acc, red);
+  if (threadIdx.x == 0) part[(long)b * RPARTS + blockIdx.x] = acc[0];
+}
+
+__global__ void k_ref_out(const double* __restrict__ part, int nb, RefScal* __restrict__ sc,
+                          float* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double d = 0;
+  for (int i = 0; i < RPARTS; ++i) d += part[(long)b * RPARTS + i];
+  sc[b].ddf = d + sc[b].dmacc * sc[b].dm;
+  out[5 * b + 0] = (float)sc[b].score;
+  out[5 * b + 1] = (float)sc[b].dpsi;
+  out[5 * b + 2] = (float)sc[b].gx;
+  out[5 * b + 3] = (float)sc[b].gy;
+  out[5 * b + 4] = (float)sc[b].ddf;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define RDISPATCH(L, FN, ...)                \
+  switch (L) {                               \
+    case 64: FN<64>(__VA_ARGS__); break;     \
+    case 96: FN<96>(__VA_ARGS__); break;     \
+    case 128: FN<128>(__VA_ARGS__); break;   \
+    case 160: FN<160>(__VA_ARGS__); break;   \
+    case 192: FN<192>(__VA_ARGS__); break;   \
+    case 256: FN<256>(__VA_ARGS__); break;   \
+    case 320: FN<320>(__VA_ARGS__); break;   \
+    case 384: FN<384>(__VA_ARGS__); break;   \
+    case 448: FN<448>(__VA_ARGS__); break;   \
+    case 512: FN<512>(__VA_ARGS__); break;   \
+    case 576: FN<576>(__VA_ARGS__); break;   \
+    case 640: FN<640>(__VA_ARGS__); break;   \
+    case 720: FN<720>(__VA_ARGS__); break;   \
+    case 768: FN<768>(__VA_ARGS__); break;   \
+    case 960: FN<960>(__VA_ARGS__); break;   \
+    case 1024: FN<1024>(__VA_ARGS__); break; \
+    default: break;                          \
+  }
+
+template <int L>
+static void ref_forward_t(const RefArgs& a, hipStream_t s) {
+  const dim3 grid(RPARTS, a.nb), blk(WG);
+  const int small = (a.nb + 63) / 64;
+  hipLaunchKernelGGL((k_ref_weight<L>), grid, blk, 0, s, a.white, a.cand, a.w, a.A, a.part);
+  hipLaunchKernelGGL(k_ref_norm_m, dim3(small), dim3(64), 0, s, a.part, a.nb, a.count, a.sc,
+                     a.pre);
+  launch_cols_inv_T(L, a.A, a.nb, a.gtmp, a.twL, s);
+  launch_rows_c2r(L, a.gtmp, a.nb, a.Tun, a.twL, a.twM, s);
+  launch_rows_r2c_rotb(L, a.Tun, a.nb, a.cs, a.pre, a.dmask, a.spec, a.rot_part, a.twL, a.twM, s);
+  hipLaunchKernelGGL(k_ref_rot_stats, dim3(small), dim3(64), 0, s, a.rot_part,
+                     (L + Geo<L>::RB - 1) / Geo<L>::RB, a.nb, (double)L * L, a.sc, a.bscale);
+  launch_cols_fwd(L, a.spec, a.nb, 1.0f, a.bscale, 1, a.twL, s);
+  hipLaunchKernelGGL((k_ref_score<L>), grid, blk, 0, s, a.tiles, a.spec, a.cand, a.part);
+  hipLaunchKernelGGL(k_ref_score_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, L, a.sc);
+}
+
+template <int L>
+static void ref_backward_t(const RefArgs& a, hipStream_t s) {
+  const dim3 grid(RPARTS, a.nb), blk(WG);
+  const int small = (a.nb + 63) / 64;
+  // d/dThat -> d/dRn (C2R, / L^2 applied in the next kernels)
+  hipLaunchKernelGGL((k_ref_grad_spec<L>), grid, blk, 0, s, a.tiles, a.spec, a.cand, a.sc, a.A);
+  launch_cols_inv_T(L, a.A, a.nb, a.gtmp, a.twL, s);
+  launch_rows_c2r(L, a.gtmp, a.nb, a.gRn, a.twL, a.twM, s);
+  hipLaunchKernelGGL((k_ref_norm_red<L>), grid, blk, 0, s, a.Tun, a.gRn, a.cs, a.pre, a.sc,
+                     a.dmask, a.part);
+  hipLaunchKernelGGL(k_ref_norm_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, (double)L * L,
+                     a.sc);
+  hipLaunchKernelGGL((k_ref_rot_grad<L>), grid, blk, 0, s, a.Tun, a.gRn, a.cs, a.pre, a.sc,
+                     a.dmask, a.gTb, a.part);
+  hipLaunchKernelGGL(k_ref_rot_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc);
+  hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.gRn);
+  // d/dT (in gRn) -> d/dA: unnormalised R2C
+  launch_rows_r2c_img(L, a.gRn, a.nb, a.A, a.twL, a.twM, s);
+  launch_cols_fwd(L, a.A, a.nb, 1.0f, nullptr, 0, a.twL, s);
+  hipLaunchKernelGGL((k_ref_df<L>), grid, blk, 0, s, a.white, a.A, a.cand, a.w, a.part);
+  hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.out);
+}
+
+void launch_ref_forward(int L, const RefArgs& a, hipStream_t s) { RDISPATCH(L, ref_forward_t, a, s); }
+void launch_ref_backward(int L, const RefArgs& a, hipStream_t s) {
+  RDISPATCH(L, ref_backward_t, a, s);
+}
+int ref_parts() { return RPARTS; }
+
+}  // namespace gs

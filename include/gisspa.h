@@ -88,6 +88,22 @@ enum {
                          kernel per stage, host reduction after every tile */
 };
 
+/* A candidate for the refinement (gs_adjoint, gs_refine). */
+typedef struct gs_peak {
+  int mic, tile, view;  /* as in gs_particle */
+  float cx, cy;         /* picked centre in micrograph pixels (tile offset + peak position) */
+  float psi;            /* in-plane rotation of the template, degrees: euler3
+                           (gs_particle.psi prints 360 - euler3) */
+  float dx, dy;         /* sub-pixel shift of the centre, pixels */
+  float defocus;        /* defocus of the template CTF weighting, um (positive underfocus) */
+} gs_peak;
+
+/* Refinement objective of a candidate and its gradient. */
+typedef struct gs_grad {
+  float score;
+  float d_psi, d_dx, d_dy, d_defocus;  /* per degree, pixel, pixel, um */
+} gs_grad;
+
 /* Per (psi, tile, view) correlation result of norm_type 0
  * (main.cu:856-881): peak of the border-excluded CC map, its position
  * y*l + x in the tile, sums over the whole map, and the score. */
@@ -196,6 +212,28 @@ int gs_get_rotated_spectra(void* ctx, int psi_index, float* out, size_t cap); /*
  * out_ms[1] = mean duration of one pass-1 / pass-2 launch; *maps_per_launch
  * = CC maps one launch covers.  No reference counterpart (measurement). */
 int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* maps_per_launch);
+
+/* Refinement objective and its gradient for n candidates of the current
+ * micrograph batch (no reference counterpart; the reference stops at the grid
+ * search of main.cu:1230-1257).  The objective is the picker's normalised
+ * CC continued to non-grid parameters: the view's whitened template spectrum
+ * weighted with the candidate's defocus (apply_weighting_function,
+ * GPU_func.cu:249-290) and normalised (main.cu:444-452), back to real space,
+ * rotated by psi with bilinear interpolation (rotate_subIMG), masked and
+ * normalised (main.cu:669-716), correlated with the tile spectrum
+ * (compute_corner_CCG), and evaluated at the shifted centre by the Fourier
+ * series of the CC map, divided by the map's standard deviation (Parseval).
+ * At a grid psi, zero shift and the picking defocus it equals the picker's
+ * score except that the standard deviation keeps the peak pixel.  The
+ * gradient comes from a hand-written adjoint of every stage (FFTs, bilinear
+ * rotation, normalisation, CTF weighting). */
+int gs_adjoint(void* ctx, const gs_peak* peaks, int n, gs_grad* out);
+/* Local refinement of n candidates in place: `iters` steps of gradient
+ * ascent on the objective of gs_adjoint (step length adapted per candidate,
+ * psi within one phistep, the shift within 1.5 pixels and the defocus within
+ * 0.5 um of the start); out (may be null) receives the final objective and
+ * gradient. */
+int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out);
 
 int gs_get_info(void* ctx, gs_info* info);
 int gs_get_timing(void* ctx, gs_timing* t);

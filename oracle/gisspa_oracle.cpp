@@ -274,6 +274,42 @@ static double template_weight(int x, int y, int l, const or_params* p, const or_
   return f;
 }
 
+/* Template J up to whitening (main.cu:182-430): pad into the centre, edge
+ * normalise, FFT / l^2, divide by the radial amplitude (whiten_Tmp). */
+static void template_whiten(const or_params* p, const float* tm, int bx, int by, cd* t) {
+  const int l = p->l;
+  const long L2 = (long)l * l;
+  /* pad into the centre, main.cu:207-216 */
+  for (long i = 0; i < L2; ++i) t[i] = 0;
+  int sx = (l - bx) / 2, sy = (l - by) / 2;
+  for (int j = 0; j < by; ++j)
+    for (int i = 0; i < bx; ++i) t[(long)(sy + j) * l + sx + i] = cd(tm[(long)j * bx + i], 0);
+  /* edge normalise, main.cu:366-383, GPU_func.cu:85-94 (em = 0) */
+  double s1 = 0, s2 = 0;
+  for (long i = 0; i < L2; ++i) { double v = t[i].real(); s1 += v; s2 += v * v; }
+  double mean = s1 / (double)L2;
+  double sigma = std::sqrt(s2 / (double)L2 - mean * mean);
+  if (sigma < 0 || !std::isfinite(sigma)) sigma = 0;
+  if (!(sigma < REF_EPS && sigma > -REF_EPS))
+    for (long i = 0; i < L2; ++i) t[i] = cd((t[i].real() - 0.0) / sigma, t[i].imag());
+  /* FFT, /l^2, main.cu:410-412 */
+  fft2d(t, l, l, -1);
+  for (long i = 0; i < L2; ++i) t[i] /= (double)L2;
+  /* radial power, GPU_func.cu:98-128 (x <= l/2 half plane) */
+  std::vector<double> ra(RA_SIZE, 0.0), rb(RA_SIZE, 0.0);
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x <= l / 2; ++x) {
+      int r = radial_bin(x, y, l, l);
+      if (r < l / 2 && r >= 0) { ra[r] += std::norm(t[(long)y * l + x]); rb[r] += 1; }
+    }
+  /* whiten, GPU_func.cu:130-152 */
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) {
+      int r = radial_bin(x, y, l, l);
+      if (r < l / 2 && r >= 0) t[(long)y * l + x] /= std::sqrt(ra[r] / rb[r]);
+    }
+}
+
 static int template_prep_impl(const or_params* p, const or_ctf* c, const float* tmpl, int ntmpl,
                               int bx, int by, double* out) {
   const int l = p->l;
@@ -282,35 +318,7 @@ static int template_prep_impl(const or_params* p, const or_ctf* c, const float* 
   cd* T = reinterpret_cast<cd*>(out);
   for (int J = 0; J < ntmpl; ++J) {
     cd* t = T + J * L2;
-    /* pad into the centre, main.cu:207-216 */
-    for (long i = 0; i < L2; ++i) t[i] = 0;
-    int sx = (l - bx) / 2, sy = (l - by) / 2;
-    for (int j = 0; j < by; ++j)
-      for (int i = 0; i < bx; ++i) t[(long)(sy + j) * l + sx + i] = cd(tmpl[(long)J * bx * by + (long)j * bx + i], 0);
-    /* edge normalise, main.cu:366-383, GPU_func.cu:85-94 (em = 0) */
-    double s1 = 0, s2 = 0;
-    for (long i = 0; i < L2; ++i) { double v = t[i].real(); s1 += v; s2 += v * v; }
-    double mean = s1 / (double)L2;
-    double sigma = std::sqrt(s2 / (double)L2 - mean * mean);
-    if (sigma < 0 || !std::isfinite(sigma)) sigma = 0;
-    if (!(sigma < REF_EPS && sigma > -REF_EPS))
-      for (long i = 0; i < L2; ++i) t[i] = cd((t[i].real() - 0.0) / sigma, t[i].imag());
-    /* FFT, /l^2, main.cu:410-412 */
-    fft2d(t, l, l, -1);
-    for (long i = 0; i < L2; ++i) t[i] /= (double)L2;
-    /* radial power, GPU_func.cu:98-128 (x <= l/2 half plane) */
-    std::vector<double> ra(RA_SIZE, 0.0), rb(RA_SIZE, 0.0);
-    for (int y = 0; y < l; ++y)
-      for (int x = 0; x <= l / 2; ++x) {
-        int r = radial_bin(x, y, l, l);
-        if (r < l / 2 && r >= 0) { ra[r] += std::norm(t[(long)y * l + x]); rb[r] += 1; }
-      }
-    /* whiten, GPU_func.cu:130-152 */
-    for (int y = 0; y < l; ++y)
-      for (int x = 0; x < l; ++x) {
-        int r = radial_bin(x, y, l, l);
-        if (r < l / 2 && r >= 0) t[(long)y * l + x] /= std::sqrt(ra[r] / rb[r]);
-      }
+    template_whiten(p, tmpl + (long)J * bx * by, bx, by, t);
     /* low pass + weighting, GPU_func.cu:249-290 */
     for (int y = 0; y < l; ++y)
       for (int x = 0; x < l; ++x) t[(long)y * l + x] *= template_weight(x, y, l, p, c);
@@ -784,5 +792,113 @@ extern "C" int or_correlate_many(const or_params* p, const double* tile_spec, in
         out[(long)t * n + J] = reduce_norm0(ccg.data(), l, p->diameter);
       }
   }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Refinement objective                                                     */
+/* ------------------------------------------------------------------------ */
+/* The reference has no refinement; this is the objective this build's
+ * adjoint differentiates, built from the picker's own stages so that at a
+ * grid psi, zero shift and the template CTF's defocus it is the picker's
+ * score up to the peak exclusion in sigma (main.cu:856-881 drops the peak
+ * pixel; here sigma covers the whole map, so the objective is smooth):
+ *   A(k)  = W(k; defocus) B(k), B = whitened template spectrum (template_whiten),
+ *           W = template_weight with theta's defocus, dfdiff/dfang of c0;
+ *   T     = Re IFFT(A / m), m = amplitude normaliser of A (main.cu:444-452);
+ *   R(x)  = bilinear(T) at the rotation of x by psi (rotate_subIMG,
+ *           GPU_func.cu:496-560, continuous angle, double coordinates);
+ *   Rn    = (mask R - mean) / sd over l^2 (apply_mask, compute_sum_sqr,
+ *           substract_by_mean, divided_by_var);
+ *   Th    = FFT(Rn) / l^2;  P(k) = I(k) conj(Th(k)) (-1)^(kx+ky) sgn
+ *           (compute_corner_CCG, GPU_func.cu:584-615);
+ *   c(u)  = Re sum_k P(k) exp(2 pi i k'.u / l), k' the signed frequency in
+ *           (-l/2, l/2], u = (x0 + dx, y0 + dy): the CC map continued between
+ *           pixels (equal to the inverse FFT of P at integer u);
+ *   sigma = sqrt(sum_k |P(k)|^2 - P(0)^2) (Parseval: the sd of c over the map);
+ *   score = c(u) / sigma.
+ * theta = {psi (degrees, the euler3 rotation), dx, dy (pixels), defocus (um)}. */
+extern "C" int or_template_whitened(const or_params* p, const float* tmpl, int ntmpl, int bx,
+                                    int by, double* out) {
+  const int l = p->l;
+  if (bx > l || by > l) return -2;
+  const long L2 = (long)l * l;
+  cd* T = reinterpret_cast<cd*>(out);
+  for (int J = 0; J < ntmpl; ++J) template_whiten(p, tmpl + (long)J * bx * by, bx, by, T + J * L2);
+  return 0;
+}
+
+extern "C" int or_refine_score(const or_params* p, const or_ctf* c0, const double* white,
+                               const double* tile, double x0, double y0, const double* theta,
+                               double* out) {
+  const int l = p->l;
+  const long L2 = (long)l * l;
+  const cd* B = reinterpret_cast<const cd*>(white);
+  const cd* I = reinterpret_cast<const cd*>(tile);
+  std::vector<cd> A(L2);
+  or_ctf c = *c0;
+  c.defocus_um = theta[3];
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x < l; ++x) A[(long)y * l + x] = B[(long)y * l + x] * template_weight(x, y, l, p, &c);
+  double S = 0, cnt = 0;
+  for (int y = 0; y < l; ++y)
+    for (int x = 0; x <= l / 2; ++x) {
+      int r = radial_bin(x, y, l, l);
+      if (r < l / 2 && r >= 0) { S += std::norm(A[(long)y * l + x]); cnt += 1; }
+    }
+  const double m = std::sqrt(S / (cnt * cnt));
+  if (m != 0)
+    for (long i = 0; i < L2; ++i) A[i] /= m;
+  fft2d(A.data(), l, l, +1);
+  /* rotation, continuous angle */
+  const double ang = theta[0] * REF_PI / 180;
+  const double cose = std::cos(ang), sine = std::sin(ang);
+  std::vector<double> R(L2, 0.0);
+  const double dm = 1.5 * p->diameter, ehw = 6;
+  double s1 = 0, s2 = 0;
+  for (int j = 0; j < l; ++j)
+    for (int i = 0; i < l; ++i) {
+      const double X = i - l / 2, Y = j - l / 2;
+      const double x2 = cose * X + sine * Y + l / 2, y2 = -sine * X + cose * Y + l / 2;
+      double v = 0;
+      if (!(x2 < 0 || x2 > l - 1.0 || y2 < 0 || y2 > l - 1.0)) {
+        int ii = (int)std::floor(x2), jj = (int)std::floor(y2);
+        long k0 = ii + (long)jj * l, k1 = k0 + 1, k2 = k0 + l + 1, k3 = k0 + l;
+        if (ii == l - 1) { k1--; k2--; }
+        if (jj == l - 1) { k2 -= l; k3 -= l; }
+        const double t = x2 - ii, u = y2 - jj, tt = 1 - t, uu = 1 - u;
+        v = A[k0].real() * tt * uu + A[k1].real() * t * uu + A[k2].real() * t * u + A[k3].real() * tt * u;
+      }
+      const double r = std::hypot(X, Y);
+      if (r > dm / 2 + 2 * ehw) v = 0;
+      else if (r >= dm / 2) v *= 0.5 * std::cos(REF_PI * (r - dm / 2) / (2 * ehw)) + 0.5;
+      R[(long)j * l + i] = v;
+      s1 += v;
+      s2 += v * v;
+    }
+  const double mean = s1 / L2, var = s2 / L2 - mean * mean;
+  const double sd = var > 0 ? std::sqrt(var) : 1.0;
+  std::vector<cd> Th(L2);
+  for (long i = 0; i < L2; ++i) Th[i] = cd((R[i] - mean) / sd, 0);
+  fft2d(Th.data(), l, l, -1);
+  const double sgn = ((l / 2) % 2 == 1) ? -1.0 : 1.0;
+  const double ux = x0 + theta[1], uy = y0 + theta[2];
+  double cc = 0, sq = 0;
+  for (int ky = 0; ky < l; ++ky) {
+    const double kys = ky <= l / 2 ? ky : ky - l;
+    for (int kx = 0; kx < l; ++kx) {
+      const double kxs = kx <= l / 2 ? kx : kx - l;
+      const cd P = I[(long)ky * l + kx] * std::conj(Th[(long)ky * l + kx] / (double)L2) *
+                   (((kx + ky) & 1) ? -sgn : sgn);
+      sq += std::norm(P);
+      const double ph = 2 * M_PI * (kxs * ux + kys * uy) / l;
+      cc += P.real() * std::cos(ph) - P.imag() * std::sin(ph);
+    }
+  }
+  const cd P0 = I[0] * std::conj(Th[0] / (double)L2) * sgn;
+  const double sigma = std::sqrt(sq - P0.real() * P0.real());
+  out[0] = sigma > 0 ? cc / sigma : 0;
+  out[1] = cc;
+  out[2] = sigma;
   return 0;
 }

@@ -89,6 +89,13 @@ int or_pick_norm1(const or_params* p, const or_ctf* c, const float* tmpl, int nt
 int or_select_norm0(const or_params* p, const or_mapstat* st, int npsi, int psi_begin, int ntmpl,
                     int nx, int ny, or_pick1* out, int cap);
 int or_num_psi(double phi_step);
+/* Refinement objective (no reference counterpart; defined in the .cpp):
+ * whitened template spectra (l x l complex per template) and the score of
+ * one candidate at theta = {psi deg, dx, dy, defocus um}; out = {score, c, sigma}. */
+int or_template_whitened(const or_params* p, const float* tmpl, int ntmpl, int bx, int by,
+                         double* out);
+int or_refine_score(const or_params* p, const or_ctf* c0, const double* white,
+                    const double* tile, double x0, double y0, const double* theta, double* out);
 float or_psi_value(double phi_step, int index);
 
 #ifdef __cplusplus

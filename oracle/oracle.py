@@ -69,6 +69,8 @@ def load() -> ctypes.CDLL:
         lib.or_select_norm0.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
         lib.or_num_psi.argtypes = [ctypes.c_double]
+        lib.or_template_whitened.argtypes = [P, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        lib.or_refine_score.argtypes = [P, C, vp, vp, ctypes.c_double, ctypes.c_double, vp, vp]
         lib.or_psi_value.argtypes = [ctypes.c_double, ctypes.c_int]
         lib.or_psi_value.restype = ctypes.c_float
         _lib = lib
@@ -142,6 +144,42 @@ def micrograph_prep(p: OrParams, c: OrCtf, img: np.ndarray) -> np.ndarray:
                                   out.ctypes.data, T)
     assert r == T, r
     return out
+
+
+def template_whitened(p: OrParams, tm: np.ndarray) -> np.ndarray:
+    """Whitened template spectra [n][l][l] (before the CTF weighting)."""
+    tm = np.ascontiguousarray(tm, dtype=np.float32)
+    n, by, bx = tm.shape
+    out = np.zeros((n, p.l, p.l), dtype=np.complex128)
+    r = load().or_template_whitened(ctypes.byref(p), tm.ctypes.data, n, bx, by, out.ctypes.data)
+    assert r == 0, r
+    return out
+
+
+def refine_score(p: OrParams, c0: OrCtf, white: np.ndarray, tile: np.ndarray, x0: float,
+                 y0: float, theta) -> tuple[float, float, float]:
+    """Refinement objective of one candidate: (score, c, sigma) at
+    theta = (psi degrees, dx, dy, defocus um)."""
+    w = np.ascontiguousarray(white, dtype=np.complex128)
+    t = np.ascontiguousarray(tile, dtype=np.complex128)
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    out = np.zeros(3, dtype=np.float64)
+    load().or_refine_score(ctypes.byref(p), ctypes.byref(c0), w.ctypes.data, t.ctypes.data,
+                           float(x0), float(y0), th.ctypes.data, out.ctypes.data)
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+def refine_grad_fd(p: OrParams, c0: OrCtf, white, tile, x0, y0, theta, h=1e-3) -> np.ndarray:
+    """FP64 central differences of refine_score in each of the 4 parameters."""
+    g = np.zeros(4)
+    for i in range(4):
+        tp = np.array(theta, dtype=np.float64)
+        tm_ = tp.copy()
+        tp[i] += h
+        tm_[i] -= h
+        g[i] = (refine_score(p, c0, white, tile, x0, y0, tp)[0] -
+                refine_score(p, c0, white, tile, x0, y0, tm_)[0]) / (2 * h)
+    return g
 
 
 def num_psi(phi_step: float) -> int:

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
+from tests import _cases
 
 RNG = np.random.default_rng(1234)
 
@@ -330,3 +331,58 @@ def test_oracle_finds_embedded_particles():
                and s["view"] == part.view and s["psi_index"] == ip]
         found += bool(hit)
     assert found == len(mg.particles)
+
+
+# ---------------------------------------------------------------------------
+# refinement objective
+# ---------------------------------------------------------------------------
+def _refine_case():
+    c = _cases.tiny(seed=0, l=128)
+    op = orc.params(c.l, **c.params)
+    ref = orc.pick_norm0(op, orc.ctf(c.defocus), c.templates, c.image)
+    ip, t, j = np.unravel_index(np.argmax(ref["score"]), ref["score"].shape)
+    white = orc.template_whitened(op, c.templates)
+    tiles = orc.micrograph_prep(op, orc.ctf(c.defocus), c.image)
+    return c, op, ref, (int(ip), int(t), int(j)), white, tiles
+
+
+def test_refine_objective_is_the_picker_score_at_the_grid():
+    """At a grid psi, zero shift and the template CTF's defocus, the
+    objective's c(u0) and sigma give back the picker's score once the peak
+    pixel is dropped from sigma the way main.cu:856-881 does."""
+    c, op, ref, (ip, t, j), white, tiles = _refine_case()
+    L = c.l
+    st = ref[ip, t, j]
+    x0, y0 = st["pos"] % L, st["pos"] // L
+    theta = (orc.psi_value(op.phi_step, ip), 0.0, 0.0, c.defocus)
+    score, cc, sigma = orc.refine_score(op, orc.ctf(c.defocus), white[j], tiles[t], x0, y0, theta)
+    L2 = L * L
+    rc = L2 - 1.0
+    s2 = L2 * sigma * sigma          # sum of c^2 (the map has zero mean)
+    sd_excl = np.sqrt((s2 - cc * cc) / rc - (-cc / rc) ** 2)
+    assert abs(cc / sd_excl - st["score"]) <= 2e-6 * abs(st["score"])
+    assert abs(cc - st["peak"]) <= 2e-6 * abs(st["peak"])
+
+
+def test_refine_objective_is_smooth_and_shift_consistent():
+    """The objective is smooth in the shift (a Fourier shift): central
+    differences at h and h/2 agree to 1e-6.  In psi (bilinear interpolation
+    cells) and defocus (|CTF| at its zeros) it is piecewise smooth: the two
+    difference quotients agree to 1e-3 of the gradient's scale.  An integer
+    shift of u lands on the picker's map value at the neighbouring pixel."""
+    c, op, ref, (ip, t, j), white, tiles = _refine_case()
+    L = c.l
+    st = ref[ip, t, j]
+    x0, y0 = st["pos"] % L, st["pos"] // L
+    theta = (orc.psi_value(op.phi_step, ip) + 0.37, 0.21, -0.33, c.defocus + 0.013)
+    g1 = orc.refine_grad_fd(op, orc.ctf(c.defocus), white[j], tiles[t], x0, y0, theta, h=1e-3)
+    g2 = orc.refine_grad_fd(op, orc.ctf(c.defocus), white[j], tiles[t], x0, y0, theta, h=5e-4)
+    assert np.all(np.abs(g1 - g2)[1:3] <= 1e-6 * np.abs(g1)[1:3]), (g1, g2)
+    assert np.all(np.abs(g1 - g2) <= 1e-3 * np.abs(g1).max()), (g1, g2)
+    # integer shift: c(u0 + (1, 0)) is the map value at pos + 1
+    treal = orc.template_prep(op, orc.ctf(c.defocus), c.templates[j:j + 1])
+    spec = orc.rotate_normalize(op, treal, orc.psi_value(op.phi_step, ip))
+    cmap = orc.cc_map(op, tiles[t], spec[0])
+    _, cc, _ = orc.refine_score(op, orc.ctf(c.defocus), white[j], tiles[t], x0, y0,
+                                (orc.psi_value(op.phi_step, ip), 1.0, 0.0, c.defocus))
+    assert abs(cc - cmap[y0, x0 + 1]) <= 1e-6 * np.abs(cmap).max()
