@@ -812,9 +812,11 @@ extern "C" int or_correlate_many(const or_params* p, const double* tile_spec, in
  *           substract_by_mean, divided_by_var);
  *   Th    = FFT(Rn) / l^2;  P(k) = I(k) conj(Th(k)) (-1)^(kx+ky) sgn
  *           (compute_corner_CCG, GPU_func.cu:584-615);
- *   c(u)  = Re sum_k P(k) exp(2 pi i k'.u / l), k' the signed frequency in
- *           (-l/2, l/2], u = (x0 + dx, y0 + dy): the CC map continued between
- *           pixels (equal to the inverse FFT of P at integer u);
+ *   c(u)  = sum_{kx <= l/2} w Re(P(k) exp(2 pi i k'.u / l)), w = 1 on the
+ *           columns kx = 0 and l/2 and 2 between them, k' = (kx, signed ky in
+ *           (-l/2, l/2]), u = (x0 + dx, y0 + dy): the Hermitian-completed
+ *           Fourier series of the CC map, equal to the inverse FFT of P at
+ *           integer u and continued between pixels;
  *   sigma = sqrt(sum_k |P(k)|^2 - P(0)^2) (Parseval: the sd of c over the map);
  *   score = c(u) / sigma.
  * theta = {psi (degrees, the euler3 rotation), dx, dy (pixels), defocus (um)}. */
@@ -887,12 +889,13 @@ extern "C" int or_refine_score(const or_params* p, const or_ctf* c0, const doubl
   for (int ky = 0; ky < l; ++ky) {
     const double kys = ky <= l / 2 ? ky : ky - l;
     for (int kx = 0; kx < l; ++kx) {
-      const double kxs = kx <= l / 2 ? kx : kx - l;
       const cd P = I[(long)ky * l + kx] * std::conj(Th[(long)ky * l + kx] / (double)L2) *
                    (((kx + ky) & 1) ? -sgn : sgn);
       sq += std::norm(P);
-      const double ph = 2 * M_PI * (kxs * ux + kys * uy) / l;
-      cc += P.real() * std::cos(ph) - P.imag() * std::sin(ph);
+      if (kx > l / 2) continue;
+      const double w = (kx == 0 || kx == l / 2) ? 1.0 : 2.0;
+      const double ph = 2 * M_PI * (kx * ux + kys * uy) / l;
+      cc += w * (P.real() * std::cos(ph) - P.imag() * std::sin(ph));
     }
   }
   const cd P0 = I[0] * std::conj(Th[0] / (double)L2) * sgn;
