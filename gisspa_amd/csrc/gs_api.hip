@@ -903,7 +903,7 @@ int ref_ensure(Ctx* c, int nb) {
   const size_t L = c->L, H = c->geo.H, HP = c->geo.HP;
   const size_t nrb = (L + c->geo.RB - 1) / c->geo.RB;
   CHK(c->r_cand.ensure(nb * sizeof(RefCand)));
-  CHK(c->r_cs.ensure(nb * sizeof(float2)));
+  CHK(c->r_cs.ensure(nb * sizeof(double2)));
   CHK(c->r_A.ensure(nb * H * L * sizeof(float2)));
   CHK(c->r_gtmp.ensure(nb * L * HP * sizeof(float2)));
   CHK(c->r_Tun.ensure(nb * L * L * sizeof(float)));
@@ -922,7 +922,7 @@ int ref_ensure(Ctx* c, int nb) {
 
 // candidates of the current batch in kernel form (checks every index)
 int ref_cands(const Ctx* c, const gs_peak* pk, int n, std::vector<RefCand>& rc,
-              std::vector<float2>& cs) {
+              std::vector<double2>& cs) {
   rc.resize(n);
   cs.resize(n);
   for (int i = 0; i < n; ++i) {
@@ -938,14 +938,14 @@ int ref_cands(const Ctx* c, const gs_peak* pk, int n, std::vector<RefCand>& rc,
       return GS_ERR_ARG;
     rc[i] = RefCand{ms.tile0 + p.tile, p.view, x0, y0, p.psi, p.dx, p.dy, p.defocus};
     const double a = (double)p.psi * kRefPiD / 180;  // rotate_subIMG's angle (GPU_func.cu:498-499)
-    cs[i] = make_float2((float)std::cos(a), (float)std::sin(a));
+    cs[i] = make_double2(std::cos(a), std::sin(a));
   }
   return GS_OK;
 }
 
 // objective and gradient of n candidates: out[5 i ..] = score, d psi, d dx,
 // d dy, d defocus
-int ref_eval(Ctx* c, const RefCand* rc, const float2* cs, int n, float* out) {
+int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out) {
   CHK(ensure_white(c));
   hipStream_t s = c->stream;
   const int nb0 = std::min(n, kRefBatch);
@@ -953,7 +953,7 @@ int ref_eval(Ctx* c, const RefCand* rc, const float2* cs, int n, float* out) {
   for (int b0 = 0; b0 < n; b0 += kRefBatch) {
     const int nb = std::min(kRefBatch, n - b0);
     HIPCHK(hipMemcpyAsync(c->r_cand.p, rc + b0, nb * sizeof(RefCand), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->r_cs.p, cs + b0, nb * sizeof(float2), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->r_cs.p, cs + b0, nb * sizeof(double2), hipMemcpyHostToDevice, s));
     RefArgs a{};
     a.nb = nb;
     a.white = c->twhite.as<float2>();
@@ -962,7 +962,7 @@ int ref_eval(Ctx* c, const RefCand* rc, const float2* cs, int n, float* out) {
     a.w = weight_args(c, c->tctf);
     a.count = c->half_count;
     a.dmask = 1.5f * c->p.diameter;
-    a.cs = c->r_cs.as<float2>();
+    a.cs = c->r_cs.as<double2>();
     a.A = c->r_A.as<float2>();
     a.gtmp = c->r_gtmp.as<float2>();
     a.Tun = c->r_Tun.as<float>();
@@ -1412,7 +1412,7 @@ int gs_adjoint(void* ctx, const gs_peak* peaks, int n, gs_grad* out) {
   if (n == 0) return GS_OK;
   HIPCHK(hipSetDevice(c->device));
   std::vector<RefCand> rc;
-  std::vector<float2> cs;
+  std::vector<double2> cs;
   CHK(ref_cands(c, peaks, n, rc, cs));
   std::vector<float> r((size_t)n * 5);
   CHK(ref_eval(c, rc.data(), cs.data(), n, r.data()));
@@ -1435,7 +1435,7 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
   const double lim[4] = {c->p.phi_step, 1.5, 1.5, 0.5};
   std::vector<gs_peak> cur(peaks, peaks + n), start(peaks, peaks + n), trial(n);
   std::vector<RefCand> rc;
-  std::vector<float2> cs;
+  std::vector<double2> cs;
   std::vector<float> fc((size_t)n * 5), ft((size_t)n * 5);
   CHK(ref_cands(c, cur.data(), n, rc, cs));
   CHK(ref_eval(c, rc.data(), cs.data(), n, fc.data()));

@@ -155,4 +155,52 @@ __device__ __forceinline__ float ctf_ast_f(int x1, int y1, int nx, int ny, const
   return v;
 }
 
+
+// The refinement's rotation sample at output pixel (x, y): bilinear value of
+// src at the rotation of (x - L/2, y - L/2) by cs = (cos, sin), coordinates
+// in double precision, the boundary rules of rotate_subIMG
+// (GPU_func.cu:496-560); also the slopes d/dx2, d/dy2 and the mask factor of
+// apply_mask (GPU_func.cu:229-247).
+struct RotSampleD {
+  float tb, dtx, dty, mask;
+  float X, Y;
+  bool in;
+};
+template <int L>
+__device__ __forceinline__ RotSampleD rot_sample_d(const float* src, int x, int y, double2 cs,
+                                                   float dmask) {
+  RotSampleD o;
+  const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
+  o.X = xf;
+  o.Y = yf;
+  const float r = hypotf(xf, yf);
+  const float dm2 = dmask * 0.5f, ehw = 6.0f;
+  o.mask = r > dm2 + 2 * ehw ? 0.0f
+                             : (r >= dm2 ? 0.5f * cosf(kRefPi * (r - dm2) / (2 * ehw)) + 0.5f : 1.0f);
+  o.tb = o.dtx = o.dty = 0.0f;
+  o.in = false;
+  if (o.mask == 0.0f) return o;
+  const double x2 = cs.x * xf + cs.y * yf + (double)(L / 2);
+  const double y2 = -cs.y * xf + cs.x * yf + (double)(L / 2);
+  if (x2 < 0 || x2 > L - 1.0 || y2 < 0 || y2 > L - 1.0) return o;
+  o.in = true;
+  const int ii = (int)floor(x2), jj = (int)floor(y2);
+  long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
+  if (ii == L - 1) { k1--; k2--; }
+  if (jj == L - 1) { k2 -= L; k3 -= L; }
+  const float t = (float)(x2 - ii), u = (float)(y2 - jj);
+  const float tt = 1.0f - t, uu = 1.0f - u;
+  const float a0 = src[k0], a1 = src[k1], a2 = src[k2], a3 = src[k3];
+  o.tb = a0 * tt * uu + a1 * t * uu + a2 * t * u + a3 * tt * u;
+  o.dtx = (a1 - a0) * uu + (a2 - a3) * u;
+  o.dty = (a3 - a0) * tt + (a2 - a1) * t;
+  return o;
+}
+template <int L>
+__device__ __forceinline__ float rot_value_d(const float* src, int x, int y, double2 cs,
+                                             float dmask, float pre) {
+  const RotSampleD o = rot_sample_d<L>(src, x, y, cs, dmask);
+  return o.tb * pre * o.mask;
+}
+
 }  // namespace gs

@@ -53,7 +53,7 @@ struct RowSrc {
   const float* treal;
   float cose, sine, dmask;
   double* partials;
-  const float2* cs_b;
+  const double2* cs_b;  // SRC_ROTB: coordinates in double precision
   const float* pre_b;
   // SRC_IMG: real images [b][L][L]
   const float* imgs;
@@ -73,10 +73,14 @@ __device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x,
     return (float)((double)s.img[(long)gy * s.nx + gx] - (double)aux);
   } else if constexpr (SRC == SRC_IMG) {
     return s.imgs[((long)b * L + y) * L + x];
+  } else if constexpr (SRC == SRC_ROTB) {
+    // the refinement's rotation: continuous angle, double-precision
+    // coordinates (the cells of the FP64 objective), same mask
+    return rot_value_d<L>(s.treal + (long)b * L * L, x, y, s.cs_b[b], s.dmask, s.pre_b[b]);
   } else {
     // rotate_subIMG (GPU_func.cu:496-560) then apply_mask (GPU_func.cu:229-247)
-    const float cose = SRC == SRC_ROTB ? s.cs_b[b].x : s.cose;
-    const float sine = SRC == SRC_ROTB ? s.cs_b[b].y : s.sine;
+    const float cose = s.cose;
+    const float sine = s.sine;
     const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
     const float r = hypotf(xf, yf);
     const float dm2 = s.dmask * 0.5f;  // d_m = 1.5 * diameter, radius d_m / 2
@@ -99,7 +103,6 @@ __device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x,
       const float p0 = src[k0] * tt * uu, p1 = src[k1] * t * uu;
       const float p3 = src[k3] * tt * u, p2 = src[k2] * t * u;
       v = p0 + p1 + p2 + p3;
-      if constexpr (SRC == SRC_ROTB) v *= s.pre_b[b];
     }
     if (r >= dm2) v *= 0.5f * cosf(kRefPi * (r - dm2) / (2 * ehw)) + 0.5f;
     return v;
@@ -1043,7 +1046,7 @@ void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float si
 }
 
 template <int L>
-static void rows_r2c_rotb_t(const float* treal, int nb, const float2* cs_b, const float* pre_b,
+static void rows_r2c_rotb_t(const float* treal, int nb, const double2* cs_b, const float* pre_b,
                             float dmask, float2* out, double* partials, const float2* twL,
                             const float2* twM, hipStream_t s) {
   RowSrc rs{};
@@ -1051,7 +1054,7 @@ static void rows_r2c_rotb_t(const float* treal, int nb, const float2* cs_b, cons
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
   hipLaunchKernelGGL((k_rows_r2c<L, SRC_ROTB>), grid, dim3(WG), 0, s, rs, out, twL, twM);
 }
-void launch_rows_r2c_rotb(int L, const float* treal, int nb, const float2* cs_b, const float* pre_b,
+void launch_rows_r2c_rotb(int L, const float* treal, int nb, const double2* cs_b, const float* pre_b,
                           float dmask, float2* out, double* partials, const float2* twL,
                           const float2* twM, hipStream_t s) {
   DISPATCH(L, rows_r2c_rotb_t, treal, nb, cs_b, pre_b, dmask, out, partials, twL, twM, s);

@@ -73,9 +73,9 @@ void launch_rows_r2c_tiles(int L, const float* img, int nx, int ny, const int* o
 void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float sine, float dmask,
                          float2* out, double* partials, const float2* twL, const float2* twM,
                          hipStream_t s);
-// Same with one angle per b (cs_b[b] = (cos, sin)) and the interpolated
-// value multiplied by pre_b[b] before the mask.
-void launch_rows_r2c_rotb(int L, const float* treal, int nb, const float2* cs_b, const float* pre_b,
+// Same with one angle per b (cs_b[b] = (cos, sin), coordinates in double
+// precision) and the interpolated value multiplied by pre_b[b].
+void launch_rows_r2c_rotb(int L, const float* treal, int nb, const double2* cs_b, const float* pre_b,
                           float dmask, float2* out, double* partials, const float2* twL,
                           const float2* twM, hipStream_t s);
 // Real rows -> half spectrum of real images imgs[b][L][L].
@@ -231,7 +231,7 @@ struct RefArgs {
   WeightArgs w;         // template CTF weighting (defocus replaced per candidate)
   double count;         // cells of the amplitude normaliser
   float dmask;          // 1.5 diameter
-  const float2* cs;     // (cos, sin) of psi per candidate
+  const double2* cs;    // (cos, sin) of psi per candidate
   float2* A;            // [nb][H][L] scratch half spectra
   float2* gtmp;         // [nb][L][HP]
   float* Tun;           // [nb][L][L]

@@ -276,62 +276,25 @@ __global__ __launch_bounds__(WG) void k_ref_grad_spec(const float2* __restrict__
   }
 }
 
-// forward rotation sample of one output pixel (the SRC_ROTB arithmetic of
-// k_rows_r2c) with the bilinear slopes
-struct RotSample {
-  float tb, dtx, dty, mask;  // bilinear value, d/dx2, d/dy2, mask factor
-  float X, Y;                // pixel relative to the centre
-  bool in;                   // inside the window
-};
-template <int L>
-__device__ __forceinline__ RotSample rot_sample(const float* src, int x, int y, float cose,
-                                                float sine, float dmask) {
-  RotSample o;
-  const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
-  o.X = xf;
-  o.Y = yf;
-  const float r = hypotf(xf, yf);
-  const float dm2 = dmask * 0.5f, ehw = 6.0f;
-  o.mask = r > dm2 + 2 * ehw ? 0.0f : (r >= dm2 ? 0.5f * cosf(kRefPi * (r - dm2) / (2 * ehw)) + 0.5f : 1.0f);
-  o.tb = o.dtx = o.dty = 0.0f;
-  o.in = false;
-  if (o.mask == 0.0f) return o;
-  const float x2 = __fadd_rn(__fadd_rn(__fmul_rn(cose, xf), __fmul_rn(sine, yf)), (float)(L / 2));
-  const float y2 = __fadd_rn(__fadd_rn(__fmul_rn(-sine, xf), __fmul_rn(cose, yf)), (float)(L / 2));
-  if (x2 < 0 || x2 > L - 1.0f || y2 < 0 || y2 > L - 1.0f) return o;
-  o.in = true;
-  const int ii = (int)floorf(x2), jj = (int)floorf(y2);
-  long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
-  if (ii == L - 1) { k1--; k2--; }
-  if (jj == L - 1) { k2 -= L; k3 -= L; }
-  const float t = x2 - (float)ii, u = y2 - (float)jj;
-  const float tt = 1.0f - t, uu = 1.0f - u;
-  const float a0 = src[k0], a1 = src[k1], a2 = src[k2], a3 = src[k3];
-  o.tb = a0 * tt * uu + a1 * t * uu + a2 * t * u + a3 * tt * u;
-  o.dtx = (a1 - a0) * uu + (a2 - a3) * u;
-  o.dty = (a3 - a0) * tt + (a2 - a1) * t;
-  return o;
-}
-
 // sums of g = d/dRn and of g Rn over the pixels (normalisation adjoint)
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_norm_red(const float* __restrict__ Tun,
                                                      const float* __restrict__ gRn,
-                                                     const float2* __restrict__ cs_b,
+                                                     const double2* __restrict__ cs_b,
                                                      const float* __restrict__ pre,
                                                      const RefScal* __restrict__ sc, float dmask,
                                                      double* __restrict__ part) {
   constexpr long L2 = (long)L * L;
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
-  const float2 cs = cs_b[b];
+  const double2 cs = cs_b[b];
   const float p = pre[b];
   const double mu = sc[b].mu, isd = 1.0 / sc[b].sd;
   const float* src = Tun + b * L2;
   double acc[2] = {0, 0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
     const int y = (int)(i / L), x = (int)(i - (long)y * L);
-    const RotSample rs = rot_sample<L>(src, x, y, cs.x, cs.y, dmask);
+    const RotSampleD rs = rot_sample_d<L>(src, x, y, cs, dmask);
     const double rn = ((double)(rs.tb * p * rs.mask) - mu) * isd;
     const double g = (double)gRn[b * L2 + i] / (double)L2;
     acc[0] += g;
@@ -361,7 +324,7 @@ __global__ void k_ref_norm_fin(const double* __restrict__ part, int nb, double L
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_rot_grad(const float* __restrict__ Tun,
                                                      const float* __restrict__ gRn,
-                                                     const float2* __restrict__ cs_b,
+                                                     const double2* __restrict__ cs_b,
                                                      const float* __restrict__ pre,
                                                      const RefScal* __restrict__ sc, float dmask,
                                                      float* __restrict__ gTb,
@@ -369,7 +332,7 @@ __global__ __launch_bounds__(WG) void k_ref_rot_grad(const float* __restrict__ T
   constexpr long L2 = (long)L * L;
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
-  const float2 cs = cs_b[b];
+  const double2 cs = cs_b[b];
   const float p = pre[b];
   const double mu = sc[b].mu, isd = 1.0 / sc[b].sd, gm = sc[b].gmean, grn = sc[b].grn;
   const float* src = Tun + b * L2;
@@ -377,7 +340,7 @@ __global__ __launch_bounds__(WG) void k_ref_rot_grad(const float* __restrict__ T
   double acc[2] = {0, 0};
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
     const int y = (int)(i / L), x = (int)(i - (long)y * L);
-    const RotSample rs = rot_sample<L>(src, x, y, cs.x, cs.y, dmask);
+    const RotSampleD rs = rot_sample_d<L>(src, x, y, cs, dmask);
     float out = 0.0f;
     if (rs.in) {
       const double rn = ((double)(rs.tb * p * rs.mask) - mu) * isd;
@@ -385,8 +348,8 @@ __global__ __launch_bounds__(WG) void k_ref_rot_grad(const float* __restrict__ T
       const double grm = (g - gm - rn * grn) * isd;  // d/dRm
       const double gv = grm * rs.mask;                // d/d(Tb p)
       out = (float)(gv * p);                          // d/dTb
-      const float dx2 = (-cs.y * rs.X + cs.x * rs.Y) * kdeg;
-      const float dy2 = (-cs.x * rs.X - cs.y * rs.Y) * kdeg;
+      const float dx2 = (float)(-cs.y * rs.X + cs.x * rs.Y) * kdeg;
+      const float dy2 = (float)(-cs.x * rs.X - cs.y * rs.Y) * kdeg;
       acc[0] += (double)out * (double)(rs.dtx * dx2 + rs.dty * dy2);
       acc[1] += gv * (double)rs.tb;
     }
@@ -419,11 +382,11 @@ __global__ void k_ref_rot_fin(const double* __restrict__ part, int nb, RefScal* 
 // of q.
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gTb,
-                                                    const float2* __restrict__ cs_b, float dmask,
+                                                    const double2* __restrict__ cs_b, float dmask,
                                                     float* __restrict__ gT) {
   constexpr long L2 = (long)L * L;
   const int b = blockIdx.y;
-  const float2 cs = cs_b[b];
+  const double2 cs = cs_b[b];
   const float* gsrc = gTb + b * L2;
   const float reach = dmask * 0.5f + 12.0f + 3.0f;
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
@@ -432,7 +395,7 @@ __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gT
     float acc = 0.0f;
     if (qX * qX + qY * qY <= reach * reach) {
       // inverse rotation: x - c = R^T (q - c)
-      const float cX = cs.x * qX - cs.y * qY, cY = cs.y * qX + cs.x * qY;
+      const float cX = (float)(cs.x * qX - cs.y * qY), cY = (float)(cs.y * qX + cs.x * qY);
       const int x0 = (int)floorf(cX) + L / 2 - 2, y0 = (int)floorf(cY) + L / 2 - 2;
       for (int dy = 0; dy < 5; ++dy) {
         const int y = y0 + dy;
@@ -443,14 +406,14 @@ __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gT
           const float g = gsrc[(long)y * L + x];
           if (g == 0.0f) continue;
           const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
-          const float x2 = __fadd_rn(__fadd_rn(__fmul_rn(cs.x, xf), __fmul_rn(cs.y, yf)), (float)(L / 2));
-          const float y2 = __fadd_rn(__fadd_rn(__fmul_rn(-cs.y, xf), __fmul_rn(cs.x, yf)), (float)(L / 2));
-          if (x2 < 0 || x2 > L - 1.0f || y2 < 0 || y2 > L - 1.0f) continue;
-          const int ii = (int)floorf(x2), jj = (int)floorf(y2);
+          const double x2 = cs.x * xf + cs.y * yf + (double)(L / 2);
+          const double y2 = -cs.y * xf + cs.x * yf + (double)(L / 2);
+          if (x2 < 0 || x2 > L - 1.0 || y2 < 0 || y2 > L - 1.0) continue;
+          const int ii = (int)floor(x2), jj = (int)floor(y2);
           long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
           if (ii == L - 1) { k1--; k2--; }
           if (jj == L - 1) { k2 -= L; k3 -= L; }
-          const float t = x2 - (float)ii, u = y2 - (float)jj;
+          const float t = (float)(x2 - ii), u = (float)(y2 - jj);
           const float tt = 1.0f - t, uu = 1.0f - u;
           float wq = 0.0f;
           if (k0 == i) wq += tt * uu;
