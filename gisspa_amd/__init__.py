@@ -70,6 +70,9 @@ PARTICLE_DTYPE = np.dtype([("psi_index", "<i4"), ("view", "<i4"), ("tile", "<i4"
 PEAK_DTYPE = np.dtype([("mic", "<i4"), ("tile", "<i4"), ("view", "<i4"), ("cx", "<f4"),
                        ("cy", "<f4"), ("psi", "<f4"), ("dx", "<f4"), ("dy", "<f4"),
                        ("defocus", "<f4")])
+# gs_pick_rec: one line of a pick list for the duplicate removal
+PICK_REC_DTYPE = np.dtype([("group", "<i4"), ("pad", "<i4"), ("x", "<f8"), ("y", "<f8"),
+                           ("alt", "<f8"), ("az", "<f8"), ("phi", "<f8"), ("score", "<f8")])
 GRAD_DTYPE = np.dtype([("score", "<f4"), ("d_psi", "<f4"), ("d_dx", "<f4"), ("d_dy", "<f4"),
                        ("d_defocus", "<f4")])
 
@@ -151,6 +154,8 @@ def load() -> ctypes.CDLL:
     lib.gs_refine.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
     lib.gs_time_kernels.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
     lib.gs_destroy.argtypes = [vp]
+    lib.gs_remove_repeats.argtypes = [ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_double,
+                                      ctypes.c_double, vp, ctypes.POINTER(ctypes.c_size_t)]
     _lib = lib
     return lib
 
@@ -166,6 +171,28 @@ def exported_symbols() -> list[str]:
             if m:
                 names.append(m.group(1))
     return names
+
+
+def pick_records(group, x, y, alt, az, phi, score) -> np.ndarray:
+    """PICK_REC_DTYPE array from per-line columns."""
+    n = len(x)
+    r = np.zeros(n, dtype=PICK_REC_DTYPE)
+    r["group"], r["x"], r["y"] = group, x, y
+    r["alt"], r["az"], r["phi"], r["score"] = alt, az, phi, score
+    return r
+
+
+def remove_repeats(recs: np.ndarray, center: float, euler_thres: float,
+                   device: int = 0) -> np.ndarray:
+    """Keep mask of the duplicate removal (gs_remove_repeats; the result of
+    python_scripts/remove_repeat_particles_from_list.py on the same lines)."""
+    recs = np.ascontiguousarray(recs, dtype=PICK_REC_DTYPE)
+    keep = np.zeros(len(recs), dtype=np.uint8)
+    nk = ctypes.c_size_t()
+    _check(load().gs_remove_repeats(device, recs.ctypes.data, len(recs), float(center),
+                                    float(euler_thres), keep.ctypes.data, ctypes.byref(nk)),
+           "gs_remove_repeats")
+    return keep.astype(bool)
 
 
 def make_params(**kw) -> GsParams:

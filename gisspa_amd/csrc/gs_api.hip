@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/gisspa.h"
+#include "gs_host.hpp"
 #include "gs_kernels.hpp"
 
 using namespace gs;
@@ -26,31 +27,6 @@ using namespace gs;
 namespace {
 
 const double kRefPiD = 3.141592654;  // GPU_func.cuh:29
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  int ensure(size_t n) {
-    if (n <= bytes) return GS_OK;
-    release();
-    if (n == 0) return GS_OK;
-    if (hipMalloc(&p, n) != hipSuccess) {
-      (void)hipGetLastError();
-      p = nullptr;
-      return GS_ERR_NOMEM;
-    }
-    bytes = n;
-    return GS_OK;
-  }
-  template <class T>
-  T* as() const { return static_cast<T*>(p); }
-};
 
 // One micrograph of the current batch.
 struct MicSlot {
@@ -131,22 +107,6 @@ struct Ctx {
   int stats_psi_begin = 0, stats_npsi = 0;
   gs_timing timing{};
 };
-
-int hip_ok(hipError_t e) {
-  if (e == hipSuccess) return GS_OK;
-  (void)hipGetLastError();
-  return GS_ERR_DEVICE;
-}
-#define HIPCHK(x)                       \
-  do {                                  \
-    int _r = hip_ok(x);                 \
-    if (_r != GS_OK) return _r;         \
-  } while (0)
-#define CHK(x)                          \
-  do {                                  \
-    int _r = (x);                       \
-    if (_r != GS_OK) return _r;         \
-  } while (0)
 
 int tile_offsets(int n, int l, int overlap, std::vector<int>& offs) {
   // main.cu:228-250

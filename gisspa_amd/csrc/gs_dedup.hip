@@ -1,0 +1,485 @@
+// gs_dedup.hip -- duplicate removal of a pick list on the GPU
+// (python_scripts/remove_repeat_particles_from_list.py:9-145).
+//
+// The script walks the picks of each micrograph in list order.  A pick i
+// that is still unmarked when its turn comes compares itself with every later
+// unmarked pick j whose centre lies closer than `center` pixels and whose
+// orientation is within `euler_thres` degrees (of 0 or of a 180-degree turn);
+// the lower-scored of the two is marked (i on a tie), and i goes on marking
+// later picks even once it has been marked itself.  Only "unmarked when its
+// turn comes" depends on the walk, so the walk is split into
+//   1. the pair relation (every pair of a micrograph, in parallel): sorted
+//      neighbour lists of every pick;
+//   2. active(i) = unmarked at its turn = no active earlier neighbour j with
+//      score_j > score_i, settled in rounds (each round settles every pick
+//      whose earlier neighbours are settled, so at least the first open one);
+//   3. the final mark of an active pick i: a later neighbour j with
+//      score_j >= score_i that no active pick before i had marked.
+// The pair test repeats the script's double-precision operations in order
+// (no FMA contraction; the Euler matrices come from the host's C library, as
+// the script's come from CPython's math module).  The one device
+// transcendental is acos: pairs whose Euler distance lies within 1e-9 degree
+// of a breakpoint of the threshold test are settled on the host with the C
+// library's acos, so the result equals the script's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/gisspa.h"
+#include "gs_host.hpp"
+
+namespace gs {
+namespace {
+
+constexpr double kScriptPi = 3.14159265359;  // remove_repeat_particles_from_list.py:13
+constexpr int DWG = 256;
+constexpr double kAmbiguity = 1e-9;          // degrees
+constexpr int kRoundsPerCheck = 4;
+
+enum : int { ST_OPEN = 0, ST_ACTIVE = 1, ST_MARKED = 2 };
+
+// Threshold test on the quaternion angle (:132-136): the distance itself or
+// its folding about 180 degrees.
+__host__ __device__ inline bool euler_close(double ed, double thr) {
+#pragma clang fp contract(off)
+  double d2 = fmod(ed, 180.0);
+  if (d2 > thr + 1) d2 = 180 - d2;
+  return ed < thr || d2 < thr;
+}
+
+__host__ __device__ inline double euler_angle(double q0) {
+#pragma clang fp contract(off)
+  return fabs(acos(q0) * 2 * 180 / kScriptPi);
+}
+
+// Orientation test of outer pick a (matrix ea) and inner pick b (matrix eb),
+// :62-136, matrices in the script's m_o layout.  Returns 0 (not a duplicate,
+// or a pair the script skips), 1 (duplicate) or 2 (undecided within the
+// acos accuracy; *q0 receives the quaternion component for the host).
+__device__ int pair_orientation(const double* __restrict__ ea, const double* __restrict__ eb,
+                                double thr, double* q0out) {
+#pragma clang fp contract(off)
+  double m[9], mo[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[r * 3 + c] = ea[c * 3 + r];  // the script's m = m_o^T
+  for (int k = 0; k < 9; ++k) mo[k] = eb[k];
+  double rm[9];
+  rm[0] = m[0] * mo[0] + m[3] * mo[1] + m[6] * mo[2];
+  rm[1] = mo[0] * m[1] + mo[1] * m[4] + mo[2] * m[7];
+  rm[2] = mo[0] * m[2] + mo[1] * m[5] + mo[2] * m[8];
+  rm[3] = mo[3] * m[0] + mo[4] * m[3] + mo[5] * m[6];
+  rm[4] = mo[3] * m[1] + mo[4] * m[4] + mo[5] * m[7];
+  rm[5] = mo[3] * m[2] + mo[4] * m[5] + mo[5] * m[8];
+  rm[6] = mo[6] * m[0] + mo[7] * m[3] + mo[8] * m[6];
+  rm[7] = mo[6] * m[1] + mo[7] * m[4] + mo[8] * m[7];
+  rm[8] = mo[6] * m[2] + mo[7] * m[5] + mo[8] * m[8];
+  double q0 = 0, t;
+  const double tr = rm[0] + rm[4] + rm[8];
+  if (tr > 0) {
+    t = sqrt(1 + rm[0] + rm[4] + rm[8]) * 2;  // left to right, as :94
+    if (t == 0) return 0;
+    q0 = 0.25 * t;
+  } else {
+    // the x-dominant case (:102-109) is overwritten by the y/z case below
+    // it in the script; only its t == 0 exit remains
+    if (rm[0] >= rm[4] && rm[0] >= rm[8]) {
+      t = sqrt(1 + rm[0] - rm[4] - rm[8]) * 2;
+      if (t == 0) return 0;
+    }
+    if (rm[4] > rm[8]) {
+      if (1 - rm[0] + rm[4] - rm[8] < 0) return 0;
+      t = sqrt(1 - rm[0] + rm[4] - rm[8]) * 2;
+      if (t == 0) return 0;
+      q0 = (rm[2] - rm[6]) / t;
+    } else {
+      if (1 - rm[0] - rm[4] + rm[8] < 0) return 0;
+      t = sqrt(1 - rm[0] - rm[4] + rm[8]) * 2;
+      if (t == 0) return 0;
+      q0 = (rm[3] - rm[1]) / t;
+    }
+  }
+  if (q0 > 1 || q0 < -1) return 0;
+  const double ed = euler_angle(q0);
+  const bool c = euler_close(ed, thr);
+  if (euler_close(ed - kAmbiguity, thr) != c || euler_close(ed + kAmbiguity, thr) != c) {
+    *q0out = q0;
+    return 2;
+  }
+  return c ? 1 : 0;
+}
+
+struct DedupArgs {
+  const double* x;
+  const double* y;
+  const double* e;     // n x 9 Euler matrices
+  const int* glo;      // first pick of each pick's group
+  const int* ghi;      // one past its last
+  int n;
+  double center, center2, thr;
+};
+
+__device__ bool resolved_duplicate(const unsigned long long* keys, int nk, unsigned long long key) {
+  int lo = 0, hi = nk;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < nk && keys[lo] == key;
+}
+
+// Neighbour relation.  One thread per pick i scans its group in tiles of DWG
+// picks staged through LDS.  FILL = false: count the duplicates of every pick
+// and list the undecided pairs (once, from their outer pick); FILL = true:
+// write each pick's duplicates in index order at off[i], the undecided pairs
+// taken from the host's sorted keys.
+template <bool FILL>
+__global__ __launch_bounds__(DWG) void k_dd_pairs(DedupArgs d, int* __restrict__ cnt,
+                                                  const int* __restrict__ off,
+                                                  int* __restrict__ nbr,
+                                                  int* __restrict__ amb_n, int amb_cap,
+                                                  int2* __restrict__ amb_pair,
+                                                  double* __restrict__ amb_q0,
+                                                  const unsigned long long* __restrict__ keys,
+                                                  int nkeys) {
+#pragma clang fp contract(off)
+  __shared__ double sx[DWG], sy[DWG];
+  const int i0 = blockIdx.x * DWG;
+  const int i = i0 + threadIdx.x;
+  const bool valid = i < d.n;
+  const int jlo = d.glo[i0], jhi = d.ghi[min(i0 + DWG, d.n) - 1];
+  const int lo = valid ? d.glo[i] : 0, hi = valid ? d.ghi[i] : 0;
+  const double xi = valid ? d.x[i] : 0.0, yi = valid ? d.y[i] : 0.0;
+  int count = 0;
+  const int w0 = (FILL && valid) ? off[i] : 0;
+  for (int t0 = jlo; t0 < jhi; t0 += DWG) {
+    __syncthreads();
+    if (t0 + (int)threadIdx.x < jhi) {
+      sx[threadIdx.x] = d.x[t0 + threadIdx.x];
+      sy[threadIdx.x] = d.y[t0 + threadIdx.x];
+    }
+    __syncthreads();
+    const int k0 = max(lo, t0) - t0, k1 = min(hi, min(t0 + DWG, jhi)) - t0;
+    for (int k = k0; k < k1; ++k) {
+      const int j = t0 + k;
+      if (j == i) continue;
+      // (x2 - x1), (y2 - y1) with pick 1 the outer (lower-index) one, :54
+      const double dx = j > i ? sx[k] - xi : xi - sx[k];
+      const double dy = j > i ? sy[k] - yi : yi - sy[k];
+      const double r2 = dx * dx + dy * dy;
+      if (r2 > d.center2) continue;  // conservative pre-test, the exact test follows
+      if (!(fabs(sqrt(r2)) < d.center)) continue;
+      const int a = min(i, j), b = max(i, j);
+      double q0 = 0;
+      int r = pair_orientation(d.e + 9 * (size_t)a, d.e + 9 * (size_t)b, d.thr, &q0);
+      if (r == 2) {
+        if (FILL) {
+          const unsigned long long key = (unsigned long long)a * (unsigned)d.n + (unsigned)b;
+          r = resolved_duplicate(keys, nkeys, key) ? 1 : 0;
+        } else {
+          if (i == a) {
+            const int slot = atomicAdd(amb_n, 1);
+            if (slot < amb_cap) {
+              amb_pair[slot] = make_int2(a, b);
+              amb_q0[slot] = q0;
+            }
+          }
+          r = 0;  // the host adds the resolved duplicates to the counts
+        }
+      }
+      if (r == 1) {
+        // both passes classify every pair alike, so the list has room; the
+        // bound only keeps a write inside it
+        if (FILL && w0 + count < off[i + 1]) nbr[w0 + count] = j;
+        ++count;
+      }
+    }
+  }
+  if (!FILL && valid) cnt[i] = count;
+}
+
+__device__ inline int load_state(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_state(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One settling round: pick i is marked at its turn iff an active earlier
+// neighbour outscores it; it is active iff every earlier neighbour that
+// outscores it is settled and none is active.  States only move from open to
+// settled, so reading a neighbour's old or new state gives the same outcome
+// (at worst one round later).
+__global__ __launch_bounds__(DWG) void k_dd_round(const int* __restrict__ off,
+                                                  const int* __restrict__ nbr,
+                                                  const double* __restrict__ s, int n,
+                                                  int* state, int* __restrict__ open_count) {
+  const int i = blockIdx.x * DWG + threadIdx.x;
+  if (i >= n || load_state(state + i) != ST_OPEN) return;
+  const double si = s[i];
+  bool marked = false, pending = false;
+  for (int k = off[i]; k < off[i + 1]; ++k) {
+    const int j = nbr[k];
+    if (j > i) break;  // lists are in index order
+    if (s[j] > si) {
+      const int st = load_state(state + j);
+      if (st == ST_ACTIVE) {
+        marked = true;
+        break;
+      }
+      if (st == ST_OPEN) pending = true;
+    }
+  }
+  if (marked) store_state(state + i, ST_MARKED);
+  else if (!pending) store_state(state + i, ST_ACTIVE);
+  else atomicAdd(open_count, 1);
+}
+
+// first_mark[j] = the first active earlier neighbour that outscores j (the
+// pick whose turn marks j), n if none.
+__global__ __launch_bounds__(DWG) void k_dd_first_mark(const int* __restrict__ off,
+                                                       const int* __restrict__ nbr,
+                                                       const double* __restrict__ s, int n,
+                                                       const int* __restrict__ state,
+                                                       int* __restrict__ first_mark) {
+  const int j = blockIdx.x * DWG + threadIdx.x;
+  if (j >= n) return;
+  int f = n;
+  for (int k = off[j]; k < off[j + 1]; ++k) {
+    const int i = nbr[k];
+    if (i > j) break;
+    if (state[i] == ST_ACTIVE && s[i] > s[j]) {
+      f = i;
+      break;
+    }
+  }
+  first_mark[j] = f;
+}
+
+// keep[i]: active at its turn and not marked during it, i.e. no later
+// neighbour j with score_j >= score_i that was still unmarked when i's turn
+// reached it (first_mark[j] > i; first_mark[j] == i cannot hold for such j).
+__global__ __launch_bounds__(DWG) void k_dd_keep(const int* __restrict__ off,
+                                                 const int* __restrict__ nbr,
+                                                 const double* __restrict__ s, int n,
+                                                 const int* __restrict__ state,
+                                                 const int* __restrict__ first_mark,
+                                                 unsigned char* __restrict__ keep) {
+  const int i = blockIdx.x * DWG + threadIdx.x;
+  if (i >= n) return;
+  bool k = state[i] == ST_ACTIVE;
+  if (k) {
+    const double si = s[i];
+    for (int q = off[i + 1] - 1; q >= off[i]; --q) {
+      const int j = nbr[q];
+      if (j < i) break;
+      if (si <= s[j] && first_mark[j] > i) {
+        k = false;
+        break;
+      }
+    }
+  }
+  keep[i] = k ? 1 : 0;
+}
+
+// Euler matrix in the script's m_o layout (:63-71), angles converted with
+// the script's pi (:43-45).
+void euler_matrix(double alt_deg, double az_deg, double phi_deg, double* e) {
+#pragma clang fp contract(off)
+  const double alt = alt_deg * kScriptPi / 180, az = az_deg * kScriptPi / 180,
+               phi = phi_deg * kScriptPi / 180;
+  e[0] = std::cos(phi) * std::cos(az) - std::sin(phi) * std::cos(alt) * std::sin(az);
+  e[1] = std::cos(phi) * std::sin(az) + std::sin(phi) * std::cos(alt) * std::cos(az);
+  e[2] = std::sin(phi) * std::sin(alt);
+  e[3] = (-1) * std::sin(phi) * std::cos(az) - std::cos(phi) * std::cos(alt) * std::sin(az);
+  e[4] = (-1) * std::sin(phi) * std::sin(az) + std::cos(phi) * std::cos(alt) * std::cos(az);
+  e[5] = std::cos(phi) * std::sin(alt);
+  e[6] = std::sin(alt) * std::sin(az);
+  e[7] = (-1) * std::sin(alt) * std::cos(az);
+  e[8] = std::cos(alt);
+}
+
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  ~StreamGuard() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center, double thr,
+                   unsigned char* keep, size_t* n_kept) {
+  if (n_kept) *n_kept = 0;
+  if ((!recs || !keep) && n_ > 0) return GS_ERR_ARG;
+  if (n_ > (size_t)(INT_MAX / 16)) return GS_ERR_ARG;
+  if (!(center == center) || !(thr == thr)) return GS_ERR_ARG;  // NaN
+  const int n = (int)n_;
+  if (n == 0) return GS_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    (void)hipGetLastError();
+    return GS_ERR_DEVICE;  // as gs_create
+  }
+  HIPCHK(hipSetDevice(device));
+
+  // host side: structure of arrays, Euler matrices, group ranges
+  std::vector<double> x(n), y(n), s(n), e((size_t)n * 9);
+  std::vector<int> glo(n), ghi(n);
+  for (int i = 0; i < n; ++i) {
+    x[i] = recs[i].x;
+    y[i] = recs[i].y;
+    s[i] = recs[i].score;
+    euler_matrix(recs[i].alt, recs[i].az, recs[i].phi, &e[(size_t)i * 9]);
+  }
+  for (int lo = 0; lo < n;) {
+    int hi = lo + 1;
+    while (hi < n && recs[hi].group == recs[lo].group) ++hi;
+    for (int i = lo; i < hi; ++i) {
+      glo[i] = lo;
+      ghi[i] = hi;
+    }
+    lo = hi;
+  }
+
+  StreamGuard sg;
+  HIPCHK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+  hipStream_t st = sg.s;
+  DevBuf dx, dy, ds, de, dglo, dghi, dcnt, doff, dnbr, damb_n, damb_pair, damb_q0, dkeys,
+      dstate, dopen, dfirst, dkeep;
+  const size_t nd = (size_t)n * sizeof(double), ni = (size_t)n * sizeof(int);
+  CHK(dx.ensure(nd));
+  CHK(dy.ensure(nd));
+  CHK(ds.ensure(nd));
+  CHK(de.ensure(nd * 9));
+  CHK(dglo.ensure(ni));
+  CHK(dghi.ensure(ni));
+  CHK(dcnt.ensure(ni));
+  CHK(doff.ensure(ni + sizeof(int)));
+  CHK(damb_n.ensure(sizeof(int)));
+  HIPCHK(hipMemcpyAsync(dx.p, x.data(), nd, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dy.p, y.data(), nd, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ds.p, s.data(), nd, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(de.p, e.data(), nd * 9, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dglo.p, glo.data(), ni, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dghi.p, ghi.data(), ni, hipMemcpyHostToDevice, st));
+
+  DedupArgs a;
+  a.x = dx.as<double>();
+  a.y = dy.as<double>();
+  a.e = de.as<double>();
+  a.glo = dglo.as<int>();
+  a.ghi = dghi.as<int>();
+  a.n = n;
+  a.center = center;
+  // every r2 with sqrt(r2) < center satisfies r2 <= center^2 (1 + 1e-12)
+  a.center2 = center > 0 ? center * center * (1 + 1e-12) : -1.0;
+  a.thr = thr;
+  const int blocks = (n + DWG - 1) / DWG;
+
+  // 1. counts and undecided pairs (a second pass if the pair list overflowed)
+  int amb_cap = 1024, amb_n = 0;
+  for (;;) {
+    CHK(damb_pair.ensure((size_t)amb_cap * sizeof(int2)));
+    CHK(damb_q0.ensure((size_t)amb_cap * sizeof(double)));
+    HIPCHK(hipMemsetAsync(damb_n.p, 0, sizeof(int), st));
+    k_dd_pairs<false><<<blocks, DWG, 0, st>>>(a, dcnt.as<int>(), nullptr, nullptr,
+                                               damb_n.as<int>(), amb_cap,
+                                               damb_pair.as<int2>(), damb_q0.as<double>(),
+                                               nullptr, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&amb_n, damb_n.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (amb_n <= amb_cap) break;
+    amb_cap = amb_n;
+  }
+  std::vector<int> cnt(n);
+  HIPCHK(hipMemcpyAsync(cnt.data(), dcnt.p, ni, hipMemcpyDeviceToHost, st));
+  std::vector<int2> amb_pair(amb_n);
+  std::vector<double> amb_q0(amb_n);
+  if (amb_n > 0) {
+    HIPCHK(hipMemcpyAsync(amb_pair.data(), damb_pair.p, amb_n * sizeof(int2),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(amb_q0.data(), damb_q0.p, amb_n * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  // undecided pairs: the C library's acos decides
+  std::vector<unsigned long long> keys;
+  for (int k = 0; k < amb_n; ++k) {
+    if (!euler_close(euler_angle(amb_q0[k]), thr)) continue;
+    const int pa = amb_pair[k].x, pb = amb_pair[k].y;
+    keys.push_back((unsigned long long)pa * (unsigned)n + (unsigned)pb);
+    cnt[pa]++;
+    cnt[pb]++;
+  }
+  std::sort(keys.begin(), keys.end());
+  std::vector<int> off(n + 1);
+  long long total = 0;
+  for (int i = 0; i < n; ++i) {
+    off[i] = (int)total;
+    total += cnt[i];
+    if (total > INT_MAX) return GS_ERR_CAPACITY;
+  }
+  off[n] = (int)total;
+  CHK(dnbr.ensure(std::max<size_t>(1, (size_t)total) * sizeof(int)));
+  CHK(dkeys.ensure(std::max<size_t>(1, keys.size()) * sizeof(unsigned long long)));
+  HIPCHK(hipMemcpyAsync(doff.p, off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+  if (!keys.empty())
+    HIPCHK(hipMemcpyAsync(dkeys.p, keys.data(), keys.size() * sizeof(unsigned long long),
+                          hipMemcpyHostToDevice, st));
+
+  // 2. neighbour lists
+  k_dd_pairs<true><<<blocks, DWG, 0, st>>>(a, nullptr, doff.as<int>(), dnbr.as<int>(), nullptr,
+                                            0, nullptr, nullptr, dkeys.as<unsigned long long>(),
+                                            (int)keys.size());
+  HIPCHK(hipGetLastError());
+
+  // 3. settle the picks that are unmarked at their turn, in rounds
+  CHK(dstate.ensure(ni));
+  CHK(dopen.ensure(kRoundsPerCheck * sizeof(int)));
+  HIPCHK(hipMemsetAsync(dstate.p, 0, ni, st));
+  int rounds = 0;
+  for (;;) {
+    HIPCHK(hipMemsetAsync(dopen.p, 0, kRoundsPerCheck * sizeof(int), st));
+    for (int r = 0; r < kRoundsPerCheck; ++r)
+      k_dd_round<<<blocks, DWG, 0, st>>>(doff.as<int>(), dnbr.as<int>(), ds.as<double>(), n,
+                                         dstate.as<int>(), dopen.as<int>() + r);
+    HIPCHK(hipGetLastError());
+    int open = 0;
+    HIPCHK(hipMemcpyAsync(&open, dopen.as<int>() + kRoundsPerCheck - 1, sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    rounds += kRoundsPerCheck;
+    if (open == 0) break;
+    if (rounds > n + kRoundsPerCheck) return GS_ERR_DEVICE;  // every round settles a pick
+  }
+
+  // 4. final marks
+  CHK(dfirst.ensure(ni));
+  CHK(dkeep.ensure(n));
+  k_dd_first_mark<<<blocks, DWG, 0, st>>>(doff.as<int>(), dnbr.as<int>(), ds.as<double>(), n,
+                                          dstate.as<int>(), dfirst.as<int>());
+  k_dd_keep<<<blocks, DWG, 0, st>>>(doff.as<int>(), dnbr.as<int>(), ds.as<double>(), n,
+                                    dstate.as<int>(), dfirst.as<int>(),
+                                    dkeep.as<unsigned char>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(keep, dkeep.p, n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (n_kept) {
+    size_t kept = 0;
+    for (int i = 0; i < n; ++i) kept += keep[i] ? 1 : 0;
+    *n_kept = kept;
+  }
+  return GS_OK;
+}
+
+}  // namespace
+}  // namespace gs
+
+extern "C" int gs_remove_repeats(int device, const gs_pick_rec* recs, size_t n, double center,
+                                 double euler_thres, unsigned char* keep, size_t* n_kept) {
+  return gs::remove_repeats(device, recs, n, center, euler_thres, keep, n_kept);
+}

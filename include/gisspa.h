@@ -235,6 +235,31 @@ int gs_adjoint(void* ctx, const gs_peak* peaks, int n, gs_grad* out);
  * gradient. */
 int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out);
 
+/* One line of a pick list as the duplicate removal reads it
+ * (python_scripts/remove_repeat_particles_from_list.py:41-46,52-61): the
+ * micrograph it belongs to, centre=, euler= and score= of the line. */
+typedef struct gs_pick_rec {
+  int group;             /* micrograph: a group is a run of equal values */
+  int pad;
+  double x, y;           /* centre (pixels) */
+  double alt, az, phi;   /* Euler angles (degrees), phi = the printed psi */
+  double score;
+} gs_pick_rec;
+
+/* Duplicate removal of a pick list on `device`; replaces
+ * python_scripts/remove_repeat_particles_from_list.py:9-145 (same result).
+ * Within each group, walking the list in order, a pick that is still kept
+ * is compared with every later kept pick whose centre lies closer than
+ * `center` pixels and whose relative orientation (quaternion angle) lies
+ * within `euler_thres` degrees of 0 or of 180 (the script's folding); of
+ * the two the lower score is dropped (the earlier one on a tie), and the
+ * earlier pick keeps dropping later ones even once it has been dropped
+ * itself.  keep[i] = 1 for the lines the script writes; *n_kept (may be
+ * null) = their number.  Needs no context; allocates and frees its own
+ * device memory. */
+int gs_remove_repeats(int device, const gs_pick_rec* recs, size_t n, double center,
+                      double euler_thres, unsigned char* keep, size_t* n_kept);
+
 int gs_get_info(void* ctx, gs_info* info);
 int gs_get_timing(void* ctx, gs_timing* t);
 /* Sample per-launch correlation times in gs_pick (see gs_timing). */

@@ -905,3 +905,130 @@ extern "C" int or_refine_score(const or_params* p, const or_ctf* c0, const doubl
   out[2] = sigma;
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Duplicate removal of a pick list
+// (python_scripts/remove_repeat_particles_from_list.py:9-145), restated in
+// C++ with the same double-precision operations in the same order (no FMA
+// contraction; cos/sin/acos/sqrt/fmod from the C library, as CPython's math
+// module uses).
+// ---------------------------------------------------------------------------
+namespace {
+
+const double kScriptPi = 3.14159265359;  // remove_repeat_particles_from_list.py:13
+
+// Euler matrix of one pick, laid out as m_o in the script (:63-71); the
+// script's m (:73-81) is its transpose with identical element values.
+void euler_matrix(double alt_deg, double az_deg, double phi_deg, double* e) {
+  const double alt = alt_deg * kScriptPi / 180, az = az_deg * kScriptPi / 180,
+               phi = phi_deg * kScriptPi / 180;
+  e[0] = std::cos(phi) * std::cos(az) - std::sin(phi) * std::cos(alt) * std::sin(az);
+  e[1] = std::cos(phi) * std::sin(az) + std::sin(phi) * std::cos(alt) * std::cos(az);
+  e[2] = std::sin(phi) * std::sin(alt);
+  e[3] = (-1) * std::sin(phi) * std::cos(az) - std::cos(phi) * std::cos(alt) * std::sin(az);
+  e[4] = (-1) * std::sin(phi) * std::sin(az) + std::cos(phi) * std::cos(alt) * std::cos(az);
+  e[5] = std::cos(phi) * std::sin(alt);
+  e[6] = std::sin(alt) * std::sin(az);
+  e[7] = (-1) * std::sin(alt) * std::cos(az);
+  e[8] = std::cos(alt);
+}
+
+// Orientation part of the pair test for outer pick i (matrix ei) and inner
+// pick j (matrix ej), :83-136.  Returns 0 when the script skips the pair
+// (`continue`), else 1 with *dist the script's euler_dist.  The quaternion
+// branches follow the script exactly: for a non-positive trace the
+// x-dominant case (:102-109) is always overwritten by the y/z case below it.
+int euler_pair(const double* ei, const double* ej, double* dist) {
+  const double* m = ei;   // transposed access below: m[r*3+c] = ei[c*3+r]
+  const double* mo = ej;
+  double mt[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) mt[r * 3 + c] = m[c * 3 + r];
+  double rm[9];
+  rm[0] = mt[0] * mo[0] + mt[3] * mo[1] + mt[6] * mo[2];
+  rm[1] = mo[0] * mt[1] + mo[1] * mt[4] + mo[2] * mt[7];
+  rm[2] = mo[0] * mt[2] + mo[1] * mt[5] + mo[2] * mt[8];
+  rm[3] = mo[3] * mt[0] + mo[4] * mt[3] + mo[5] * mt[6];
+  rm[4] = mo[3] * mt[1] + mo[4] * mt[4] + mo[5] * mt[7];
+  rm[5] = mo[3] * mt[2] + mo[4] * mt[5] + mo[5] * mt[8];
+  rm[6] = mo[6] * mt[0] + mo[7] * mt[3] + mo[8] * mt[6];
+  rm[7] = mo[6] * mt[1] + mo[7] * mt[4] + mo[8] * mt[7];
+  rm[8] = mo[6] * mt[2] + mo[7] * mt[5] + mo[8] * mt[8];
+  double q0 = 0, t;
+  const double tr = rm[0] + rm[4] + rm[8];
+  if (tr > 0) {
+    t = std::sqrt(1 + rm[0] + rm[4] + rm[8]) * 2;  // left to right, as :94
+    if (t == 0) return 0;
+    q0 = 0.25 * t;
+  }
+  if (tr <= 0) {
+    if (rm[0] >= rm[4] && rm[0] >= rm[8]) {
+      t = std::sqrt(1 + rm[0] - rm[4] - rm[8]) * 2;
+      if (t == 0) return 0;
+      q0 = (rm[7] - rm[5]) / t;
+    }
+    if (rm[4] > rm[8]) {
+      if (1 - rm[0] + rm[4] - rm[8] < 0) return 0;
+      t = std::sqrt(1 - rm[0] + rm[4] - rm[8]) * 2;
+      if (t == 0) return 0;
+      q0 = (rm[2] - rm[6]) / t;
+    } else {
+      if (1 - rm[0] - rm[4] + rm[8] < 0) return 0;
+      t = std::sqrt(1 - rm[0] - rm[4] + rm[8]) * 2;
+      if (t == 0) return 0;
+      q0 = (rm[3] - rm[1]) / t;
+    }
+  }
+  if (q0 > 1 || q0 < -1) return 0;
+  *dist = std::fabs(std::acos(q0) * 2 * 180 / kScriptPi);
+  return 1;
+}
+
+bool euler_close(double euler_dist, double thr) {
+  double dist2 = std::fmod(euler_dist, 180.0);  // Python % on non-negative floats
+  if (dist2 > thr + 1) dist2 = 180 - dist2;
+  return euler_dist < thr || dist2 < thr;
+}
+
+}  // namespace
+
+extern "C" int or_euler_distance(double alt1, double az1, double phi1, double alt2, double az2,
+                                 double phi2, double* dist) {
+  double ei[9], ej[9];
+  euler_matrix(alt1, az1, phi1, ei);
+  euler_matrix(alt2, az2, phi2, ej);
+  return euler_pair(ei, ej, dist);
+}
+
+extern "C" int or_remove_repeats(int n, const int* group, const double* x, const double* y,
+                                 const double* alt, const double* az, const double* phi,
+                                 const double* score, double center, double euler_thres,
+                                 unsigned char* keep) {
+  if (n < 0) return -1;
+  std::vector<double> e((size_t)n * 9);
+  for (int i = 0; i < n; ++i) euler_matrix(alt[i], az[i], phi[i], &e[(size_t)i * 9]);
+  std::vector<unsigned char> a(n, 0);
+  int lo = 0;
+  while (lo < n) {  // groups: runs of equal micrograph (:25-36)
+    int hi = lo + 1;
+    while (hi < n && group[hi] == group[lo]) ++hi;
+    for (int i = lo; i < hi; ++i) {  // :39-141
+      if (a[i]) continue;
+      for (int j = i + 1; j < hi; ++j) {
+        if (a[j]) continue;
+        const double dx = x[j] - x[i], dy = y[j] - y[i];
+        const double dist = std::fabs(std::sqrt(dx * dx + dy * dy));
+        if (!(dist < center)) continue;
+        double ed;
+        if (!euler_pair(&e[(size_t)i * 9], &e[(size_t)j * 9], &ed)) continue;
+        if (euler_close(ed, euler_thres)) {
+          if (score[i] > score[j] && a[j] == 0) a[j] = 1;
+          if (score[i] <= score[j] && a[i] == 0) a[i] = 1;
+        }
+      }
+    }
+    lo = hi;
+  }
+  for (int i = 0; i < n; ++i) keep[i] = a[i] ? 0 : 1;
+  return 0;
+}

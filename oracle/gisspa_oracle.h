@@ -98,6 +98,19 @@ int or_refine_score(const or_params* p, const or_ctf* c0, const double* white,
                     const double* tile, double x0, double y0, const double* theta, double* out);
 float or_psi_value(double phi_step, int index);
 
+/* ---- duplicate removal of a pick list
+ *      (python_scripts/remove_repeat_particles_from_list.py:9-145) ---- */
+/* The script's orientation distance (degrees) between an outer pick 1 and
+ * an inner pick 2; returns 0 where the script skips the pair. */
+int or_euler_distance(double alt1, double az1, double phi1, double alt2, double az2,
+                      double phi2, double* dist);
+/* keep[i] = 0 for the picks the script drops.  Groups are runs of equal
+ * `group` values (the script's runs of equal micrograph path). */
+int or_remove_repeats(int n, const int* group, const double* x, const double* y,
+                      const double* alt, const double* az, const double* phi,
+                      const double* score, double center, double euler_thres,
+                      unsigned char* keep);
+
 #ifdef __cplusplus
 }
 #endif

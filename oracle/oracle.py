@@ -73,6 +73,9 @@ def load() -> ctypes.CDLL:
         lib.or_refine_score.argtypes = [P, C, vp, vp, ctypes.c_double, ctypes.c_double, vp, vp]
         lib.or_psi_value.argtypes = [ctypes.c_double, ctypes.c_int]
         lib.or_psi_value.restype = ctypes.c_float
+        d = ctypes.c_double
+        lib.or_euler_distance.argtypes = [d] * 6 + [vp]
+        lib.or_remove_repeats.argtypes = [ctypes.c_int] + [vp] * 7 + [d, d, vp]
         _lib = lib
     return _lib
 
@@ -265,3 +268,25 @@ def correlate_many(p: OrParams, tile_spec: np.ndarray, tspec: np.ndarray, nthrea
     load().or_correlate_many(ctypes.byref(p), a.ctypes.data, a.shape[0], b.ctypes.data,
                              b.shape[0], out.ctypes.data, nthreads)
     return out
+
+
+def euler_distance(e1, e2):
+    """The duplicate-removal script's orientation distance in degrees between
+    outer pick e1 = (alt, az, phi) and inner pick e2, or None where the script
+    skips the pair (remove_repeat_particles_from_list.py:83-132)."""
+    out = ctypes.c_double()
+    ok = load().or_euler_distance(*(float(v) for v in e1), *(float(v) for v in e2),
+                                  ctypes.byref(out))
+    return out.value if ok else None
+
+
+def remove_repeats(group, x, y, alt, az, phi, score, center: float, euler_thres: float):
+    """Keep mask of remove_repeat_particles_from_list.py (True = line kept)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, alt, az, phi, score)]
+    g = np.ascontiguousarray(group, dtype=np.int32)
+    n = len(g)
+    keep = np.zeros(n, dtype=np.uint8)
+    r = load().or_remove_repeats(n, g.ctypes.data, *(a.ctypes.data for a in arrs),
+                                 float(center), float(euler_thres), keep.ctypes.data)
+    assert r == 0, r
+    return keep.astype(bool)

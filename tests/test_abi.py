@@ -65,3 +65,38 @@ def test_cli_help_and_missing_keys(tmp_path):
     assert "Error : template file is requested." in r.stdout
     assert "--angpix           1.000000" in r.stdout
     assert "--phistep          5.000000" in r.stdout
+
+
+def test_remove_repeats_entry_and_tool_errors(tmp_path):
+    lib = gisspa_amd.load()
+    nk = ctypes.c_size_t(7)
+    assert lib.gs_remove_repeats(0, None, 0, 10.0, 5.0, None, ctypes.byref(nk)) == 0
+    assert nk.value == 0
+    recs = gisspa_amd.pick_records([0], [1.0], [2.0], [0.0], [0.0], [0.0], [8.0])
+    keep = np.zeros(1, np.uint8)
+    assert lib.gs_remove_repeats(0, None, 1, 10.0, 5.0, keep.ctypes.data, None) == -1
+    if not gpu_here():
+        assert lib.gs_remove_repeats(0, recs.ctypes.data, 1, 10.0, 5.0, keep.ctypes.data,
+                                     None) == -3
+    tool = os.path.join(os.path.dirname(gisspa_amd.LIB_PATH), "remove_repeats")
+    run = lambda *a: subprocess.run([tool, *map(str, a)], capture_output=True, text=True,
+                                    timeout=30)
+    assert run().returncode == 1
+    empty = tmp_path / "empty.lst"
+    empty.write_text("")
+    assert run(empty, 1, 10, 5, tmp_path / "o").returncode == 1
+    bad = tmp_path / "bad.lst"
+    bad.write_text("0\ta.mrc\tdefocus=1\n")
+    r = run(bad, 1, 10, 5, tmp_path / "o")
+    assert r.returncode == 1 and "not a pick line" in r.stderr
+    line = ("0\t{}\tdefocus=1.0\tdfdiff=0\tdfang=0\teuler=1,2,3\tcenter=10,20\t"
+            "score=8.5\n")
+    two = tmp_path / "two.lst"
+    two.write_text(line.format("a.mrc") + line.format("b.mrc"))
+    r = run(two, 1, 10, 5, tmp_path / "o")
+    assert r.returncode == 1 and "exceed" in r.stderr
+
+
+def gpu_here() -> bool:
+    from tests.conftest import gpu_available
+    return gpu_available()
