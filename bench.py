@@ -144,6 +144,33 @@ def plain_rate(templates, d_img, nx, ny, defocus, npsi: int):
                        f"{t_psi:.2f}s each, extrapolated to 72 psi")
 
 
+def dedupe_rate(picks, n_views: int, no_cpu: bool, center: float = 20.0,
+                euler_thres: float = 10.0):
+    """Duplicate removal (gs_remove_repeats) of the run's pick list, one
+    micrograph: Euler angles of the synthetic views, the printed psi as phi.
+    The oracle's single-thread time on the same list is reported beside it."""
+    import gisspa_amd
+    from gisspa_amd import synth
+    eul = synth.sphere_eulers(n_views)
+    v = picks["view"]
+    recs = gisspa_amd.pick_records(np.zeros(len(picks), np.int32), picks["cx"], picks["cy"],
+                                   eul[v, 0], eul[v, 1], picks["psi"], picks["score"])
+    gisspa_amd.remove_repeats(recs[:64], center, euler_thres)      # warm-up
+    t = time.time()
+    keep = gisspa_amd.remove_repeats(recs, center, euler_thres)
+    gpu_ms = (time.time() - t) * 1e3
+    out = {"picks": int(len(recs)), "kept": int(keep.sum()), "center": center,
+           "euler_thres": euler_thres, "gpu_ms": gpu_ms, "cpu_ms": None, "identical": None}
+    if not no_cpu:
+        from oracle import oracle as orc
+        t = time.time()
+        ref = orc.remove_repeats(recs["group"], recs["x"], recs["y"], recs["alt"], recs["az"],
+                                 recs["phi"], recs["score"], center, euler_thres)
+        out["cpu_ms"] = (time.time() - t) * 1e3
+        out["identical"] = bool(np.array_equal(ref, keep))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +194,8 @@ def main():
                     help="refine the top N picks (perturbed by +-2.5 deg, +-1 px, +-200 A) "
                          "for --refine-iters steps after the timed run (0 = skip)")
     ap.add_argument("--refine-iters", type=int, default=20)
+    ap.add_argument("--dedupe", type=int, default=1,
+                    help="time the duplicate removal of the pick list (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--cache", default=os.environ.get("TMPDIR", "/tmp"))
@@ -265,6 +294,13 @@ def main():
             f"(at the picks {s_grid.mean():.3f})")
     pk.close()
 
+    dedupe = None
+    if args.dedupe and len(picks):
+        dedupe = dedupe_rate(picks, N, args.no_cpu)
+        log(f"[bench] dedupe: {dedupe['picks']} picks -> {dedupe['kept']} in "
+            f"{dedupe['gpu_ms']:.2f} ms" + (f" (oracle {dedupe['cpu_ms']:.0f} ms, 1 thread)"
+                                              if dedupe.get("cpu_ms") else ""))
+
     # Roofline of the correlation kernel, from hipEvents bracketing every
     # correlation launch of the first psi of each timed step on the stream it
     # runs on.  Algorithmic bytes per CC map: 24 L H = pass 1 (template half
@@ -330,6 +366,7 @@ def main():
                      "psi_correlation_achieved": pair,
                      "psi_correlation_frac": (pair / HBM_PEAK_GBS) if pair else None},
         "refine": refine,
+        "dedupe": dedupe,
         "plain": plain,
         "speedup_vs_plain": (mph / plain["value"]) if plain else None,
         "cpu_baseline": cpu,

@@ -132,14 +132,17 @@ __device__ bool resolved_duplicate(const unsigned long long* keys, int nk, unsig
   return lo < nk && keys[lo] == key;
 }
 
-// Neighbour relation.  One thread per pick i scans its group in tiles of DWG
-// picks staged through LDS.  FILL = false: count the duplicates of every pick
-// and list the undecided pairs (once, from their outer pick); FILL = true:
-// write each pick's duplicates in index order at off[i], the undecided pairs
-// taken from the host's sorted keys.
+// Neighbour relation.  Workgroup (bx, by) takes picks i of block bx and the
+// candidates j of chunk by (ch picks wide) of the block's group range, staged
+// through LDS in tiles of DWG (x, y).  FILL = false: count the duplicates per
+// (chunk, pick) and list the undecided pairs (once, from their outer pick);
+// FILL = true: write them at off[i] + off2[by * n + i] (chunks in order, so
+// every pick's list is in index order), the undecided pairs taken from the
+// host's sorted keys.
 template <bool FILL>
-__global__ __launch_bounds__(DWG) void k_dd_pairs(DedupArgs d, int* __restrict__ cnt,
+__global__ __launch_bounds__(DWG) void k_dd_pairs(DedupArgs d, int ch, int* __restrict__ cnt,
                                                   const int* __restrict__ off,
+                                                  const int* __restrict__ off2,
                                                   int* __restrict__ nbr,
                                                   int* __restrict__ amb_n, int amb_cap,
                                                   int2* __restrict__ amb_pair,
@@ -151,11 +154,15 @@ __global__ __launch_bounds__(DWG) void k_dd_pairs(DedupArgs d, int* __restrict__
   const int i0 = blockIdx.x * DWG;
   const int i = i0 + threadIdx.x;
   const bool valid = i < d.n;
-  const int jlo = d.glo[i0], jhi = d.ghi[min(i0 + DWG, d.n) - 1];
+  const int glo = d.glo[i0];
+  const int jlo = glo + blockIdx.y * ch;
+  const int jhi = min(jlo + ch, d.ghi[min(i0 + DWG, d.n) - 1]);
   const int lo = valid ? d.glo[i] : 0, hi = valid ? d.ghi[i] : 0;
   const double xi = valid ? d.x[i] : 0.0, yi = valid ? d.y[i] : 0.0;
   int count = 0;
-  const int w0 = (FILL && valid) ? off[i] : 0;
+  const long slot = (long)blockIdx.y * d.n + i;
+  const int w0 = (FILL && valid) ? off[i] + off2[slot] : 0;
+  const int wend = (FILL && valid) ? w0 + cnt[slot] : 0;
   for (int t0 = jlo; t0 < jhi; t0 += DWG) {
     __syncthreads();
     if (t0 + (int)threadIdx.x < jhi) {
@@ -194,12 +201,62 @@ __global__ __launch_bounds__(DWG) void k_dd_pairs(DedupArgs d, int* __restrict__
       if (r == 1) {
         // both passes classify every pair alike, so the list has room; the
         // bound only keeps a write inside it
-        if (FILL && w0 + count < off[i + 1]) nbr[w0 + count] = j;
+        if (FILL && w0 + count < wend) nbr[w0 + count] = j;
         ++count;
       }
     }
   }
-  if (!FILL && valid) cnt[i] = count;
+  if (!FILL && valid) cnt[slot] = count;
+}
+
+// Resolved undecided pairs join the counts (one slot per side).
+__global__ void k_dd_bump(int* __restrict__ cnt, const long* __restrict__ slots, int ns) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ns) atomicAdd(cnt + slots[k], 1);
+}
+
+// Per pick: chunk offsets relative to the pick's list (off2) and its total.
+__global__ __launch_bounds__(DWG) void k_dd_chunk_offsets(const int* __restrict__ cnt, int n,
+                                                          int nch, int* __restrict__ off2,
+                                                          int* __restrict__ tot) {
+  const int i = blockIdx.x * DWG + threadIdx.x;
+  if (i >= n) return;
+  int r = 0;
+  for (int c = 0; c < nch; ++c) {
+    off2[(long)c * n + i] = r;
+    r += cnt[(long)c * n + i];
+  }
+  tot[i] = r;
+}
+
+// Exclusive prefix sum of tot[0..n) into off[0..n], one workgroup of SWG
+// threads (each a contiguous run), the grand total also in *total.
+constexpr int SWG = 1024;
+__global__ __launch_bounds__(SWG) void k_dd_scan(const int* __restrict__ tot, int n,
+                                                 int* __restrict__ off,
+                                                 long long* __restrict__ total) {
+  __shared__ long long part[SWG];
+  const int per = (n + SWG - 1) / SWG;
+  const int a = min(n, threadIdx.x * per), b = min(n, a + per);
+  long long sum = 0;
+  for (int i = a; i < b; ++i) sum += tot[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < SWG; d <<= 1) {  // inclusive scan of the run sums
+    const long long v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  long long r = part[threadIdx.x] - sum;
+  for (int i = a; i < b; ++i) {
+    off[i] = (int)min(r, (long long)INT_MAX);
+    r += tot[i];
+  }
+  if (threadIdx.x == SWG - 1) {
+    off[n] = (int)min(part[SWG - 1], (long long)INT_MAX);
+    *total = part[SWG - 1];
+  }
 }
 
 __device__ inline int load_state(const int* p) {
@@ -292,15 +349,19 @@ void euler_matrix(double alt_deg, double az_deg, double phi_deg, double* e) {
 #pragma clang fp contract(off)
   const double alt = alt_deg * kScriptPi / 180, az = az_deg * kScriptPi / 180,
                phi = phi_deg * kScriptPi / 180;
-  e[0] = std::cos(phi) * std::cos(az) - std::sin(phi) * std::cos(alt) * std::sin(az);
-  e[1] = std::cos(phi) * std::sin(az) + std::sin(phi) * std::cos(alt) * std::cos(az);
-  e[2] = std::sin(phi) * std::sin(alt);
-  e[3] = (-1) * std::sin(phi) * std::cos(az) - std::cos(phi) * std::cos(alt) * std::sin(az);
-  e[4] = (-1) * std::sin(phi) * std::sin(az) + std::cos(phi) * std::cos(alt) * std::cos(az);
-  e[5] = std::cos(phi) * std::sin(alt);
-  e[6] = std::sin(alt) * std::sin(az);
-  e[7] = (-1) * std::sin(alt) * std::cos(az);
-  e[8] = std::cos(alt);
+  // each function value once (the script calls them again per element; the
+  // values are the same)
+  const double ca = std::cos(alt), sa = std::sin(alt), cz = std::cos(az), sz = std::sin(az),
+               cp = std::cos(phi), sp = std::sin(phi);
+  e[0] = cp * cz - sp * ca * sz;
+  e[1] = cp * sz + sp * ca * cz;
+  e[2] = sp * sa;
+  e[3] = (-1) * sp * cz - cp * ca * sz;
+  e[4] = (-1) * sp * sz + cp * ca * cz;
+  e[5] = cp * sa;
+  e[6] = sa * sz;
+  e[7] = (-1) * sa * cz;
+  e[8] = ca;
 }
 
 struct StreamGuard {
@@ -347,8 +408,8 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
   StreamGuard sg;
   HIPCHK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
   hipStream_t st = sg.s;
-  DevBuf dx, dy, ds, de, dglo, dghi, dcnt, doff, dnbr, damb_n, damb_pair, damb_q0, dkeys,
-      dstate, dopen, dfirst, dkeep;
+  DevBuf dx, dy, ds, de, dglo, dghi, dcnt, doff, doff2, dtot, dtotal, dbump, dnbr, damb_n,
+      damb_pair, damb_q0, dkeys, dstate, dopen, dfirst, dkeep;
   const size_t nd = (size_t)n * sizeof(double), ni = (size_t)n * sizeof(int);
   CHK(dx.ensure(nd));
   CHK(dy.ensure(nd));
@@ -356,7 +417,6 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
   CHK(de.ensure(nd * 9));
   CHK(dglo.ensure(ni));
   CHK(dghi.ensure(ni));
-  CHK(dcnt.ensure(ni));
   CHK(doff.ensure(ni + sizeof(int)));
   CHK(damb_n.ensure(sizeof(int)));
   HIPCHK(hipMemcpyAsync(dx.p, x.data(), nd, hipMemcpyHostToDevice, st));
@@ -378,25 +438,35 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
   a.center2 = center > 0 ? center * center * (1 + 1e-12) : -1.0;
   a.thr = thr;
   const int blocks = (n + DWG - 1) / DWG;
+  // j chunks: about 4096 workgroups in all, chunks a multiple of DWG wide
+  int maxrange = 0;
+  for (int bx = 0; bx < blocks; ++bx)
+    maxrange = std::max(maxrange, ghi[std::min(n, (bx + 1) * DWG) - 1] - glo[bx * DWG]);
+  const int want = std::max(1, 4096 / blocks);
+  const int ch = std::max(DWG, (((maxrange + want - 1) / want) + DWG - 1) / DWG * DWG);
+  const int nch = (maxrange + ch - 1) / ch;
+  const dim3 grid2(blocks, nch);
+  const size_t nslot = (size_t)n * nch;
+  CHK(dcnt.ensure(nslot * sizeof(int)));
+  CHK(doff2.ensure(nslot * sizeof(int)));
+  CHK(dtot.ensure(ni));
+  CHK(dtotal.ensure(sizeof(long long)));
 
   // 1. counts and undecided pairs (a second pass if the pair list overflowed)
-  int amb_cap = 1024, amb_n = 0;
+  int amb_cap = 1 << 16, amb_n = 0;
   for (;;) {
     CHK(damb_pair.ensure((size_t)amb_cap * sizeof(int2)));
     CHK(damb_q0.ensure((size_t)amb_cap * sizeof(double)));
     HIPCHK(hipMemsetAsync(damb_n.p, 0, sizeof(int), st));
-    k_dd_pairs<false><<<blocks, DWG, 0, st>>>(a, dcnt.as<int>(), nullptr, nullptr,
-                                               damb_n.as<int>(), amb_cap,
-                                               damb_pair.as<int2>(), damb_q0.as<double>(),
-                                               nullptr, 0);
+    k_dd_pairs<false><<<grid2, DWG, 0, st>>>(a, ch, dcnt.as<int>(), nullptr, nullptr, nullptr,
+                                              damb_n.as<int>(), amb_cap, damb_pair.as<int2>(),
+                                              damb_q0.as<double>(), nullptr, 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&amb_n, damb_n.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (amb_n <= amb_cap) break;
     amb_cap = amb_n;
   }
-  std::vector<int> cnt(n);
-  HIPCHK(hipMemcpyAsync(cnt.data(), dcnt.p, ni, hipMemcpyDeviceToHost, st));
   std::vector<int2> amb_pair(amb_n);
   std::vector<double> amb_q0(amb_n);
   if (amb_n > 0) {
@@ -404,37 +474,49 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(amb_q0.data(), damb_q0.p, amb_n * sizeof(double),
                           hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
   }
-  HIPCHK(hipStreamSynchronize(st));
-  // undecided pairs: the C library's acos decides
+  // undecided pairs: the C library's acos decides; a duplicate joins the
+  // counts of both picks, in the chunk that holds the other pick
   std::vector<unsigned long long> keys;
+  std::vector<long> bumps;
+  auto chunk_slot = [&](int i, int j) -> long {
+    return (long)((j - glo[(i / DWG) * DWG]) / ch) * n + i;
+  };
   for (int k = 0; k < amb_n; ++k) {
     if (!euler_close(euler_angle(amb_q0[k]), thr)) continue;
     const int pa = amb_pair[k].x, pb = amb_pair[k].y;
     keys.push_back((unsigned long long)pa * (unsigned)n + (unsigned)pb);
-    cnt[pa]++;
-    cnt[pb]++;
+    bumps.push_back(chunk_slot(pa, pb));
+    bumps.push_back(chunk_slot(pb, pa));
   }
   std::sort(keys.begin(), keys.end());
-  std::vector<int> off(n + 1);
-  long long total = 0;
-  for (int i = 0; i < n; ++i) {
-    off[i] = (int)total;
-    total += cnt[i];
-    if (total > INT_MAX) return GS_ERR_CAPACITY;
-  }
-  off[n] = (int)total;
-  CHK(dnbr.ensure(std::max<size_t>(1, (size_t)total) * sizeof(int)));
   CHK(dkeys.ensure(std::max<size_t>(1, keys.size()) * sizeof(unsigned long long)));
-  HIPCHK(hipMemcpyAsync(doff.p, off.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, st));
-  if (!keys.empty())
+  CHK(dbump.ensure(std::max<size_t>(1, bumps.size()) * sizeof(long)));
+  if (!keys.empty()) {
     HIPCHK(hipMemcpyAsync(dkeys.p, keys.data(), keys.size() * sizeof(unsigned long long),
                           hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dbump.p, bumps.data(), bumps.size() * sizeof(long),
+                          hipMemcpyHostToDevice, st));
+    const int nb = (int)bumps.size();
+    k_dd_bump<<<(nb + 255) / 256, 256, 0, st>>>(dcnt.as<int>(), dbump.as<long>(), nb);
+  }
+  k_dd_chunk_offsets<<<blocks, DWG, 0, st>>>(dcnt.as<int>(), n, nch, doff2.as<int>(),
+                                             dtot.as<int>());
+  k_dd_scan<<<1, SWG, 0, st>>>(dtot.as<int>(), n, doff.as<int>(), dtotal.as<long long>());
+  HIPCHK(hipGetLastError());
+  long long total = 0;
+  HIPCHK(hipMemcpyAsync(&total, dtotal.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // (keeps the host copies of the keys and bumps alive until here)
+  if (total > INT_MAX) return GS_ERR_CAPACITY;
+  CHK(dnbr.ensure(std::max<size_t>(1, (size_t)total) * sizeof(int)));
 
   // 2. neighbour lists
-  k_dd_pairs<true><<<blocks, DWG, 0, st>>>(a, nullptr, doff.as<int>(), dnbr.as<int>(), nullptr,
-                                            0, nullptr, nullptr, dkeys.as<unsigned long long>(),
-                                            (int)keys.size());
+  k_dd_pairs<true><<<grid2, DWG, 0, st>>>(a, ch, dcnt.as<int>(), doff.as<int>(),
+                                           doff2.as<int>(), dnbr.as<int>(), nullptr, 0, nullptr,
+                                           nullptr, dkeys.as<unsigned long long>(),
+                                           (int)keys.size());
   HIPCHK(hipGetLastError());
 
   // 3. settle the picks that are unmarked at their turn, in rounds

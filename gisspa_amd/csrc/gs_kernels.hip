@@ -447,6 +447,16 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
   constexpr Plan PL = tuned_plan(M, RB, WG, MAP_J);
   constexpr int RL = PL.r[PL.count - 1];  // radix of the last pass
   constexpr int NSL = M / RL;             // its butterflies per transform
+  if (rb < blo / RB || rb > bhi / RB) {
+    // every pixel of the block is a border pixel (value 0, never a strict
+    // maximum): the partial the transform would give, without it
+    if (threadIdx.x == 0) {
+      CorrPartial cp;
+      cp.vmax = 0.0f; cp.pos = INT_MAX; cp.s1 = 0.0f; cp.s2 = 0.0f;
+      part[(long)m * nparts + rb] = cp;
+    }
+    return;
+  }
   const unsigned brange = (unsigned)(bhi - blo);
   const bool rows_in = y0 >= blo && y0 + RB - 1 <= bhi;
   float vmax = 0.0f;
