@@ -171,6 +171,29 @@ def dedupe_rate(picks, n_views: int, no_cpu: bool, center: float = 20.0,
     return out
 
 
+def project_rate(n_views: int, no_cpu: bool, box: int = 180):
+    """Template projection (gs_project) of the full workload's 180^3 density
+    (the 60-blob map its templates come from) at all n_views views; the
+    oracle on 2 of them, extrapolated."""
+    import gisspa_amd
+    from gisspa_amd import synth
+    vol = synth.render_volume(synth.blob_density(60, box, 1), box)
+    eul = np.zeros((n_views, 3), np.float32)
+    eul[:, :2] = synth.sphere_eulers(n_views)
+    gisspa_amd.project(vol, eul[:8])                       # warm-up
+    t = time.time()
+    gisspa_amd.project(vol, eul)
+    gpu_s = time.time() - t
+    out = {"views": n_views, "box": box, "gpu_ms": gpu_s * 1e3, "views_per_s": n_views / gpu_s,
+           "samples_per_s": n_views * box ** 3 / gpu_s, "cpu_views_per_s": None}
+    if not no_cpu:
+        from oracle import oracle as orc
+        t = time.time()
+        orc.project(vol, eul[:2].astype(np.float64))
+        out["cpu_views_per_s"] = 2 / (time.time() - t)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +219,8 @@ def main():
     ap.add_argument("--refine-iters", type=int, default=20)
     ap.add_argument("--dedupe", type=int, default=1,
                     help="time the duplicate removal of the pick list (0 = skip)")
+    ap.add_argument("--project", type=int, default=1,
+                    help="time the template projection of the density (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--cache", default=os.environ.get("TMPDIR", "/tmp"))
@@ -301,6 +326,13 @@ def main():
             f"{dedupe['gpu_ms']:.2f} ms" + (f" (oracle {dedupe['cpu_ms']:.0f} ms, 1 thread)"
                                               if dedupe.get("cpu_ms") else ""))
 
+    proj = None
+    if args.project:
+        proj = project_rate(N, args.no_cpu)
+        log(f"[bench] project: {proj['views']} views of {proj['box']}^3 in {proj['gpu_ms']:.1f} ms"
+            + (f" (oracle {proj['cpu_views_per_s']:.2f} views/s, 1 thread)"
+               if proj["cpu_views_per_s"] else ""))
+
     # Roofline of the correlation kernel, from hipEvents bracketing every
     # correlation launch of the first psi of each timed step on the stream it
     # runs on.  Algorithmic bytes per CC map: 24 L H = pass 1 (template half
@@ -367,6 +399,7 @@ def main():
                      "psi_correlation_frac": (pair / HBM_PEAK_GBS) if pair else None},
         "refine": refine,
         "dedupe": dedupe,
+        "project": proj,
         "plain": plain,
         "speedup_vs_plain": (mph / plain["value"]) if plain else None,
         "cpu_baseline": cpu,

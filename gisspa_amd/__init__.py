@@ -156,6 +156,7 @@ def load() -> ctypes.CDLL:
     lib.gs_destroy.argtypes = [vp]
     lib.gs_remove_repeats.argtypes = [ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_double,
                                       ctypes.c_double, vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.gs_project.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
     _lib = lib
     return lib
 
@@ -193,6 +194,20 @@ def remove_repeats(recs: np.ndarray, center: float, euler_thres: float,
                                     float(euler_thres), keep.ctypes.data, ctypes.byref(nk)),
            "gs_remove_repeats")
     return keep.astype(bool)
+
+
+def project(vol: np.ndarray, eulers, device: int = 0) -> np.ndarray:
+    """Projections [nviews][n][n] float32 of an n^3 volume [z][y][x] at
+    (alt, az, phi) degrees (gs_project, EMAN convention)."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    n = v.shape[0]
+    if v.shape != (n, n, n):
+        raise ValueError("a cubic volume is needed")
+    e = np.ascontiguousarray(np.asarray(eulers, dtype=np.float32).reshape(-1, 3))
+    out = np.zeros((len(e), n, n), dtype=np.float32)
+    _check(load().gs_project(device, v.ctypes.data, n, e.ctypes.data, len(e), out.ctypes.data),
+           "gs_project")
+    return out
 
 
 def make_params(**kw) -> GsParams:

@@ -496,8 +496,11 @@ void drop_graphs(Ctx* c) {
 int ensure_corr_buffers(Ctx* c) {
   const int L = c->L;
   const long per_map = c->geo.gmap * (long)sizeof(float2);
-  // default batch: about 384 MB of intermediate per stream (189 maps at L = 720)
-  if (c->batch <= 0) c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (384l << 20) / per_map));
+  // default batch: about 1.5 GiB of intermediate per stream (759 maps at
+  // L = 720); measured at the full workload, 189 maps 270, 384 286.6, 768 288.0
+  // micrographs/hour (fewer launch boundaries where the two streams drain)
+  if (c->batch <= 0)
+    c->batch = (int)std::max(1l, std::min((long)c->T * c->N, (1536l << 20) / per_map));
   const int nb = std::min(c->batch, c->T * c->N);
   const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
   // G[0] serves norm_type 1 and the kernel timing in every schedule; G[1]

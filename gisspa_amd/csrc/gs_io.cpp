@@ -183,10 +183,11 @@ bool read_euler(const std::string& path, Euler& e) {
   std::string buf;
   while (std::getline(in, buf)) {
     if (buf.empty() || buf[0] == '#') continue;
-    float a = 0, b = 0;
-    sscanf(buf.c_str(), "%f %f", &a, &b);
+    float a = 0, b = 0, c = 0;
+    sscanf(buf.c_str(), "%f %f %f", &a, &b, &c);
     e.alt.push_back(a);
     e.az.push_back(b);
+    e.phi.push_back(c);
   }
   return true;
 }

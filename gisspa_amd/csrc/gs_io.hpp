@@ -30,6 +30,7 @@ void print_help(FILE* out);
 
 struct Euler {
   std::vector<float> alt, az;
+  std::vector<float> phi;  // third column (0 if absent); the picker ignores it
 };
 // readEulerData (DataReader.cpp:267-284): "alt az ..." per line, '#' comments.
 bool read_euler(const std::string& path, Euler& e);

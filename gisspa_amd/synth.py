@@ -91,6 +91,19 @@ def render_blobs(c2: np.ndarray, sig: np.ndarray, amp: np.ndarray, nx: int, ny: 
     return out
 
 
+def render_volume(dens: BlobDensity, box: int) -> np.ndarray:
+    """The density on a box^3 grid [z][y][x] (float32), voxel (x, y, z) at
+    (x, y, z) - box//2 relative to the centre: sum_b amp_b
+    exp(-|p - c_b|^2 / 2 s_b^2).  Its projection along z is project_templates'
+    image."""
+    vol = np.zeros((box, box, box), dtype=np.float64)
+    ax = np.arange(box, dtype=np.float64) - box // 2
+    for (cx, cy, cz), s, a in zip(dens.centres, dens.sigmas, dens.amps):
+        g = [np.exp(-((ax - c) ** 2) / (2 * s * s)) for c in (cx, cy, cz)]
+        vol += a * g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    return vol.astype(np.float32)
+
+
 def project_templates(dens: BlobDensity, box: int, eulers: np.ndarray) -> np.ndarray:
     """Stack (n, box, box) float32 of projections at the given (alt, az)."""
     out = np.zeros((len(eulers), box, box), dtype=np.float32)

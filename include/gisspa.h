@@ -260,6 +260,18 @@ typedef struct gs_pick_rec {
 int gs_remove_repeats(int device, const gs_pick_rec* recs, size_t n, double center,
                       double euler_thres, unsigned char* keep, size_t* n_kept);
 
+/* Projections of a density map at Euler angles: the templates of a picking
+ * run, made by EMAN's project3d in the reference's protocol (step 5; not
+ * part of the reference's code).  vol: box^3 float32 [z][y][x]; euler:
+ * nviews x (alt, az, phi) degrees, R = Rz(phi) Rx(alt) Rz(az) (EMAN);
+ * out: nviews x box x box float32,
+ *   out[j][row][col] = sum_w trilinear(vol, R_j^T (col-c, row-c, w) + c),
+ * c = box/2, w = -c .. box-1-c, voxels outside the box 0 (the volume rotated
+ * into its box, summed along z).  Needs no context; allocates and frees its
+ * own device memory. */
+int gs_project(int device, const float* vol, int box, const float* euler, int nviews,
+               float* out);
+
 int gs_get_info(void* ctx, gs_info* info);
 int gs_get_timing(void* ctx, gs_timing* t);
 /* Sample per-launch correlation times in gs_pick (see gs_timing). */

@@ -1032,3 +1032,50 @@ extern "C" int or_remove_repeats(int n, const int* group, const double* x, const
   for (int i = 0; i < n; ++i) keep[i] = a[i] ? 0 : 1;
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Template projection of a density map (the step before the path: the
+// reference takes templates from EMAN's project3d, protocol step 5, not
+// shipped).  out[j](row, col) = sum_{w=-c}^{box-1-c} trilinear(vol,
+// R_j^T (col-c, row-c, w) + c), c = box/2, R = Rz(phi) Rx(alt) Rz(az), zero
+// outside the box; every sample of every ray, in FP64.
+// ---------------------------------------------------------------------------
+extern "C" int or_project(const float* vol, int n, const double* euler, int nviews, double* out) {
+  if (!vol || !euler || !out || n < 2) return -1;
+  const int c = n / 2;
+  auto vox = [&](long x, long y, long z) -> double {
+    if (x < 0 || y < 0 || z < 0 || x >= n || y >= n || z >= n) return 0.0;
+    return vol[(z * n + y) * n + x];
+  };
+  for (int j = 0; j < nviews; ++j) {
+    const double d = M_PI / 180.0;
+    const double alt = euler[3 * j] * d, az = euler[3 * j + 1] * d, phi = euler[3 * j + 2] * d;
+    const double ca = std::cos(alt), sa = std::sin(alt), cz = std::cos(az), sz = std::sin(az),
+                 cp = std::cos(phi), sp = std::sin(phi);
+    const double R[9] = {cp * cz - ca * sz * sp,  cp * sz + ca * cz * sp,  sp * sa,
+                         -sp * cz - ca * sz * cp, -sp * sz + ca * cz * cp, cp * sa,
+                         sa * sz,                 -sa * cz,                ca};
+    for (int row = 0; row < n; ++row)
+      for (int col = 0; col < n; ++col) {
+        const double u = col - c, v = row - c;
+        double s = 0;
+        for (int w = -c; w <= n - 1 - c; ++w) {
+          const double x = u * R[0] + v * R[3] + w * R[6] + c;
+          const double y = u * R[1] + v * R[4] + w * R[7] + c;
+          const double z = u * R[2] + v * R[5] + w * R[8] + c;
+          const double fx0 = std::floor(x), fy0 = std::floor(y), fz0 = std::floor(z);
+          const long x0 = (long)fx0, y0 = (long)fy0, z0 = (long)fz0;
+          const double fx = x - fx0, fy = y - fy0, fz = z - fz0;
+          double acc = 0;
+          for (int k = 0; k < 8; ++k) {
+            const int ix = k & 1, iy = (k >> 1) & 1, iz = k >> 2;
+            const double wgt = (ix ? fx : 1 - fx) * (iy ? fy : 1 - fy) * (iz ? fz : 1 - fz);
+            acc += wgt * vox(x0 + ix, y0 + iy, z0 + iz);
+          }
+          s += acc;
+        }
+        out[((size_t)j * n + row) * n + col] = s;
+      }
+  }
+  return 0;
+}

@@ -111,6 +111,12 @@ int or_remove_repeats(int n, const int* group, const double* x, const double* y,
                       const double* score, double center, double euler_thres,
                       unsigned char* keep);
 
+/* ---- template projection (EMAN project3d's role, protocol step 5) ---- */
+/* out[j][row][col] = sum over w of the trilinear value of vol ([z][y][x],
+ * n^3) at R_j^T (col-c, row-c, w) + c, c = n/2, R = Rz(phi) Rx(alt) Rz(az)
+ * from euler[3j..3j+2] = (alt, az, phi) degrees; zero outside the box. */
+int or_project(const float* vol, int n, const double* euler, int nviews, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,7 @@ def load() -> ctypes.CDLL:
         d = ctypes.c_double
         lib.or_euler_distance.argtypes = [d] * 6 + [vp]
         lib.or_remove_repeats.argtypes = [ctypes.c_int] + [vp] * 7 + [d, d, vp]
+        lib.or_project.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp]
         _lib = lib
     return _lib
 
@@ -290,3 +291,16 @@ def remove_repeats(group, x, y, alt, az, phi, score, center: float, euler_thres:
                                  float(center), float(euler_thres), keep.ctypes.data)
     assert r == 0, r
     return keep.astype(bool)
+
+
+def project(vol: np.ndarray, eulers) -> np.ndarray:
+    """Projections [nviews][n][n] (FP64) of an n^3 volume [z][y][x] at
+    (alt, az, phi) degrees, EMAN convention."""
+    v = np.ascontiguousarray(vol, dtype=np.float32)
+    n = v.shape[0]
+    assert v.shape == (n, n, n)
+    e = np.ascontiguousarray(np.asarray(eulers, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros((len(e), n, n), dtype=np.float64)
+    r = load().or_project(v.ctypes.data, n, e.ctypes.data, len(e), out.ctypes.data)
+    assert r == 0, r
+    return out

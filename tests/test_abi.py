@@ -100,3 +100,23 @@ def test_remove_repeats_entry_and_tool_errors(tmp_path):
 def gpu_here() -> bool:
     from tests.conftest import gpu_available
     return gpu_available()
+
+
+def test_project_entry_and_tool_errors(tmp_path):
+    lib = gisspa_amd.load()
+    vol = np.zeros((4, 4, 4), np.float32)
+    eul = np.zeros(3, np.float32)
+    out = np.zeros((1, 4, 4), np.float32)
+    assert lib.gs_project(0, None, 4, eul.ctypes.data, 1, out.ctypes.data) == -1
+    assert lib.gs_project(0, vol.ctypes.data, 1, eul.ctypes.data, 1, out.ctypes.data) == -1
+    assert lib.gs_project(0, vol.ctypes.data, 4, eul.ctypes.data, 0, out.ctypes.data) == 0
+    if not gpu_here():
+        assert lib.gs_project(0, vol.ctypes.data, 4, eul.ctypes.data, 1, out.ctypes.data) == -3
+    from gisspa_amd import mrc
+    tool = os.path.join(os.path.dirname(gisspa_amd.LIB_PATH), "project3d")
+    assert subprocess.run([tool], capture_output=True, timeout=30).returncode == 1
+    mrc.write_mrc(str(tmp_path / "flat.mrc"), np.zeros((3, 4, 4), np.float32))
+    (tmp_path / "e.lst").write_text("0 0 0\n")
+    r = subprocess.run([tool, str(tmp_path / "flat.mrc"), str(tmp_path / "e.lst"),
+                        str(tmp_path / "o.mrcs")], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 1 and "cubic" in r.stderr
