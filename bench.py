@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per CC map at window 720 from rocprofv3 FETCH_SIZE (x2.0) and
 # WRITE_SIZE (x1.0) counters, scaled by a 1 GiB copy kernel's readings
 # (profiles/corr_720_full_workload.md)
-TRAFFIC_PER_MAP_720 = {"k_corr_pass1": 4.272e6, "k_corr_pass2": 2.128e6}
+TRAFFIC_PER_MAP_720 = {"k_corr_pass1": 4.262e6, "k_corr_pass2": 1.933e6}
 
 
 def log(*a):
