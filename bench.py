@@ -308,13 +308,15 @@ def main():
         _, fin = pk.refine(pert, args.refine_iters)
         hip.sync()
         tr = time.time() - t
+        nev = pk.timing().refine_evals
         refine = {"candidates": int(len(pert)), "iters": args.refine_iters, "seconds": tr,
-                  "candidates_per_s": len(pert) / tr,
-                  "evaluations_per_s": len(pert) * (args.refine_iters + 1) / tr,
+                  "candidates_per_s": len(pert) / tr, "evaluations": int(nev),
+                  "evaluations_per_s": nev / tr,
                   "mean_score_perturbed": float(s0.mean()),
                   "mean_score_refined": float(fin["score"].mean()),
                   "mean_score_at_picks": float(s_grid.mean())}
-        log(f"[bench] refine: {len(pert)} candidates x {args.refine_iters} steps in {tr:.2f}s; "
+        log(f"[bench] refine: {len(pert)} candidates, up to {args.refine_iters} steps "
+            f"({nev} evaluations) in {tr:.2f}s; "
             f"mean objective {s0.mean():.3f} -> {fin['score'].mean():.3f} "
             f"(at the picks {s_grid.mean():.3f})")
     pk.close()

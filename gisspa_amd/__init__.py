@@ -58,7 +58,8 @@ class GsInfo(ctypes.Structure):
 class GsTiming(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("pick_total_ms", "micrograph_prep_ms", "corr_ms",
                                                "corr_pass2_ms", "corr_psi_ms")] + \
-               [(n, ctypes.c_longlong) for n in ("corr_launches", "corr_sampled_maps", "corr_maps")]
+               [(n, ctypes.c_longlong) for n in ("corr_launches", "corr_sampled_maps", "corr_maps",
+                                                  "refine_evals")]
 
 
 MAPSTAT_DTYPE = np.dtype([("score", "<f4"), ("pos", "<i4"), ("peak", "<f4"), ("sd", "<f4"),

@@ -1396,15 +1396,24 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
   // the step by 1.25, rejected ones halve it
   const double scl[4] = {1.0, 1.0, 1.0, 0.1};
   const double lim[4] = {c->p.phi_step, 1.5, 1.5, 0.5};
-  std::vector<gs_peak> cur(peaks, peaks + n), start(peaks, peaks + n), trial(n);
+  std::vector<gs_peak> cur(peaks, peaks + n), start(peaks, peaks + n);
   std::vector<RefCand> rc;
   std::vector<double2> cs;
   std::vector<float> fc((size_t)n * 5), ft((size_t)n * 5);
   CHK(ref_cands(c, cur.data(), n, rc, cs));
   CHK(ref_eval(c, rc.data(), cs.data(), n, fc.data()));
+  c->timing.refine_evals = n;
   std::vector<double> step(n, 0.5);
-  for (int it = 0; it < iters; ++it) {
-    for (int i = 0; i < n; ++i) {
+  // a candidate whose step has shrunk below kMinStep (0.01 degree / pixel /
+  // 0.001 um) has converged and leaves the evaluated set
+  constexpr double kMinStep = 0.01;
+  std::vector<int> act(n);
+  for (int i = 0; i < n; ++i) act[i] = i;
+  std::vector<gs_peak> tact;
+  for (int it = 0; it < iters && !act.empty(); ++it) {
+    tact.resize(act.size());
+    for (size_t a = 0; a < act.size(); ++a) {
+      const int i = act[a];
       const float* g = &fc[5 * (size_t)i + 1];
       double nrm = 0;
       for (int k = 0; k < 4; ++k) nrm += (scl[k] * g[k]) * (scl[k] * g[k]);
@@ -1415,23 +1424,29 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
         if (nrm > 0) th[k] += step[i] * scl[k] * scl[k] * g[k] / nrm;
         th[k] = std::min(std::max(th[k], t0[k] - lim[k]), t0[k] + lim[k]);
       }
-      trial[i] = cur[i];
-      trial[i].psi = (float)th[0];
-      trial[i].dx = (float)th[1];
-      trial[i].dy = (float)th[2];
-      trial[i].defocus = (float)th[3];
+      tact[a] = cur[i];
+      tact[a].psi = (float)th[0];
+      tact[a].dx = (float)th[1];
+      tact[a].dy = (float)th[2];
+      tact[a].defocus = (float)th[3];
     }
-    CHK(ref_cands(c, trial.data(), n, rc, cs));
-    CHK(ref_eval(c, rc.data(), cs.data(), n, ft.data()));
-    for (int i = 0; i < n; ++i) {
-      if (ft[5 * (size_t)i] > fc[5 * (size_t)i]) {
-        cur[i] = trial[i];
-        std::copy(&ft[5 * (size_t)i], &ft[5 * (size_t)i] + 5, &fc[5 * (size_t)i]);
+    const int na = (int)act.size();
+    CHK(ref_cands(c, tact.data(), na, rc, cs));
+    CHK(ref_eval(c, rc.data(), cs.data(), na, ft.data()));
+    size_t keep = 0;
+    for (int a = 0; a < na; ++a) {
+      const int i = act[a];
+      if (ft[5 * (size_t)a] > fc[5 * (size_t)i]) {
+        cur[i] = tact[a];
+        std::copy(&ft[5 * (size_t)a], &ft[5 * (size_t)a] + 5, &fc[5 * (size_t)i]);
         step[i] *= 1.25;
       } else {
         step[i] *= 0.5;
       }
+      if (step[i] >= kMinStep) act[keep++] = i;
     }
+    act.resize(keep);
+    c->timing.refine_evals += na;
   }
   for (int i = 0; i < n; ++i) {
     peaks[i] = cur[i];

@@ -394,13 +394,17 @@ __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gT
     const float qX = (float)(qx - L / 2), qY = (float)(qy - L / 2);
     float acc = 0.0f;
     if (qX * qX + qY * qY <= reach * reach) {
-      // inverse rotation: x - c = R^T (q - c)
+      // inverse rotation: x - c = R^T (q - c).  An output pixel samples q's
+      // cells only if its rotated position lies within 1 of q on both axes,
+      // so within sqrt(2) of R^T (q - c) on each: the 4 x 4 window
+      // floor(c) - 1 .. floor(c) + 2 holds all of them (0.41 px to spare for
+      // the float rounding of c); the others carry weight 0.
       const float cX = (float)(cs.x * qX - cs.y * qY), cY = (float)(cs.y * qX + cs.x * qY);
-      const int x0 = (int)floorf(cX) + L / 2 - 2, y0 = (int)floorf(cY) + L / 2 - 2;
-      for (int dy = 0; dy < 5; ++dy) {
+      const int x0 = (int)floorf(cX) + L / 2 - 1, y0 = (int)floorf(cY) + L / 2 - 1;
+      for (int dy = 0; dy < 4; ++dy) {
         const int y = y0 + dy;
         if (y < 0 || y >= L) continue;
-        for (int dx = 0; dx < 5; ++dx) {
+        for (int dx = 0; dx < 4; ++dx) {
           const int x = x0 + dx;
           if (x < 0 || x >= L) continue;
           const float g = gsrc[(long)y * L + x];

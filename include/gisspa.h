@@ -142,6 +142,8 @@ typedef struct gs_timing {
   long long corr_launches;
   long long corr_sampled_maps; /* CC maps those launches covered (both passes) */
   long long corr_maps;         /* CC maps evaluated by the last gs_pick */
+  long long refine_evals;      /* candidate evaluations (objective + gradient) of the
+                                  last gs_refine */
 } gs_timing;
 
 /* Fill p with the reference's defaults (DataReader.h:16-42); required keys
@@ -228,10 +230,12 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
  * gradient comes from a hand-written adjoint of every stage (FFTs, bilinear
  * rotation, normalisation, CTF weighting). */
 int gs_adjoint(void* ctx, const gs_peak* peaks, int n, gs_grad* out);
-/* Local refinement of n candidates in place: `iters` steps of gradient
- * ascent on the objective of gs_adjoint (step length adapted per candidate,
- * psi within one phistep, the shift within 1.5 pixels and the defocus within
- * 0.5 um of the start); out (may be null) receives the final objective and
+/* Local refinement of n candidates in place: up to `iters` steps of
+ * gradient ascent on the objective of gs_adjoint (step length adapted per
+ * candidate: x1.25 after an accepted step, x0.5 after a rejected one; psi
+ * within one phistep, the shift within 1.5 pixels and the defocus within
+ * 0.5 um of the start); a candidate stops once its step is below 0.01 degree
+ * / pixel (0.001 um).  out (may be null) receives the final objective and
  * gradient. */
 int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out);
 
