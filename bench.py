@@ -67,6 +67,24 @@ class Hip:
     def sync(self) -> None:
         self.lib.hipDeviceSynchronize()
 
+    def copy_rate(self, nbytes: int, reps: int = 3) -> float:
+        """Best device-to-device copy rate (bytes read + written per second)
+        of an nbytes buffer: what the HBM sustains for a plain stream."""
+        a, b = self.malloc(nbytes), self.malloc(nbytes)
+        best = 0.0
+        try:
+            self.d2d(b, a, nbytes)
+            for _ in range(reps):
+                self.sync()
+                t = time.perf_counter()
+                self.d2d(b, a, nbytes)
+                self.sync()
+                best = max(best, 2.0 * nbytes / (time.perf_counter() - t))
+        finally:
+            self.free(a)
+            self.free(b)
+        return best
+
     def free(self, p: int) -> None:
         self.lib.hipFree(ctypes.c_void_p(p))
 
@@ -361,6 +379,10 @@ def main():
         log(f"[bench] psi correlation: {tpsi / max(args.steps, 1):.1f} ms, {pair:.0f} GB/s "
             f"algorithmic ({pair / HBM_PEAK_GBS:.3f} of HBM peak)")
 
+    copy_gbs = hip.copy_rate(4 << 30) / 1e9
+    log(f"[bench] 4 GiB device copy: {copy_gbs:.0f} GB/s (read + write); psi correlation at "
+        f"{(pair or 0) / copy_gbs:.3f} of it")
+
     plain = None
     if not args.no_plain:
         plain = plain_rate(templates, d_img, nx, ny, defocus, args.plain_psi)
@@ -398,7 +420,8 @@ def main():
                      else None,
                      "launch_ms": t_launch, "maps_per_launch": mpl, "bytes_per_map": bpm,
                      "psi_correlation_achieved": pair,
-                     "psi_correlation_frac": (pair / HBM_PEAK_GBS) if pair else None},
+                     "psi_correlation_frac": (pair / HBM_PEAK_GBS) if pair else None,
+                     "copy_gbs": copy_gbs},
         "refine": refine,
         "dedupe": dedupe,
         "project": proj,
