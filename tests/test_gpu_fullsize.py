@@ -1,0 +1,77 @@
+"""GPU parity at the benchmark shapes, against the FP64 oracle:
+  * the plain-method small configuration: a 1024 x 1024 micrograph with 20
+    particles, 64 views of a 128^3 blob density, window 512 / overlap 128
+    (9 tiles), 10 degree in-plane steps (the first 4 of 36);
+  * the full-size shape of the benchmark: a 5760 x 4092 micrograph, window
+    720 / overlap 180 (88 tiles), 180 px templates (8 of the 1649 views),
+    one in-plane step, batches of 100 maps (the last one ragged).
+Same tolerances as test_gpu_parity.py: scores within 1e-4 * max(|oracle|, 1),
+identical peak positions and identical pick lists."""
+import os
+
+import numpy as np
+import pytest
+
+from gisspa_amd import synth
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def _compare(got, ref, what):
+    rs, gs_ = ref["score"], got["score"].astype(np.float64)
+    rel = np.abs(gs_ - rs) / np.maximum(np.abs(rs), 1.0)
+    mism = np.count_nonzero(got["pos"] != ref["pos"])
+    print(f"{what}: {ref.size} maps, score max rel err {rel.max():.2e}, "
+          f"position mismatches {mism}, max score {rs.max():.2f}")
+    assert rel.max() < 1e-4
+    assert mism == 0
+
+
+def _compare_picks(picks, sel):
+    print("picks gpu", len(picks), "oracle", len(sel))
+    assert len(picks) == len(sel)
+    for k in ("psi_index", "view"):
+        assert np.array_equal(picks[k], sel[k])
+    assert np.array_equal(picks["cx"], sel["cx"].astype(np.float32))
+    assert np.array_equal(picks["cy"], sel["cy"].astype(np.float32))
+    assert np.allclose(picks["score"], sel["score"], rtol=1e-4)
+
+
+def test_small_configuration(gpu):
+    import gisspa_amd
+    wl = synth.small_workload(seed=0)
+    img, df = wl.micrographs[0]
+    p = orc.params(512, **wl.params)
+    ref = orc.pick_norm0(p, orc.ctf(df), wl.templates, img, 0, 4, nthreads=NTHREADS)
+    with gisspa_amd.Picker(0, **wl.params) as pk:
+        pk.set_templates(wl.templates, df)
+        pk.set_micrograph(img, df)
+        picks = pk.pick(0, 4)
+        got = pk.mapstats().reshape(ref.shape)
+    _compare(got, ref, "1024^2 / window 512")
+    _compare_picks(picks, orc.select_norm0(p, ref, 0, img.shape[1], img.shape[0]))
+
+
+def test_full_size_shape(gpu):
+    import gisspa_amd
+    dens = synth.blob_density(60, 180, 1)
+    views = synth.sphere_eulers(1649)
+    tm = synth.project_templates(dens, 180, views[::206][:8])
+    mg = synth.make_micrograph(5760, 4092, dens, views, 300, 0.05, 0, apix=1.0,
+                               defocus_um=1.8, ampconst=0.1, box=180, psi_step=5.0)
+    params = dict(angpix=1.0, phistep=5.0, kk=3, energy=300, cs=2.7, Highres=8, Lowres=100,
+                  diameter=135.0, threshold=7.0, window_size=720, overlap=180, ampconst=0.1)
+    p = orc.params(720, **params)
+    ref = orc.pick_norm0(p, orc.ctf(1.8), tm, mg.image, 0, 1, nthreads=NTHREADS)
+    with gisspa_amd.Picker(0, **params) as pk:
+        pk.set_batch(100)
+        pk.set_templates(tm, 1.8)
+        pk.set_micrograph(mg.image, 1.8)
+        picks = pk.pick(0, 1)
+        got = pk.mapstats().reshape(ref.shape)
+    assert ref.shape == (1, 88, 8)
+    _compare(got, ref, "5760x4092 / window 720")
+    _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))

@@ -3,8 +3,8 @@ refinement, against the FP64 oracle (oracle/gisspa_oracle.cpp,
 or_refine_score).
 
   * objective: GPU value within 1e-4 * max(|oracle|, 1) of the oracle's;
-  * gradient: against FP64 central differences of the oracle objective,
-    within 1e-3 relative (of the component, or of 5 % of the largest
+  * gradient: against FP64 central differences of the oracle objective on
+    12 and on 256 candidates, within 1e-3 relative (of the component, or of 5 % of the largest
     component for a near-zero one).  Steps: 1e-3 pixel for the shift,
     1e-3 A (1e-7 um) for the defocus and 1e-6 degree for psi: the objective
     is piecewise smooth in psi (bilinear cells) and defocus (|CTF| at its
@@ -96,6 +96,31 @@ def test_objective_and_gradient_match_oracle(gpu, setup):
         g = np.array([got["d_psi"][i], got["d_dx"][i], got["d_dy"][i], got["d_defocus"][i]],
                      dtype=np.float64)
         _check_grad(g, _fd(args, theta), f"peak {i}")
+
+
+def test_gradient_on_256_candidates(gpu, setup):
+    """Gradients of 256 candidates (the top picks under 22 perturbations each,
+    +-2.5 degree, +-1 px, +-200 A) against FP64 central differences."""
+    c, op, pk, top, white, tiles, offs = setup
+    rows = []
+    seed = 10
+    while sum(len(r) for r in rows) < 256:
+        rows.append(_perturbed(c, top, seed)[1])
+        seed += 1
+    peaks = np.concatenate(rows)[:256]
+    got = pk.adjoint(peaks)
+    worst = 0.0
+    for i, p in enumerate(peaks):
+        args, theta = _oracle_args(c, op, white, tiles, offs, p)
+        ref = orc.refine_score(*args, theta)[0]
+        assert abs(got["score"][i] - ref) <= 1e-4 * max(abs(ref), 1.0)
+        g = np.array([got["d_psi"][i], got["d_dx"][i], got["d_dy"][i], got["d_defocus"][i]],
+                     dtype=np.float64)
+        fd = _fd(args, theta)
+        tol = 1e-3 * np.maximum(np.abs(fd), 0.05 * np.abs(fd).max())
+        worst = max(worst, float((np.abs(g - fd) / tol).max()))
+        assert np.all(np.abs(g - fd) <= tol), (i, g, fd)
+    print(f"256 candidates: worst gradient error {worst:.3f} of the tolerance")
 
 
 def test_autograd_function_gradient(gpu, setup):
