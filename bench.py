@@ -237,6 +237,9 @@ def main():
     ap.add_argument("--refine-iters", type=int, default=20)
     ap.add_argument("--dedupe", type=int, default=1,
                     help="time the duplicate removal of the pick list (0 = skip)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="choose the correlation schedule on the device before timing (0 = the "
+                         "built-in defaults)")
     ap.add_argument("--project", type=int, default=1,
                     help="time the template projection of the density (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -276,6 +279,19 @@ def main():
     pk.set_templates(templates, defocus)
     log(f"[bench] template prep {time.time() - t:.2f}s")
     npsi = 72 if args.psi_limit <= 0 else min(72, args.psi_limit)
+    tune = None
+    if args.autotune:
+        t = time.time()
+        pk.set_micrograph_device(d_img, nx, ny, defocus)
+        tr = pk.autotune(0)
+        tune = {"pass1_variant": tr.pass1_variant, "pass2_variant": tr.pass2_variant,
+                "batch": tr.batch, "pipeline": tr.pipeline, "psi_ms": tr.psi_ms,
+                "pass1_us_per_map": list(tr.pass1_us), "pass2_us_per_map": list(tr.pass2_us),
+                "seconds": time.time() - t}
+        log(f"[bench] autotune ({tune['seconds']:.1f}s): FFT variants {tr.pass1_variant}/"
+            f"{tr.pass2_variant}, batch {tr.batch}, pipeline {tr.pipeline}, psi {tr.psi_ms:.1f} ms;"
+            f" pass 1 us/map {[round(v, 3) for v in tr.pass1_us]}, pass 2 "
+            f"{[round(v, 3) for v in tr.pass2_us]}")
 
     def step():
         pk.set_micrograph_device(d_img, nx, ny, defocus)
@@ -412,7 +428,7 @@ def main():
                                "180px templates, window 720 overlap 180 (88 tiles)",
                    "views": N, "psi": 72, "tiles": T, "window": L, "psi_timed": npsi,
                    "maps_per_micrograph": N * 72 * T, "threshold": args.threshold,
-                   "mode": "fused" + ("+graphs" if args.graphs else "")},
+                   "mode": "fused" + ("+graphs" if args.graphs else ""), "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": (TRAFFIC_PER_MAP_720[kern] * mpl)

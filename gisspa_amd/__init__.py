@@ -62,6 +62,13 @@ class GsTiming(ctypes.Structure):
                                                   "refine_evals")]
 
 
+class GsTune(ctypes.Structure):
+    _fields_ = [("pass1_variant", ctypes.c_int), ("pass2_variant", ctypes.c_int),
+                ("batch", ctypes.c_int), ("pipeline", ctypes.c_int),
+                ("candidates", ctypes.c_int), ("pad", ctypes.c_int), ("psi_ms", ctypes.c_double),
+                ("pass1_us", ctypes.c_double * 4), ("pass2_us", ctypes.c_double * 4)]
+
+
 MAPSTAT_DTYPE = np.dtype([("score", "<f4"), ("pos", "<i4"), ("peak", "<f4"), ("sd", "<f4"),
                           ("sum", "<f8"), ("sum2", "<f8")])
 PARTICLE_DTYPE = np.dtype([("psi_index", "<i4"), ("view", "<i4"), ("tile", "<i4"),
@@ -158,6 +165,8 @@ def load() -> ctypes.CDLL:
     lib.gs_remove_repeats.argtypes = [ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_double,
                                       ctypes.c_double, vp, ctypes.POINTER(ctypes.c_size_t)]
     lib.gs_project.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
+    lib.gs_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(GsTune)]
+    lib.gs_set_fft_variants.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     _lib = lib
     return lib
 
@@ -354,6 +363,16 @@ class Picker:
 
     def set_tile_block(self, n: int) -> None:
         _check(self.lib.gs_set_tile_block(self.ctx, n), "gs_set_tile_block")
+
+    def autotune(self, psi_index: int = 0) -> GsTune:
+        """Time the correlation schedules on the device and keep the fastest
+        (gs_autotune)."""
+        t = GsTune()
+        _check(self.lib.gs_autotune(self.ctx, psi_index, ctypes.byref(t)), "gs_autotune")
+        return t
+
+    def set_fft_variants(self, pass1: int, pass2: int) -> None:
+        _check(self.lib.gs_set_fft_variants(self.ctx, pass1, pass2), "gs_set_fft_variants")
 
     def set_batch(self, n: int) -> None:
         _check(self.lib.gs_set_batch(self.ctx, n), "gs_set_batch")

@@ -82,6 +82,8 @@ struct Ctx {
   // `lag` rounds back beside pass 1 of its current map, intermediates in a
   // ring of `ring` slots per XCD
   int pipeline = GS_PIPE_LANES, lag = 2, ring = 4;  // lag, ring: rounds of 8 maps
+  // FFT schedule variant of each correlation pass (GS_PIPE_LANES launches)
+  int var1 = 0, var2 = 0;
   DevBuf Gring, counters, err;
   // GS_PIPE_LANES: two streams, each running pass 1 then pass 2 of every
   // other batch
@@ -603,11 +605,12 @@ int correlate_psi(Ctx* c, MapStat* out) {
       if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k], ls));
       launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
                         c->maps_blk.as<MapRef>() + m0, cnt, c->G[k % 2].as<float2>(),
-                        c->psum.as<float2>() + (size_t)m0 * ncb, c->twL.as<float2>(), ls);
+                        c->psum.as<float2>() + (size_t)m0 * ncb, c->twL.as<float2>(), ls,
+                        c->var1);
       if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k + 1], ls));
       launch_corr_pass2(L, c->G[k % 2].as<float2>(), cnt, lo, hi,
                         c->part.as<CorrPartial>() + (size_t)m0 * np, c->twL.as<float2>(),
-                        c->twM.as<float2>(), ls);
+                        c->twM.as<float2>(), ls, c->var2);
       if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k + 2], ls));
     }
     if (c->profile_now) c->prof_batches = nbatch;
@@ -1349,11 +1352,11 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
   for (int i = 0; i < reps; ++i)
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps_blk.as<MapRef>(), nb,
                       c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(),
-                      c->stream);
+                      c->stream, c->var1);
   HIPCHK(hipEventRecord(e[1], c->stream));
   for (int i = 0; i < reps; ++i)
     launch_corr_pass2(L, c->G[0].as<float2>(), nb, lo, hi, c->part.as<CorrPartial>(),
-                      c->twL.as<float2>(), c->twM.as<float2>(), c->stream);
+                      c->twL.as<float2>(), c->twM.as<float2>(), c->stream, c->var2);
   HIPCHK(hipEventRecord(e[2], c->stream));
   HIPCHK(hipEventSynchronize(e[2]));
   HIPCHK(hipGetLastError());
@@ -1454,6 +1457,120 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
       out[i] = gs_grad{fc[5 * (size_t)i], fc[5 * (size_t)i + 1], fc[5 * (size_t)i + 2],
                        fc[5 * (size_t)i + 3], fc[5 * (size_t)i + 4]};
   }
+  return GS_OK;
+}
+
+int gs_set_fft_variants(void* ctx, int pass1, int pass2) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return GS_ERR_ARG;
+  // windows compiled without variants run variant 0 whatever is set
+  const int nv = corr_max_variants();
+  if (pass1 < 0 || pass2 < 0 || pass1 >= nv || pass2 >= nv) return GS_ERR_ARG;
+  c->var1 = pass1;
+  c->var2 = pass2;
+  drop_graphs(c);  // captured with the old schedule
+  return GS_OK;
+}
+
+int gs_autotune(void* ctx, int psi_index, gs_tune* out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || psi_index < 0) return GS_ERR_ARG;
+  if (c->N == 0 || !c->have_mic) return GS_ERR_STATE;
+  if (c->mode != GS_MODE_FUSED || c->p.norm_type != 0) return GS_ERR_UNSUPPORTED;
+  if (psi_index >= num_psi(c->p.phi_step)) return GS_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  gs_tune t{};
+  const int L = c->L, nv = corr_variants(L);
+  const bool graphs = c->use_graphs, prof = c->profiling;
+  c->use_graphs = false;
+  // 1. FFT schedule of each pass, timed alone on one batch of this psi
+  CHK(ensure_corr_buffers(c));
+  CHK(rotate_templates(c, psi_index));
+  const int nb = std::min(c->batch, c->T * c->N);
+  const float lo = c->p.diameter / 4.0f, hi = (float)L - c->p.diameter / 4.0f;
+  hipEvent_t e[2];
+  for (auto& x : e) HIPCHK(hipEventCreate(&x));
+  auto timed = [&](bool pass1, int v, double* us) -> int {
+    for (int rep = 0; rep < 4; ++rep) {  // the first launch warms up
+      if (rep == 1) HIPCHK(hipEventRecord(e[0], c->stream));
+      if (pass1)
+        launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
+                          c->maps_blk.as<MapRef>(), nb, c->G[0].as<float2>(),
+                          c->psum.as<float2>(), c->twL.as<float2>(), c->stream, v);
+      else
+        launch_corr_pass2(L, c->G[0].as<float2>(), nb, lo, hi, c->part.as<CorrPartial>(),
+                          c->twL.as<float2>(), c->twM.as<float2>(), c->stream, v);
+    }
+    HIPCHK(hipEventRecord(e[1], c->stream));
+    HIPCHK(hipEventSynchronize(e[1]));
+    HIPCHK(hipGetLastError());
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e[0], e[1]));
+    *us = 1e3 * ms / 3.0 / nb;
+    return GS_OK;
+  };
+  int best1 = 0, best2 = 0;
+  for (int pass = 1; pass <= 2; ++pass)
+    for (int v = 0; v < nv && v < 4; ++v) {
+      double us = 0;
+      CHK(timed(pass == 1, v, &us));
+      double* row = pass == 1 ? t.pass1_us : t.pass2_us;
+      row[v] = us;
+      int& best = pass == 1 ? best1 : best2;
+      if (us < row[best]) best = v;
+    }
+  for (auto& x : e) (void)hipEventDestroy(x);
+  c->var1 = best1;
+  c->var2 = best2;
+  // 2. launch schedule: batch and pipeline, timed on the whole correlation
+  //    of the psi (gs_pick with launch sampling)
+  const int b0 = c->batch;
+  const int batches[3] = {std::max(1, b0 / 2), b0, 2 * b0};
+  const int pipes[2] = {GS_PIPE_LANES, GS_PIPE_LAUNCH};
+  const int p0 = c->pipeline;
+  double best_ms = 1e300, cur_ms = 1e300;
+  int best_b = b0, best_p = p0;
+  c->profiling = true;
+  std::vector<gs_particle> keep_picks = c->picks;
+  for (int bi = 0; bi < 3; ++bi)
+    for (int pi = 0; pi < 2; ++pi) {
+      c->batch = batches[bi];
+      c->pipeline = pipes[pi];
+      double ms = 1e300;
+      for (int rep = 0; rep < 3; ++rep) {  // the first run allocates and warms up
+        size_t np = 0;
+        const int r = gs_pick(ctx, psi_index, psi_index + 1, nullptr, 0, &np);
+        if (r != GS_OK && r != GS_ERR_CAPACITY) return r;
+        if (rep > 0) ms = std::min(ms, c->timing.corr_psi_ms);
+      }
+      t.candidates++;
+      if (batches[bi] == b0 && pipes[pi] == p0) cur_ms = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_b = batches[bi];
+        best_p = pipes[pi];
+      }
+    }
+  // the schedule in use stays unless another is clearly (1 %) faster: the
+  // candidates often differ by less than the run-to-run spread
+  if (cur_ms < 1e300 && best_ms > 0.99 * cur_ms) {
+    best_ms = cur_ms;
+    best_b = b0;
+    best_p = p0;
+  }
+  c->picks = keep_picks;
+  c->stats_npsi = 0;
+  c->batch = best_b;
+  c->pipeline = best_p;
+  c->profiling = prof;
+  c->use_graphs = graphs;
+  drop_graphs(c);  // captured with the old schedule
+  t.pass1_variant = best1;
+  t.pass2_variant = best2;
+  t.batch = best_b;
+  t.pipeline = best_p;
+  t.psi_ms = best_ms;
+  if (out) *out = t;
   return GS_OK;
 }
 

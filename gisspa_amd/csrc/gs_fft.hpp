@@ -403,6 +403,53 @@ constexpr Plan tuned_plan(int N, int NB, int WG, int MAPL) {
   return make_plan(N, NB, WG);
 }
 
+// Schedule variants (selected at run time by gs_autotune): variant 0 is the
+// tuned plan above; variant v >= 1 is the v-th cheapest other factorisation
+// of N into two or three radices <= 32 by plan_cost (ties keep the larger
+// leading radices), with the tuned plan's LDS padding.  A variant past the
+// last alternative falls back to the tuned plan.
+constexpr int kPlanVariants = 4;
+constexpr Plan variant_plan(int N, int NB, int WG, int MAPL, int V) {
+  const Plan t = tuned_plan(N, NB, WG, MAPL);
+  if (V <= 0) return t;
+  constexpr int CAP = 256;
+  int rr[CAP][3] = {{0}};
+  int cnt[CAP] = {0};
+  double cost[CAP] = {0};
+  int n = 0;
+  for (int a = 32; a >= 2; --a) {
+    if (N % a) continue;
+    for (int b = 32; b >= 2; --b) {
+      if ((N / a) % b) continue;
+      const int c = N / a / b;
+      if (c > 32 || n >= CAP) continue;
+      const int k = c == 1 ? 2 : 3;
+      const int r[3] = {a, b, c};
+      bool same = t.count == k && t.r[0] == a && t.r[1] == b && (k == 2 || t.r[2] == c);
+      if (same) continue;
+      rr[n][0] = a;
+      rr[n][1] = b;
+      rr[n][2] = c;
+      cnt[n] = k;
+      cost[n] = plan_cost(N, NB, WG, r, k);
+      ++n;
+    }
+  }
+  bool used[CAP] = {false};
+  int pick = -1;
+  for (int v = 0; v < V; ++v) {
+    pick = -1;
+    for (int i = 0; i < n; ++i)
+      if (!used[i] && (pick < 0 || cost[i] < cost[pick])) pick = i;
+    if (pick < 0) return t;
+    used[pick] = true;
+  }
+  Plan p{cnt[pick], {rr[pick][0], rr[pick][1], cnt[pick] == 3 ? rr[pick][2] : 0}};
+  p.pad_shift = t.pad_shift;
+  p.pad_mult = t.pad_mult;
+  return p;
+}
+
 constexpr bool plan_ok(int n) {
   while (n % 2 == 0) n /= 2;
   while (n % 3 == 0) n /= 3;
@@ -493,19 +540,19 @@ template <int SHIFT, int MULT>
 __device__ __forceinline__ int pidx_sm(int i) { return i + (i >> SHIFT) * MULT; }
 
 // LDS elements (per transform, padding included) the executor needs
-template <int N, int NB, int WG, int MAPL>
+template <int N, int NB, int WG, int MAPL, int V = 0>
 __host__ __device__ constexpr int exec_stride() {
-  constexpr Plan P = tuned_plan(N, NB, WG, MAPL);
+  constexpr Plan P = variant_plan(N, NB, WG, MAPL, V);
   return padded_len_sm<P.pad_shift, P.pad_mult>(N);
 }
 
 enum { MAP_J = 0, MAP_B = 1 };
 
-template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int PASS, int NS, class LoadF,
+template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int PASS, int NS, int V, class LoadF,
           class StoreF>
 __device__ __forceinline__ void fft_exec_pass(float2* lds, const float2* tw, LoadF& load,
                                               StoreF& store) {
-  constexpr Plan P = tuned_plan(N, NB, WG, MAPL);
+  constexpr Plan P = variant_plan(N, NB, WG, MAPL, V);
   constexpr int R = P.r[PASS];
   constexpr bool FIRST = PASS == 0, LAST = PASS == P.count - 1;
   constexpr int M = N / R;
@@ -566,7 +613,7 @@ __device__ __forceinline__ void fft_exec_pass(float2* lds, const float2* tw, Loa
   }
   if constexpr (!LAST) {
     __syncthreads();
-    fft_exec_pass<N, NB, WG, DIR, MAP0, MAPL, PASS + 1, NS * R>(lds, tw, load, store);
+    fft_exec_pass<N, NB, WG, DIR, MAP0, MAPL, PASS + 1, NS * R, V>(lds, tw, load, store);
   }
 }
 
@@ -574,14 +621,15 @@ __device__ __forceinline__ void fft_exec_pass(float2* lds, const float2* tw, Loa
 // element i of sequence b; store(b, i, v) receives output element i.  The
 // mapping of the first pass is MAP0 and of the last pass MAPL (a single-pass
 // plan uses MAP0).  Requires tw to be visible (barrier) before the call.
-template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, class LoadF, class StoreF>
+template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int V = 0, class LoadF,
+          class StoreF>
 __device__ __forceinline__ void fft_exec(float2* lds, const float2* tw, LoadF& load,
                                          StoreF& store) {
   static_assert(plan_ok(N), "FFT length must factor into 2, 3, 5, 7");
-  constexpr Plan P = tuned_plan(N, NB, WG, MAPL);
+  constexpr Plan P = variant_plan(N, NB, WG, MAPL, V);
   static_assert(P.count != 1 || MAP0 == MAPL, "single-pass plans use one mapping");
   constexpr int MAPL_EFF = MAPL;
-  fft_exec_pass<N, NB, WG, DIR, MAP0, MAPL_EFF, 0, 1>(lds, tw, load, store);
+  fft_exec_pass<N, NB, WG, DIR, MAP0, MAPL_EFF, 0, 1, V>(lds, tw, load, store);
 }
 
 // fill an LDS twiddle table exp(-2 pi i m / N) from a global table

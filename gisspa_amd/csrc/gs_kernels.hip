@@ -289,7 +289,7 @@ constexpr int corr_lds2() { return Geo<L>::RB * exec_stride<L / 2, Geo<L>::RB, W
 
 // Pass 1 of map m, column block cb: P = tile * conj(template) on the paired
 // columns, inverse column transforms, C2R pre-processing, blocked store.
-template <int L, bool COH = false>
+template <int L, bool COH = false, int V = 0>
 __device__ __forceinline__ void corr_pass1_body(float2* lds, float* red_q, int cb, int m, int gm,
                                                 const float2* __restrict__ tiles,
                                                 const float2* __restrict__ tspec,
@@ -341,7 +341,7 @@ __device__ __forceinline__ void corr_pass1_body(float2* lds, float* red_q, int c
     if constexpr (COH) store_coh(gb + y * CB + c, c2r_pre(v, pv, wk));
     else store_nt(gb + y * CB + c, c2r_pre(v, pv, wk));
   };
-  fft_exec<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
+  fft_exec<L, CB, WG, +1, MAP_J, MAP_B, V>(lds, twL, load, store);
   sumsq = wave_sum_f(sumsq);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) red_q[wv] = sumsq;
@@ -366,7 +366,7 @@ __device__ __forceinline__ bool corr_pass1_item(int w, int nmaps, int* cb, int* 
   return *cb < NCB && *m < nmaps;
 }
 
-template <int L>
+template <int L, int V>
 __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ tiles,
                                                    const float2* __restrict__ tspec,
                                                    const MapRef* __restrict__ maps, int nmaps,
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ ti
   __shared__ float red_q[WG / 64];
   int cb, m;
   if (!corr_pass1_item<L>(blockIdx.x, nmaps, &cb, &m)) return;
-  corr_pass1_body<L>(lds, red_q, cb, m, m, tiles, tspec, maps, Gout, psum, twL);
+  corr_pass1_body<L, false, V>(lds, red_q, cb, m, m, tiles, tspec, maps, Gout, psum, twL);
 }
 
 // (-1)^(kx+ky) sgn on half spectra [t][kx][ky], in place
@@ -418,7 +418,7 @@ __device__ void corr_finalize_map(const CorrPartial* p, int nparts, const float2
 
 // Pass 2 of map m, row block rb: row transforms of Z' and the border-
 // excluded maximum with its position.
-template <int L, bool COH = false>
+template <int L, bool COH = false, int V = 0>
 __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* red_p, int m,
                                                 int gm, int rb, const float2* __restrict__ Gin,
                                                 int blo,
@@ -444,7 +444,7 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
   // values; the running maximum records a compile-time code of the winning
   // (round, slot, half) and the position is decoded once at the end.  The
   // border tests are uniform: rows per workgroup, columns per output slot.
-  constexpr Plan PL = tuned_plan(M, RB, WG, MAP_J);
+  constexpr Plan PL = variant_plan(M, RB, WG, MAP_J, V);
   constexpr int RL = PL.r[PL.count - 1];  // radix of the last pass
   constexpr int NSL = M / RL;             // its butterflies per transform
   if (rb < blo / RB || rb > bhi / RB) {
@@ -476,7 +476,7 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
     if (a > vmax) { vmax = a; vcode = code; }
     if (b > vmax) { vmax = b; vcode = code + 1; }
   };
-  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+  fft_exec<M, RB, WG, +1, MAP_J, MAP_J, V>(lds, twM, load, store);
   int pos = INT_MAX;
   if (vcode >= 0) {
     const int pr = vcode / (2 * RL), slot = (vcode >> 1) % RL, half = vcode & 1;
@@ -498,15 +498,15 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
   }
 }
 
-template <int L>
+template <int L, int V>
 __global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gin, int blo,
                                                    int bhi, CorrPartial* __restrict__ part,
                                                    const float2* __restrict__ twM) {
   __shared__ float2 lds[corr_lds2<L>()];
   __shared__ float red_v[WG / 64];
   __shared__ int red_p[WG / 64];
-  corr_pass2_body<L>(lds, red_v, red_p, blockIdx.y, blockIdx.y, blockIdx.x, Gin, blo, bhi, part,
-                     gridDim.x, twM);
+  corr_pass2_body<L, false, V>(lds, red_v, red_p, blockIdx.y, blockIdx.y, blockIdx.x, Gin, blo,
+                               bhi, part, gridDim.x, twM);
 }
 
 // Software-pipelined correlation: one launch runs pass 1 of batch k and
@@ -1121,13 +1121,27 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
   DISPATCH(L, rows_c2r_t, G, nb, out, twL, twM, s);
 }
 
+// windows whose correlation passes are compiled in every schedule variant
+// (the others run variant 0)
+constexpr bool has_variants(int L) { return L == 512 || L == 720; }
+
 template <int L>
 static void corr_pass1_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
-                         float2* G, float2* psum, const float2* twL, hipStream_t s) {
+                         float2* G, float2* psum, const float2* twL, int v, hipStream_t s) {
   constexpr int NCB8 = (Geo<L>::NCB + 7) / 8;
   dim3 grid(8 * NCB8 * nmaps);
-  hipLaunchKernelGGL((k_corr_pass1<L>), grid, dim3(WG), 0, s, tiles, tspec, maps, nmaps, G, psum,
-                     twL);
+#define P1(V_) hipLaunchKernelGGL((k_corr_pass1<L, V_>), grid, dim3(WG), 0, s, tiles, tspec, maps, \
+                                  nmaps, G, psum, twL)
+  if constexpr (has_variants(L)) {
+    switch (v) {
+      case 1: P1(1); return;
+      case 2: P1(2); return;
+      case 3: P1(3); return;
+      default: break;
+    }
+  }
+  P1(0);
+#undef P1
 }
 
 template <int L>
@@ -1145,9 +1159,13 @@ static void corr_fused_t(const float2* tiles, const float2* tspec, const MapRef*
                      nmaps1, G1, psum1, ng1, G2, nmaps2, ng2, blo, bhi, part2, NP, twL, twM);
 }
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s) {
-  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, G, psum, twL, s);
+                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s,
+                       int variant) {
+  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, G, psum, twL, variant, s);
 }
+
+int corr_variants(int L) { return has_variants(L) ? kPlanVariants : 1; }
+int corr_max_variants() { return kPlanVariants; }
 
 void launch_checker_sign(int L, float2* spec, int ntiles, hipStream_t s) {
   const long n = (long)ntiles * (L / 2 + 1) * L;
@@ -1160,9 +1178,19 @@ void launch_checker_sign(int L, float2* spec, int ntiles, hipStream_t s) {
 
 template <int L>
 static void corr_pass2_t(const float2* G, int nmaps, int blo, int bhi, CorrPartial* part,
-                         const float2* twL, const float2* twM, hipStream_t s) {
+                         const float2* twL, const float2* twM, int v, hipStream_t s) {
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
-  hipLaunchKernelGGL((k_corr_pass2<L>), grid, dim3(WG), 0, s, G, blo, bhi, part, twM);
+#define P2(V_) hipLaunchKernelGGL((k_corr_pass2<L, V_>), grid, dim3(WG), 0, s, G, blo, bhi, part, twM)
+  if constexpr (has_variants(L)) {
+    switch (v) {
+      case 1: P2(1); return;
+      case 2: P2(2); return;
+      case 3: P2(3); return;
+      default: break;
+    }
+  }
+  P2(0);
+#undef P2
 }
 template <int L>
 static void corr_stream_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
@@ -1188,10 +1216,10 @@ static void border_range(int L, float lo, float hi, int* blo, int* bhi) {
   if (*bhi < *blo) { *blo = L + 1; *bhi = L + 1; }  // nothing valid
 }
 void launch_corr_pass2(int L, const float2* G, int nmaps, float lo, float hi, CorrPartial* part,
-                       const float2* twL, const float2* twM, hipStream_t s) {
+                       const float2* twL, const float2* twM, hipStream_t s, int variant) {
   int blo, bhi;
   border_range(L, lo, hi, &blo, &bhi);
-  DISPATCH(L, corr_pass2_t, G, nmaps, blo, bhi, part, twL, twM, s);
+  DISPATCH(L, corr_pass2_t, G, nmaps, blo, bhi, part, twL, twM, variant, s);
 }
 void launch_fill_int(int* p, long n, int v, hipStream_t s) {
   if (n > 0)

@@ -99,8 +99,15 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
 // (-1)^(kx+ky) sgn), column inverse and C2R pre-processing, written to the
 // paired-column blocks of G (Geometry::gmap elements per map).
 // psum[m * corr_ncb(L) + cb] = (sum of weighted |P|^2, P(0,0).re) of the block
+// `variant` selects the FFT schedule (radix order) of the pass: 0 is the
+// tuned plan, 1 .. corr_variants(L) - 1 the alternatives gs_autotune times.
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s);
+                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s,
+                       int variant = 0);
+// schedule variants compiled for window L (1 = the tuned plan only), and
+// the most any window has
+int corr_variants(int L);
+int corr_max_variants();
 // one launch: pass 1 of maps1 (into G1/psum1) and pass 2 of the maps whose
 // intermediate G2 a previous launch wrote (into part2); either count may be 0
 void launch_corr_fused(int L, const float2* tiles, const float2* tspec, const MapRef* maps1,
@@ -121,7 +128,8 @@ int corr_ncb(int L);
 // Correlation pass 2: row C2R of G and the border-excluded max / sums, one
 // CorrPartial per (map, row block): part[m * corr_nparts(L) + block].
 void launch_corr_pass2(int L, const float2* G, int nmaps, float dm4_lo, float dm4_hi,
-                       CorrPartial* part, const float2* twL, const float2* twM, hipStream_t s);
+                       CorrPartial* part, const float2* twL, const float2* twM, hipStream_t s,
+                       int variant = 0);
 int corr_nparts(int L);
 // Combine the partials of nmaps maps (processing order `maps`) into
 // out[tile * nviews + view] (score in FP64).

@@ -276,6 +276,36 @@ int gs_remove_repeats(int device, const gs_pick_rec* recs, size_t n, double cent
 int gs_project(int device, const float* vol, int box, const float* euler, int nviews,
                float* out);
 
+/* Result of gs_autotune. */
+typedef struct gs_tune {
+  int pass1_variant, pass2_variant;  /* FFT schedule (radix order) of the two
+                                        correlation passes */
+  int batch;                         /* CC maps per launch batch */
+  int pipeline;                      /* GS_PIPE_* */
+  int candidates;                    /* launch schedules timed */
+  int pad;
+  double psi_ms;                     /* correlation of one psi at the choice */
+  double pass1_us[4], pass2_us[4];   /* each FFT variant alone, us per CC map
+                                        (0: not compiled for this window) */
+} gs_tune;
+
+/* Choose the correlation schedule on the device (no counterpart in the
+ * reference, whose cuFFT plans are fixed, main.cu:256-308): the FFT schedule
+ * of each correlation pass among the compiled variants (radix orders; timed
+ * alone on one batch of in-plane step psi_index), then the batch size (half,
+ * the current, double) and the pipeline (GS_PIPE_LANES, GS_PIPE_LAUNCH),
+ * timed on the whole correlation of that step (best of two runs each; the
+ * schedule in use stays unless another is more than 1 % faster).  Applies
+ * the choice to the context and reports it in out (may be null).  Needs templates and a
+ * micrograph; replaces the last gs_pick's map statistics (its picks are
+ * kept).  All schedules give the same scores within FP32 rounding (the
+ * pipelines bit-identical). */
+int gs_autotune(void* ctx, int psi_index, gs_tune* out);
+/* Set the FFT schedule variants (0..3) of the correlation passes directly
+ * (0 = the tuned plan; gs_autotune's choices).  Windows compiled without
+ * variants (all but 512 and 720) run variant 0. */
+int gs_set_fft_variants(void* ctx, int pass1, int pass2);
+
 int gs_get_info(void* ctx, gs_info* info);
 int gs_get_timing(void* ctx, gs_timing* t);
 /* Sample per-launch correlation times in gs_pick (see gs_timing). */

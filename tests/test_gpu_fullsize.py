@@ -55,8 +55,8 @@ def test_small_configuration(gpu):
     _compare_picks(picks, orc.select_norm0(p, ref, 0, img.shape[1], img.shape[0]))
 
 
-def test_full_size_shape(gpu):
-    import gisspa_amd
+@pytest.fixture(scope="module")
+def full_case():
     dens = synth.blob_density(60, 180, 1)
     views = synth.sphere_eulers(1649)
     tm = synth.project_templates(dens, 180, views[::206][:8])
@@ -66,12 +66,51 @@ def test_full_size_shape(gpu):
                   diameter=135.0, threshold=7.0, window_size=720, overlap=180, ampconst=0.1)
     p = orc.params(720, **params)
     ref = orc.pick_norm0(p, orc.ctf(1.8), tm, mg.image, 0, 1, nthreads=NTHREADS)
+    return tm, mg.image, params, p, ref
+
+
+def test_full_size_shape(gpu, full_case):
+    import gisspa_amd
+    tm, image, params, p, ref = full_case
     with gisspa_amd.Picker(0, **params) as pk:
         pk.set_batch(100)
         pk.set_templates(tm, 1.8)
-        pk.set_micrograph(mg.image, 1.8)
+        pk.set_micrograph(image, 1.8)
         picks = pk.pick(0, 1)
         got = pk.mapstats().reshape(ref.shape)
     assert ref.shape == (1, 88, 8)
     _compare(got, ref, "5760x4092 / window 720")
+    _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
+
+
+@pytest.mark.parametrize("v1,v2", [(1, 1), (2, 2), (3, 3)])
+def test_fft_schedule_variants(gpu, full_case, v1, v2):
+    """Every compiled FFT schedule of the correlation passes meets the same
+    bar as the tuned one."""
+    import gisspa_amd
+    tm, image, params, p, ref = full_case
+    with gisspa_amd.Picker(0, **params) as pk:
+        pk.set_fft_variants(v1, v2)
+        pk.set_templates(tm, 1.8)
+        pk.set_micrograph(image, 1.8)
+        picks = pk.pick(0, 1)
+        got = pk.mapstats().reshape(ref.shape)
+    _compare(got, ref, f"window 720, FFT variants {v1}/{v2}")
+    _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
+
+
+def test_autotune(gpu, full_case):
+    import gisspa_amd
+    tm, image, params, p, ref = full_case
+    with gisspa_amd.Picker(0, **params) as pk:
+        pk.set_templates(tm, 1.8)
+        pk.set_micrograph(image, 1.8)
+        t = pk.autotune(0)
+        print("autotune:", t.pass1_variant, t.pass2_variant, t.batch, t.pipeline, t.psi_ms,
+              list(t.pass1_us), list(t.pass2_us))
+        assert 0 <= t.pass1_variant < 4 and 0 <= t.pass2_variant < 4
+        assert t.candidates == 6 and t.psi_ms > 0 and min(t.pass1_us) > 0
+        picks = pk.pick(0, 1)
+        got = pk.mapstats().reshape(ref.shape)
+    _compare(got, ref, "window 720 after autotune")
     _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
