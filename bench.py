@@ -437,7 +437,15 @@ def main():
                      "launch_ms": t_launch, "maps_per_launch": mpl, "bytes_per_map": bpm,
                      "psi_correlation_achieved": pair,
                      "psi_correlation_frac": (pair / HBM_PEAK_GBS) if pair else None,
-                     "copy_gbs": copy_gbs},
+                     "copy_gbs": copy_gbs,
+                     # each pass alone (autotune's timing of the chosen FFT schedule, one
+                     # stream, no overlap): algorithmic bytes / isolated time per map
+                     "pass1_isolated_gbs": (16.0 * L * H / (tune["pass1_us_per_map"]
+                                            [tune["pass1_variant"]] * 1e-6) / 1e9)
+                     if tune else None,
+                     "pass2_isolated_gbs": (8.0 * L * H / (tune["pass2_us_per_map"]
+                                            [tune["pass2_variant"]] * 1e-6) / 1e9)
+                     if tune else None},
         "refine": refine,
         "dedupe": dedupe,
         "project": proj,
