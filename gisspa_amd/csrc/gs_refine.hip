@@ -106,45 +106,16 @@ __device__ __forceinline__ void block_sum_n(double (&v)[N], double* scratch) {
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-template <int L>
-__global__ __launch_bounds__(WG) void k_ref_weight(const float2* __restrict__ white,
-                                                   const RefCand* __restrict__ cand, WeightArgs w0,
-                                                   float2* __restrict__ A,
-                                                   double* __restrict__ part) {
-  constexpr int H = L / 2 + 1;
-  __shared__ double red[2 * (WG / 64)];
-  const int b = blockIdx.y;
-  const RefCand cd = cand[b];
-  const WeightArgs w = cand_weight(w0, cd.df);
-  const float2* Bj = white + (long)cd.view * H * L;
-  double acc[2] = {0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < (long)H * L; i += (long)RPARTS * WG) {
-    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
-    const WeightD wd = weight_deriv<L>(kx, ky, w);
-    const float2 bv = Bj[i];
-    A[(long)b * H * L + i] = cscale(bv, wd.f);
-    if (wd.rbin < L / 2 && wd.rbin >= 0) {
-      const double b2 = (double)bv.x * bv.x + (double)bv.y * bv.y;
-      acc[0] += b2 * wd.f * wd.f;
-      acc[1] += b2 * wd.f * wd.fd;
-    }
-  }
-  block_sum_n<2>(acc, red);
-  if (threadIdx.x == 0) {
-    part[((long)b * RPARTS + blockIdx.x) * 2] = acc[0];
-    part[((long)b * RPARTS + blockIdx.x) * 2 + 1] = acc[1];
-  }
-}
-
 // m = sqrt(S) / count and dm / d defocus = (sum |B|^2 f f') / (count sqrt(S))
 __global__ void k_ref_norm_m(const double* __restrict__ part, int nb, double count,
-                             RefScal* __restrict__ sc, float* __restrict__ pre) {
+                             RefScal* __restrict__ sc, float* __restrict__ pre, int np,
+                             int stride) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double S = 0, SW = 0;
-  for (int i = 0; i < RPARTS; ++i) {
-    S += part[((long)b * RPARTS + i) * 2];
-    SW += part[((long)b * RPARTS + i) * 2 + 1];
+  for (int i = 0; i < np; ++i) {
+    S += part[((long)b * stride + i) * 2];
+    SW += part[((long)b * stride + i) * 2 + 1];
   }
   const double rs = sqrt(S);
   const double m = rs / count;
@@ -184,6 +155,173 @@ __device__ __forceinline__ void phase_tables(float2* ex, float2* ey, double ux, 
     (col ? ex[kk] : ey[kk]) = make_float2((float)cs, (float)sn);
   }
   __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// column kernels of both passes
+// ---------------------------------------------------------------------------
+// Column kernels with a computed load or a reducing store.  One workgroup
+// per (column block of CB half-spectrum columns, candidate), the layout of
+// k_cols_inv_T / k_cols_fwd; their partials use FSLOTS slots per candidate
+// (one per column block).
+constexpr int FSLOTS = 2 * RPARTS;
+template <int L>
+constexpr int ncol_blocks() { return (Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB; }
+
+// inverse columns of the transposed store of k_cols_inv_T
+template <int L>
+__device__ __forceinline__ void cols_inv_store(const float2* lds, float2* __restrict__ gb,
+                                               int kx0, int ncol) {
+  constexpr int HP = Geo<L>::HP, CB = Geo<L>::CB;
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int y = g / CB, c = g - y * CB;
+    if (c < ncol) gb[(long)y * HP + kx0 + c] = lds[c * L + y];
+  }
+}
+
+// Forward weighting fused into the inverse column pass: A = f(k; defocus) B_j
+// is formed while loading (never stored), with the partials of sum |A|^2 and
+// sum |B|^2 f f' (over the radial bins the normaliser counts), then the
+// inverse column transform writes the transposed intermediate of the C2R.
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict__ white,
+                                                        const RefCand* __restrict__ cand,
+                                                        WeightArgs w0, float2* __restrict__ Gout,
+                                                        double* __restrict__ part,
+                                                        const float2* __restrict__ twL) {
+  using Gm = Geo<L>;
+  constexpr int H = Gm::H, HP = Gm::HP, CB = Gm::CB;
+  static_assert(ncol_blocks<L>() <= FSLOTS, "partial slots");
+  __shared__ float2 lds[CB * L];
+  __shared__ float2 tw[L];
+  __shared__ double red[2 * (WG / 64)];
+  const int b = blockIdx.y, kx0 = blockIdx.x * CB;
+  const RefCand cd = cand[b];
+  const WeightArgs w = cand_weight(w0, cd.df);
+  load_twiddles<L, WG>(tw, twL);
+  const float2* Bj = white + (long)cd.view * H * L;
+  const int ncol = min(CB, H - kx0);
+  double acc[2] = {0, 0};
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int c = g / L, ky = g - c * L;
+    float2 v = make_float2(0.f, 0.f);
+    if (c < ncol) {
+      const int kx = kx0 + c;
+      const WeightD wd = weight_deriv<L>(kx, ky, w);
+      const float2 bv = Bj[(long)kx * L + ky];
+      v = cscale(bv, wd.f);
+      if (wd.rbin < L / 2 && wd.rbin >= 0) {
+        const double b2 = (double)bv.x * bv.x + (double)bv.y * bv.y;
+        acc[0] += b2 * wd.f * wd.f;
+        acc[1] += b2 * wd.f * wd.fd;
+      }
+    }
+    lds[g] = v;
+  }
+  block_sum_n<2>(acc, red);  // (its barriers also order the LDS fill)
+  if (threadIdx.x == 0) {
+    part[((long)b * FSLOTS + blockIdx.x) * 2] = acc[0];
+    part[((long)b * FSLOTS + blockIdx.x) * 2 + 1] = acc[1];
+  }
+  __syncthreads();
+  lds_fft<L, CB, WG, +1, L>(lds, tw);
+  cols_inv_store<L>(lds, Gout + (long)b * L * HP, kx0, ncol);
+}
+
+// Backward: d score / d That(k) = g I with g = E / sigma - (c / sigma^3)
+// conj(P), formed while loading; the DC is zeroed in the forward pass, so it
+// carries no gradient.  The real gradient image is (1 / L^2) sum_half w
+// Re(G e^{+2 pi i k.x / L}); the C2R computes exactly that once the kx = 0
+// and kx = L/2 columns are Hermitian in ky, so those two columns are loaded
+// as (G(ky) + conj G(-ky)) / 2.  The inverse column transform then writes the
+// intermediate of the C2R that yields d/dRn.
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__ tiles,
+                                                      const float2* __restrict__ spec,
+                                                      const RefCand* __restrict__ cand,
+                                                      const RefScal* __restrict__ sc,
+                                                      float2* __restrict__ Gout,
+                                                      const float2* __restrict__ twL) {
+  using Gm = Geo<L>;
+  constexpr int H = Gm::H, HP = Gm::HP, CB = Gm::CB;
+  __shared__ float2 lds[CB * L];
+  __shared__ float2 tw[L];
+  __shared__ float2 ex[H], ey[L];
+  const int b = blockIdx.y, kx0 = blockIdx.x * CB;
+  const RefCand cd = cand[b];
+  load_twiddles<L, WG>(tw, twL);
+  phase_tables<L>(ex, ey, (double)cd.x0 + cd.dx, (double)cd.y0 + cd.dy);
+  const float2* I = tiles + (long)cd.ptile * H * L;
+  const float2* T = spec + (long)b * H * L;
+  const double sg = sc[b].sigma;
+  const float is = sg > 0 ? (float)(1.0 / sg) : 0.0f;
+  const float cs3 = sg > 0 ? (float)(sc[b].c / (sg * sg * sg)) : 0.0f;
+  auto grad = [&](int kx, int ky) -> float2 {
+    if (kx == 0 && ky == 0) return make_float2(0.f, 0.f);
+    const long i = (long)kx * L + ky;
+    const float2 iv = I[i];
+    const float2 P = cmulc(iv, T[i]);
+    const float2 e = cmul(ex[kx], ey[ky]);
+    const float2 g = make_float2(e.x * is - cs3 * P.x, e.y * is + cs3 * P.y);
+    return cmul(g, iv);
+  };
+  const int ncol = min(CB, H - kx0);
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int c = g / L, ky = g - c * L;
+    float2 gv = make_float2(0.f, 0.f);
+    if (c < ncol) {
+      const int kx = kx0 + c;
+      gv = grad(kx, ky);
+      if (kx == 0 || kx == L / 2) {
+        const float2 gm = grad(kx, (L - ky) % L);
+        gv = make_float2(0.5f * (gv.x + gm.x), 0.5f * (gv.y - gm.y));
+      }
+    }
+    lds[g] = gv;
+  }
+  __syncthreads();
+  lds_fft<L, CB, WG, +1, L>(lds, tw);
+  cols_inv_store<L>(lds, Gout + (long)b * L * HP, kx0, ncol);
+}
+
+// Backward: forward column transform of the row R2C of d/dT (d/dA), reduced
+// on the spot to the defocus partial sum_half w Re(conj(d/dA) f' B) of
+// d score / d defocus through A = f(defocus) B, so d/dA is never stored.
+template <int L>
+__global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ rows,
+                                                    const float2* __restrict__ white,
+                                                    const RefCand* __restrict__ cand,
+                                                    WeightArgs w0, double* __restrict__ part,
+                                                    const float2* __restrict__ twL) {
+  using Gm = Geo<L>;
+  constexpr int H = Gm::H, CB = Gm::CB;
+  __shared__ float2 lds[CB * L];
+  __shared__ float2 tw[L];
+  __shared__ double red[1 * (WG / 64)];
+  const int b = blockIdx.y, kx0 = blockIdx.x * CB;
+  const RefCand cd = cand[b];
+  const WeightArgs w = cand_weight(w0, cd.df);
+  load_twiddles<L, WG>(tw, twL);
+  const float2* base = rows + ((long)b * H + kx0) * L;
+  const int ncol = min(CB, H - kx0);
+  for (int g = threadIdx.x; g < CB * L; g += WG) {
+    const int c = g / L;
+    lds[g] = (c < ncol) ? base[g] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft<L, CB, WG, -1, L>(lds, tw);
+  const float2* Bj = white + (long)cd.view * H * L;
+  double acc[1] = {0};
+  for (int g = threadIdx.x; g < ncol * L; g += WG) {
+    const int c = g / L, ky = g - c * L, kx = kx0 + c;
+    const WeightD wd = weight_deriv<L>(kx, ky, w);
+    if (wd.fd == 0.0f) continue;
+    const float2 ga = lds[g], bv = Bj[(long)kx * L + ky];
+    const float wh = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
+    acc[0] += (double)(wh * wd.fd * (ga.x * bv.x + ga.y * bv.y));
+  }
+  block_sum_n<1>(acc, red);
+  if (threadIdx.x == 0) part[(long)b * FSLOTS + blockIdx.x] = acc[0];
 }
 
 template <int L>
@@ -235,47 +373,6 @@ __global__ void k_ref_score_fin(const double* __restrict__ part, int nb, int L,
 // ---------------------------------------------------------------------------
 // backward
 // ---------------------------------------------------------------------------
-// d score / d That(k) = g I with g = E / sigma - (c / sigma^3) conj(P); the
-// DC is zeroed in the forward pass, so it carries no gradient.  The real
-// gradient image is (1 / L^2) sum_half w Re(G e^{+2 pi i k.x / L}); the C2R
-// computes exactly that once the kx = 0 and kx = L/2 columns are Hermitian in
-// ky, so those two columns are written as (G(ky) + conj G(-ky)) / 2.
-template <int L>
-__global__ __launch_bounds__(WG) void k_ref_grad_spec(const float2* __restrict__ tiles,
-                                                      const float2* __restrict__ spec,
-                                                      const RefCand* __restrict__ cand,
-                                                      const RefScal* __restrict__ sc,
-                                                      float2* __restrict__ G) {
-  constexpr int H = L / 2 + 1;
-  __shared__ float2 ex[H], ey[L];
-  const int b = blockIdx.y;
-  const RefCand cd = cand[b];
-  phase_tables<L>(ex, ey, (double)cd.x0 + cd.dx, (double)cd.y0 + cd.dy);
-  const float2* I = tiles + (long)cd.ptile * H * L;
-  const float2* T = spec + (long)b * H * L;
-  const double sg = sc[b].sigma;
-  const float is = sg > 0 ? (float)(1.0 / sg) : 0.0f;
-  const float cs3 = sg > 0 ? (float)(sc[b].c / (sg * sg * sg)) : 0.0f;
-  auto grad = [&](int kx, int ky) -> float2 {
-    if (kx == 0 && ky == 0) return make_float2(0.f, 0.f);
-    const long i = (long)kx * L + ky;
-    const float2 iv = I[i];
-    const float2 P = cmulc(iv, T[i]);
-    const float2 e = cmul(ex[kx], ey[ky]);
-    const float2 g = make_float2(e.x * is - cs3 * P.x, e.y * is + cs3 * P.y);
-    return cmul(g, iv);
-  };
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < (long)H * L; i += (long)RPARTS * WG) {
-    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
-    float2 gv = grad(kx, ky);
-    if (kx == 0 || kx == L / 2) {
-      const float2 gm = grad(kx, (L - ky) % L);
-      gv = make_float2(0.5f * (gv.x + gm.x), 0.5f * (gv.y - gm.y));
-    }
-    G[(long)b * H * L + i] = gv;
-  }
-}
-
 // sums of g = d/dRn and of g Rn over the pixels (normalisation adjoint)
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_norm_red(const float* __restrict__ Tun,
@@ -432,37 +529,12 @@ __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gT
   }
 }
 
-// d score / d defocus through A = f(defocus) B: sum_half w Re(conj(G_A) f' B)
-template <int L>
-__global__ __launch_bounds__(WG) void k_ref_df(const float2* __restrict__ white,
-                                               const float2* __restrict__ GA,
-                                               const RefCand* __restrict__ cand, WeightArgs w0,
-                                               double* __restrict__ part) {
-  constexpr int H = L / 2 + 1;
-  __shared__ double red[1 * (WG / 64)];
-  const int b = blockIdx.y;
-  const RefCand cd = cand[b];
-  const WeightArgs w = cand_weight(w0, cd.df);
-  const float2* Bj = white + (long)cd.view * H * L;
-  double acc[1] = {0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < (long)H * L; i += (long)RPARTS * WG) {
-    const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
-    const WeightD wd = weight_deriv<L>(kx, ky, w);
-    if (wd.fd == 0.0f) continue;
-    const float2 ga = GA[(long)b * H * L + i], bv = Bj[i];
-    const float wh = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
-    acc[0] += (double)(wh * wd.fd * (ga.x * bv.x + ga.y * bv.y));
-  }
-  block_sum_n<1>(acc, red);
-  if (threadIdx.x == 0) part[(long)b * RPARTS + blockIdx.x] = acc[0];
-}
-
 __global__ void k_ref_out(const double* __restrict__ part, int nb, RefScal* __restrict__ sc,
-                          float* __restrict__ out) {
+                          float* __restrict__ out, int np, int stride) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double d = 0;
-  for (int i = 0; i < RPARTS; ++i) d += part[(long)b * RPARTS + i];
+  for (int i = 0; i < np; ++i) d += part[(long)b * stride + i];
   sc[b].ddf = d + sc[b].dmacc * sc[b].dm;
   out[5 * b + 0] = (float)sc[b].score;
   out[5 * b + 1] = (float)sc[b].dpsi;
@@ -499,10 +571,11 @@ template <int L>
 static void ref_forward_t(const RefArgs& a, hipStream_t s) {
   const dim3 grid(RPARTS, a.nb), blk(WG);
   const int small = (a.nb + 63) / 64;
-  hipLaunchKernelGGL((k_ref_weight<L>), grid, blk, 0, s, a.white, a.cand, a.w, a.A, a.part);
+  constexpr int NCB = ncol_blocks<L>();
+  hipLaunchKernelGGL((k_ref_weight_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.white, a.cand, a.w,
+                     a.gtmp, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_norm_m, dim3(small), dim3(64), 0, s, a.part, a.nb, a.count, a.sc,
-                     a.pre);
-  launch_cols_inv_T(L, a.A, a.nb, a.gtmp, a.twL, s);
+                     a.pre, NCB, FSLOTS);
   launch_rows_c2r(L, a.gtmp, a.nb, a.Tun, a.twL, a.twM, s);
   launch_rows_r2c_rotb(L, a.Tun, a.nb, a.cs, a.pre, a.dmask, a.spec, a.rot_part, a.twL, a.twM, s);
   hipLaunchKernelGGL(k_ref_rot_stats, dim3(small), dim3(64), 0, s, a.rot_part,
@@ -516,9 +589,10 @@ template <int L>
 static void ref_backward_t(const RefArgs& a, hipStream_t s) {
   const dim3 grid(RPARTS, a.nb), blk(WG);
   const int small = (a.nb + 63) / 64;
+  constexpr int NCB = ncol_blocks<L>();
   // d/dThat -> d/dRn (C2R, / L^2 applied in the next kernels)
-  hipLaunchKernelGGL((k_ref_grad_spec<L>), grid, blk, 0, s, a.tiles, a.spec, a.cand, a.sc, a.A);
-  launch_cols_inv_T(L, a.A, a.nb, a.gtmp, a.twL, s);
+  hipLaunchKernelGGL((k_ref_grad_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.tiles, a.spec, a.cand,
+                     a.sc, a.gtmp, a.twL);
   launch_rows_c2r(L, a.gtmp, a.nb, a.gRn, a.twL, a.twM, s);
   hipLaunchKernelGGL((k_ref_norm_red<L>), grid, blk, 0, s, a.Tun, a.gRn, a.cs, a.pre, a.sc,
                      a.dmask, a.part);
@@ -530,9 +604,10 @@ static void ref_backward_t(const RefArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.gRn);
   // d/dT (in gRn) -> d/dA: unnormalised R2C
   launch_rows_r2c_img(L, a.gRn, a.nb, a.A, a.twL, a.twM, s);
-  launch_cols_fwd(L, a.A, a.nb, 1.0f, nullptr, 0, a.twL, s);
-  hipLaunchKernelGGL((k_ref_df<L>), grid, blk, 0, s, a.white, a.A, a.cand, a.w, a.part);
-  hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.out);
+  hipLaunchKernelGGL((k_ref_cols_df<L>), dim3(NCB, a.nb), blk, 0, s, a.A, a.white, a.cand, a.w,
+                     a.part, a.twL);
+  hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.out, NCB,
+                     FSLOTS);
 }
 
 void launch_ref_forward(int L, const RefArgs& a, hipStream_t s) { RDISPATCH(L, ref_forward_t, a, s); }
