@@ -212,6 +212,66 @@ def project_rate(n_views: int, no_cpu: bool, box: int = 180):
     return out
 
 
+def batched_rate(templates, n_mics: int, npsi: int, seed: int = 1, tile_block: int = 0,
+                 autotune: bool = False):
+    """Many micrographs in one tile pool (gs_set_micrographs): n_mics images
+    of sizes drawn from 3838x3710, 4096x4096, 5760x4092, each with its own
+    defocus (0.8-2.5 um) and 200-400 particles, templates of the 1649 views in
+    mixed boxes of 128, 180 and 256 px (cropped / padded from the 180 px
+    projections); npsi in-plane steps timed, extrapolated to 72."""
+    import gisspa_amd
+    from gisspa_amd import synth
+    rng = np.random.default_rng(seed)
+    sizes = [(3838, 3710), (4096, 4096), (5760, 4092)]
+    dens = synth.blob_density(60, 180, 1)
+    views = synth.sphere_eulers(templates.shape[0])
+    imgs, dfs, shapes = [], [], []
+    t = time.time()
+    for i in range(n_mics):
+        nx, ny = sizes[int(rng.integers(0, 3))]
+        df = float(rng.uniform(0.8, 2.5))
+        npart = int(rng.integers(200, 401))
+        mg = synth.make_micrograph(nx, ny, dens, views, npart, 0.05, seed * 100 + i, apix=1.0,
+                                   defocus_um=df, ampconst=0.1, box=180, psi_step=5.0)
+        imgs.append(mg.image)
+        dfs.append(df)
+        shapes.append(f"{nx}x{ny}")
+        log(f"[bench] batched: micrograph {i} {nx}x{ny} defocus {df:.2f} um, {npart} particles")
+    log(f"[bench] batched: generated {n_mics} micrographs in {time.time() - t:.1f}s")
+    mixed = []
+    for j, tm in enumerate(templates):
+        b = (128, 180, 256)[j % 3]
+        if b <= 180:
+            o = (180 - b) // 2
+            mixed.append(np.ascontiguousarray(tm[o:o + b, o:o + b]))
+        else:
+            z = np.zeros((b, b), np.float32)
+            o = (b - 180) // 2
+            z[o:o + 180, o:o + 180] = tm
+            mixed.append(z)
+    pk = gisspa_amd.Picker(0, **FULL_PARAMS)
+    if tile_block:
+        pk.set_tile_block(tile_block)
+    pk.set_templates_mixed(mixed, dfs[0])
+    t = time.time()
+    pk.set_micrographs(imgs, dfs)
+    t_prep = time.time() - t
+    info = pk.info()
+    tune = pk.autotune(0) if autotune else None
+    pk.pick(0, 1)                      # warm-up
+    t = time.time()
+    picks = pk.pick(0, npsi)
+    t_psi = (time.time() - t) / npsi
+    pk.close()
+    sec = t_prep + 72 * t_psi
+    return {"micrographs": n_mics, "sizes": shapes, "tiles": info.n_tiles,
+            "maps_per_psi": info.n_tiles * templates.shape[0], "prep_s": t_prep,
+            "psi_s": t_psi, "picks_timed": int(len(picks)), "value": n_mics * 3600.0 / sec,
+            "tile_block": tune.tile_block if tune else tile_block,
+            "unit": "micrographs/hour",
+            "sample": f"{npsi} psi timed (after one warm-up psi), extrapolated to 72"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +300,9 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="choose the correlation schedule on the device before timing (0 = the "
                          "built-in defaults)")
+    ap.add_argument("--batched", type=int, default=0, metavar="N",
+                    help="also time N micrographs of uneven sizes in one tile pool with mixed "
+                         "template boxes (0 = skip)")
     ap.add_argument("--project", type=int, default=1,
                     help="time the template projection of the density (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -395,6 +458,13 @@ def main():
         log(f"[bench] psi correlation: {tpsi / max(args.steps, 1):.1f} ms, {pair:.0f} GB/s "
             f"algorithmic ({pair / HBM_PEAK_GBS:.3f} of HBM peak)")
 
+    batched = None
+    if args.batched:
+        batched = batched_rate(templates, args.batched, max(1, min(npsi, 4)),
+                               tile_block=args.tile_block, autotune=bool(args.autotune))
+        log(f"[bench] batched: {batched['micrographs']} micrographs ({batched['tiles']} tiles), "
+            f"{batched['psi_s']:.2f}s per psi, {batched['value']:.1f} micrographs/hour")
+
     copy_gbs = hip.copy_rate(4 << 30) / 1e9
     log(f"[bench] 4 GiB device copy: {copy_gbs:.0f} GB/s (read + write); psi correlation at "
         f"{(pair or 0) / copy_gbs:.3f} of it")
@@ -448,6 +518,7 @@ def main():
                      if tune else None},
         "refine": refine,
         "dedupe": dedupe,
+        "batched": batched,
         "project": proj,
         "plain": plain,
         "speedup_vs_plain": (mph / plain["value"]) if plain else None,

@@ -65,7 +65,8 @@ class GsTiming(ctypes.Structure):
 class GsTune(ctypes.Structure):
     _fields_ = [("pass1_variant", ctypes.c_int), ("pass2_variant", ctypes.c_int),
                 ("batch", ctypes.c_int), ("pipeline", ctypes.c_int),
-                ("candidates", ctypes.c_int), ("pad", ctypes.c_int), ("psi_ms", ctypes.c_double),
+                ("candidates", ctypes.c_int), ("tile_block", ctypes.c_int),
+                ("psi_ms", ctypes.c_double),
                 ("pass1_us", ctypes.c_double * 4), ("pass2_us", ctypes.c_double * 4)]
 
 
@@ -166,6 +167,8 @@ def load() -> ctypes.CDLL:
                                       ctypes.c_double, vp, ctypes.POINTER(ctypes.c_size_t)]
     lib.gs_project.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
     lib.gs_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(GsTune)]
+    lib.gs_set_templates_mixed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), vp,
+                                           vp, ctypes.POINTER(GsCtf)]
     lib.gs_set_fft_variants.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     _lib = lib
     return lib
@@ -274,6 +277,18 @@ class Picker:
         c = GsCtf(defocus, dfdiff, dfang)
         _check(self.lib.gs_set_templates(self.ctx, tm.ctypes.data, n, bx, by, ctypes.byref(c)),
                "gs_set_templates")
+
+    def set_templates_mixed(self, templates, defocus: float, dfdiff: float = 0.0,
+                            dfang: float = 0.0) -> None:
+        """Templates of different boxes (a list of 2D arrays)."""
+        arrs = [np.ascontiguousarray(t, dtype=np.float32) for t in templates]
+        n = len(arrs)
+        ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in arrs])
+        bx = np.array([a.shape[1] for a in arrs], dtype=np.int32)
+        by = np.array([a.shape[0] for a in arrs], dtype=np.int32)
+        c = GsCtf(defocus, dfdiff, dfang)
+        _check(self.lib.gs_set_templates_mixed(self.ctx, n, ptrs, bx.ctypes.data, by.ctypes.data,
+                                               ctypes.byref(c)), "gs_set_templates_mixed")
 
     def set_micrograph(self, img: np.ndarray, defocus: float, dfdiff: float = 0.0,
                        dfang: float = 0.0) -> None:

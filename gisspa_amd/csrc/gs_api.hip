@@ -93,7 +93,7 @@ struct Ctx {
   DevBuf psum;  // column-block Parseval sums of every map of one psi
   DevBuf maps, stats, sel, sel_count;
   DevBuf maps_blk;
-  int tile_block = 1 << 30;  // one block: view-major (each template read once per psi)
+  int tile_block = 0;  // tiles per block of the map order; 0: as many as fit in 192 MiB
   int maps_T = -1, maps_N = -1, maps_tb = -1;
   DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_z, n1_rec;  // norm_type 1
   // plain mode buffers
@@ -521,19 +521,23 @@ int ensure_corr_buffers(Ctx* c) {
   // through the MapRef of each map.
   //  * maps (norm_type 1): view-major, so a batch of at most T maps never
   //    holds one tile twice;
-  //  * maps_blk (norm_type 0): blocks of tile_block tiles, all views of a
-  //    block before the next block, tiles fastest.  The default single block
-  //    is view-major: the template spectra (N half spectra per psi, far
-  //    beyond any cache) are read once per psi and stay in L2 across the
-  //    tiles, while the tile spectra (T half spectra) are re-read from the
-  //    last-level cache.  Smaller blocks trade that for tile reuse.
+  //  * maps_blk (norm_type 0): blocks of tb tiles, all views of a block
+  //    before the next block, tiles fastest.  Within a block each template
+  //    half spectrum is read from HBM once and stays in L2 across the block's
+  //    tiles, while the block's tile spectra are re-read for every view from
+  //    the 256 MB last-level cache, so the default block is as many tiles as
+  //    fit in 192 MiB (96 at L = 720: a whole 5760 x 4092 micrograph, 88
+  //    tiles, is one block; a batch of 16 micrographs, 1078 tiles, runs 2.09
+  //    instead of 2.62 s per psi compared with one block of all tiles).
   const size_t nm = (size_t)c->T * c->N;
-  if (c->maps_T != c->T || c->maps_N != c->N || c->maps_tb != c->tile_block) {
+  const long tile_bytes = (long)(L / 2 + 1) * L * (long)sizeof(float2);
+  const int tb = c->tile_block > 0 ? c->tile_block
+                                   : (int)std::max(1l, (192l << 20) / tile_bytes);
+  if (c->maps_T != c->T || c->maps_N != c->N || c->maps_tb != tb) {
     std::vector<MapRef> mr(nm), mb;
     mb.reserve(nm);
     for (int j = 0; j < c->N; ++j)
       for (int t = 0; t < c->T; ++t) mr[(size_t)j * c->T + t] = MapRef{t, j};
-    const int tb = std::max(1, c->tile_block);
     for (int t0 = 0; t0 < c->T; t0 += tb)
       for (int j = 0; j < c->N; ++j)
         for (int t = t0; t < std::min(c->T, t0 + tb); ++t) mb.push_back(MapRef{t, j});
@@ -545,7 +549,7 @@ int ensure_corr_buffers(Ctx* c) {
     HIPCHK(hipStreamSynchronize(c->stream));  // mr, mb are released on return
     c->maps_T = c->T;
     c->maps_N = c->N;
-    c->maps_tb = c->tile_block;
+    c->maps_tb = tb;
   }
   return GS_OK;
 }
@@ -1055,6 +1059,34 @@ int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const 
   return prep_templates(c, host, n, bx, by);
 }
 
+int gs_set_templates_mixed(void* ctx, int n, const float* const* host, const int* bx,
+                           const int* by, const gs_ctf* tctf) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !host || !bx || !by || n <= 0 || !tctf) return GS_ERR_ARG;
+  int B = 0;
+  for (int j = 0; j < n; ++j) {
+    if (!host[j] || bx[j] <= 0 || by[j] <= 0) return GS_ERR_ARG;
+    B = std::max(B, std::max(bx[j], by[j]));
+  }
+  int L = c->p.window_size;
+  if (L < B) {  // adjustPaddingSize (main.cu:158-178) on the largest box
+    L = 0;
+    while (L < B) L += 32;
+  }
+  // Canvas B with L - B even: then centring a template in B and B in L puts
+  // it at ((L - bx) / 2, (L - by) / 2), the offset readAndPaddingTemplate
+  // gives each template from its own header (main.cu:206-207).
+  if ((L - B) % 2) ++B;
+  std::vector<float> canvas((size_t)n * B * B, 0.0f);
+  for (int j = 0; j < n; ++j) {
+    const int ox = (B - bx[j]) / 2, oy = (B - by[j]) / 2;
+    for (int y = 0; y < by[j]; ++y)
+      std::copy(host[j] + (size_t)y * bx[j], host[j] + (size_t)(y + 1) * bx[j],
+                canvas.begin() + ((size_t)j * B + oy + y) * B + ox);
+  }
+  return gs_set_templates(ctx, canvas.data(), n, B, B, tctf);
+}
+
 static int set_mics(Ctx* c, int n, const float* const* images, const int* nx, const int* ny,
                     const gs_ctf* ctfs, bool dev) {
   if (!c || n <= 0 || !images || !nx || !ny || !ctfs) return GS_ERR_ARG;
@@ -1136,10 +1168,10 @@ int gs_set_pipeline(void* ctx, int mode, int lag, int ring) {
 
 int gs_set_tile_block(void* ctx, int tiles) {
   Ctx* c = static_cast<Ctx*>(ctx);
-  if (!c || tiles < 1) return GS_ERR_ARG;
+  if (!c || tiles < 0) return GS_ERR_ARG;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipDeviceSynchronize());
-  c->tile_block = tiles;
+  c->tile_block = tiles;  // 0: the cache-sized default
   drop_graphs(c);
   return GS_OK;
 }
@@ -1522,28 +1554,47 @@ int gs_autotune(void* ctx, int psi_index, gs_tune* out) {
   for (auto& x : e) (void)hipEventDestroy(x);
   c->var1 = best1;
   c->var2 = best2;
-  // 2. launch schedule: batch and pipeline, timed on the whole correlation
-  //    of the psi (gs_pick with launch sampling)
+  // 2. launch schedule, timed on the whole correlation of the psi (gs_pick
+  //    with launch sampling): the tile block (the cache-sized default or all
+  //    tiles in one block), then batch and pipeline
+  c->profiling = true;
+  std::vector<gs_particle> keep_picks = c->picks;
+  auto psi_time = [&](double* ms) -> int {
+    *ms = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {  // the first run allocates and warms up
+      size_t np = 0;
+      const int r = gs_pick(ctx, psi_index, psi_index + 1, nullptr, 0, &np);
+      if (r != GS_OK && r != GS_ERR_CAPACITY) return r;
+      if (rep > 0) *ms = std::min(*ms, c->timing.corr_psi_ms);
+    }
+    t.candidates++;
+    return GS_OK;
+  };
+  {
+    const int tb0 = c->tile_block;
+    const long tile_bytes = (long)(L / 2 + 1) * L * (long)sizeof(float2);
+    const int tb_auto = (int)std::max(1l, (192l << 20) / tile_bytes);
+    const int cur_tb = tb0 > 0 ? tb0 : tb_auto;
+    if (c->T > cur_tb) {  // otherwise every block size gives the same order
+      double ms_cur = 0, ms_all = 0;
+      CHK(psi_time(&ms_cur));
+      c->tile_block = c->T;
+      CHK(psi_time(&ms_all));
+      c->tile_block = ms_all < 0.99 * ms_cur ? c->T : tb0;
+    }
+  }
   const int b0 = c->batch;
   const int batches[3] = {std::max(1, b0 / 2), b0, 2 * b0};
   const int pipes[2] = {GS_PIPE_LANES, GS_PIPE_LAUNCH};
   const int p0 = c->pipeline;
   double best_ms = 1e300, cur_ms = 1e300;
   int best_b = b0, best_p = p0;
-  c->profiling = true;
-  std::vector<gs_particle> keep_picks = c->picks;
   for (int bi = 0; bi < 3; ++bi)
     for (int pi = 0; pi < 2; ++pi) {
       c->batch = batches[bi];
       c->pipeline = pipes[pi];
       double ms = 1e300;
-      for (int rep = 0; rep < 3; ++rep) {  // the first run allocates and warms up
-        size_t np = 0;
-        const int r = gs_pick(ctx, psi_index, psi_index + 1, nullptr, 0, &np);
-        if (r != GS_OK && r != GS_ERR_CAPACITY) return r;
-        if (rep > 0) ms = std::min(ms, c->timing.corr_psi_ms);
-      }
-      t.candidates++;
+      CHK(psi_time(&ms));
       if (batches[bi] == b0 && pipes[pi] == p0) cur_ms = ms;
       if (ms < best_ms) {
         best_ms = ms;
@@ -1567,6 +1618,7 @@ int gs_autotune(void* ctx, int psi_index, gs_tune* out) {
   drop_graphs(c);  // captured with the old schedule
   t.pass1_variant = best1;
   t.pass2_variant = best2;
+  t.tile_block = c->tile_block;
   t.batch = best_b;
   t.pipeline = best_p;
   t.psi_ms = best_ms;

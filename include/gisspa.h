@@ -161,6 +161,14 @@ int gs_create(const gs_params* p, int device, void** ctx);
  * If bx or by exceeds window_size the window is enlarged to the next
  * multiple of 32 (adjustPaddingSize, main.cu:158-178). */
 int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const gs_ctf* tctf);
+/* Templates of different boxes in one set: template j is by[j] x bx[j]
+ * floats at host[j].  The reference reads every template with its own
+ * header and centres it in the window by that size (main.cu:182-223); the
+ * window grows to the largest box when needed (main.cu:158-178).  Everything
+ * after the padding is as for gs_set_templates; gs_get_info reports the
+ * common canvas as the box. */
+int gs_set_templates_mixed(void* ctx, int n, const float* const* host, const int* bx,
+                           const int* by, const gs_ctf* tctf);
 
 /* Upload and prepare a micrograph (replaces init_d_image,
  * split_normalize_image and fft_on_img, main.cu:523-655, 758-769).
@@ -283,7 +291,8 @@ typedef struct gs_tune {
   int batch;                         /* CC maps per launch batch */
   int pipeline;                      /* GS_PIPE_* */
   int candidates;                    /* launch schedules timed */
-  int pad;
+  int tile_block;                    /* tiles per block of the map order (0: the
+                                        cache-sized default) */
   double psi_ms;                     /* correlation of one psi at the choice */
   double pass1_us[4], pass2_us[4];   /* each FFT variant alone, us per CC map
                                         (0: not compiled for this window) */
@@ -292,10 +301,12 @@ typedef struct gs_tune {
 /* Choose the correlation schedule on the device (no counterpart in the
  * reference, whose cuFFT plans are fixed, main.cu:256-308): the FFT schedule
  * of each correlation pass among the compiled variants (radix orders; timed
- * alone on one batch of in-plane step psi_index), then the batch size (half,
- * the current, double) and the pipeline (GS_PIPE_LANES, GS_PIPE_LAUNCH),
- * timed on the whole correlation of that step (best of two runs each; the
- * schedule in use stays unless another is more than 1 % faster).  Applies
+ * alone on one batch of in-plane step psi_index), then the tile block (the
+ * one in use or all tiles, when the tiles span more than one block), the
+ * batch size (half, the current, double) and the pipeline (GS_PIPE_LANES,
+ * GS_PIPE_LAUNCH), timed on the whole correlation of that step (best of two
+ * runs each; the schedule in use stays unless another is more than 1 %
+ * faster).  Applies
  * the choice to the context and reports it in out (may be null).  Needs templates and a
  * micrograph; replaces the last gs_pick's map statistics (its picks are
  * kept).  All schedules give the same scores within FP32 rounding (the
@@ -325,8 +336,10 @@ int gs_set_batch(void* ctx, int maps_per_batch);
  * All schedules give bit-identical results. */
 enum { GS_PIPE_STREAM = 0, GS_PIPE_LAUNCH = 1, GS_PIPE_LANES = 2 };
 int gs_set_pipeline(void* ctx, int mode, int lag, int ring);
-/* Tiles per block of the norm_type 0 processing order (tuning; the default,
- * all tiles in one block, is view-major). */
+/* Tiles per block of the norm_type 0 processing order: all views of a block
+ * before the next block.  0 (the default) takes as many tiles as fit in
+ * 192 MiB, so the block's tile spectra stay in the last-level cache (96 at
+ * window 720: one 5760 x 4092 micrograph is one block). */
 int gs_set_tile_block(void* ctx, int tiles);
 int gs_destroy(void* ctx);
 const char* gs_strerror(int code);

@@ -149,3 +149,61 @@ def test_pipelines_agree(gpu, lag, ring):
     for b in (out[gisspa_amd.GS_PIPE_STREAM], out[gisspa_amd.GS_PIPE_LANES]):
         assert np.array_equal(a[1], b[1])
         assert np.array_equal(a[0], b[0])
+
+
+def test_mixed_template_boxes(gpu):
+    """Templates of boxes 64, 48 and 57 in one set: each is centred in the
+    window by its own size (main.cu:206-207), then picked like any set; the
+    oracle gets the same templates centred in the window on the host."""
+    import gisspa_amd
+    c = _cases.tiny(seed=6, l=128, box=64, n_views=6)
+    boxes = [64, 48, 57, 64, 48, 57]
+    tms = []
+    for t, b in zip(c.templates, boxes):
+        o = (64 - b) // 2
+        tms.append(np.ascontiguousarray(t[o:o + b, o:o + b]))
+    L = c.l
+    padded = np.zeros((len(tms), L, L), np.float32)
+    for j, t in enumerate(tms):
+        b = t.shape[0]
+        o = (L - b) // 2
+        padded[j, o:o + b, o:o + b] = t
+    op = orc.params(L, **c.params)
+    ref = orc.pick_norm0(op, orc.ctf(c.defocus), padded, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates_mixed(tms, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        picks = pk.pick()
+        got = pk.mapstats().reshape(ref.shape)
+    rel = np.abs(got["score"] - ref["score"]) / np.maximum(np.abs(ref["score"]), 1.0)
+    print("mixed boxes: score max rel err", rel.max(), "picks", len(picks))
+    assert rel.max() < 1e-4
+    assert np.array_equal(got["pos"], ref["pos"])
+    sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
+    assert len(picks) == len(sel)
+    assert np.array_equal(picks["view"], sel["view"])
+
+
+@pytest.mark.parametrize("tile_block", [1, 5, 1000])
+def test_tile_block_order_is_invisible(gpu, tile_block):
+    """The map order (tile blocks of any size, the default cache-sized one)
+    changes which maps share a launch, never a map's result: statistics and
+    picks are bit-identical."""
+    import gisspa_amd
+    cases = _batch_cases()
+    c0 = cases[0]
+    defs = [1.5, 1.2, 1.9]
+
+    def run(tb):
+        with gisspa_amd.Picker(0, **c0.params) as pk:
+            if tb is not None:
+                pk.set_tile_block(tb)
+            pk.set_batch(37)   # batches that cut through tile blocks
+            pk.set_templates(c0.templates, defs[0])
+            pk.set_micrographs([c.image for c in cases], defs)
+            return pk.pick(), pk.mapstats()
+
+    p0, s0 = run(None)
+    p1, s1 = run(tile_block)
+    assert np.array_equal(s0, s1)
+    assert np.array_equal(p0, p1)

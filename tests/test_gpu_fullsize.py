@@ -109,7 +109,7 @@ def test_autotune(gpu, full_case):
         print("autotune:", t.pass1_variant, t.pass2_variant, t.batch, t.pipeline, t.psi_ms,
               list(t.pass1_us), list(t.pass2_us))
         assert 0 <= t.pass1_variant < 4 and 0 <= t.pass2_variant < 4
-        assert t.candidates == 6 and t.psi_ms > 0 and min(t.pass1_us) > 0
+        assert t.candidates >= 6 and t.psi_ms > 0 and min(t.pass1_us) > 0
         picks = pk.pick(0, 1)
         got = pk.mapstats().reshape(ref.shape)
     _compare(got, ref, "window 720 after autotune")
