@@ -272,6 +272,37 @@ def batched_rate(templates, n_mics: int, npsi: int, seed: int = 1, tile_block: i
             "sample": f"{npsi} psi timed (after one warm-up psi), extrapolated to 72"}
 
 
+def rocfft_compare(L: int, batch: int = 189, reps: int = 5):
+    """Vendor FFT timing comparison (torch.fft on ROCm runs rocFFT): one
+    batched inverse 2D transform per map, as C2R from the half spectrum (the
+    transform inside the fused correlation) and as full complex C2C (the
+    reference's cuFFT call, main.cu:848).  Only the transform: no product,
+    no peak search, no partial sums."""
+    import torch
+    dev = torch.device("cuda:0")
+    H = L // 2 + 1
+    out = {"maps_per_call": batch}
+    half = torch.randn(batch, L, H, dtype=torch.complex64, device=dev)
+    full = torch.randn(batch, L, L, dtype=torch.complex64, device=dev)
+    for name, fn, nbytes in (("c2r", lambda: torch.fft.irfft2(half, s=(L, L)),
+                              8 * L * H + 4 * L * L),
+                             ("c2c", lambda: torch.fft.ifft2(full), 16 * L * L)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps / batch
+        out[f"{name}_us_per_map"] = us
+        out[f"{name}_gbs"] = nbytes / (us * 1e-6) / 1e9
+    del half, full
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +334,8 @@ def main():
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="also time N micrographs of uneven sizes in one tile pool with mixed "
                          "template boxes (0 = skip)")
+    ap.add_argument("--rocfft", type=int, default=1,
+                    help="time rocFFT's inverse 2D transforms of the same size (0 = skip)")
     ap.add_argument("--project", type=int, default=1,
                     help="time the template projection of the density (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -465,6 +498,17 @@ def main():
         log(f"[bench] batched: {batched['micrographs']} micrographs ({batched['tiles']} tiles), "
             f"{batched['psi_s']:.2f}s per psi, {batched['value']:.1f} micrographs/hour")
 
+    rocfft = None
+    if args.rocfft:
+        rocfft = rocfft_compare(L)
+        if pair:
+            rocfft["fused_correlation_us_per_map"] = tpsi / max(args.steps, 1) * 1e3 / (
+                nmaps / max(args.steps, 1))
+        log(f"[bench] rocFFT inverse 2D {L}^2: C2R {rocfft['c2r_us_per_map']:.3f} us/map "
+            f"({rocfft['c2r_gbs']:.0f} GB/s), C2C {rocfft['c2c_us_per_map']:.3f} us/map; fused "
+            f"correlation (products, transforms, peak search) "
+            f"{rocfft.get('fused_correlation_us_per_map', 0):.3f} us/map")
+
     copy_gbs = hip.copy_rate(4 << 30) / 1e9
     log(f"[bench] 4 GiB device copy: {copy_gbs:.0f} GB/s (read + write); psi correlation at "
         f"{(pair or 0) / copy_gbs:.3f} of it")
@@ -519,6 +563,7 @@ def main():
         "refine": refine,
         "dedupe": dedupe,
         "batched": batched,
+        "rocfft": rocfft,
         "project": proj,
         "plain": plain,
         "speedup_vs_plain": (mph / plain["value"]) if plain else None,
