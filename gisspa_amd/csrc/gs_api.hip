@@ -60,6 +60,7 @@ struct Ctx {
   int bins_L = 0;
   DevBuf twhite;                          // whitened spectra (refinement)
   bool white_ok = false;
+  DevBuf r_wtab;  // refinement weighting table, valid with white_ok
   // refinement batch buffers
   int ref_cap = 0;
   DevBuf r_cand, r_cs, r_A, r_gtmp, r_Tun, r_spec, r_rot, r_bscale, r_pre, r_gRn, r_gTb, r_part,
@@ -280,6 +281,9 @@ int ensure_white(Ctx* c) {
   if (c->white_ok) return GS_OK;
   CHK(c->twhite.ensure((size_t)c->N * c->geo.H * c->L * sizeof(float2)));
   CHK(template_spectra(c, c->twhite.as<float2>(), 0));
+  CHK(c->r_wtab.ensure((size_t)c->geo.H * c->L * sizeof(float4)));
+  launch_ref_wtable(c->L, weight_args(c, c->tctf), c->r_wtab.as<float4>(), c->stream);
+  HIPCHK(hipGetLastError());
   c->white_ok = true;
   return GS_OK;
 }
@@ -930,6 +934,7 @@ int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out) {
     a.tiles = c->tiles.as<float2>();
     a.cand = c->r_cand.as<RefCand>();
     a.w = weight_args(c, c->tctf);
+    a.wtab = c->r_wtab.as<float4>();
     a.count = c->half_count;
     a.dmask = 1.5f * c->p.diameter;
     a.cs = c->r_cs.as<double2>();

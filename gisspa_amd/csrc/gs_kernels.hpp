@@ -237,6 +237,7 @@ struct RefArgs {
   const float2* tiles;  // tile pool
   const RefCand* cand;
   WeightArgs w;         // template CTF weighting (defocus replaced per candidate)
+  const float4* wtab;   // [H][L] its defocus-independent part (launch_ref_wtable)
   double count;         // cells of the amplitude normaliser
   float dmask;          // 1.5 diameter
   const double2* cs;    // (cos, sin) of psi per candidate
@@ -255,6 +256,9 @@ struct RefArgs {
   const float2* twL;
   const float2* twM;
 };
+// defocus-independent part of the refinement's CTF weighting per half-spectrum
+// cell ([H][L] float4), built once per template set
+void launch_ref_wtable(int L, const WeightArgs& w, float4* tab, hipStream_t s);
 void launch_ref_forward(int L, const RefArgs& a, hipStream_t s);
 void launch_ref_backward(int L, const RefArgs& a, hipStream_t s);
 int ref_parts();

@@ -24,21 +24,31 @@ namespace gs {
 
 constexpr int RPARTS = 64;  // partial blocks per candidate in each reduction
 
-// Template weighting factor of apply_weighting_function (GPU_func.cu:249-290,
-// the same arithmetic as k_tmpl_weight) and its derivative by the defocus
-// (um, positive underfocus: df_ast = -defocus + dfdiff cos(...), so
-// d gamma / d defocus = +2 pi 5000 lambda ss).
+// A cell's template weight f and its derivative fd by the defocus.
 struct WeightD {
   float f, fd;
   int rbin;
 };
+
+// The template weighting of apply_weighting_function (GPU_func.cu:249-290,
+// the arithmetic of k_tmpl_weight: low-pass factor, |CTF_AST| of
+// GPU_func.cu:292-315, noise / signal curves) and its derivative by the
+// defocus (um, positive underfocus: df_ast = -defocus + dfdiff cos(...)),
+// split into a defocus-independent table per half-spectrum cell, so the
+// refinement kernels evaluate the weight of a candidate's defocus with one
+// sincos: with the template CTF's dfdiff / dfang fixed, gamma = g0 + g1
+// defocus (um), and g1 is also d gamma / d defocus.  Entry = (flp, ncurve,
+// g0, g1); ncurve < 0 marks cells outside the |CTF| rings (weight = flp,
+// slope 0, not in the normaliser's sums).
 template <int L>
-__device__ __forceinline__ WeightD weight_deriv(int kx, int ky, const WeightArgs& w) {
-  WeightD o;
+__global__ __launch_bounds__(WG) void k_ref_wtable(WeightArgs w, float4* __restrict__ tab) {
+  constexpr int H = L / 2 + 1;
+  const long i = (long)blockIdx.x * WG + threadIdx.x;
+  if (i >= (long)H * L) return;
+  const int kx = (int)(i / L), ky = (int)(i - (long)kx * L);
   const float hy = (float)min(ky, L - ky), hx = (float)min(kx, L - kx);
   const float rr = hypotf(hy, hx);
   const int rbin = (int)floorf(rr + 0.5f) - 1;
-  o.rbin = rbin;
   const float hi = L * w.apix / w.highres, lo = L * w.apix / w.lowres;
   float flp = 1.0f;
   if (rbin < hi && rbin >= 0) {
@@ -49,10 +59,8 @@ __device__ __forceinline__ WeightD weight_deriv(int kx, int ky, const WeightArgs
   } else {
     flp = 0.0f;
   }
-  o.f = flp;
-  o.fd = 0.0f;
+  float4 t = make_float4(flp, -1.0f, 0.0f, 0.0f);
   if (rbin < L / 2 && rbin >= 0) {
-    // CTF_AST (GPU_func.cu:292-315) at (kx, (ky + L/2) % L), value and slope
     int x1 = kx, y1 = (ky + L / 2) % L;
     y1 = x1 > L / 2 ? L - y1 : y1;
     x1 = min(x1, L - x1);
@@ -60,28 +68,40 @@ __device__ __forceinline__ WeightD weight_deriv(int kx, int ky, const WeightArgs
     const float dy = (float)(y1 - L / 2);
     const float sc = (dx * dx / (float)(L * L) + dy * dy / (float)(L * L)) / (w.apix * w.apix);
     const float ag = atan2f((float)(y1 - L / 2), (float)x1);
-    const float df_ast =
-        0.5f * (w.dfu + w.dfv + 2 * w.dfdiff * cosf(2 * (w.dfang * kRefPi / 180 - ag)));
-    const float gamma = -2 * kRefPi * (w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * sc * sc +
-                                       df_ast * 5000.0f * w.lambda * sc);
-    const float sa = sqrtf(1.0f - w.ampconst * w.ampconst);
-    const float v = sa * sinf(gamma) + w.ampconst * cosf(gamma);
-    const float dv = (sa * cosf(gamma) - w.ampconst * sinf(gamma)) * (2 * kRefPi * 5000.0f * w.lambda * sc);
-    const float vv = fabsf(v), dvv = v >= 0 ? dv : -dv;
+    // df_ast = -defocus + dfdiff cos(2 (dfang - ag)) (dfu + dfv = -2 defocus)
+    const float ast = w.dfdiff * cosf(2 * (w.dfang * kRefPi / 180 - ag));
+    const float g1 = 2 * kRefPi * 5000.0f * w.lambda * sc;
+    t.z = -2 * kRefPi * (w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * sc * sc +
+                         ast * 5000.0f * w.lambda * sc);
+    t.w = g1;
     const float ss = rr * w.ds;
     const float signal = expf(w.bf * ss * ss + w.bf2 * ss + w.bf3) / (w.kk + 1);
-    const float ncurve = expf(w.a * ss * ss + w.b * ss + w.b2) / signal;
-    const float q = ncurve + w.kk * vv * vv;
-    o.f = flp * vv * sqrtf(1 / q);
-    o.fd = flp * ncurve / (q * sqrtf(q)) * dvv;
+    t.y = expf(w.a * ss * ss + w.b * ss + w.b2) / signal;
   }
-  return o;
+  tab[i] = t;
 }
 
-__device__ __forceinline__ WeightArgs cand_weight(WeightArgs w, float defocus) {
-  w.dfu = -defocus + w.dfdiff;  // defocus is negated at parse (DataReader.cpp:378)
-  w.dfv = -defocus - w.dfdiff;
-  return w;
+// weight_deriv of one cell at a candidate's defocus from its table entry
+__device__ __forceinline__ WeightD weight_tab(float4 t, float defocus, const WeightArgs& w) {
+  WeightD o;
+  o.rbin = t.y < 0.0f ? -1 : 0;  // only the in-range test is used
+  if (t.y < 0.0f) {
+    o.f = t.x;
+    o.fd = 0.0f;
+    return o;
+  }
+  const float gamma = t.z + t.w * defocus;
+  float sg, cg;
+  sincosf(gamma, &sg, &cg);
+  const float sa = sqrtf(1.0f - w.ampconst * w.ampconst);
+  const float v = sa * sg + w.ampconst * cg;
+  const float dv = (sa * cg - w.ampconst * sg) * t.w;
+  const float vv = fabsf(v), dvv = v >= 0 ? dv : -dv;
+  const float q = t.y + w.kk * vv * vv;
+  const float rq = 1.0f / sqrtf(q);
+  o.f = t.x * vv * rq;
+  o.fd = t.x * t.y * rq * rq * rq * dvv;
+  return o;
 }
 
 // block reduction of N doubles; valid in thread 0
@@ -186,7 +206,9 @@ __device__ __forceinline__ void cols_inv_store(const float2* lds, float2* __rest
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict__ white,
                                                         const RefCand* __restrict__ cand,
-                                                        WeightArgs w0, float2* __restrict__ Gout,
+                                                        WeightArgs w0,
+                                                        const float4* __restrict__ wtab,
+                                                        float2* __restrict__ Gout,
                                                         double* __restrict__ part,
                                                         const float2* __restrict__ twL) {
   using Gm = Geo<L>;
@@ -197,7 +219,6 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   const RefCand cd = cand[b];
-  const WeightArgs w = cand_weight(w0, cd.df);
   load_twiddles<L, WG>(tw, twL);
   const float2* Bj = white + (long)cd.view * H * L;
   const int ncol = min(CB, H - kx0);
@@ -207,10 +228,10 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
     float2 v = make_float2(0.f, 0.f);
     if (c < ncol) {
       const int kx = kx0 + c;
-      const WeightD wd = weight_deriv<L>(kx, ky, w);
+      const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0);
       const float2 bv = Bj[(long)kx * L + ky];
       v = cscale(bv, wd.f);
-      if (wd.rbin < L / 2 && wd.rbin >= 0) {
+      if (wd.rbin >= 0) {  // inside the |CTF| rings
         const double b2 = (double)bv.x * bv.x + (double)bv.y * bv.y;
         acc[0] += b2 * wd.f * wd.f;
         acc[1] += b2 * wd.f * wd.fd;
@@ -291,7 +312,9 @@ template <int L>
 __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ rows,
                                                     const float2* __restrict__ white,
                                                     const RefCand* __restrict__ cand,
-                                                    WeightArgs w0, double* __restrict__ part,
+                                                    WeightArgs w0,
+                                                    const float4* __restrict__ wtab,
+                                                    double* __restrict__ part,
                                                     const float2* __restrict__ twL) {
   using Gm = Geo<L>;
   constexpr int H = Gm::H, CB = Gm::CB;
@@ -300,7 +323,6 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
   __shared__ double red[1 * (WG / 64)];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   const RefCand cd = cand[b];
-  const WeightArgs w = cand_weight(w0, cd.df);
   load_twiddles<L, WG>(tw, twL);
   const float2* base = rows + ((long)b * H + kx0) * L;
   const int ncol = min(CB, H - kx0);
@@ -314,7 +336,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
   double acc[1] = {0};
   for (int g = threadIdx.x; g < ncol * L; g += WG) {
     const int c = g / L, ky = g - c * L, kx = kx0 + c;
-    const WeightD wd = weight_deriv<L>(kx, ky, w);
+    const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0);
     if (wd.fd == 0.0f) continue;
     const float2 ga = lds[g], bv = Bj[(long)kx * L + ky];
     const float wh = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
@@ -573,7 +595,7 @@ static void ref_forward_t(const RefArgs& a, hipStream_t s) {
   const int small = (a.nb + 63) / 64;
   constexpr int NCB = ncol_blocks<L>();
   hipLaunchKernelGGL((k_ref_weight_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.white, a.cand, a.w,
-                     a.gtmp, a.part, a.twL);
+                     a.wtab, a.gtmp, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_norm_m, dim3(small), dim3(64), 0, s, a.part, a.nb, a.count, a.sc,
                      a.pre, NCB, FSLOTS);
   launch_rows_c2r(L, a.gtmp, a.nb, a.Tun, a.twL, a.twM, s);
@@ -605,11 +627,20 @@ static void ref_backward_t(const RefArgs& a, hipStream_t s) {
   // d/dT (in gRn) -> d/dA: unnormalised R2C
   launch_rows_r2c_img(L, a.gRn, a.nb, a.A, a.twL, a.twM, s);
   hipLaunchKernelGGL((k_ref_cols_df<L>), dim3(NCB, a.nb), blk, 0, s, a.A, a.white, a.cand, a.w,
-                     a.part, a.twL);
+                     a.wtab, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.out, NCB,
                      FSLOTS);
 }
 
+template <int L>
+static void ref_wtable_t(const WeightArgs& w, float4* tab, hipStream_t s) {
+  const long n = (long)(L / 2 + 1) * L;
+  hipLaunchKernelGGL((k_ref_wtable<L>), dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, s, w,
+                     tab);
+}
+void launch_ref_wtable(int L, const WeightArgs& w, float4* tab, hipStream_t s) {
+  RDISPATCH(L, ref_wtable_t, w, tab, s);
+}
 void launch_ref_forward(int L, const RefArgs& a, hipStream_t s) { RDISPATCH(L, ref_forward_t, a, s); }
 void launch_ref_backward(int L, const RefArgs& a, hipStream_t s) {
   RDISPATCH(L, ref_backward_t, a, s);
