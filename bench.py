@@ -303,6 +303,75 @@ def rocfft_compare(L: int, batch: int = 189, reps: int = 5):
     return out
 
 
+def stress_rate(n_mics: int, npsi: int, n_refine: int, refine_iters: int, seed: int = 3):
+    """The combined stress shape: n_mics 5760x4092 micrographs (defocus
+    0.8-3.0 um, 300 particles each) in one tile pool, 256 px templates of a
+    60-blob 256^3 density at 4600 views of a ~3 degree sphere made by the GPU
+    projector (gs_project), 3 degree in-plane steps (120 psi); npsi steps
+    timed (after one warm-up step) and extrapolated to 120, then local
+    refinement of the top n_refine picks of the timed steps."""
+    import gisspa_amd
+    from gisspa_amd import synth
+    rng = np.random.default_rng(seed)
+    box, nviews = 256, 4600
+    t = time.time()
+    dens = synth.blob_density(60, box, 1)
+    vol = synth.render_volume(dens, box)
+    views = synth.sphere_eulers(nviews)
+    eul = np.zeros((nviews, 3), np.float32)
+    eul[:, :2] = views
+    t1 = time.time()
+    templates = gisspa_amd.project(vol, eul)
+    t_proj = time.time() - t1
+    log(f"[bench] stress: {nviews} templates of {box}^2 projected in {t_proj:.2f}s")
+    imgs, dfs = [], []
+    for i in range(n_mics):
+        df = float(rng.uniform(0.8, 3.0))
+        mg = synth.make_micrograph(5760, 4092, dens, views, 300, 0.05, seed * 100 + i, apix=1.0,
+                                   defocus_um=df, ampconst=0.1, box=box, psi_step=3.0)
+        imgs.append(mg.image)
+        dfs.append(df)
+        log(f"[bench] stress: micrograph {i} defocus {df:.2f} um")
+    log(f"[bench] stress: inputs in {time.time() - t:.1f}s")
+    # a lower threshold than the headline run, so that two timed steps hold
+    # the 50,000 peaks the refinement takes
+    params = dict(FULL_PARAMS, phistep=3.0, diameter=0.75 * box, threshold=8.0)
+    pk = gisspa_amd.Picker(0, **params)
+    pk.set_templates(templates, dfs[0])
+    t = time.time()
+    pk.set_micrographs(imgs, dfs)
+    t_prep = time.time() - t
+    info = pk.info()
+    pk.pick(0, 1)                      # warm-up
+    t = time.time()
+    picks = pk.pick(1, 1 + npsi)
+    t_psi = (time.time() - t) / npsi
+    out = {"micrographs": n_mics, "views": nviews, "box": box, "psi": info.n_psi,
+           "tiles": info.n_tiles, "maps_per_psi": info.n_tiles * nviews, "prep_s": t_prep,
+           "psi_s": t_psi, "project_s": t_proj,
+           "value": n_mics * 3600.0 / (t_prep + info.n_psi * t_psi), "unit": "micrographs/hour",
+           "sample": f"{npsi} of {info.n_psi} psi timed, extrapolated", "picks_timed": len(picks)}
+    if n_refine and len(picks):
+        top = picks[np.argsort(-picks["score"])[:n_refine]]
+        peaks = gisspa_amd.peaks_from_picks(top, params["phistep"], 0.0)
+        peaks["defocus"] = np.asarray(dfs, np.float32)[top["mic"]]
+        pert = peaks.copy()
+        pert["psi"] += rng.uniform(-2.5, 2.5, len(pert)).astype(np.float32)
+        pert["dx"] += rng.uniform(-1, 1, len(pert)).astype(np.float32)
+        pert["dy"] += rng.uniform(-1, 1, len(pert)).astype(np.float32)
+        pert["defocus"] += rng.uniform(-0.02, 0.02, len(pert)).astype(np.float32)
+        s0 = pk.adjoint(pert)["score"]
+        t = time.time()
+        _, fin = pk.refine(pert, refine_iters)
+        tr = time.time() - t
+        out["refine"] = {"candidates": int(len(pert)), "seconds": tr,
+                         "evaluations": int(pk.timing().refine_evals),
+                         "mean_score_perturbed": float(s0.mean()),
+                         "mean_score_refined": float(fin["score"].mean())}
+    pk.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +400,10 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="choose the correlation schedule on the device before timing (0 = the "
                          "built-in defaults)")
+    ap.add_argument("--stress", type=int, default=0, metavar="N",
+                    help="also run the combined stress shape on N 5760x4092 micrographs "
+                         "(4600 views of 256 px, 120 psi, refinement of the top 50,000 picks; "
+                         "0 = skip)")
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="also time N micrographs of uneven sizes in one tile pool with mixed "
                          "template boxes (0 = skip)")
@@ -509,6 +582,14 @@ def main():
             f"correlation (products, transforms, peak search) "
             f"{rocfft.get('fused_correlation_us_per_map', 0):.3f} us/map")
 
+    stress = None
+    if args.stress:
+        stress = stress_rate(args.stress, max(1, min(npsi, 2)), 50000, args.refine_iters)
+        log(f"[bench] stress: {stress['micrographs']} micrographs, {stress['maps_per_psi']} maps "
+            f"per psi, {stress['psi_s']:.2f}s per psi, {stress['value']:.1f} micrographs/hour"
+            + (f"; refine {stress['refine']['candidates']} in {stress['refine']['seconds']:.1f}s"
+               if stress.get("refine") else ""))
+
     copy_gbs = hip.copy_rate(4 << 30) / 1e9
     log(f"[bench] 4 GiB device copy: {copy_gbs:.0f} GB/s (read + write); psi correlation at "
         f"{(pair or 0) / copy_gbs:.3f} of it")
@@ -563,6 +644,7 @@ def main():
         "refine": refine,
         "dedupe": dedupe,
         "batched": batched,
+        "stress": stress,
         "rocfft": rocfft,
         "project": proj,
         "plain": plain,
