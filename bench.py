@@ -372,6 +372,54 @@ def stress_rate(n_mics: int, npsi: int, n_refine: int, refine_iters: int, seed: 
     return out
 
 
+def cli_rate(templates, image, defocus, n_mics: int, cache_dir: str, groups=(1, 4)):
+    """The ./main command line end to end on n_mics full-size micrographs
+    (the benchmark micrograph shifted by a different offset for each, defocus
+    spread 1.6-2.0 um), files in the reference's formats; one run per
+    micrograph_batch value in groups.  Reports the program's own loop time
+    (main.cu:1271-1273 prints the same span) and the process wall time
+    (template file reading and preparation included)."""
+    import re
+    import subprocess
+
+    import gisspa_amd
+    from gisspa_amd import synth
+    d = os.path.join(cache_dir, "gisspa_cli_bench")
+    mics = []
+    for i in range(n_mics):
+        mics.append((np.roll(image, (97 * i, 211 * i), axis=(0, 1)),
+                     float(defocus + 0.4 * (i / max(1, n_mics - 1) - 0.5))))
+    wl = synth.Workload("cli", templates, synth.sphere_eulers(templates.shape[0]), mics,
+                        dict(FULL_PARAMS), [])
+    t = time.time()
+    out = {"micrographs": n_mics, "runs": {}}
+    outputs = {}
+    for g in groups:
+        cfg = synth.write_workload(wl, d, extra=dict(micrograph_batch=g))
+        if g == groups[0]:
+            log(f"[bench] cli: wrote {n_mics} micrographs and the templates in "
+                f"{time.time() - t:.1f}s")
+        t0 = time.time()
+        r = subprocess.run([gisspa_amd.MAIN_PATH, cfg], capture_output=True, text=True,
+                           timeout=1200)
+        wall = time.time() - t0
+        if r.returncode != 0:
+            raise RuntimeError(f"./main failed ({r.returncode}): {r.stderr[-2000:]}")
+        m = re.search(r"Total consumed time is: ([\d.]+) seconds", r.stdout)
+        loop = float(m.group(1))
+        with open(os.path.join(d, "picks.lst")) as f:
+            outputs[g] = f.read()
+        out["runs"][str(g)] = {"loop_s": loop, "wall_s": wall,
+                               "micrographs_per_hour": n_mics * 3600.0 / loop,
+                               "lines": outputs[g].count("\n")}
+        log(f"[bench] cli: micrograph_batch {g}: loop {loop:.1f}s, wall {wall:.1f}s, "
+            f"{n_mics * 3600.0 / loop:.1f} micrographs/hour")
+    out["identical_outputs"] = len(set(outputs.values())) == 1
+    out["unit"] = "micrographs/hour"
+    out["sample"] = f"{n_mics} micrographs, all 72 psi, one run per micrograph_batch"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +455,9 @@ def main():
     ap.add_argument("--batched", type=int, default=0, metavar="N",
                     help="also time N micrographs of uneven sizes in one tile pool with mixed "
                          "template boxes (0 = skip)")
+    ap.add_argument("--cli", type=int, default=0, metavar="N",
+                    help="also run the ./main command line on N full-size micrographs, once per "
+                         "micrograph_batch of 1 and 4 (0 = skip)")
     ap.add_argument("--rocfft", type=int, default=1,
                     help="time rocFFT's inverse 2D transforms of the same size (0 = skip)")
     ap.add_argument("--project", type=int, default=1,
@@ -590,6 +641,10 @@ def main():
             + (f"; refine {stress['refine']['candidates']} in {stress['refine']['seconds']:.1f}s"
                if stress.get("refine") else ""))
 
+    cli = None
+    if args.cli:
+        cli = cli_rate(templates, image, defocus, args.cli, args.cache)
+
     copy_gbs = hip.copy_rate(4 << 30) / 1e9
     log(f"[bench] 4 GiB device copy: {copy_gbs:.0f} GB/s (read + write); psi correlation at "
         f"{(pair or 0) / copy_gbs:.3f} of it")
@@ -645,6 +700,7 @@ def main():
         "dedupe": dedupe,
         "batched": batched,
         "stress": stress,
+        "cli": cli,
         "rocfft": rocfft,
         "project": proj,
         "plain": plain,

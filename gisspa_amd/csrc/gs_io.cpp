@@ -53,6 +53,7 @@ void parse_line(const std::string& line, Config& c) {
   else if (option == "norm_type") p.norm_type = atoi(v);
   else if (option == "invert") p.invert = atoi(v);
   else if (option == "ampconst") p.ampconst = (float)atof(v);  // not a reference key (fixed 0.07 there)
+  else if (option == "micrograph_batch") c.mic_batch = atoi(v);  // not a reference key
   else {
     c.unknown.push_back(value);
     printf("Undefined option: %s .Abort.\n", value.c_str());
@@ -103,6 +104,7 @@ bool read_config(const std::string& path, Config& c) {
   req(c.p.highres < 0, "highres is requested.");
   req(c.p.lowres < 0, "lowres is requested.");
   req(c.p.diameter < 0, "diameter is requested.");
+  req(c.mic_batch < 1, "micrograph_batch must be at least 1.");
   return true;
 }
 
@@ -133,6 +135,7 @@ void print_config(const Config& c, FILE* o) {
   fprintf(o, "--norm_type        %d\n", p.norm_type);
   fprintf(o, "--invert           %d\n", p.invert);
   fprintf(o, "--ampconst         %f\n", p.ampconst);
+  fprintf(o, "--micrograph_batch %d\n", c.mic_batch);
   fprintf(o, "Parameter fixed:\n");
   fprintf(o, "--a                %f\n", p.a);
   fprintf(o, "--b                %f\n", p.b);
@@ -172,6 +175,8 @@ void print_help(FILE* o) {
   fprintf(o, "phase_flip       = Whether use phase filp operation(1) or not(0).\n");
   fprintf(o, "overlap          = size of overlap between diff window.\n");
   fprintf(o, "ampconst         = amplitude contrast of the CTF (default 0.07).\n");
+  fprintf(o, "micrograph_batch = micrographs searched together, sharing each template rotation\n");
+  fprintf(o, "                   (default 4; output unchanged).\n");
 }
 
 bool read_euler(const std::string& path, Euler& e) {

@@ -17,6 +17,9 @@ struct Config {
   std::string output = "./outlst";
   int first = -1, last = -1;
   int device_id = 0;
+  // micrographs per correlation batch (not a reference key; the reference
+  // processes one per loop iteration, main.cu:1149)
+  int mic_batch = 4;
   gs_params p{};
   std::vector<std::string> errors;    // missing required keys
   std::vector<std::string> unknown;   // "Undefined option" lines

@@ -4,6 +4,7 @@
 // Same config keys, Euler .lst, micrograph .lst and output lines.  Errors end
 // the run with a message and a non-zero status instead of continuing on a
 // 10x10 zero image (emdata_.cpp:184-203) or garbage defaults.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -46,7 +47,7 @@ int main(int argc, char** argv) {
     return 1;
   }
 
-  // first image: size and the CTF used for the template weighting (main.cu:1106)
+  // first image: its CTF weights the templates (main.cu:1106)
   MicEntry first;
   int r = read_lst_entry(cfg.input, cfg.first, first);
   if (r < 0) {
@@ -59,7 +60,7 @@ int main(int argc, char** argv) {
     fprintf(stderr, "Error: %s\n", err.c_str());
     return 1;
   }
-  const int nx = img.nx, ny = img.ny;
+  img = Image();
 
   // templates: the first N sections of the template stack
   const int nsec = mrc_sections(cfg.templ);
@@ -95,26 +96,36 @@ int main(int argc, char** argv) {
     printf("Aujust padding size to %d\n", info.l);
 
   const auto t_start = std::chrono::steady_clock::now();
+  // Micrographs go to the picker in groups of micrograph_batch (any sizes,
+  // each with its own CTF): one search covers the group's tiles, so each
+  // template rotation serves all of them.  Picks come back micrograph by
+  // micrograph, and the lines and "finished" messages follow the order of
+  // the reference's one-image loop (main.cu:1149-1269).
   std::vector<gs_particle> picks(1 << 16);
-  for (int n = cfg.first; n < cfg.last; ++n) {
-    MicEntry e;
-    if ((r = read_lst_entry(cfg.input, n, e)) < 0) {
-      printf("Error %d occured in readRawImage!\n", r);
-      return 1;
+  for (int n0 = cfg.first; n0 < cfg.last; n0 += cfg.mic_batch) {
+    const int k = std::min(cfg.mic_batch, cfg.last - n0);
+    std::vector<MicEntry> es(k);
+    std::vector<Image> ims(k);
+    std::vector<const float*> ptr(k);
+    std::vector<int> nxs(k), nys(k);
+    std::vector<gs_ctf> ctfs(k);
+    for (int i = 0; i < k; ++i) {
+      if ((r = read_lst_entry(cfg.input, n0 + i, es[i])) < 0) {
+        printf("Error %d occured in readRawImage!\n", r);
+        return 1;
+      }
+      if (read_mrc(es[i].path, es[i].nn, ims[i], &err)) {
+        fprintf(stderr, "Error: %s\n", err.c_str());
+        return 1;
+      }
+      ptr[i] = ims[i].data.data();
+      nxs[i] = ims[i].nx;
+      nys[i] = ims[i].ny;
+      ctfs[i] = gs_ctf{es[i].defocus, es[i].dfdiff, es[i].dfang};
     }
-    Image im;
-    if (read_mrc(e.path, e.nn, im, &err)) {
-      fprintf(stderr, "Error: %s\n", err.c_str());
-      return 1;
-    }
-    if (im.nx != nx || im.ny != ny) {
-      fprintf(stderr, "Error: %s is %dx%d, the first micrograph is %dx%d\n", e.path.c_str(), im.nx,
-              im.ny, nx, ny);
-      return 1;
-    }
-    gs_ctf c{e.defocus, e.dfdiff, e.dfang};
-    if ((r = gs_set_micrograph(ctx, im.data.data(), nx, ny, &c)) != GS_OK)
-      return fail("gs_set_micrograph", r);
+    if ((r = gs_set_micrographs(ctx, k, ptr.data(), nxs.data(), nys.data(), ctfs.data(), 0)) !=
+        GS_OK)
+      return fail("gs_set_micrographs", r);
     size_t np = 0;
     r = gs_pick(ctx, 0, -1, picks.data(), picks.size(), &np);
     if (r == GS_ERR_CAPACITY) {
@@ -122,13 +133,21 @@ int main(int argc, char** argv) {
       r = gs_get_picks(ctx, picks.data(), picks.size(), &np);
     }
     if (r != GS_OK) return fail("gs_pick", r);
-    for (size_t i = 0; i < np; ++i) {
-      const gs_particle& pk = picks[i];
-      write_pick_line(fp, e.nn, e.path, e, euler.alt[pk.view], euler.az[pk.view], pk.psi, pk.cx,
-                      pk.cy, pk.score);
+    size_t i = 0;
+    for (int m = 0; m < k; ++m) {
+      const MicEntry& e = es[m];
+      for (; i < np && picks[i].mic == m; ++i) {
+        const gs_particle& pk = picks[i];
+        write_pick_line(fp, e.nn, e.path, e, euler.alt[pk.view], euler.az[pk.view], pk.psi, pk.cx,
+                        pk.cy, pk.score);
+      }
+      fflush(fp);
+      printf("------------------------Image %d finished--------------------------------\n", n0 + m);
     }
-    fflush(fp);
-    printf("------------------------Image %d finished--------------------------------\n", n);
+    if (i != np) {
+      fprintf(stderr, "Error: picks out of micrograph order\n");
+      return 1;
+    }
   }
   const double secs =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

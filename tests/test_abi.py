@@ -65,6 +65,10 @@ def test_cli_help_and_missing_keys(tmp_path):
     assert "Error : template file is requested." in r.stdout
     assert "--angpix           1.000000" in r.stdout
     assert "--phistep          5.000000" in r.stdout
+    assert "--micrograph_batch 4" in r.stdout
+    cfg.write_text("angpix = 1.0\nmicrograph_batch = 0\n")
+    r = subprocess.run([main, str(cfg)], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 1 and "micrograph_batch must be at least 1." in r.stdout
 
 
 def test_remove_repeats_entry_and_tool_errors(tmp_path):

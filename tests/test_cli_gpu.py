@@ -47,3 +47,51 @@ def test_cli_matches_oracle(gpu, tmp_path):
         assert float(m.group(8)) == pytest.approx(e3, abs=1e-6)
         assert float(m.group(9)) == s["cx"] and float(m.group(10)) == s["cy"]
         assert float(m.group(11)) == pytest.approx(s["score"], rel=1e-4, abs=2e-6)
+
+
+def test_cli_micrograph_batches(gpu, tmp_path):
+    """Five micrographs of different sizes and defoci: searched one by one
+    (micrograph_batch = 1, the reference's loop), in groups of two (a ragged
+    last group) and all five together, the output files are identical (up
+    to the folder in the micrograph paths)
+    and the "finished" messages come in micrograph order.  The one-by-one
+    run's picks equal the oracle's, micrograph by micrograph (templates
+    weighted by the first micrograph's CTF, main.cu:1106)."""
+    sizes = [(320, 288), (256, 352), (400, 200), (288, 288), (352, 256)]
+    defs = [1.5, 1.2, 1.9, 1.6, 2.1]
+    cases = [_cases.tiny(seed=30 + i, l=96, size=sz, threshold=4.0) for i, sz in enumerate(sizes)]
+    c0 = cases[0]
+    eul = synth.random_eulers(len(c0.templates), 9)
+    mics = [(c.image, d) for c, d in zip(cases, defs)]
+    outs = {}
+    for group in (1, 2, 5):
+        d = tmp_path / f"g{group}"
+        wl = synth.Workload("cli", c0.templates, eul, mics, dict(c0.params), [])
+        cfg = synth.write_workload(wl, str(d), extra=dict(micrograph_batch=group))
+        r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+        print(r.stdout[-1000:], r.stderr[-1000:])
+        assert r.returncode == 0
+        done = [int(m) for m in re.findall(r"Image (\d+) finished", r.stdout)]
+        assert done == list(range(5))
+        # lines name the micrographs by path: compare with the folder masked
+        outs[group] = open(os.path.join(d, "picks.lst")).read().replace(str(d), "DIR")
+    assert outs[1] == outs[2] == outs[5]
+    lines = outs[1].splitlines()
+    op = orc.params(c0.l, **c0.params)
+    treal = orc.template_prep(op, orc.ctf(defs[0]), c0.templates)
+    P = orc.num_psi(op.phi_step)
+    k = 0
+    for i, c in enumerate(cases):
+        tiles = orc.micrograph_prep(op, orc.ctf(defs[i]), c.image)
+        st = np.stack([orc.correlate_many(op, tiles, orc.rotate_normalize(
+            op, treal, orc.psi_value(op.phi_step, ip))) for ip in range(P)])
+        sel = orc.select_norm0(op, st, 0, c.image.shape[1], c.image.shape[0])
+        mine = lines[k:k + len(sel)]
+        k += len(sel)
+        for ln, s in zip(mine, sel):
+            m = LINE.match(ln)
+            assert m, ln
+            assert int(m.group(1)) == i and m.group(2).endswith(f"mic{i:03d}.mrc")
+            assert float(m.group(9)) == s["cx"] and float(m.group(10)) == s["cy"]
+            assert float(m.group(11)) == pytest.approx(s["score"], rel=1e-4, abs=2e-6)
+    assert k == len(lines)
