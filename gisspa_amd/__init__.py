@@ -18,6 +18,7 @@ MAIN_PATH = os.path.join(_HERE, "main")
 
 GS_OK = 0
 GS_ERR_CAPACITY = -6
+GS_ERR_IO = -7
 
 
 class GsParams(ctypes.Structure):
@@ -170,6 +171,16 @@ def load() -> ctypes.CDLL:
     lib.gs_set_templates_mixed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), vp,
                                            vp, ctypes.POINTER(GsCtf)]
     lib.gs_set_fft_variants.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.gs_io_error.argtypes = []
+    lib.gs_io_error.restype = ctypes.c_char_p
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.gs_image_count.argtypes = [ctypes.c_char_p, ip]
+    lib.gs_read_image.argtypes = [ctypes.c_char_p, ctypes.c_int, vp, ctypes.c_size_t, ip, ip, ip]
+    lib.gs_hdf5_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, ctypes.c_size_t, ip,
+                                 ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+    lib.gs_hdf5_attr.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, vp,
+                                 ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), vp,
+                                 ctypes.c_size_t, ip]
     _lib = lib
     return lib
 
@@ -221,6 +232,61 @@ def project(vol: np.ndarray, eulers, device: int = 0) -> np.ndarray:
     _check(load().gs_project(device, v.ctypes.data, n, e.ctypes.data, len(e), out.ctypes.data),
            "gs_project")
     return out
+
+
+def _io_check(code: int, what: str) -> None:
+    if code == GS_ERR_IO:
+        raise OSError(f"{what}: {load().gs_io_error().decode(errors='replace')}")
+    _check(code, what)
+
+
+def image_count(path: str) -> int:
+    """Images in an MRC or EMAN HDF5 file (gs_image_count)."""
+    n = ctypes.c_int()
+    _io_check(load().gs_image_count(os.fsencode(path), ctypes.byref(n)), "gs_image_count")
+    return n.value
+
+
+def read_image(path: str, index: int = -1) -> np.ndarray:
+    """Image `index` of an MRC or EMAN HDF5 file as float32 [ny][nx] (or
+    [nz][ny][nx]); index < 0 reads the whole file (gs_read_image, the
+    reference's emdata::readImage)."""
+    lib = load()
+    nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    p = os.fsencode(path)
+    _io_check(lib.gs_read_image(p, index, None, 0, ctypes.byref(nx), ctypes.byref(ny),
+                                ctypes.byref(nz)), "gs_read_image")
+    out = np.zeros((nz.value, ny.value, nx.value), np.float32)
+    _io_check(lib.gs_read_image(p, index, out.ctypes.data, out.size, ctypes.byref(nx),
+                                ctypes.byref(ny), ctypes.byref(nz)), "gs_read_image")
+    return out[0] if nz.value == 1 else out
+
+
+def hdf5_read(path: str, obj: str) -> np.ndarray:
+    """An HDF5 dataset as float32 in its stored shape (gs_hdf5_read)."""
+    lib = load()
+    rank = ctypes.c_int()
+    dims = (ctypes.c_longlong * 32)()
+    p, o = os.fsencode(path), obj.encode()
+    _io_check(lib.gs_hdf5_read(p, o, None, 0, ctypes.byref(rank), dims, 32), "gs_hdf5_read")
+    out = np.zeros([dims[k] for k in range(rank.value)], np.float32)
+    _io_check(lib.gs_hdf5_read(p, o, out.ctypes.data, out.size, ctypes.byref(rank), dims, 32),
+              "gs_hdf5_read")
+    return out
+
+
+def hdf5_attr(path: str, obj: str, name: str):
+    """An attribute of an HDF5 object: str, or a numpy float64 array of its
+    numeric values (gs_hdf5_attr)."""
+    lib = load()
+    num = np.zeros(4096, np.float64)
+    s = ctypes.create_string_buffer(1 << 16)
+    n = ctypes.c_size_t()
+    is_str = ctypes.c_int()
+    _io_check(lib.gs_hdf5_attr(os.fsencode(path), obj.encode(), name.encode(), num.ctypes.data,
+                               num.size, ctypes.byref(n), s, len(s), ctypes.byref(is_str)),
+              "gs_hdf5_attr")
+    return s.value.decode() if is_str.value else num[:n.value].copy()
 
 
 def make_params(**kw) -> GsParams:

@@ -994,6 +994,7 @@ const char* gs_strerror(int code) {
     case GS_ERR_NOMEM: return "device memory allocation failed";
     case GS_ERR_STATE: return "templates or micrograph not set";
     case GS_ERR_CAPACITY: return "output buffer too small";
+    case GS_ERR_IO: return "file missing, unreadable or in an unsupported format";
     default: return "unknown error";
   }
 }

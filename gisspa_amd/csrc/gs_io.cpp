@@ -2,8 +2,11 @@
 // drop-in surface (Autumn1998/GisSPA EMReader/ and main.cu).
 #include "gs_io.hpp"
 
+#include "gs_hdf.hpp"
+
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cstdint>
 #include <cstdlib>
@@ -358,6 +361,174 @@ int read_mrc(const std::string& path, int n, Image& img, std::string* err) {
       memcpy(&img.data[i], b, 4);
     }
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// HDF5 images: the HDF branches of emdata::readImage (EMReader/emdata_.cpp:174-199)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Hdf {
+  gshdf::File f;
+  uint64_t root = 0;
+  bool old = false;   // EMAN1 layout
+};
+
+int hdf_open(const std::string& path, Hdf& h, std::string* err) {
+  if (!h.f.open(path, err) || !h.f.find("/", &h.root, err)) return -1;
+  // EMhdf::init_test (EMReader/emhdf_.cpp:103-122): a root attribute
+  // num_dataset marks the old layout
+  gshdf::Value v;
+  std::string e;
+  h.old = h.f.attribute(h.root, "num_dataset", &v, &e);
+  if (!e.empty()) {
+    if (err) *err = path + ": " + e;
+    return -1;
+  }
+  return 0;
+}
+
+bool numeric_name(const std::string& s) {
+  if (s.empty() || s.size() > 9) return false;
+  for (char ch : s)
+    if (ch < '0' || ch > '9') return false;
+  return true;
+}
+
+// number of images: the old layout's num_dataset (EMReader/emhdf_.cpp:229-233
+// iterates the root datasets), the new layout's members of /MDF/images
+int hdf_count(Hdf& h, std::string* err) {
+  std::vector<std::pair<std::string, uint64_t>> mem;
+  uint64_t g = h.root;
+  if (!h.old && !h.f.find("/MDF/images", &g, err)) return -1;
+  if (!h.f.members(g, mem, err)) return -1;
+  int n = 0;
+  for (const auto& m : mem)
+    if (numeric_name(m.first)) n = std::max(n, atoi(m.first.c_str()) + 1);
+  return n;
+}
+
+int hdf_image(Hdf& h, int idx, Image& img, std::string* err) {
+  if (idx < 0) idx = 0;   // readHDF / readHDF2: a negative index reads image 0
+  uint64_t obj;
+  std::vector<uint64_t> dims;
+  if (h.old) {
+    // readHDF (emdata_.cpp:216-250) with EMhdf::read_dims (emhdf_.cpp:124-150):
+    // dataset "<idx>" under the root, nx = dims[0], ny = dims[1], nz = dims[2]
+    if (!h.f.find("/" + std::to_string(idx), &obj, err)) return -1;
+    if (!h.f.read(obj, img.data, &dims, err)) return -1;
+    if (dims.size() != 2 && dims.size() != 3) {
+      if (err) *err = "HDF image " + std::to_string(idx) + " is neither 2D nor 3D";
+      return -1;
+    }
+    img.nx = (int)dims[0];
+    img.ny = (int)dims[1];
+    img.nz = dims.size() == 3 ? (int)dims[2] : 1;
+    return 0;
+  }
+  // readHDF2 (emdata_.cpp:330-380) with EMhdf2::read_header / read_data
+  // (emhdf2_.cpp:245-363): the size from the EMAN.nx/ny/nz attributes of
+  // /MDF/images/<idx>, the pixels from its dataset "image"
+  const std::string grp = "/MDF/images/" + std::to_string(idx);
+  if (!h.f.find(grp, &obj, err)) return -1;
+  int n[3] = {0, 0, 0};
+  const char* names[3] = {"EMAN.nx", "EMAN.ny", "EMAN.nz"};
+  for (int k = 0; k < 3; ++k) {
+    gshdf::Value v;
+    std::string e;
+    if (h.f.attribute(obj, names[k], &v, &e) && !v.num.empty()) n[k] = (int)v.num[0];
+    else if (!e.empty()) {
+      if (err) *err = grp + ": " + e;
+      return -1;
+    }
+  }
+  if (n[2] > 1) { img.nx = n[0]; img.ny = n[1]; img.nz = n[2]; }
+  else if (n[1] > 1) { img.nx = n[0]; img.ny = n[1]; img.nz = 1; }
+  else if (n[0] > 1) { img.nx = n[0]; img.ny = 1; img.nz = 1; }
+  else {
+    if (err) *err = grp + ": no EMAN.nx/ny/nz size";
+    return -1;
+  }
+  if (!h.f.find(grp + "/image", &obj, err) || !h.f.read(obj, img.data, &dims, err)) return -1;
+  if (img.data.size() != (size_t)img.nx * img.ny * img.nz) {
+    if (err) *err = grp + "/image holds " + std::to_string(img.data.size()) +
+                    " values, its header says " + std::to_string(img.nx) + "x" +
+                    std::to_string(img.ny) + "x" + std::to_string(img.nz);
+    return -1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int image_count(const std::string& path, std::string* err) {
+  if (!gshdf::is_hdf5(path)) {
+    const int n = mrc_sections(path);
+    if (n < 0 && err) *err = "cannot read " + path;
+    return n;
+  }
+  Hdf h;
+  if (hdf_open(path, h, err)) return -1;
+  return hdf_count(h, err);
+}
+
+int read_image(const std::string& path, int n, Image& img, std::string* err) {
+  if (!gshdf::is_hdf5(path)) return read_mrc(path, n, img, err);
+  Hdf h;
+  if (hdf_open(path, h, err)) return -1;
+  if (n >= 0) return hdf_image(h, n, img, err);
+  // the whole file: one image, or a stack of equal-size 2D images
+  const int cnt = hdf_count(h, err);
+  if (cnt <= 0) {
+    if (err && cnt == 0) *err = path + " holds no images";
+    return -1;
+  }
+  if (hdf_image(h, 0, img, err)) return -1;
+  if (cnt == 1) return 0;
+  if (img.nz != 1) {
+    if (err) *err = path + ": a stack of 3D images";
+    return -1;
+  }
+  const size_t per = (size_t)img.nx * img.ny;
+  img.data.resize(per * cnt);
+  Image one;
+  for (int k = 1; k < cnt; ++k) {
+    if (hdf_image(h, k, one, err)) return -1;
+    if (one.nx != img.nx || one.ny != img.ny || one.nz != 1) {
+      if (err) *err = path + ": images of different sizes in one stack";
+      return -1;
+    }
+    std::copy(one.data.begin(), one.data.end(), img.data.begin() + per * k);
+  }
+  img.nz = cnt;
+  return 0;
+}
+
+int read_images(const std::string& path, int count, std::vector<Image>& out, std::string* err) {
+  out.clear();
+  if (!gshdf::is_hdf5(path)) {
+    Image all;
+    if (read_mrc(path, -1, all, err)) return -1;
+    if (all.nz < count) {
+      if (err) *err = path + " holds " + std::to_string(all.nz) + " images";
+      return -1;
+    }
+    const size_t per = (size_t)all.nx * all.ny;
+    out.resize(count);
+    for (int k = 0; k < count; ++k) {
+      out[k].nx = all.nx;
+      out[k].ny = all.ny;
+      out[k].nz = 1;
+      out[k].data.assign(all.data.begin() + per * k, all.data.begin() + per * (k + 1));
+    }
+    return 0;
+  }
+  Hdf h;
+  if (hdf_open(path, h, err)) return -1;
+  out.resize(count);
+  for (int k = 0; k < count; ++k)
+    if (hdf_image(h, k, out[k], err)) return -1;
   return 0;
 }
 

@@ -57,6 +57,16 @@ struct Image {
 int read_mrc(const std::string& path, int n, Image& img, std::string* err);
 // number of sections
 int mrc_sections(const std::string& path);
+// emdata::readImage (EMReader/emdata_.cpp:174-205): an HDF5 file (EMAN1 or
+// EMAN2 layout) or else an MRC file.  n >= 0 reads image n; n < 0 the whole
+// file (an MRC stack or volume; an HDF stack of equal-size 2D images as nz
+// sections).  Returns 0, or a negative value with *err set.
+int read_image(const std::string& path, int n, Image& img, std::string* err);
+// the first count images of a stack, each with its own size (HDF stacks may
+// mix sizes; MRC stacks share one)
+int read_images(const std::string& path, int count, std::vector<Image>& out, std::string* err);
+// number of images (MRC sections, HDF datasets); negative on error
+int image_count(const std::string& path, std::string* err);
 // write a mode-2 MRC (for tools and tests)
 int write_mrc(const std::string& path, const float* data, int nx, int ny, int nz);
 

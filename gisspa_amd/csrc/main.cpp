@@ -56,40 +56,52 @@ int main(int argc, char** argv) {
   }
   Image img;
   std::string err;
-  if (read_mrc(first.path, first.nn, img, &err)) {
+  if (read_image(first.path, first.nn, img, &err)) {
     fprintf(stderr, "Error: %s\n", err.c_str());
     return 1;
   }
   img = Image();
 
-  // templates: the first N sections of the template stack
-  const int nsec = mrc_sections(cfg.templ);
+  // templates: the first N images of the template file (MRC stack, or HDF
+  // with one size per image, main.cu:182-223)
+  const int nsec = image_count(cfg.templ, &err);
   if (nsec < N) {
-    fprintf(stderr, "Error: %s holds %d templates, the Euler file lists %d\n", cfg.templ.c_str(),
-            nsec, N);
+    if (nsec < 0) fprintf(stderr, "Error: %s\n", err.c_str());
+    else fprintf(stderr, "Error: %s holds %d templates, the Euler file lists %d\n",
+                 cfg.templ.c_str(), nsec, N);
     return 1;
   }
-  Image t0;
-  if (read_mrc(cfg.templ, 0, t0, &err)) {
+  std::vector<Image> tms;
+  if (read_images(cfg.templ, N, tms, &err)) {
     fprintf(stderr, "Error: %s\n", err.c_str());
     return 1;
   }
-  const int bx = t0.nx, by = t0.ny;
-  std::vector<float> tm((size_t)N * bx * by);
-  {
-    Image all;
-    if (read_mrc(cfg.templ, -1, all, &err)) {
-      fprintf(stderr, "Error: %s\n", err.c_str());
-      return 1;
-    }
-    memcpy(tm.data(), all.data.data(), tm.size() * sizeof(float));
-  }
+  bool one_box = true;
+  for (const Image& t : tms) one_box = one_box && t.nx == tms[0].nx && t.ny == tms[0].ny;
 
   void* ctx = nullptr;
   if ((r = gs_create(&cfg.p, cfg.device_id, &ctx)) != GS_OK) return fail("gs_create", r);
   gs_ctf tctf{first.defocus, first.dfdiff, first.dfang};
-  if ((r = gs_set_templates(ctx, tm.data(), N, bx, by, &tctf)) != GS_OK)
-    return fail("gs_set_templates", r);
+  if (one_box) {
+    const int bx = tms[0].nx, by = tms[0].ny;
+    std::vector<float> tm((size_t)N * bx * by);
+    for (int j = 0; j < N; ++j)
+      std::copy(tms[j].data.begin(), tms[j].data.end(), tm.begin() + (size_t)j * bx * by);
+    tms.clear();
+    if ((r = gs_set_templates(ctx, tm.data(), N, bx, by, &tctf)) != GS_OK)
+      return fail("gs_set_templates", r);
+  } else {
+    std::vector<const float*> ptr(N);
+    std::vector<int> bx(N), by(N);
+    for (int j = 0; j < N; ++j) {
+      ptr[j] = tms[j].data.data();
+      bx[j] = tms[j].nx;
+      by[j] = tms[j].ny;
+    }
+    if ((r = gs_set_templates_mixed(ctx, N, ptr.data(), bx.data(), by.data(), &tctf)) != GS_OK)
+      return fail("gs_set_templates_mixed", r);
+    tms.clear();
+  }
   gs_info info;
   gs_get_info(ctx, &info);
   if (info.l != cfg.p.window_size)
@@ -114,7 +126,7 @@ int main(int argc, char** argv) {
         printf("Error %d occured in readRawImage!\n", r);
         return 1;
       }
-      if (read_mrc(es[i].path, es[i].nn, ims[i], &err)) {
+      if (read_image(es[i].path, es[i].nn, ims[i], &err)) {
         fprintf(stderr, "Error: %s\n", err.c_str());
         return 1;
       }

@@ -23,7 +23,7 @@ int main(int argc, char** argv) {
   const int device = argc > 4 ? std::atoi(argv[4]) : 0;
   gsio::Image vol;
   std::string err;
-  if (gsio::read_mrc(argv[1], -1, vol, &err) != 0) {
+  if (gsio::read_image(argv[1], -1, vol, &err) != 0) {
     std::fprintf(stderr, "%s: %s\n", argv[1], err.c_str());
     return 1;
   }

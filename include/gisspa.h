@@ -32,7 +32,9 @@ typedef enum gs_status {
   GS_ERR_DEVICE = -3,       /* HIP runtime error */
   GS_ERR_NOMEM = -4,        /* device allocation failed */
   GS_ERR_STATE = -5,        /* call order (templates / micrograph not set) */
-  GS_ERR_CAPACITY = -6      /* output buffer too small (n_out holds the need) */
+  GS_ERR_CAPACITY = -6,     /* output buffer too small (n_out holds the need) */
+  GS_ERR_IO = -7            /* file missing, unreadable or in an unsupported format
+                               (gs_io_error() holds the message) */
 } gs_status;
 
 /* Run parameters: the reference's Parameters (EMReader/DataReader.h:14-42)
@@ -343,6 +345,39 @@ int gs_set_pipeline(void* ctx, int mode, int lag, int ring);
 int gs_set_tile_block(void* ctx, int tiles);
 int gs_destroy(void* ctx);
 const char* gs_strerror(int code);
+
+/* ---- Image files (host only; no device is touched) ------------------------
+ * emdata::readImage (EMReader/emdata_.cpp:174-205) reads templates and
+ * micrographs from HDF5 files in two layouts, or else from MRC files:
+ *   - EMAN1 ("old-style", a root attribute num_dataset, EMReader/emhdf_.cpp:
+ *     103-122): image i is the root dataset "i", nx = dims[0], ny = dims[1],
+ *     nz = dims[2] (emdata_.cpp:216-250, emhdf_.cpp:124-150);
+ *   - EMAN2 ("new-style"): image i is the group /MDF/images/i, its size the
+ *     attributes EMAN.nx/ny/nz, its pixels the dataset "image"
+ *     (emdata_.cpp:330-380, emhdf2_.cpp:245-363);
+ *   - MRC modes 0, 1, 2, 6, either byte order (emdata_.cpp:384-617).
+ * The HDF5 files are parsed without libhdf5: superblocks 0-3, symbol-table
+ * and compact groups, contiguous / compact / chunked datasets (deflate,
+ * shuffle, fletcher32), integer and IEEE float elements, converted to float.
+ * Errors return GS_ERR_IO with the message in gs_io_error() (per thread). */
+const char* gs_io_error(void);
+/* Number of images in a file (MRC sections, HDF images). */
+int gs_image_count(const char* path, int* n);
+/* Image `index` (index < 0: the whole file -- an MRC stack or volume, or an
+ * HDF stack of equal-size 2D images as nz sections) as float, x fastest.
+ * Sizes are always set; out may be NULL to query them; a cap (in floats)
+ * below nx*ny*nz returns GS_ERR_CAPACITY. */
+int gs_read_image(const char* path, int index, float* out, size_t cap, int* nx, int* ny, int* nz);
+/* Any HDF5 dataset by path ("/MDF/images/0/image"), elements converted to
+ * float in storage order; *rank and dims[0..rank-1] (slowest first, at most
+ * max_rank entries) are always set; out may be NULL to query them. */
+int gs_hdf5_read(const char* path, const char* object, float* out, size_t cap, int* rank,
+                 long long* dims, int max_rank);
+/* An attribute of an HDF5 object: numeric values into num (*n_num of them,
+ * GS_ERR_CAPACITY beyond cap), a string into str (NUL-terminated,
+ * GS_ERR_CAPACITY if it does not fit).  *is_string tells which. */
+int gs_hdf5_attr(const char* path, const char* object, const char* name, double* num, size_t cap,
+                 size_t* n_num, char* str, size_t str_cap, int* is_string);
 
 #ifdef __cplusplus
 }

@@ -95,3 +95,92 @@ def test_cli_micrograph_batches(gpu, tmp_path):
             assert float(m.group(9)) == s["cx"] and float(m.group(10)) == s["cy"]
             assert float(m.group(11)) == pytest.approx(s["score"], rel=1e-4, abs=2e-6)
     assert k == len(lines)
+
+
+def _lines_without_paths(path):
+    out = []
+    for ln in open(path).read().splitlines():
+        f = ln.split("\t")
+        out.append("\t".join([f[0]] + f[2:]))
+    return out
+
+
+def test_cli_reads_hdf_files(gpu, tmp_path):
+    """The same workload with its templates and micrographs in HDF files
+    (emdata::readImage, EMReader/emdata_.cpp:174-205) gives the lines of the
+    MRC run.  Run A: EMAN2 layout, templates with version-2 metadata and one
+    deflated single-chunk dataset per image, the two micrographs (of
+    different sizes) as images 0 and 1 of one chunked, shuffled and deflated
+    stack.  Run B: EMAN1 layout, templates behind a 512-byte user block."""
+    from tests import _hdf5 as H
+    c = _cases.tiny(seed=2, l=128, n_views=5, phistep=45.0, threshold=4.5)
+    c2 = _cases.tiny(seed=3, l=128, n_views=5, phistep=45.0, threshold=4.5, size=(288, 320))
+    eul = synth.random_eulers(len(c.templates), 7)
+    mics = [(c.image, c.defocus), (c2.image, 1.7)]
+    wl = synth.Workload("cli", c.templates, eul, mics, dict(c.params), [])
+    d = tmp_path / "mrc"
+    cfg = synth.write_workload(wl, str(d))
+    r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = _lines_without_paths(os.path.join(d, "picks.lst"))
+    assert len(ref) > 0
+    tm = c.templates.shape[1]
+    runs = {
+        "A": (lambda p: H.Writer(2).write(p, H.eman2_stack(list(c.templates), layout="chunked",
+                                                           chunks=(tm, tm), deflate=4)),
+              lambda p: H.Writer(0).write(p, H.eman2_stack([c.image, c2.image], layout="chunked",
+                                                           chunks=(64, 96), shuffle=True,
+                                                           deflate=6))),
+        "B": (lambda p: H.Writer(0, 512).write(p, H.eman1_stack(list(c.templates))),
+              lambda p: H.Writer(0).write(p, H.eman1_stack([c.image, c2.image]))),
+    }
+    for name, (write_templates, write_mics) in runs.items():
+        h = tmp_path / name
+        cfg = synth.write_workload(wl, str(h))
+        write_templates(str(h / "templates.hdf"))
+        write_mics(str(h / "micrographs.hdf"))
+        synth.write_micrograph_lst(str(h / "micrographs.lst"),
+                                   [(i, "micrographs.hdf", df, 0.0, 0.0) for i, (_, df) in
+                                    enumerate(mics)])
+        text = open(cfg).read().replace("templates.mrcs", "templates.hdf")
+        open(cfg, "w").write(text)
+        r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (name, r.stderr[-2000:])
+        got = open(os.path.join(h, "picks.lst")).read().splitlines()
+        assert all(ln.split("\t")[1].endswith("micrographs.hdf") for ln in got)
+        assert _lines_without_paths(os.path.join(h, "picks.lst")) == ref, name
+
+
+def test_cli_hdf_templates_of_mixed_boxes(gpu, tmp_path):
+    """An EMAN2 HDF template stack whose images have their own sizes (64,
+    48 and 57 px): the CLI centres each in the window by its size
+    (main.cu:182-223) and prints the picks the library gives for the same
+    templates (gs_set_templates_mixed)."""
+    import gisspa_amd
+    from tests import _hdf5 as H
+    c = _cases.tiny(seed=6, l=128, box=64, n_views=6, threshold=4.0)
+    boxes = [64, 48, 57, 64, 48, 57]
+    tms = []
+    for t, b in zip(c.templates, boxes):
+        o = (64 - b) // 2
+        tms.append(np.ascontiguousarray(t[o:o + b, o:o + b]))
+    eul = synth.random_eulers(len(tms), 5)
+    wl = synth.Workload("cli", c.templates, eul, [(c.image, c.defocus)], dict(c.params), [])
+    cfg = synth.write_workload(wl, str(tmp_path))
+    H.Writer(0).write(str(tmp_path / "templates.hdf"), H.eman2_stack(tms))
+    text = open(cfg).read().replace("templates.mrcs", "templates.hdf")
+    open(cfg, "w").write(text)
+    r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = open(os.path.join(tmp_path, "picks.lst")).read().splitlines()
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates_mixed(tms, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        picks = pk.pick()
+    assert len(lines) == len(picks) and len(picks) > 0
+    for ln, p in zip(lines, picks):
+        m = LINE.match(ln)
+        assert m, ln
+        assert float(m.group(6)) == pytest.approx(float(np.float32(eul[p["view"]][0])), abs=1e-5)
+        assert float(m.group(9)) == p["cx"] and float(m.group(10)) == p["cy"]
+        assert float(m.group(11)) == pytest.approx(float(p["score"]), rel=1e-5, abs=2e-6)
