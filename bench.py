@@ -130,14 +130,21 @@ def cpu_baseline(templates, image, defocus, nthreads: int, n_views: int):
     t0 = time.time()
     orc.correlate_many(p, tiles, spec, nthreads)
     t_maps = time.time() - t0
+    # the same maps on one thread: one view against all tiles
+    t0 = time.time()
+    orc.correlate_many(p, tiles, spec[:1], 1)
+    t_map1 = (time.time() - t0) / tiles.shape[0]
     T = tiles.shape[0]
     N = templates.shape[0]
     sec = t_prep + 72 * (N / n_views) * (t_rot + t_maps)
+    sec1 = t_prep + 72 * N * (t_rot / n_views + T * t_map1)
     return dict(value=3600.0 / sec, unit="micrographs/hour", cores=nthreads, kind="port",
+                single_thread_value=3600.0 / sec1,
                 sample=f"FP64 oracle on the host: micrograph prep {t_prep:.1f}s (1 thread), "
                        f"rotation of {n_views} views {t_rot:.2f}s (1 thread), "
                        f"{n_views} views x {T} tiles = {n_views * T} CC maps {t_maps:.1f}s "
-                       f"({nthreads} threads); extrapolated to {N} views x 72 psi")
+                       f"({nthreads} threads), 1 view x {T} tiles {T * t_map1:.1f}s "
+                       f"(1 thread); extrapolated to {N} views x 72 psi")
 
 
 def plain_rate(templates, d_img, nx, ny, defocus, npsi: int):
