@@ -169,6 +169,26 @@ def plain_rate(templates, d_img, nx, ny, defocus, npsi: int):
                        f"{t_psi:.2f}s each, extrapolated to 72 psi")
 
 
+def norm1_rate(templates, d_img, nx, ny, defocus, n_views: int):
+    """norm_type 1 (per-pixel normalisation over all views x psi,
+    main.cu:1170-1229) on the same micrograph with the first n_views views:
+    both passes over all 72 psi, wall clock, scaled to all views (the work is
+    linear in the number of views)."""
+    import gisspa_amd
+    pk = gisspa_amd.Picker(0, **dict(FULL_PARAMS, norm_type=1))
+    pk.set_templates(np.ascontiguousarray(templates[:n_views]), defocus)
+    pk.set_micrograph_device(d_img, nx, ny, defocus)
+    t0 = time.time()
+    picks = pk.pick()
+    t = time.time() - t0
+    pk.close()
+    sec = t * templates.shape[0] / n_views
+    return dict(value=3600.0 / sec, unit="micrographs/hour", views=n_views, seconds=t,
+                picks=int(len(picks)),
+                sample=f"norm_type 1, {n_views} of {templates.shape[0]} views x 72 psi (both "
+                       f"passes) in {t:.2f}s, scaled by the view count")
+
+
 def dedupe_rate(picks, n_views: int, no_cpu: bool, center: float = 20.0,
                 euler_thres: float = 10.0):
     """Duplicate removal (gs_remove_repeats) of the run's pick list, one
@@ -465,6 +485,9 @@ def main():
     ap.add_argument("--cli", type=int, default=0, metavar="N",
                     help="also run the ./main command line on N full-size micrographs, once per "
                          "micrograph_batch of 1 and 4 (0 = skip)")
+    ap.add_argument("--norm1", type=int, default=0, metavar="VIEWS",
+                    help="also time the norm_type 1 picker with the first VIEWS views, scaled "
+                         "to all (0 = skip)")
     ap.add_argument("--rocfft", type=int, default=1,
                     help="time rocFFT's inverse 2D transforms of the same size (0 = skip)")
     ap.add_argument("--project", type=int, default=1,
@@ -660,6 +683,10 @@ def main():
     if not args.no_plain:
         plain = plain_rate(templates, d_img, nx, ny, defocus, args.plain_psi)
         log(f"[bench] plain: {plain['sample']}: {plain['value']:.2f} micrographs/hour")
+    norm1 = None
+    if args.norm1:
+        norm1 = norm1_rate(templates, d_img, nx, ny, defocus, args.norm1)
+        log(f"[bench] norm_type 1: {norm1['sample']}: {norm1['value']:.2f} micrographs/hour")
     hip.free(d_img)
 
     cpu = None
@@ -708,6 +735,7 @@ def main():
         "batched": batched,
         "stress": stress,
         "cli": cli,
+        "norm1": norm1,
         "rocfft": rocfft,
         "project": proj,
         "plain": plain,

@@ -95,7 +95,8 @@ struct Ctx {
   DevBuf maps, stats, sel, sel_count;
   DevBuf maps_blk;
   int tile_block = 0;  // tiles per block of the map order; 0: as many as fit in 192 MiB
-  int maps_T = -1, maps_N = -1, maps_tb = -1;
+  int maps_T = -1, maps_N = -1, maps_tb = -1, maps_nv = -1;
+  int n1_nv = 1;   // views per norm_type 1 batch (maps order)
   DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_z, n1_rec;  // norm_type 1
   // plain mode buffers
   DevBuf pl_tspec, pl_tiles, pl_ccg, pl_part, pl_rotpart, pl_mean, pl_var;
@@ -523,8 +524,9 @@ int ensure_corr_buffers(Ctx* c) {
   CHK(c->psum.ensure((size_t)c->T * c->N * corr_ncb(L) * sizeof(float2)));
   // Processing orders; results land in the canonical [tile][view] order
   // through the MapRef of each map.
-  //  * maps (norm_type 1): view-major, so a batch of at most T maps never
-  //    holds one tile twice;
+  //  * maps (norm_type 1): groups of nv views (nv = batch / T), tile-major
+  //    within a group (maps t * nv .. t * nv + nv - 1 are the group's views of
+  //    tile t); one group is one batch;
   //  * maps_blk (norm_type 0): blocks of tb tiles, all views of a block
   //    before the next block, tiles fastest.  Within a block each template
   //    half spectrum is read from HBM once and stays in L2 across the block's
@@ -537,11 +539,14 @@ int ensure_corr_buffers(Ctx* c) {
   const long tile_bytes = (long)(L / 2 + 1) * L * (long)sizeof(float2);
   const int tb = c->tile_block > 0 ? c->tile_block
                                    : (int)std::max(1l, (192l << 20) / tile_bytes);
-  if (c->maps_T != c->T || c->maps_N != c->N || c->maps_tb != tb) {
-    std::vector<MapRef> mr(nm), mb;
+  const int nv1 = std::max(1, std::min(c->N, nb / c->T));
+  if (c->maps_T != c->T || c->maps_N != c->N || c->maps_tb != tb || c->maps_nv != nv1) {
+    std::vector<MapRef> mr, mb;
+    mr.reserve(nm);
     mb.reserve(nm);
-    for (int j = 0; j < c->N; ++j)
-      for (int t = 0; t < c->T; ++t) mr[(size_t)j * c->T + t] = MapRef{t, j};
+    for (int j0 = 0; j0 < c->N; j0 += nv1)
+      for (int t = 0; t < c->T; ++t)
+        for (int j = j0; j < std::min(c->N, j0 + nv1); ++j) mr.push_back(MapRef{t, j});
     for (int t0 = 0; t0 < c->T; t0 += tb)
       for (int j = 0; j < c->N; ++j)
         for (int t = t0; t < std::min(c->T, t0 + tb); ++t) mb.push_back(MapRef{t, j});
@@ -554,7 +559,9 @@ int ensure_corr_buffers(Ctx* c) {
     c->maps_T = c->T;
     c->maps_N = c->N;
     c->maps_tb = tb;
+    c->maps_nv = nv1;
   }
+  c->n1_nv = nv1;
   return GS_OK;
 }
 
@@ -701,16 +708,23 @@ int fused_psi(Ctx* c, int ip, MapStat* out) {
 
 // norm_type 1 (main.cu:1170-1229): pass 1 accumulates per-pixel sums of c/L^2
 // and its square over every view x psi, pass 2 turns each map into z-scores
-// and reports 1024-pixel runs whose maximum reaches the threshold.  One
-// stream, and batches that never repeat a tile, so the FP64 accumulators see
-// a fixed order of additions.
+// and reports 1024-pixel runs whose maximum reaches the threshold.  Batches
+// are groups of nv views over all tiles (tile-major, see ensure_corr_buffers):
+// the accumulation sums each tile's nv maps in registers before touching the
+// FP64 accumulators, and z-maps of one tile are neighbours, so the tile's
+// mean / sd maps are read from cache.  One stream; every accumulator sees a
+// fixed order of additions.
 int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& picks) {
   const int L = c->L;
   const long L2 = (long)L * L;
   const int P = num_psi(c->p.phi_step);
   const int nm = c->T * c->N;
-  const int nb = std::min({c->batch, nm, c->T});
+  const int nv1 = c->n1_nv;
+  const long per_map = c->geo.gmap * (long)sizeof(float2);
+  const int nb = c->T * nv1;   // maps of a full batch
   const size_t npix = (size_t)c->T * L2;
+  CHK(c->G[0].ensure((size_t)nb * per_map));
+  CHK(c->psum.ensure((size_t)nm * corr_ncb(L) * sizeof(float2)));
   CHK(c->n1_s1.ensure(npix * sizeof(double)));
   CHK(c->n1_s2.ensure(npix * sizeof(double)));
   CHK(c->n1_avg.ensure(npix * sizeof(float)));
@@ -733,9 +747,8 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
     for (int m0 = 0; m0 < nm; m0 += nb) {
       const int cnt = std::min(nb, nm - m0);
       run_pass1(m0, cnt);
-      launch_norm1_rows(L, 0, c->G[0].as<float2>(), maps + m0, cnt, c->n1_s1.as<double>(),
-                        c->n1_s2.as<double>(), nullptr, nullptr, nullptr, c->twL.as<float2>(),
-                        c->twM.as<float2>(), s);
+      launch_norm1_acc(L, c->G[0].as<float2>(), maps + m0, c->T, cnt / c->T,
+                       c->n1_s1.as<double>(), c->n1_s2.as<double>(), c->twM.as<float2>(), s);
     }
   }
   launch_norm1_stats(c->n1_s1.as<double>(), c->n1_s2.as<double>(), (long)npix, (double)c->N * P,
@@ -746,10 +759,9 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
     for (int m0 = 0; m0 < nm; m0 += nb) {
       const int cnt = std::min(nb, nm - m0);
       run_pass1(m0, cnt);
-      launch_norm1_rows(L, 1, c->G[0].as<float2>(), maps + m0, cnt, nullptr, nullptr,
-                        c->n1_avg.as<float>(), c->n1_sd.as<float>(), c->n1_z.as<float>(),
-                        c->twL.as<float2>(), c->twM.as<float2>(), s);
-      launch_norm1_blockmax(c->n1_z.as<float>(), L, cnt, (ip - psi_begin) * nm + m0, maps + m0,
+      launch_norm1_zmap(L, c->G[0].as<float2>(), maps + m0, cnt, c->n1_avg.as<float>(),
+                        c->n1_sd.as<float>(), c->n1_z.as<float>(), c->twM.as<float2>(), s);
+      launch_norm1_blockmax(c->n1_z.as<float>(), L, cnt, (ip - psi_begin) * nm, c->T, maps + m0,
                             c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
                             c->sel_count.as<int>(), c->n1_rec.p, cap, s);
     }
@@ -761,11 +773,12 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   if (count > cap) return GS_ERR_CAPACITY;
   std::vector<N1Rec> recs(count);
   if (count) CHK(copy_out(c, recs.data(), c->n1_rec.p, (size_t)count * sizeof(N1Rec)));
-  // canonical order: micrograph, psi, tile, view, block (main.cu:784-819)
+  // canonical order: micrograph, psi, tile, view, block (main.cu:784-819);
+  // r.m = psi offset * nm + view * T + tile
   const int nblk = (int)((L2 + 1023) / 1024);
   auto key = [&](const N1Rec& r) {
     const int ipl = r.m / nm, mm = r.m % nm;
-    const MapRef mr{mm % c->T, mm / c->T};  // view-major processing order
+    const MapRef mr{mm % c->T, mm / c->T};
     return (((long long)ipl * c->T + mr.tile) * c->N + mr.view) * nblk + r.block;
   };
   std::sort(recs.begin(), recs.end(), [&](const N1Rec& a, const N1Rec& b) { return key(a) < key(b); });

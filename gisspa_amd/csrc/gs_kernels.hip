@@ -649,24 +649,86 @@ __global__ void k_corr_finalize(const CorrPartial* __restrict__ part, int nparts
 }
 
 // ---------------------------------------------------------------------------
-// norm_type 1 ("global" per-pixel normalisation, main.cu:1170-1229):
-// the same row C2R as pass 2, then either accumulate c/L^2 and its square per
+// norm_type 1 ("global" per-pixel normalisation, main.cu:1170-1229): the
+// same row C2R as pass 2, then either accumulate c/L^2 and its square per
 // tile pixel over all views x psi (add_CCG_to_sum, GPU_func.cu:618-633), or
-// write the z-score map (update_CCG, GPU_func.cu:649-662).  Accumulators are
-// FP64; a batch never holds two maps of one tile and batches run in stream
-// order, so every accumulator sees a fixed order of additions.
+// write the z-score map (update_CCG, GPU_func.cu:649-662).
 // ---------------------------------------------------------------------------
-enum { N1_ACCUM = 0, N1_ZMAP = 1 };
 
-template <int L, int MODE>
-__global__ __launch_bounds__(WG) void k_norm1_rows(const float2* __restrict__ Gin,
+// Accumulation.  The batch is tile-major: maps t * nv .. t * nv + nv - 1 are
+// nv views of tile maps[t * nv].tile.  A workgroup transforms the rows of its
+// row block for those nv maps in order and keeps the per-pixel sums of c/L^2
+// and (c/L^2)^2 in registers: every output of the last transform pass lands
+// in the same thread and the same (round, slot) for every map, so the
+// accumulator of an output is a compile-time register.  The nv-map partial
+// sums (FP32) are then added once to the tile's FP64 accumulators; batches
+// run in stream order and each (tile, row block) belongs to one workgroup, so
+// every accumulator sees a fixed order of additions.
+template <int L>
+__global__ __launch_bounds__(WG) void k_norm1_acc(const float2* __restrict__ Gin,
+                                                  const MapRef* __restrict__ maps, int nv,
+                                                  double* __restrict__ s1,
+                                                  double* __restrict__ s2,
+                                                  const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
+  constexpr long L2 = (long)L * L;
+  constexpr Plan PL = variant_plan(M, RB, WG, MAP_J, 0);
+  constexpr int RL = PL.r[PL.count - 1];   // radix of the last pass
+  constexpr int NSL = M / RL;              // its butterflies per row
+  constexpr int TOT = RB * NSL;
+  constexpr int PER = (TOT + WG - 1) / WG;
+  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  const int y0 = blockIdx.x * RB;
+  const int mfirst = blockIdx.y * nv;
+  const int t = maps[mfirst].tile;
+  const float inv = 1.0f / (float)L2;
+  float acc1[PER][RL][2], acc2[PER][RL][2];
+#pragma unroll
+  for (int p = 0; p < PER; ++p)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      acc1[p][r][0] = acc1[p][r][1] = 0.0f;
+      acc2[p][r][0] = acc2[p][r][1] = 0.0f;
+    }
+  for (int i = 0; i < nv; ++i) {
+    const float2* gb = Gin + (long)(mfirst + i) * NCB * L * CB;
+    auto load = [&](int r, int k) -> float2 { return gb[pair_slot<L>(k, min(y0 + r, L - 1))]; };
+    auto store = [&](int, int, float2 z, int p, int r) {
+      const float a = z.x * inv, b = z.y * inv;
+      acc1[p][r][0] += a;
+      acc1[p][r][1] += b;
+      acc2[p][r][0] += a * a;
+      acc2[p][r][1] += b * b;
+    };
+    fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+    __syncthreads();  // the last pass's LDS reads end before the next map's first writes
+  }
+  // output (row, n) of round p, slot r: row = g / NSL, n = g % NSL + r * NSL
+  // with g = thread + p * WG (the last pass's butterfly mapping)
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int g = threadIdx.x + p * WG;
+    const int y = y0 + g / NSL;
+    if ((TOT % WG != 0 && g >= TOT) || y >= L) continue;
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const long pix = t * L2 + (long)y * L + 2 * (g % NSL + r * NSL);
+      s1[pix] += (double)acc1[p][r][0];
+      s1[pix + 1] += (double)acc1[p][r][1];
+      s2[pix] += (double)acc2[p][r][0];
+      s2[pix + 1] += (double)acc2[p][r][1];
+    }
+  }
+}
+
+// z-score maps of a batch (update_CCG): map m's z values at z + m * L^2
+template <int L>
+__global__ __launch_bounds__(WG) void k_norm1_zmap(const float2* __restrict__ Gin,
                                                    const MapRef* __restrict__ maps,
-                                                   double* __restrict__ s1,
-                                                   double* __restrict__ s2,
                                                    const float* __restrict__ avg,
                                                    const float* __restrict__ sd,
                                                    float* __restrict__ zout,
-                                                   const float2* __restrict__ twL,
                                                    const float2* __restrict__ twM) {
   using G = Geo<L>;
   constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
@@ -683,21 +745,13 @@ __global__ __launch_bounds__(WG) void k_norm1_rows(const float2* __restrict__ Gi
     const int y = y0 + r;
     if (y >= L) return;
     const long pix = (long)y * L + 2 * n;
-    if constexpr (MODE == N1_ACCUM) {
-      const double a = (double)(z.x * inv), b = (double)(z.y * inv);
-      double* p1 = s1 + t * L2 + pix;
-      double* p2 = s2 + t * L2 + pix;
-      p1[0] += a; p1[1] += b;
-      p2[0] += a * a; p2[1] += b * b;
-    } else {
-      const float* av = avg + t * L2 + pix;
-      const float* sv = sd + t * L2 + pix;
-      const float a = z.x * inv, b = z.y * inv;
-      float2 o;
-      o.x = sv[0] > 0 ? (a - av[0]) / sv[0] : 0.0f;
-      o.y = sv[1] > 0 ? (b - av[1]) / sv[1] : 0.0f;
-      *reinterpret_cast<float2*>(zout + (long)m * L2 + pix) = o;
-    }
+    const float* av = avg + t * L2 + pix;
+    const float* sv = sd + t * L2 + pix;
+    const float a = z.x * inv, b = z.y * inv;
+    float2 o;
+    o.x = sv[0] > 0 ? (a - av[0]) / sv[0] : 0.0f;
+    o.y = sv[1] > 0 ? (b - av[1]) / sv[1] : 0.0f;
+    *reinterpret_cast<float2*>(zout + (long)m * L2 + pix) = o;
   };
   fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
 }
@@ -716,10 +770,11 @@ __global__ void k_norm1_stats(const double* __restrict__ s1, const double* __res
 // get_peak_pos (GPU_func.cu:711-745) + host loop (main.cu:796-819): maximum
 // of each run of 1024 pixels of a z-map (lowest index on ties); runs above
 // the threshold whose centre is at least `dmin` from the micrograph edges
-// are appended to `out` (order restored on the host).
+// are appended to `out` with the map's canonical index mbase + view *
+// ntiles + tile (order restored on the host).
 
 __global__ __launch_bounds__(256) void k_norm1_blockmax(const float* __restrict__ z, long L2,
-                                                        int nblk, int L, int mbase,
+                                                        int nblk, int L, int mbase, int ntiles,
                                                         const MapRef* maps,
                                                         const TileDesc* __restrict__ tdesc,
                                                         float thres, float dmin,
@@ -746,7 +801,13 @@ __global__ __launch_bounds__(256) void k_norm1_blockmax(const float* __restrict_
   const int cx = td.ox + p % L, cy = td.oy + p / L;
   if (cx >= dmin && cx < td.nx - dmin && cy >= dmin && cy < td.ny - dmin) {
     const int i = atomicAdd(count, 1);
-    if (i < cap) { N1Rec r; r.m = mbase + m; r.block = b; r.pos = p; r.z = v; out[i] = r; }
+    if (i < cap) {
+      // the map's canonical (view-major) index, whatever the processing order
+      N1Rec r;
+      r.m = mbase + maps[m].view * ntiles + maps[m].tile;
+      r.block = b; r.pos = p; r.z = v;
+      out[i] = r;
+    }
   }
 }
 
@@ -1322,34 +1383,37 @@ void launch_img_mean(const double* partials, int nblocks, long n, double* mean, 
 }
 
 template <int L>
-static void norm1_rows_t(int mode, const float2* G, const MapRef* maps, int nmaps, double* s1,
-                         double* s2, const float* avg, const float* sd, float* z,
-                         const float2* twL, const float2* twM, hipStream_t s) {
-  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
-  if (mode == N1_ACCUM)
-    hipLaunchKernelGGL((k_norm1_rows<L, N1_ACCUM>), grid, dim3(WG), 0, s, G, maps, s1, s2, avg, sd,
-                       z, twL, twM);
-  else
-    hipLaunchKernelGGL((k_norm1_rows<L, N1_ZMAP>), grid, dim3(WG), 0, s, G, maps, s1, s2, avg, sd,
-                       z, twL, twM);
+static void norm1_acc_t(const float2* G, const MapRef* maps, int ntiles, int nv, double* s1,
+                        double* s2, const float2* twM, hipStream_t s) {
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, ntiles);
+  hipLaunchKernelGGL(k_norm1_acc<L>, grid, dim3(WG), 0, s, G, maps, nv, s1, s2, twM);
 }
-void launch_norm1_rows(int L, int mode, const float2* G, const MapRef* maps, int nmaps,
-                       double* s1, double* s2, const float* avg, const float* sd, float* z,
-                       const float2* twL, const float2* twM, hipStream_t s) {
-  DISPATCH(L, norm1_rows_t, mode, G, maps, nmaps, s1, s2, avg, sd, z, twL, twM, s);
+void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, int nv, double* s1,
+                      double* s2, const float2* twM, hipStream_t s) {
+  DISPATCH(L, norm1_acc_t, G, maps, ntiles, nv, s1, s2, twM, s);
+}
+template <int L>
+static void norm1_zmap_t(const float2* G, const MapRef* maps, int nmaps, const float* avg,
+                         const float* sd, float* z, const float2* twM, hipStream_t s) {
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
+  hipLaunchKernelGGL(k_norm1_zmap<L>, grid, dim3(WG), 0, s, G, maps, avg, sd, z, twM);
+}
+void launch_norm1_zmap(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
+                       const float* sd, float* z, const float2* twM, hipStream_t s) {
+  DISPATCH(L, norm1_zmap_t, G, maps, nmaps, avg, sd, z, twM, s);
 }
 void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
                         float* sd, hipStream_t s) {
   hipLaunchKernelGGL(k_norm1_stats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, s1, s2, n,
                      total, avg, sd);
 }
-void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, const MapRef* maps,
-                           const TileDesc* tdesc, float thres, float dmin, int* count, void* out,
-                           int cap, hipStream_t s) {
+void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, int ntiles,
+                           const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
+                           int* count, void* out, int cap, hipStream_t s) {
   const long L2 = (long)L * L;
   const int nblk = (int)((L2 + 1023) / 1024);
   hipLaunchKernelGGL(k_norm1_blockmax, dim3(nblk, nmaps), dim3(256), 0, s, z, L2, nblk, L, mbase,
-                     maps, tdesc, thres, dmin, count, static_cast<N1Rec*>(out), cap);
+                     ntiles, maps, tdesc, thres, dmin, count, static_cast<N1Rec*>(out), cap);
 }
 
 __global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T, int N, int L,

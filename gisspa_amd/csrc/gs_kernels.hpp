@@ -137,20 +137,24 @@ void launch_corr_finalize(int L, const CorrPartial* part, const float2* psum, in
                           const MapRef* maps, int nviews, MapStat* out, hipStream_t s);
 
 // ---- norm_type 1 ----
-// mode 0: s1/s2[tile][pixel] += c / L^2, (c / L^2)^2 ; mode 1: z[m][pixel]
-void launch_norm1_rows(int L, int mode, const float2* G, const MapRef* maps, int nmaps,
-                       double* s1, double* s2, const float* avg, const float* sd, float* z,
-                       const float2* twL, const float2* twM, hipStream_t s);
+// s1/s2[tile][pixel] += sums of c / L^2 and (c / L^2)^2 over a tile-major
+// batch of ntiles x nv maps (maps t * nv .. t * nv + nv - 1 share a tile)
+void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, int nv, double* s1,
+                      double* s2, const float2* twM, hipStream_t s);
+// z[m][pixel] of a batch of maps
+void launch_norm1_zmap(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
+                       const float* sd, float* z, const float2* twM, hipStream_t s);
 void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
                         float* sd, hipStream_t s);
-// records {int mbase + m, block, pos; float z} of 1024-pixel runs above threshold
+// records {int m, block, pos; float z} of 1024-pixel runs above threshold;
+// m = mbase + view * ntiles + tile (canonical view-major map index)
 struct N1Rec {
   int m, block, pos;
   float z;
 };
-void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, const MapRef* maps,
-                           const TileDesc* tdesc, float thres, float dmin, int* count, void* out,
-                           int cap, hipStream_t s);
+void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, int ntiles,
+                           const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
+                           int* count, void* out, int cap, hipStream_t s);
 // threshold + centre-bounds selection of norm_type 0 maps (main.cu:871-891)
 void launch_select(const MapStat* st, long long nmaps, int T, int N, int L, const TileDesc* tdesc,
                    float thres, float dmin, int* count, SelRec* out, int cap, hipStream_t s);

@@ -98,20 +98,38 @@ def test_mapstats_and_picks(gpu, tiny_case):
     assert np.allclose(picks["score"], sel["score"], rtol=1e-4)
 
 
-def test_norm_type1_picks(gpu):
+N1_CASES = {
+    # (case arguments, views per batch: None = the default, one batch per psi)
+    "l64": (dict(seed=3, l=64, box=48, n_views=4, size=(200, 176), phistep=45.0, overlap=16,
+                 threshold=3.0), None),
+    # batches of 2 views over all tiles, the last one ragged (5 views = 2 + 2 + 1)
+    "l96-groups": (dict(seed=9, l=96, box=48, n_views=5, size=(256, 224), phistep=60.0,
+                        overlap=24, threshold=3.0), 2),
+    "l128-one-view": (dict(seed=10, l=128, box=64, n_views=3, size=(320, 288), phistep=90.0,
+                           overlap=32, threshold=3.0), 1),
+}
+
+
+@pytest.mark.parametrize("name", list(N1_CASES))
+def test_norm_type1_picks(gpu, name):
     """norm_type 1: per-pixel mean/sd over every view x psi, z-scores, and one
-    candidate per 1024-pixel run (main.cu:1170-1229).  Picks must match the
-    oracle's record for record; a record whose oracle z-score lies within
-    1e-4 (relative) of the threshold may be missing on either side."""
+    candidate per 1024-pixel run (main.cu:1170-1229).  Batches hold groups of
+    views over all tiles (the accumulation sums a tile's views of a batch in
+    registers); the cases cover one group per psi, several groups with a
+    ragged last one, and one view per group.  Picks must match the oracle's
+    record for record; a record whose oracle z-score lies within 1e-4
+    (relative) of the threshold may be missing on either side."""
     import gisspa_amd
-    c = _cases.tiny(seed=3, l=64, box=48, n_views=4, size=(200, 176), phistep=45.0, overlap=16,
-                    threshold=3.0)
+    args, views_per_batch = N1_CASES[name]
+    c = _cases.tiny(**args)
     c.params["norm_type"] = 1
     op = _oparams(c)
     ref = orc.pick_norm1(op, orc.ctf(c.defocus), c.templates, c.image)
     with gisspa_amd.Picker(0, **c.params) as pk:
         pk.set_templates(c.templates, c.defocus)
         pk.set_micrograph(c.image, c.defocus)
+        if views_per_batch:
+            pk.set_batch(pk.info().n_tiles * views_per_batch)
         got = pk.pick()
     thr = c.params["threshold"]
     key = lambda r: (int(r["psi_index"]), int(r["view"]), float(r["cx"]), float(r["cy"]))
