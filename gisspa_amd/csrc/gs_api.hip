@@ -97,7 +97,7 @@ struct Ctx {
   int tile_block = 0;  // tiles per block of the map order; 0: as many as fit in 192 MiB
   int maps_T = -1, maps_N = -1, maps_tb = -1, maps_nv = -1;
   int n1_nv = 1;   // views per norm_type 1 batch (maps order)
-  DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_z, n1_rec;  // norm_type 1
+  DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_keys, n1_rec;  // norm_type 1
   // plain mode buffers
   DevBuf pl_tspec, pl_tiles, pl_ccg, pl_part, pl_rotpart, pl_mean, pl_var;
   // per-psi graphs of the fused correlation (gs_set_graphs)
@@ -711,9 +711,10 @@ int fused_psi(Ctx* c, int ip, MapStat* out) {
 // and reports 1024-pixel runs whose maximum reaches the threshold.  Batches
 // are groups of nv views over all tiles (tile-major, see ensure_corr_buffers):
 // the accumulation sums each tile's nv maps in registers before touching the
-// FP64 accumulators, and z-maps of one tile are neighbours, so the tile's
-// mean / sd maps are read from cache.  One stream; every accumulator sees a
-// fixed order of additions.
+// FP64 accumulators; the second pass reduces the z-scores to per-run maxima
+// inside the row transform (no z-maps), and the maps of one tile are
+// neighbours, so the tile's mean / sd maps are read from cache.  One stream;
+// every accumulator sees a fixed order of additions.
 int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& picks) {
   const int L = c->L;
   const long L2 = (long)L * L;
@@ -729,7 +730,7 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   CHK(c->n1_s2.ensure(npix * sizeof(double)));
   CHK(c->n1_avg.ensure(npix * sizeof(float)));
   CHK(c->n1_sd.ensure(npix * sizeof(float)));
-  CHK(c->n1_z.ensure((size_t)nb * L2 * sizeof(float)));
+  CHK(c->n1_keys.ensure((size_t)nb * norm1_key_bytes(L)));
   const int cap = 1 << 22;
   CHK(c->n1_rec.ensure((size_t)cap * sizeof(N1Rec)));
   CHK(c->sel_count.ensure(sizeof(int)));
@@ -759,11 +760,11 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
     for (int m0 = 0; m0 < nm; m0 += nb) {
       const int cnt = std::min(nb, nm - m0);
       run_pass1(m0, cnt);
-      launch_norm1_zmap(L, c->G[0].as<float2>(), maps + m0, cnt, c->n1_avg.as<float>(),
-                        c->n1_sd.as<float>(), c->n1_z.as<float>(), c->twM.as<float2>(), s);
-      launch_norm1_blockmax(c->n1_z.as<float>(), L, cnt, (ip - psi_begin) * nm, c->T, maps + m0,
-                            c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
-                            c->sel_count.as<int>(), c->n1_rec.p, cap, s);
+      launch_norm1_zrun(L, c->G[0].as<float2>(), maps + m0, cnt, c->n1_avg.as<float>(),
+                        c->n1_sd.as<float>(), c->n1_keys.p, c->twM.as<float2>(), s);
+      launch_norm1_runfin(L, c->n1_keys.p, cnt, (ip - psi_begin) * nm, c->T, maps + m0,
+                          c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
+                          c->sel_count.as<int>(), c->n1_rec.p, cap, s);
     }
   }
   int count = 0;

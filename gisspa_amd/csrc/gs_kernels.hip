@@ -722,38 +722,114 @@ __global__ __launch_bounds__(WG) void k_norm1_acc(const float2* __restrict__ Gin
   }
 }
 
-// z-score maps of a batch (update_CCG): map m's z values at z + m * L^2
+// Ordered key of a run candidate: larger z first, then the lower pixel index
+// (vmax_combine's rule as one unsigned 64-bit maximum); +0 and -0 share a key.
+__device__ __forceinline__ unsigned long long run_key(float v, int pos) {
+  v = v == 0.0f ? 0.0f : v;
+  const unsigned u = __float_as_uint(v);
+  const unsigned o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((unsigned long long)o << 32) | (unsigned)(~pos);
+}
+__device__ __forceinline__ float run_key_value(unsigned long long k) {
+  const unsigned o = (unsigned)(k >> 32);
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// z-scores of a batch (update_CCG, GPU_func.cu:649-662) reduced on the fly to
+// the maximum of each run of 1024 pixels (get_peak_pos, GPU_func.cu:711-745)
+// without writing the z-map: each thread keeps a running maximum of the run
+// it is in (its outputs come in increasing pixel order) and folds it into
+// the workgroup's per-run keys with an LDS atomic maximum when the run
+// changes.  A run spans at most two row blocks (RB * L >= 1024); the row
+// block holding its first pixel writes slot 0 of keys[m][run], the next one
+// slot 1.
 template <int L>
-__global__ __launch_bounds__(WG) void k_norm1_zmap(const float2* __restrict__ Gin,
+__global__ __launch_bounds__(WG) void k_norm1_zrun(const float2* __restrict__ Gin,
                                                    const MapRef* __restrict__ maps,
                                                    const float* __restrict__ avg,
                                                    const float* __restrict__ sd,
-                                                   float* __restrict__ zout,
+                                                   unsigned long long* __restrict__ keys,
                                                    const float2* __restrict__ twM) {
   using G = Geo<L>;
   constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
   constexpr long L2 = (long)L * L;
+  constexpr int NBLK = (int)((L2 + 1023) / 1024);
+  constexpr int NRUN = RB * L / 1024 + 2;
+  static_assert(RB * L >= 1024, "a run spans at most two row blocks");
   __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  __shared__ unsigned long long rk[NRUN];
   const int m = blockIdx.y;
   const int y0 = blockIdx.x * RB;
+  const long p0 = (long)y0 * L, p1 = (long)min(y0 + RB, L) * L;
+  const int k0 = (int)(p0 >> 10), k1 = (int)((p1 - 1) >> 10);
+  for (int i = threadIdx.x; i <= k1 - k0; i += WG) rk[i] = 0ull;
+  __syncthreads();
   const int t = maps[m].tile;
   const float2* gb = Gin + (long)m * NCB * L * CB;
-  // Z'[k] of row y, pre-processed by pass 1
   auto load = [&](int r, int k) -> float2 { return gb[pair_slot<L>(k, min(y0 + r, L - 1))]; };
   const float inv = 1.0f / (float)L2;
+  int cur = -1, cp = 0;
+  float cv = 0.0f;
+  auto visit = [&](float v, int q) {
+    const int k = q >> 10;
+    if (k != cur) {
+      if (cur >= 0) atomicMax(&rk[cur - k0], run_key(cv, cp));
+      cur = k; cv = v; cp = q;
+    } else if (v > cv || (v == cv && q < cp)) {
+      cv = v; cp = q;
+    }
+  };
   auto store = [&](int r, int n, float2 z) {
     const int y = y0 + r;
     if (y >= L) return;
-    const long pix = (long)y * L + 2 * n;
+    const int pix = y * L + 2 * n;
     const float* av = avg + t * L2 + pix;
     const float* sv = sd + t * L2 + pix;
     const float a = z.x * inv, b = z.y * inv;
-    float2 o;
-    o.x = sv[0] > 0 ? (a - av[0]) / sv[0] : 0.0f;
-    o.y = sv[1] > 0 ? (b - av[1]) / sv[1] : 0.0f;
-    *reinterpret_cast<float2*>(zout + (long)m * L2 + pix) = o;
+    visit(sv[0] > 0 ? (a - av[0]) / sv[0] : 0.0f, pix);
+    visit(sv[1] > 0 ? (b - av[1]) / sv[1] : 0.0f, pix + 1);
   };
   fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+  if (cur >= 0) atomicMax(&rk[cur - k0], run_key(cv, cp));
+  __syncthreads();
+  for (int i = threadIdx.x; i <= k1 - k0; i += WG) {
+    const int k = k0 + i;
+    const int slot = ((long)k << 10) >= p0 ? 0 : 1;
+    keys[((long)m * NBLK + k) * 2 + slot] = rk[i];
+  }
+}
+
+// Per run: combine the slots of the row blocks it spans, then the threshold
+// and centre bounds of the host loop (main.cu:796-819); records carry the
+// map's canonical index mbase + view * ntiles + tile.
+template <int L>
+__global__ void k_norm1_runfin(const unsigned long long* __restrict__ keys, int mbase,
+                               int ntiles, const MapRef* __restrict__ maps,
+                               const TileDesc* __restrict__ tdesc, float thres, float dmin,
+                               int* __restrict__ count, N1Rec* __restrict__ out, int cap) {
+  constexpr long L2 = (long)L * L;
+  constexpr int NBLK = (int)((L2 + 1023) / 1024);
+  constexpr int RB = Geo<L>::RB;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (k >= NBLK) return;
+  const long first = (long)k << 10, last = min(first + 1023, L2 - 1);
+  unsigned long long key = keys[((long)m * NBLK + k) * 2];
+  if ((first / L) / RB != (last / L) / RB) key = max(key, keys[((long)m * NBLK + k) * 2 + 1]);
+  const float v = run_key_value(key);
+  const int p = (int)(~(unsigned)key);
+  if (!(v >= thres)) return;
+  const TileDesc td = tdesc[maps[m].tile];
+  const int cx = td.ox + p % L, cy = td.oy + p / L;
+  if (cx >= dmin && cx < td.nx - dmin && cy >= dmin && cy < td.ny - dmin) {
+    const int i = atomicAdd(count, 1);
+    if (i < cap) {
+      N1Rec r;
+      r.m = mbase + maps[m].view * ntiles + maps[m].tile;
+      r.block = k; r.pos = p; r.z = v;
+      out[i] = r;
+    }
+  }
 }
 
 // set_CCG_mean (GPU_func.cu:635-646): mean and standard deviation per pixel
@@ -765,50 +841,6 @@ __global__ void k_norm1_stats(const double* __restrict__ s1, const double* __res
   const double a = s1[i] / total;
   avg[i] = (float)a;
   sd[i] = (float)sqrt(s2[i] / total - a * a);
-}
-
-// get_peak_pos (GPU_func.cu:711-745) + host loop (main.cu:796-819): maximum
-// of each run of 1024 pixels of a z-map (lowest index on ties); runs above
-// the threshold whose centre is at least `dmin` from the micrograph edges
-// are appended to `out` with the map's canonical index mbase + view *
-// ntiles + tile (order restored on the host).
-
-__global__ __launch_bounds__(256) void k_norm1_blockmax(const float* __restrict__ z, long L2,
-                                                        int nblk, int L, int mbase, int ntiles,
-                                                        const MapRef* maps,
-                                                        const TileDesc* __restrict__ tdesc,
-                                                        float thres, float dmin,
-                                                        int* __restrict__ count,
-                                                        N1Rec* __restrict__ out, int cap) {
-  __shared__ float rv[4];
-  __shared__ int rp[4];
-  const int m = blockIdx.y, b = blockIdx.x;
-  const long beg = (long)b * 1024;
-  float v = -INFINITY;
-  int p = INT_MAX;
-  for (int i = threadIdx.x; i < 1024; i += 256) {
-    const long k = beg + i;
-    if (k < L2) vmax_combine(v, p, z[(long)m * L2 + k], (int)k);
-  }
-  wave_max_idx(v, p);
-  if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = v; rp[threadIdx.x >> 6] = p; }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  v = rv[0]; p = rp[0];
-  for (int i = 1; i < 4; ++i) vmax_combine(v, p, rv[i], rp[i]);
-  if (!(v >= thres)) return;
-  const TileDesc td = tdesc[maps[m].tile];
-  const int cx = td.ox + p % L, cy = td.oy + p / L;
-  if (cx >= dmin && cx < td.nx - dmin && cy >= dmin && cy < td.ny - dmin) {
-    const int i = atomicAdd(count, 1);
-    if (i < cap) {
-      // the map's canonical (view-major) index, whatever the processing order
-      N1Rec r;
-      r.m = mbase + maps[m].view * ntiles + maps[m].tile;
-      r.block = b; r.pos = p; r.z = v;
-      out[i] = r;
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1393,27 +1425,36 @@ void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, in
   DISPATCH(L, norm1_acc_t, G, maps, ntiles, nv, s1, s2, twM, s);
 }
 template <int L>
-static void norm1_zmap_t(const float2* G, const MapRef* maps, int nmaps, const float* avg,
-                         const float* sd, float* z, const float2* twM, hipStream_t s) {
+static void norm1_zrun_t(const float2* G, const MapRef* maps, int nmaps, const float* avg,
+                         const float* sd, unsigned long long* keys, const float2* twM,
+                         hipStream_t s) {
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
-  hipLaunchKernelGGL(k_norm1_zmap<L>, grid, dim3(WG), 0, s, G, maps, avg, sd, z, twM);
+  hipLaunchKernelGGL(k_norm1_zrun<L>, grid, dim3(WG), 0, s, G, maps, avg, sd, keys, twM);
 }
-void launch_norm1_zmap(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
-                       const float* sd, float* z, const float2* twM, hipStream_t s) {
-  DISPATCH(L, norm1_zmap_t, G, maps, nmaps, avg, sd, z, twM, s);
+void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
+                       const float* sd, void* keys, const float2* twM, hipStream_t s) {
+  DISPATCH(L, norm1_zrun_t, G, maps, nmaps, avg, sd, static_cast<unsigned long long*>(keys), twM,
+           s);
 }
+template <int L>
+static void norm1_runfin_t(const unsigned long long* keys, int nmaps, int mbase, int ntiles,
+                           const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
+                           int* count, N1Rec* out, int cap, hipStream_t s) {
+  const int nblk = (int)(((long)L * L + 1023) / 1024);
+  hipLaunchKernelGGL(k_norm1_runfin<L>, dim3((nblk + 255) / 256, nmaps), dim3(256), 0, s, keys,
+                     mbase, ntiles, maps, tdesc, thres, dmin, count, out, cap);
+}
+void launch_norm1_runfin(int L, const void* keys, int nmaps, int mbase, int ntiles,
+                         const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
+                         int* count, void* out, int cap, hipStream_t s) {
+  DISPATCH(L, norm1_runfin_t, static_cast<const unsigned long long*>(keys), nmaps, mbase, ntiles,
+           maps, tdesc, thres, dmin, count, static_cast<N1Rec*>(out), cap, s);
+}
+long norm1_key_bytes(int L) { return (((long)L * L + 1023) / 1024) * 2 * 8; }
 void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
                         float* sd, hipStream_t s) {
   hipLaunchKernelGGL(k_norm1_stats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, s1, s2, n,
                      total, avg, sd);
-}
-void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, int ntiles,
-                           const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
-                           int* count, void* out, int cap, hipStream_t s) {
-  const long L2 = (long)L * L;
-  const int nblk = (int)((L2 + 1023) / 1024);
-  hipLaunchKernelGGL(k_norm1_blockmax, dim3(nblk, nmaps), dim3(256), 0, s, z, L2, nblk, L, mbase,
-                     ntiles, maps, tdesc, thres, dmin, count, static_cast<N1Rec*>(out), cap);
 }
 
 __global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T, int N, int L,

@@ -141,20 +141,22 @@ void launch_corr_finalize(int L, const CorrPartial* part, const float2* psum, in
 // batch of ntiles x nv maps (maps t * nv .. t * nv + nv - 1 share a tile)
 void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, int nv, double* s1,
                       double* s2, const float2* twM, hipStream_t s);
-// z[m][pixel] of a batch of maps
-void launch_norm1_zmap(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
-                       const float* sd, float* z, const float2* twM, hipStream_t s);
 void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
                         float* sd, hipStream_t s);
+// per-run maxima of the z-scores of a batch of maps, without z-maps:
+// keys[m][run][2] (norm1_key_bytes(L) bytes per map)
+void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
+                       const float* sd, void* keys, const float2* twM, hipStream_t s);
+long norm1_key_bytes(int L);
 // records {int m, block, pos; float z} of 1024-pixel runs above threshold;
 // m = mbase + view * ntiles + tile (canonical view-major map index)
 struct N1Rec {
   int m, block, pos;
   float z;
 };
-void launch_norm1_blockmax(const float* z, int L, int nmaps, int mbase, int ntiles,
-                           const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
-                           int* count, void* out, int cap, hipStream_t s);
+void launch_norm1_runfin(int L, const void* keys, int nmaps, int mbase, int ntiles,
+                         const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
+                         int* count, void* out, int cap, hipStream_t s);
 // threshold + centre-bounds selection of norm_type 0 maps (main.cu:871-891)
 void launch_select(const MapStat* st, long long nmaps, int T, int N, int L, const TileDesc* tdesc,
                    float thres, float dmin, int* count, SelRec* out, int cap, hipStream_t s);
