@@ -97,7 +97,8 @@ struct Ctx {
   int tile_block = 0;  // tiles per block of the map order; 0: as many as fit in 192 MiB
   int maps_T = -1, maps_N = -1, maps_tb = -1, maps_nv = -1;
   int n1_nv = 1;   // views per norm_type 1 batch (maps order)
-  DevBuf n1_s1, n1_s2, n1_avg, n1_sd, n1_keys, n1_rec;  // norm_type 1
+  bool batch_set = false;   // gs_set_batch chose the batch size
+  DevBuf n1_s1, n1_s2, n1_norm, n1_keys, n1_rec;  // norm_type 1
   // plain mode buffers
   DevBuf pl_tspec, pl_tiles, pl_ccg, pl_part, pl_rotpart, pl_mean, pl_var;
   // per-psi graphs of the fused correlation (gs_set_graphs)
@@ -539,7 +540,11 @@ int ensure_corr_buffers(Ctx* c) {
   const long tile_bytes = (long)(L / 2 + 1) * L * (long)sizeof(float2);
   const int tb = c->tile_block > 0 ? c->tile_block
                                    : (int)std::max(1l, (192l << 20) / tile_bytes);
-  const int nv1 = std::max(1, std::min(c->N, nb / c->T));
+  // norm_type 1 batches: nv views of every tile; by default about 4 GiB of
+  // intermediate (23 views at L = 720), so that the per-batch flush of the
+  // FP64 accumulators is shared by many maps
+  const long nb1 = c->batch_set ? nb : std::max(1l, (4096l << 20) / per_map);
+  const int nv1 = (int)std::max(1l, std::min((long)c->N, nb1 / c->T));
   if (c->maps_T != c->T || c->maps_N != c->N || c->maps_tb != tb || c->maps_nv != nv1) {
     std::vector<MapRef> mr, mb;
     mr.reserve(nm);
@@ -728,8 +733,7 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   CHK(c->psum.ensure((size_t)nm * corr_ncb(L) * sizeof(float2)));
   CHK(c->n1_s1.ensure(npix * sizeof(double)));
   CHK(c->n1_s2.ensure(npix * sizeof(double)));
-  CHK(c->n1_avg.ensure(npix * sizeof(float)));
-  CHK(c->n1_sd.ensure(npix * sizeof(float)));
+  CHK(c->n1_norm.ensure(npix * 2 * sizeof(float)));
   CHK(c->n1_keys.ensure((size_t)nb * norm1_key_bytes(L)));
   const int cap = 1 << 22;
   CHK(c->n1_rec.ensure((size_t)cap * sizeof(N1Rec)));
@@ -753,15 +757,15 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
     }
   }
   launch_norm1_stats(c->n1_s1.as<double>(), c->n1_s2.as<double>(), (long)npix, (double)c->N * P,
-                     c->n1_avg.as<float>(), c->n1_sd.as<float>(), s);
+                     c->n1_norm.p, s);
   HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), s));
   for (int ip = psi_begin; ip < psi_end; ++ip) {
     CHK(rotate_templates(c, ip));
     for (int m0 = 0; m0 < nm; m0 += nb) {
       const int cnt = std::min(nb, nm - m0);
       run_pass1(m0, cnt);
-      launch_norm1_zrun(L, c->G[0].as<float2>(), maps + m0, cnt, c->n1_avg.as<float>(),
-                        c->n1_sd.as<float>(), c->n1_keys.p, c->twM.as<float2>(), s);
+      launch_norm1_zrun(L, c->G[0].as<float2>(), maps + m0, cnt, c->n1_norm.p, c->n1_keys.p,
+                        c->twM.as<float2>(), s);
       launch_norm1_runfin(L, c->n1_keys.p, cnt, (ip - psi_begin) * nm, c->T, maps + m0,
                           c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
                           c->sel_count.as<int>(), c->n1_rec.p, cap, s);
@@ -1202,6 +1206,7 @@ int gs_set_batch(void* ctx, int maps_per_batch) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipDeviceSynchronize());
   c->batch = maps_per_batch;
+  c->batch_set = maps_per_batch > 0;
   drop_graphs(c);
   for (int i = 0; i < Ctx::NLANE; ++i) c->G[i].release();
   return GS_OK;

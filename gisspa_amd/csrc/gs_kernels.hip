@@ -735,7 +735,8 @@ __device__ __forceinline__ float run_key_value(unsigned long long k) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
 }
 
-// z-scores of a batch (update_CCG, GPU_func.cu:649-662) reduced on the fly to
+// z-scores of a batch (update_CCG, GPU_func.cu:649-662: (c/L^2 - mean) / sd,
+// here times the stored 1/sd) reduced on the fly to
 // the maximum of each run of 1024 pixels (get_peak_pos, GPU_func.cu:711-745)
 // without writing the z-map: each thread keeps a running maximum of the run
 // it is in (its outputs come in increasing pixel order) and folds it into
@@ -746,8 +747,7 @@ __device__ __forceinline__ float run_key_value(unsigned long long k) {
 template <int L>
 __global__ __launch_bounds__(WG) void k_norm1_zrun(const float2* __restrict__ Gin,
                                                    const MapRef* __restrict__ maps,
-                                                   const float* __restrict__ avg,
-                                                   const float* __restrict__ sd,
+                                                   const float4* __restrict__ norm,
                                                    unsigned long long* __restrict__ keys,
                                                    const float2* __restrict__ twM) {
   using G = Geo<L>;
@@ -783,11 +783,9 @@ __global__ __launch_bounds__(WG) void k_norm1_zrun(const float2* __restrict__ Gi
     const int y = y0 + r;
     if (y >= L) return;
     const int pix = y * L + 2 * n;
-    const float* av = avg + t * L2 + pix;
-    const float* sv = sd + t * L2 + pix;
-    const float a = z.x * inv, b = z.y * inv;
-    visit(sv[0] > 0 ? (a - av[0]) / sv[0] : 0.0f, pix);
-    visit(sv[1] > 0 ? (b - av[1]) / sv[1] : 0.0f, pix + 1);
+    const float4 q = norm[(t * L2 + pix) >> 1];   // {mean, mean, 1/sd, 1/sd} of the pair
+    visit((z.x * inv - q.x) * q.z, pix);
+    visit((z.y * inv - q.y) * q.w, pix + 1);
   };
   fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
   if (cur >= 0) atomicMax(&rk[cur - k0], run_key(cv, cp));
@@ -832,15 +830,21 @@ __global__ void k_norm1_runfin(const unsigned long long* __restrict__ keys, int 
   }
 }
 
-// set_CCG_mean (GPU_func.cu:635-646): mean and standard deviation per pixel
+// set_CCG_mean (GPU_func.cu:635-646): mean and standard deviation per pixel,
+// stored per pixel pair as {mean, mean, 1/sd, 1/sd} (0 where sd is not
+// positive: update_CCG's z is 0 there)
 __global__ void k_norm1_stats(const double* __restrict__ s1, const double* __restrict__ s2,
-                              long n, double total, float* __restrict__ avg,
-                              float* __restrict__ sd) {
+                              long npairs, double total, float4* __restrict__ norm) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double a = s1[i] / total;
-  avg[i] = (float)a;
-  sd[i] = (float)sqrt(s2[i] / total - a * a);
+  if (i >= npairs) return;
+  float r[4];
+  for (int h = 0; h < 2; ++h) {
+    const double a = s1[2 * i + h] / total;
+    const double sd = sqrt(s2[2 * i + h] / total - a * a);
+    r[h] = (float)a;
+    r[2 + h] = sd > 0 ? (float)(1.0 / sd) : 0.0f;
+  }
+  norm[i] = make_float4(r[0], r[1], r[2], r[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1425,16 +1429,15 @@ void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, in
   DISPATCH(L, norm1_acc_t, G, maps, ntiles, nv, s1, s2, twM, s);
 }
 template <int L>
-static void norm1_zrun_t(const float2* G, const MapRef* maps, int nmaps, const float* avg,
-                         const float* sd, unsigned long long* keys, const float2* twM,
-                         hipStream_t s) {
+static void norm1_zrun_t(const float2* G, const MapRef* maps, int nmaps, const void* norm,
+                         unsigned long long* keys, const float2* twM, hipStream_t s) {
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
-  hipLaunchKernelGGL(k_norm1_zrun<L>, grid, dim3(WG), 0, s, G, maps, avg, sd, keys, twM);
+  hipLaunchKernelGGL(k_norm1_zrun<L>, grid, dim3(WG), 0, s, G, maps,
+                     static_cast<const float4*>(norm), keys, twM);
 }
-void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
-                       const float* sd, void* keys, const float2* twM, hipStream_t s) {
-  DISPATCH(L, norm1_zrun_t, G, maps, nmaps, avg, sd, static_cast<unsigned long long*>(keys), twM,
-           s);
+void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, const void* norm,
+                       void* keys, const float2* twM, hipStream_t s) {
+  DISPATCH(L, norm1_zrun_t, G, maps, nmaps, norm, static_cast<unsigned long long*>(keys), twM, s);
 }
 template <int L>
 static void norm1_runfin_t(const unsigned long long* keys, int nmaps, int mbase, int ntiles,
@@ -1451,10 +1454,11 @@ void launch_norm1_runfin(int L, const void* keys, int nmaps, int mbase, int ntil
            maps, tdesc, thres, dmin, count, static_cast<N1Rec*>(out), cap, s);
 }
 long norm1_key_bytes(int L) { return (((long)L * L + 1023) / 1024) * 2 * 8; }
-void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
-                        float* sd, hipStream_t s) {
-  hipLaunchKernelGGL(k_norm1_stats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, s1, s2, n,
-                     total, avg, sd);
+void launch_norm1_stats(const double* s1, const double* s2, long n, double total, void* norm,
+                        hipStream_t s) {
+  const long npairs = n / 2;
+  hipLaunchKernelGGL(k_norm1_stats, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s, s1,
+                     s2, npairs, total, static_cast<float4*>(norm));
 }
 
 __global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T, int N, int L,

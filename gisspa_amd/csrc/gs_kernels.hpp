@@ -141,12 +141,13 @@ void launch_corr_finalize(int L, const CorrPartial* part, const float2* psum, in
 // batch of ntiles x nv maps (maps t * nv .. t * nv + nv - 1 share a tile)
 void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, int nv, double* s1,
                       double* s2, const float2* twM, hipStream_t s);
-void launch_norm1_stats(const double* s1, const double* s2, long n, double total, float* avg,
-                        float* sd, hipStream_t s);
+// per pixel pair of n (even) pixels: {mean, mean, 1/sd, 1/sd} (float4)
+void launch_norm1_stats(const double* s1, const double* s2, long n, double total, void* norm,
+                        hipStream_t s);
 // per-run maxima of the z-scores of a batch of maps, without z-maps:
 // keys[m][run][2] (norm1_key_bytes(L) bytes per map)
-void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, const float* avg,
-                       const float* sd, void* keys, const float2* twM, hipStream_t s);
+void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, const void* norm,
+                       void* keys, const float2* twM, hipStream_t s);
 long norm1_key_bytes(int L);
 // records {int m, block, pos; float z} of 1024-pixel runs above threshold;
 // m = mbase + view * ntiles + tile (canonical view-major map index)
