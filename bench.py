@@ -189,6 +189,60 @@ def norm1_rate(templates, d_img, nx, ny, defocus, n_views: int):
                        f"passes) in {t:.2f}s, scaled by the view count")
 
 
+def io_rate(templates, image, cache_dir: str, n_lines: int = 100000, seed: int = 5):
+    """Host-side formats around the path: reading the template stack and the
+    micrograph through gs_read_image as MRC and as EMAN2 HDF5 (contiguous;
+    chunked + deflate), bit-checked against the arrays written; the score
+    filter and the lst -> RELION STAR conversion of a synthetic pick list of
+    n_lines lines."""
+    import gisspa_amd
+    from gisspa_amd import lstconv, mrc
+    from tests import _hdf5 as H
+    d = os.path.join(cache_dir, "gisspa_io_bench")
+    os.makedirs(d, exist_ok=True)
+    out = {"reads": {}}
+    cases = [
+        ("templates, MRC stack", "t.mrcs", lambda p: mrc.write_mrc(p, templates), templates),
+        ("templates, EMAN2 HDF5 contiguous", "t.hdf",
+         lambda p: H.Writer(0).write(p, H.eman2_stack(list(templates))), templates),
+        ("templates, EMAN2 HDF5 deflate", "tz.hdf",
+         lambda p: H.Writer(0).write(p, H.eman2_stack(list(templates), layout="chunked",
+                                                      chunks=templates.shape[1:], deflate=1)),
+         templates),
+        ("micrograph, MRC", "m.mrc", lambda p: mrc.write_mrc(p, image[None]), image),
+        ("micrograph, EMAN1 HDF5", "m.hdf", lambda p: H.Writer(0).write(p, H.eman1_stack([image])),
+         image),
+    ]
+    for name, fn, write, ref in cases:
+        p = os.path.join(d, fn)
+        write(p)
+        t0 = time.time()
+        got = gisspa_amd.read_image(p, -1)
+        t = time.time() - t0
+        out["reads"][name] = {"seconds": t, "MB_per_s": ref.nbytes / 1e6 / t,
+                              "identical": bool(np.array_equal(got, ref))}
+        log(f"[bench] io: {name}: {ref.nbytes / 1e6:.0f} MB in {t * 1e3:.0f} ms "
+            f"({ref.nbytes / 1e6 / t:.0f} MB/s), identical {out['reads'][name]['identical']}")
+    rng = np.random.default_rng(seed)
+    lines = ["%d\t/data/mic%03d.mrc\tdefocus=%f\tdfdiff=%f\tdfang=%f\teuler=%f,%f,%f\t"
+             "center=%f,%f\tscore=%f\n" % (i % 300, i // 300, rng.uniform(1, 3), 0.0, 0.0,
+                                             rng.uniform(0, 180), rng.uniform(0, 360),
+                                             rng.uniform(0, 360), rng.uniform(0, 5760),
+                                             rng.uniform(0, 4092), rng.uniform(5, 20))
+             for i in range(n_lines)]
+    t0 = time.time()
+    kept = lstconv.delete_bad_score(lines, 10.0)
+    t_f = time.time() - t0
+    t0 = time.time()
+    star = lstconv.lst_to_star(lines, 1, 1.0, "/data/")
+    t_s = time.time() - t0
+    out["lst"] = {"lines": n_lines, "score_filter_s": t_f, "kept": len(kept),
+                  "lst_to_star_s": t_s, "star_bytes": len(star)}
+    log(f"[bench] io: score filter of {n_lines} lines {t_f * 1e3:.0f} ms ({len(kept)} kept), "
+        f"lst -> STAR {t_s * 1e3:.0f} ms")
+    return out
+
+
 def dedupe_rate(picks, n_views: int, no_cpu: bool, center: float = 20.0,
                 euler_thres: float = 10.0):
     """Duplicate removal (gs_remove_repeats) of the run's pick list, one
@@ -488,6 +542,9 @@ def main():
     ap.add_argument("--norm1", type=int, default=0, metavar="VIEWS",
                     help="also time the norm_type 1 picker with the first VIEWS views, scaled "
                          "to all (0 = skip)")
+    ap.add_argument("--io", type=int, default=0,
+                    help="also time the host-side formats: MRC / HDF5 reading, score filter, "
+                         "lst -> STAR (0 = skip)")
     ap.add_argument("--rocfft", type=int, default=1,
                     help="time rocFFT's inverse 2D transforms of the same size (0 = skip)")
     ap.add_argument("--project", type=int, default=1,
@@ -671,6 +728,8 @@ def main():
             + (f"; refine {stress['refine']['candidates']} in {stress['refine']['seconds']:.1f}s"
                if stress.get("refine") else ""))
 
+    io = io_rate(templates, image, args.cache) if args.io else None
+
     cli = None
     if args.cli:
         cli = cli_rate(templates, image, defocus, args.cli, args.cache)
@@ -736,6 +795,7 @@ def main():
         "stress": stress,
         "cli": cli,
         "norm1": norm1,
+        "io": io,
         "rocfft": rocfft,
         "project": proj,
         "plain": plain,

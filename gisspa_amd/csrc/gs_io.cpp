@@ -336,19 +336,32 @@ int read_mrc(const std::string& path, int n, Image& img, std::string* err) {
   else { nz = nzf; startz = 0; }
   const int bpp = (mode == 0) ? 1 : (mode == 1 ? 2 : 4);
   const size_t npix = (size_t)nx * ny * nz;
+  if (fseeko(fp, (off_t)1024 + nsym + (off_t)startz * nx * ny * bpp, SEEK_SET) != 0) {
+    if (err) *err = "Incomplete data read " + path;
+    fclose(fp);
+    return -1;
+  }
+  img.nx = nx; img.ny = ny; img.nz = nz;
+  img.data.resize(npix);
+  if (mode == 2 && !swap) {   // native floats: straight into the image
+    const size_t got = fread(img.data.data(), sizeof(float), npix, fp);
+    fclose(fp);
+    if (got != npix) {
+      if (err) *err = "Incomplete data read " + path;
+      return -1;
+    }
+    return 0;
+  }
   std::vector<unsigned char> raw(npix * bpp);
-  if (fseeko(fp, (off_t)1024 + nsym + (off_t)startz * nx * ny * bpp, SEEK_SET) != 0 ||
-      fread(raw.data(), 1, raw.size(), fp) != raw.size()) {
+  if (fread(raw.data(), 1, raw.size(), fp) != raw.size()) {
     if (err) *err = "Incomplete data read " + path;
     fclose(fp);
     return -1;
   }
   fclose(fp);
-  img.nx = nx; img.ny = ny; img.nz = nz;
-  img.data.resize(npix);
   for (size_t i = 0; i < npix; ++i) {
     if (mode == 0) {
-      img.data[i] = (float)raw[i] / 100.0f - 1.28f;  // emdata_.cpp:510
+      img.data[i] = (float)((double)raw[i] / 100.0 - 1.28);  // emdata_.cpp:503, in double
     } else if (mode == 1) {
       unsigned char b[2] = {raw[2 * i], raw[2 * i + 1]};
       if (swap) std::swap(b[0], b[1]);

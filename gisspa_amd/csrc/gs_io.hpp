@@ -52,8 +52,10 @@ struct Image {
   int nx = 0, ny = 0, nz = 0;
   std::vector<float> data;
 };
-// readMRC (EMReader/emdata_.cpp:384-617): modes 0, 1, 2, 6, either byte
-// order; section n of a stack (n < 0 or out of range: whole file).
+// readMRC (EMReader/emdata_.cpp:384-617): modes 0, 1 and 2 (modes above 4
+// rejected as there, emdata_.cpp:457; complex modes 3 and 4 are not
+// images), either byte order; section n of a stack (n < 0 or out of range:
+// whole file).
 int read_mrc(const std::string& path, int n, Image& img, std::string* err);
 // number of sections
 int mrc_sections(const std::string& path);

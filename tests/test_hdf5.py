@@ -224,3 +224,41 @@ def test_cli_template_count_from_hdf(tmp_path):
     r = subprocess.run([gisspa_amd.MAIN_PATH, str(cfg)], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "Error" in r.stderr
+
+
+def _mrc_bytes(data: np.ndarray, mode: int, big: bool) -> bytes:
+    """An MRC file: 1024-byte header (nx, ny, nz, mode, ..., nlabl = 0) and
+    the data, both in the given byte order."""
+    nz, ny, nx = data.shape
+    e = ">" if big else "<"
+    hdr = np.zeros(256, dtype=e + "i4")
+    hdr[0:4] = (nx, ny, nz, mode)
+    dt = {0: "u1", 1: e + "i2", 2: e + "f4", 6: e + "u2"}[mode]
+    return hdr.tobytes() + np.ascontiguousarray(data).astype(dt).tobytes()
+
+
+@pytest.mark.parametrize("big", [False, True])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_mrc_modes_and_byte_orders(tmp_path, mode, big):
+    """readMRC's conversions (EMReader/emdata_.cpp:494-560): mode 0 bytes as
+    b / 100.0 - 1.28 evaluated in double, mode 1 signed 16-bit, mode 2
+    float; big-endian files detected from the header and swapped."""
+    rng = np.random.default_rng(mode + 10 * big)
+    if mode == 0:
+        raw = rng.integers(0, 256, (3, 9, 11)).astype(np.uint8)
+        want = (raw.astype(np.float64) / 100.0 - 1.28).astype(np.float32)
+    elif mode == 1:
+        raw = rng.integers(-32768, 32768, (3, 9, 11)).astype(np.int16)
+        want = raw.astype(np.float32)
+    else:
+        raw = rng.standard_normal((3, 9, 11)).astype(np.float32)
+        want = raw
+    p = tmp_path / "x.mrc"
+    p.write_bytes(_mrc_bytes(raw, mode, big))
+    assert gisspa_amd.image_count(str(p)) == 3
+    assert np.array_equal(gisspa_amd.read_image(str(p), -1), want)
+    assert np.array_equal(gisspa_amd.read_image(str(p), 2), want[2])
+    bad = tmp_path / "bad.mrc"
+    bad.write_bytes(_mrc_bytes(raw, 6, big))   # modes above 4 are refused (emdata_.cpp:457)
+    with pytest.raises(OSError):
+        gisspa_amd.read_image(str(bad), 0)
