@@ -710,6 +710,13 @@ bool File::read(uint64_t obj, std::vector<float>& out, std::vector<uint64_t>* di
   }
   if (cls == 1) {   // contiguous
     if (undefined(addr, so_) || n == 0) return true;   // never written: fill value 0
+    if (t.cls == 1 && t.size == 4 && !t.big) {   // native floats: straight into the output
+      const uint64_t a = base_ + addr;
+      if (a > size_ || n * 4 > size_ - a || fseeko(fp_, (off_t)a, SEEK_SET) != 0 ||
+          fread(out.data(), 4, n, fp_) != n)
+        return fail(err, "dataset data outside the file");
+      return true;
+    }
     const uint64_t step = std::max<uint64_t>(1, (64ull << 20) / es);
     std::vector<uint8_t> buf;
     for (uint64_t k = 0; k < n; k += step) {

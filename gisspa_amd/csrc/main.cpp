@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,34 @@ using namespace gsio;
 static int fail(const char* what, int code) {
   fprintf(stderr, "Error: %s: %s (%d)\n", what, gs_strerror(code), code);
   return 1;
+}
+
+// One group of consecutive micrographs read from disk: their lst entries and
+// images, or the first error met (the lst code of readRawImage, or a file
+// error message).
+struct MicGroup {
+  std::vector<MicEntry> es;
+  std::vector<Image> ims;
+  int lst_error = 0;
+  std::string err;
+};
+
+static MicGroup load_group(const Config& cfg, int n0, int k) {
+  MicGroup g;
+  g.es.resize(k);
+  g.ims.resize(k);
+  for (int i = 0; i < k; ++i) {
+    const int r = read_lst_entry(cfg.input, n0 + i, g.es[i]);
+    if (r < 0) {
+      g.lst_error = r;
+      return g;
+    }
+    if (read_image(g.es[i].path, g.es[i].nn, g.ims[i], &g.err)) {
+      if (g.err.empty()) g.err = "cannot read " + g.es[i].path;
+      return g;
+    }
+  }
+  return g;
 }
 
 int main(int argc, char** argv) {
@@ -113,31 +142,40 @@ int main(int argc, char** argv) {
   // template rotation serves all of them.  Picks come back micrograph by
   // micrograph, and the lines and "finished" messages follow the order of
   // the reference's one-image loop (main.cu:1149-1269).
+  // The next group is read from disk while the GPU searches the current one.
   std::vector<gs_particle> picks(1 << 16);
+  auto group_size = [&](int n0) { return std::min(cfg.mic_batch, cfg.last - n0); };
+  std::future<MicGroup> next;
+  if (cfg.first < cfg.last)
+    next = std::async(std::launch::async, load_group, std::cref(cfg), cfg.first,
+                      group_size(cfg.first));
   for (int n0 = cfg.first; n0 < cfg.last; n0 += cfg.mic_batch) {
-    const int k = std::min(cfg.mic_batch, cfg.last - n0);
-    std::vector<MicEntry> es(k);
-    std::vector<Image> ims(k);
+    const int k = group_size(n0);
+    MicGroup g = next.get();
+    if (g.lst_error < 0) {
+      printf("Error %d occured in readRawImage!\n", g.lst_error);
+      return 1;
+    }
+    if (!g.err.empty()) {
+      fprintf(stderr, "Error: %s\n", g.err.c_str());
+      return 1;
+    }
     std::vector<const float*> ptr(k);
     std::vector<int> nxs(k), nys(k);
     std::vector<gs_ctf> ctfs(k);
     for (int i = 0; i < k; ++i) {
-      if ((r = read_lst_entry(cfg.input, n0 + i, es[i])) < 0) {
-        printf("Error %d occured in readRawImage!\n", r);
-        return 1;
-      }
-      if (read_image(es[i].path, es[i].nn, ims[i], &err)) {
-        fprintf(stderr, "Error: %s\n", err.c_str());
-        return 1;
-      }
-      ptr[i] = ims[i].data.data();
-      nxs[i] = ims[i].nx;
-      nys[i] = ims[i].ny;
-      ctfs[i] = gs_ctf{es[i].defocus, es[i].dfdiff, es[i].dfang};
+      ptr[i] = g.ims[i].data.data();
+      nxs[i] = g.ims[i].nx;
+      nys[i] = g.ims[i].ny;
+      ctfs[i] = gs_ctf{g.es[i].defocus, g.es[i].dfdiff, g.es[i].dfang};
     }
     if ((r = gs_set_micrographs(ctx, k, ptr.data(), nxs.data(), nys.data(), ctfs.data(), 0)) !=
         GS_OK)
       return fail("gs_set_micrographs", r);
+    g.ims.clear();   // copied to the device
+    const int n1 = n0 + cfg.mic_batch;
+    if (n1 < cfg.last)
+      next = std::async(std::launch::async, load_group, std::cref(cfg), n1, group_size(n1));
     size_t np = 0;
     r = gs_pick(ctx, 0, -1, picks.data(), picks.size(), &np);
     if (r == GS_ERR_CAPACITY) {
@@ -147,7 +185,7 @@ int main(int argc, char** argv) {
     if (r != GS_OK) return fail("gs_pick", r);
     size_t i = 0;
     for (int m = 0; m < k; ++m) {
-      const MicEntry& e = es[m];
+      const MicEntry& e = g.es[m];
       for (; i < np && picks[i].mic == m; ++i) {
         const gs_particle& pk = picks[i];
         write_pick_line(fp, e.nn, e.path, e, euler.alt[pk.view], euler.az[pk.view], pk.psi, pk.cx,
