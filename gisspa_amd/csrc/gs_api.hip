@@ -728,48 +728,79 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   const int nv1 = c->n1_nv;
   const long per_map = c->geo.gmap * (long)sizeof(float2);
   const int nb = c->T * nv1;   // maps of a full batch
+  const int ncb = corr_ncb(L);
   const size_t npix = (size_t)c->T * L2;
-  CHK(c->G[0].ensure((size_t)nb * per_map));
-  CHK(c->psum.ensure((size_t)nm * corr_ncb(L) * sizeof(float2)));
+  for (int i = 0; i < Ctx::NLANE; ++i) CHK(c->G[i].ensure((size_t)nb * per_map));
+  CHK(c->psum.ensure((size_t)Ctx::NLANE * nb * ncb * sizeof(float2)));
   CHK(c->n1_s1.ensure(npix * sizeof(double)));
   CHK(c->n1_s2.ensure(npix * sizeof(double)));
   CHK(c->n1_norm.ensure(npix * 2 * sizeof(float)));
-  CHK(c->n1_keys.ensure((size_t)nb * norm1_key_bytes(L)));
+  CHK(c->n1_keys.ensure((size_t)Ctx::NLANE * nb * norm1_key_bytes(L)));
   const int cap = 1 << 22;
   CHK(c->n1_rec.ensure((size_t)cap * sizeof(N1Rec)));
   CHK(c->sel_count.ensure(sizeof(int)));
   hipStream_t s = c->stream;
-  if (!c->G[0].p || !c->maps.p || !c->psum.p) return GS_ERR_STATE;
+  if (!c->maps.p) return GS_ERR_STATE;
   HIPCHK(hipMemsetAsync(c->n1_s1.p, 0, npix * sizeof(double), s));
   HIPCHK(hipMemsetAsync(c->n1_s2.p, 0, npix * sizeof(double), s));
+  HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), s));
   const MapRef* maps = c->maps.as<MapRef>();
-  auto run_pass1 = [&](int m0, int cnt) {
+  // Batches alternate between the two lane streams, each with its own
+  // intermediate, partial sums and run keys, so that pass 1 of one batch runs
+  // beside the row kernel of the previous one.  The accumulations run in
+  // batch order: each waits for the previous batch's (ev_lane of the other
+  // lane), so every FP64 accumulator sees the same order of additions as on
+  // one stream.  The next psi's rotation waits for both lanes.
+  auto lanes_start = [&]() -> int {
+    HIPCHK(hipEventRecord(c->ev_ready, s));
+    for (int i = 0; i < Ctx::NLANE; ++i)
+      HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
+    return GS_OK;
+  };
+  auto lanes_join = [&]() -> int {
+    for (int i = 0; i < Ctx::NLANE; ++i) {
+      HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
+      HIPCHK(hipStreamWaitEvent(s, c->ev_lane[i], 0));
+    }
+    return GS_OK;
+  };
+  auto pass1 = [&](int m0, int cnt, int lane) {
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), maps + m0, cnt,
-                      c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(), s);
+                      c->G[lane].as<float2>(), c->psum.as<float2>() + (size_t)lane * nb * ncb,
+                      c->twL.as<float2>(), c->lane_stream[lane]);
   };
   for (int ip = 0; ip < P; ++ip) {
     CHK(rotate_templates(c, ip));
-    for (int m0 = 0; m0 < nm; m0 += nb) {
-      const int cnt = std::min(nb, nm - m0);
-      run_pass1(m0, cnt);
-      launch_norm1_acc(L, c->G[0].as<float2>(), maps + m0, c->T, cnt / c->T,
-                       c->n1_s1.as<double>(), c->n1_s2.as<double>(), c->twM.as<float2>(), s);
+    CHK(lanes_start());
+    for (int m0 = 0, k = 0; m0 < nm; m0 += nb, ++k) {
+      const int cnt = std::min(nb, nm - m0), lane = k % 2;
+      hipStream_t ls = c->lane_stream[lane];
+      pass1(m0, cnt, lane);
+      if (k > 0) HIPCHK(hipStreamWaitEvent(ls, c->ev_lane[1 - lane], 0));
+      launch_norm1_acc(L, c->G[lane].as<float2>(), maps + m0, c->T, cnt / c->T,
+                       c->n1_s1.as<double>(), c->n1_s2.as<double>(), c->twM.as<float2>(), ls);
+      HIPCHK(hipEventRecord(c->ev_lane[lane], ls));
     }
+    CHK(lanes_join());
   }
   launch_norm1_stats(c->n1_s1.as<double>(), c->n1_s2.as<double>(), (long)npix, (double)c->N * P,
                      c->n1_norm.p, s);
-  HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), s));
+  const long kb = norm1_key_bytes(L);
   for (int ip = psi_begin; ip < psi_end; ++ip) {
     CHK(rotate_templates(c, ip));
-    for (int m0 = 0; m0 < nm; m0 += nb) {
-      const int cnt = std::min(nb, nm - m0);
-      run_pass1(m0, cnt);
-      launch_norm1_zrun(L, c->G[0].as<float2>(), maps + m0, cnt, c->n1_norm.p, c->n1_keys.p,
-                        c->twM.as<float2>(), s);
-      launch_norm1_runfin(L, c->n1_keys.p, cnt, (ip - psi_begin) * nm, c->T, maps + m0,
+    CHK(lanes_start());
+    for (int m0 = 0, k = 0; m0 < nm; m0 += nb, ++k) {
+      const int cnt = std::min(nb, nm - m0), lane = k % 2;
+      hipStream_t ls = c->lane_stream[lane];
+      void* keys = static_cast<char*>(c->n1_keys.p) + (size_t)lane * nb * kb;
+      pass1(m0, cnt, lane);
+      launch_norm1_zrun(L, c->G[lane].as<float2>(), maps + m0, cnt, c->n1_norm.p, keys,
+                        c->twM.as<float2>(), ls);
+      launch_norm1_runfin(L, keys, cnt, (ip - psi_begin) * nm, c->T, maps + m0,
                           c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
-                          c->sel_count.as<int>(), c->n1_rec.p, cap, s);
+                          c->sel_count.as<int>(), c->n1_rec.p, cap, ls);
     }
+    CHK(lanes_join());
   }
   int count = 0;
   HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -131,6 +131,10 @@ def test_norm_type1_picks(gpu, name):
         if views_per_batch:
             pk.set_batch(pk.info().n_tiles * views_per_batch)
         got = pk.pick()
+        # the accumulations run in batch order on both lane streams: a second
+        # search gives the same bits
+        again = pk.pick()
+    assert np.array_equal(got, again)
     thr = c.params["threshold"]
     key = lambda r: (int(r["psi_index"]), int(r["view"]), float(r["cx"]), float(r["cy"]))
     rk = {key(r): float(r["score"]) for r in ref}
