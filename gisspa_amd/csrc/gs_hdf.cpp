@@ -381,14 +381,16 @@ bool File::header(uint64_t addr, std::vector<Msg>& msgs, std::string* err) {
 }
 
 bool File::group_btree(uint64_t node, const std::vector<uint8_t>& heap,
-                       std::vector<std::pair<std::string, uint64_t>>& out, int depth,
+                       std::vector<std::pair<std::string, uint64_t>>& out, int level_want,
                        std::string* err) {
-  if (depth > kMaxDepth) return fail(err, "group B-tree too deep");
   std::vector<uint8_t> h;
   const size_t hs = 8 + 2 * (size_t)so_;
   if (!bytes(node, hs, h) || memcmp(h.data(), "TREE", 4) != 0 || h[4] != 0)
     return fail(err, "bad group B-tree node");
   const int level = h[5];
+  // children sit exactly one level lower, so a malformed tree cannot loop
+  if ((level_want >= 0 && level != level_want) || level > kMaxDepth)
+    return fail(err, "group B-tree levels out of order");
   const size_t entries = h[6] | (size_t)h[7] << 8;
   std::vector<uint8_t> n;
   if (!bytes(node, hs + (entries + 1) * sl_ + entries * so_, n))
@@ -398,7 +400,7 @@ bool File::group_btree(uint64_t node, const std::vector<uint8_t>& heap,
     c.u(sl_);   // key (heap offset of the left-most name)
     const uint64_t child = c.u(so_);
     if (level > 0) {
-      if (!group_btree(child, heap, out, depth + 1, err)) return false;
+      if (!group_btree(child, heap, out, level - 1, err)) return false;
       continue;
     }
     std::vector<uint8_t> s;
@@ -408,6 +410,7 @@ bool File::group_btree(uint64_t node, const std::vector<uint8_t>& heap,
     const size_t es = 2 * (size_t)so_ + 24;
     if (!bytes(child, 8 + nsym * es, s)) return fail(err, "symbol table node outside the file");
     Cur e(s.data() + 8, nsym * es);
+    if (out.size() + nsym > (1u << 24)) return fail(err, "group with more than 2^24 members");
     for (size_t k = 0; k < nsym; ++k) {
       const uint64_t name_off = e.u(so_), obj = e.u(so_);
       e.take(24);
@@ -444,7 +447,7 @@ bool File::members(uint64_t group, std::vector<std::pair<std::string, uint64_t>>
       const uint64_t daddr = q.u(so_);
       std::vector<uint8_t> heap;
       if (!bytes(daddr, dsize, heap)) return fail(err, "local heap data outside the file");
-      if (!group_btree(bt, heap, out, 0, err)) return false;
+      if (!group_btree(bt, heap, out, -1, err)) return false;
     } else if (m.type == 0x06) {   // link (compact storage)
       is_group = true;
       if (c.u(1) != 1) return fail(err, "link message version");
@@ -564,14 +567,15 @@ bool File::shape(uint64_t obj, std::vector<uint64_t>* dims, std::string* err) {
   return fail(err, "object is not a dataset (no dataspace)");
 }
 
-bool File::chunk_btree(uint64_t node, int ndim, std::vector<Chunk>& chunks, int depth,
+bool File::chunk_btree(uint64_t node, int ndim, std::vector<Chunk>& chunks, int level_want,
                        std::string* err) {
-  if (depth > kMaxDepth) return fail(err, "chunk B-tree too deep");
   std::vector<uint8_t> h;
   const size_t hs = 8 + 2 * (size_t)so_;
   if (!bytes(node, hs, h) || memcmp(h.data(), "TREE", 4) != 0 || h[4] != 1)
     return fail(err, "bad chunk B-tree node");
   const int level = h[5];
+  if ((level_want >= 0 && level != level_want) || level > kMaxDepth)
+    return fail(err, "chunk B-tree levels out of order");
   const size_t entries = h[6] | (size_t)h[7] << 8;
   const size_t ks = 8 + 8 * (size_t)ndim;
   std::vector<uint8_t> n;
@@ -587,7 +591,7 @@ bool File::chunk_btree(uint64_t node, int ndim, std::vector<Chunk>& chunks, int 
     ch.addr = c.u(so_);
     if (!c.ok) return fail(err, "truncated chunk B-tree node");
     if (level > 0) {
-      if (!chunk_btree(ch.addr, ndim, chunks, depth + 1, err)) return false;
+      if (!chunk_btree(ch.addr, ndim, chunks, level - 1, err)) return false;
     } else {
       chunks.push_back(ch);
     }
@@ -736,7 +740,7 @@ bool File::read(uint64_t obj, std::vector<float>& out, std::vector<uint64_t>* di
   }
   std::vector<Chunk> chunks;
   if (index == 0) {
-    if (!undefined(addr, so_) && !chunk_btree(addr, rank + 1, chunks, 0, err)) return false;
+    if (!undefined(addr, so_) && !chunk_btree(addr, rank + 1, chunks, -1, err)) return false;
   } else if (index == 1) {
     if (!undefined(addr, so_))
       chunks.push_back({addr, single_filtered ? (uint32_t)single_size : (uint32_t)(cn * es),

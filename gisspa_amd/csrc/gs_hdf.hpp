@@ -77,10 +77,12 @@ class File {
     uint32_t size, mask;
     std::vector<uint64_t> off;   // element offsets, one per dimension (+ element size)
   };
+  // level_want: the level the node must have (its parent's minus one; -1 at
+  // the root)
   bool group_btree(uint64_t node, const std::vector<uint8_t>& heap,
-                   std::vector<std::pair<std::string, uint64_t>>& out, int depth,
+                   std::vector<std::pair<std::string, uint64_t>>& out, int level_want,
                    std::string* err);
-  bool chunk_btree(uint64_t node, int ndim, std::vector<Chunk>& chunks, int depth,
+  bool chunk_btree(uint64_t node, int ndim, std::vector<Chunk>& chunks, int level_want,
                    std::string* err);
 
   std::map<uint64_t, std::vector<std::pair<std::string, uint64_t>>> members_cache_;

@@ -262,3 +262,21 @@ def test_mrc_modes_and_byte_orders(tmp_path, mode, big):
     bad.write_bytes(_mrc_bytes(raw, 6, big))   # modes above 4 are refused (emdata_.cpp:457)
     with pytest.raises(OSError):
         gisspa_amd.read_image(str(bad), 0)
+
+
+def test_cyclic_group_btree_is_refused(tmp_path):
+    """A group B-tree node that claims level 1 and names itself as its child
+    (a cycle) is refused: children must sit exactly one level lower."""
+    import struct
+    ims = _images(8, 2, (8, 8))
+    p = tmp_path / "c.hdf"
+    H.Writer(0).write(str(p), H.eman1_stack(ims))
+    b = bytearray(p.read_bytes())
+    off = b.find(b"TREE")
+    assert off > 0 and b[off + 4] == 0            # the root group's node
+    b[off + 5] = 1                                # level 1 ...
+    child = off + 8 + 16 + 8                      # after signature, siblings and key 0
+    b[child:child + 8] = struct.pack("<Q", off)   # ... whose child is itself
+    p.write_bytes(bytes(b))
+    with pytest.raises(OSError, match="levels out of order"):
+        gisspa_amd.image_count(str(p))
