@@ -539,7 +539,7 @@ def main():
     ap.add_argument("--cli", type=int, default=0, metavar="N",
                     help="also run the ./main command line on N full-size micrographs, once per "
                          "micrograph_batch of 1 and 4 (0 = skip)")
-    ap.add_argument("--norm1", type=int, default=0, metavar="VIEWS",
+    ap.add_argument("--norm1", type=int, default=206, metavar="VIEWS",
                     help="also time the norm_type 1 picker with the first VIEWS views, scaled "
                          "to all (0 = skip)")
     ap.add_argument("--io", type=int, default=0,
