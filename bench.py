@@ -169,6 +169,28 @@ def plain_rate(templates, d_img, nx, ny, defocus, npsi: int):
                        f"{t_psi:.2f}s each, extrapolated to 72 psi")
 
 
+def prep_times(templates, d_img, nx, ny, defocus, hip):
+    """Micrograph preparation alone (statistics, hot-pixel clamp, tiles and
+    their spectra; with phase_flip 1 also the whole-micrograph CTF flip,
+    whitening and band pass, main.cu:523-655), device synchronised, best of
+    three, for phase_flip 0 and 1."""
+    import gisspa_amd
+    out = {}
+    for pf in (0, 1):
+        pk = gisspa_amd.Picker(0, **dict(FULL_PARAMS, phase_flip=pf))
+        pk.set_templates(np.ascontiguousarray(templates[:8]), defocus)
+        best = 1e30
+        for _ in range(3):
+            hip.sync()
+            t0 = time.perf_counter()
+            pk.set_micrograph_device(d_img, nx, ny, defocus)
+            hip.sync()
+            best = min(best, time.perf_counter() - t0)
+        pk.close()
+        out[f"phase_flip_{pf}_ms"] = best * 1e3
+    return out
+
+
 def norm1_rate(templates, d_img, nx, ny, defocus, n_views: int):
     """norm_type 1 (per-pixel normalisation over all views x psi,
     main.cu:1170-1229) on the same micrograph with the first n_views views:
@@ -742,6 +764,10 @@ def main():
     if not args.no_plain:
         plain = plain_rate(templates, d_img, nx, ny, defocus, args.plain_psi)
         log(f"[bench] plain: {plain['sample']}: {plain['value']:.2f} micrographs/hour")
+    prep = prep_times(templates, d_img, nx, ny, defocus, hip)
+    log(f"[bench] micrograph prep: phase_flip 0 {prep['phase_flip_0_ms']:.1f} ms, phase_flip 1 "
+        f"{prep['phase_flip_1_ms']:.1f} ms")
+
     norm1 = None
     if args.norm1:
         norm1 = norm1_rate(templates, d_img, nx, ny, defocus, args.norm1)
@@ -795,6 +821,7 @@ def main():
         "stress": stress,
         "cli": cli,
         "norm1": norm1,
+        "prep": prep,
         "io": io,
         "rocfft": rocfft,
         "project": proj,
