@@ -4,7 +4,9 @@
 
 #include "gs_hdf.hpp"
 
+#include <dlfcn.h>
 #include <sys/stat.h>
+#include <zlib.h>
 
 #include <algorithm>
 #include <cctype>
@@ -12,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 
 namespace gsio {
@@ -278,10 +281,103 @@ struct MrcHeader {
 };
 static_assert(sizeof(MrcHeader) == 1024, "MRC header is 1024 bytes");
 
-int open_mrc(const std::string& path, FILE** fp, MrcHeader& h, bool& swap, std::string* err) {
-  *fp = fopen(path.c_str(), "rb");
+bool ends_with(const std::string& s, const char* suffix) {
+  const size_t n = strlen(suffix);
+  return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
+}
+
+// bzip2 decompression through the system's libbz2 (loaded at run time; its
+// public stream interface: bzlib.h of bzip2 1.0)
+struct BzStream {
+  char* next_in;
+  unsigned int avail_in, total_in_lo32, total_in_hi32;
+  char* next_out;
+  unsigned int avail_out, total_out_lo32, total_out_hi32;
+  void* state;
+  void* (*bzalloc)(void*, int, int);
+  void (*bzfree)(void*, void*);
+  void* opaque;
+};
+
+bool bunzip2(const std::vector<char>& in, std::vector<char>& out) {
+  void* lib = dlopen("libbz2.so.1", RTLD_NOW | RTLD_LOCAL);
+  if (!lib) lib = dlopen("libbz2.so.1.0", RTLD_NOW | RTLD_LOCAL);
+  if (!lib) return false;
+  using InitF = int (*)(BzStream*, int, int);
+  using StepF = int (*)(BzStream*);
+  auto init = reinterpret_cast<InitF>(dlsym(lib, "BZ2_bzDecompressInit"));
+  auto step = reinterpret_cast<StepF>(dlsym(lib, "BZ2_bzDecompress"));
+  auto end = reinterpret_cast<StepF>(dlsym(lib, "BZ2_bzDecompressEnd"));
+  bool ok = false;
+  if (init && step && end) {
+    const int kOk = 0, kStreamEnd = 4;
+    size_t pos = 0;   // concatenated streams (as bzcat reads them)
+    ok = true;
+    while (ok && pos < in.size()) {
+      BzStream z{};
+      if (init(&z, 0, 0) != kOk) {
+        ok = false;
+        break;
+      }
+      z.next_in = const_cast<char*>(in.data()) + pos;
+      z.avail_in = (unsigned)std::min<size_t>(in.size() - pos, 1u << 30);
+      int r = kOk;
+      char buf[1 << 16];
+      while (r == kOk) {
+        z.next_out = buf;
+        z.avail_out = sizeof(buf);
+        r = step(&z);
+        out.insert(out.end(), buf, buf + (sizeof(buf) - z.avail_out));
+        if (r == kOk && z.avail_in == 0 && z.avail_out != 0) r = -1;   // truncated
+      }
+      pos = (size_t)(z.next_in - in.data());
+      end(&z);
+      ok = r == kStreamEnd;
+    }
+  }
+  dlclose(lib);
+  return ok;
+}
+
+// Open an MRC file for reading.  readMRC (emdata_.cpp:403-413) meant to read
+// .gz / .Z files through zcat and reads .bz / .bz2 files through bzcat; here
+// .gz (zlib) and .bz / .bz2 (libbz2) files are decompressed into `mem` and
+// read from memory, so stack sections can still be addressed.  (.Z is not
+// read: the reference's zcat stream is replaced by a plain fopen before use,
+// emdata_.cpp:403-413, so it does not read such files either.)
+FILE* open_mrc_stream(const std::string& path, std::vector<char>& mem, std::string* err) {
+  const bool gz = ends_with(path, ".gz");
+  const bool bz = ends_with(path, ".bz") || ends_with(path, ".bz2");
+  if (!gz && !bz) return fopen(path.c_str(), "rb");
+  mem.clear();
+  bool ok = false;
+  if (gz) {
+    gzFile g = gzopen(path.c_str(), "rb");
+    if (g) {
+      char buf[1 << 16];
+      int k;
+      while ((k = gzread(g, buf, sizeof(buf))) > 0) mem.insert(mem.end(), buf, buf + k);
+      ok = k == 0;
+      gzclose(g);
+    }
+  } else {
+    std::ifstream f(path, std::ios::binary);
+    std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    ok = f.good() || f.eof();
+    ok = ok && !raw.empty() && bunzip2(raw, mem);
+  }
+  if (!ok || mem.empty()) {
+    if (err) *err = "cannot decompress " + path;
+    return nullptr;
+  }
+  return fmemopen(mem.data(), mem.size(), "rb");
+}
+
+int open_mrc(const std::string& path, FILE** fp, MrcHeader& h, bool& swap, std::string* err,
+             std::vector<char>& mem) {
+  *fp = open_mrc_stream(path, mem, err);
   if (!*fp) {
-    if (err) *err = "Cannot open MRC file " + path;
+    if (err && err->empty()) *err = "Cannot open MRC file " + path;
     return -1;
   }
   if (fread(&h, sizeof(h), 1, *fp) != 1) {
@@ -314,7 +410,8 @@ int mrc_sections(const std::string& path) {
   FILE* fp;
   MrcHeader h;
   bool sw;
-  if (open_mrc(path, &fp, h, sw, nullptr)) return -1;
+  std::vector<char> mem;
+  if (open_mrc(path, &fp, h, sw, nullptr, mem)) return -1;
   fclose(fp);
   return h.w[2];
 }
@@ -323,7 +420,8 @@ int read_mrc(const std::string& path, int n, Image& img, std::string* err) {
   FILE* fp;
   MrcHeader h;
   bool swap;
-  int r = open_mrc(path, &fp, h, swap, err);
+  std::vector<char> mem;   // decompressed file, when compressed
+  int r = open_mrc(path, &fp, h, swap, err, mem);
   if (r) return r;
   const int nx = h.w[0], ny = h.w[1], nzf = h.w[2], mode = h.w[3], nsym = h.w[23];
   if (mode == 3 || mode == 4) {

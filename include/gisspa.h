@@ -360,7 +360,9 @@ const char* gs_strerror(int code);
  *     attributes EMAN.nx/ny/nz, its pixels the dataset "image"
  *     (emdata_.cpp:330-380, emhdf2_.cpp:245-363);
  *   - MRC modes 0, 1 and 2, either byte order (emdata_.cpp:384-617; the
- *     reference also rejects modes above 4, emdata_.cpp:457).
+ *     reference also rejects modes above 4, emdata_.cpp:457), plain or
+ *     compressed as .gz (zlib) or .bz / .bz2 (the system's libbz2; the
+ *     reference pipes these through bzcat, emdata_.cpp:403-413).
  * The HDF5 files are parsed without libhdf5: superblocks 0-3, symbol-table
  * and compact groups, contiguous / compact / chunked datasets (deflate,
  * shuffle, fletcher32), integer and IEEE float elements, converted to float.

@@ -280,3 +280,23 @@ def test_cyclic_group_btree_is_refused(tmp_path):
     p.write_bytes(bytes(b))
     with pytest.raises(OSError, match="levels out of order"):
         gisspa_amd.image_count(str(p))
+
+
+@pytest.mark.parametrize("suffix", [".mrc.gz", ".mrcs.bz2", ".mrc.bz"])
+def test_compressed_mrc(tmp_path, suffix):
+    """readMRC reads .bz / .bz2 files through bzcat (EMReader/emdata_.cpp:
+    408-412); .gz files are read here too (zlib), the stack's sections
+    included."""
+    import bz2
+    import gzip
+    ims = np.stack(_images(9, 4, (20, 24)))
+    raw = _mrc_bytes(ims, 2, False)
+    p = tmp_path / ("s" + suffix)
+    p.write_bytes(gzip.compress(raw) if suffix.endswith(".gz") else bz2.compress(raw))
+    assert gisspa_amd.image_count(str(p)) == 4
+    assert np.array_equal(gisspa_amd.read_image(str(p), -1), ims)
+    assert np.array_equal(gisspa_amd.read_image(str(p), 3), ims[3])
+    bad = tmp_path / ("t" + suffix)
+    bad.write_bytes(b"not compressed data" * 10)
+    with pytest.raises(OSError):   # zlib passes unknown bytes through: a short header
+        gisspa_amd.read_image(str(bad), 0)
