@@ -169,6 +169,48 @@ def plain_rate(templates, d_img, nx, ny, defocus, npsi: int):
                        f"{t_psi:.2f}s each, extrapolated to 72 psi")
 
 
+def small_config():
+    """The small plain-method configuration run in full on the GPU: one
+    1024^2 micrograph, 64 views of a 128^3 density, window 512 / overlap 128
+    (9 tiles), 36 psi = 20,736 CC maps; wall clock of micrograph preparation
+    and the whole pick, after one warm-up pick.  (cpu_baseline times the
+    FP64 oracle on the same maps and compares the pick lists.)"""
+    import gisspa_amd
+    from gisspa_amd import synth
+    wl = synth.small_workload(seed=0)
+    img, df = wl.micrographs[0]
+    pk = gisspa_amd.Picker(0, **wl.params)
+    pk.set_templates(wl.templates, df)
+    pk.set_micrograph(img, df)
+    pk.pick()
+    t0 = time.perf_counter()
+    pk.set_micrograph(img, df)
+    picks = pk.pick()
+    t_gpu = time.perf_counter() - t0
+    pk.close()
+    out = {"maps": 9 * 64 * 36, "gpu_s": t_gpu, "gpu_micrographs_per_hour": 3600.0 / t_gpu,
+           "picks": int(len(picks)), "cpu_s": None, "cpu_threads": None, "identical_picks": None}
+    return out, wl, picks
+
+
+def cpu_baseline_small(wl, picks, nthreads: int):
+    """The FP64 oracle on the small configuration's 20,736 maps (micrograph
+    and template preparation included) on nthreads threads, and whether its
+    pick list equals the GPU's (psi, view, centre)."""
+    from oracle import oracle as orc
+    img, df = wl.micrographs[0]
+    p = orc.params(512, **wl.params)
+    t0 = time.perf_counter()
+    st = orc.pick_norm0(p, orc.ctf(df), wl.templates, img, nthreads=nthreads)
+    sel = orc.select_norm0(p, st, 0, img.shape[1], img.shape[0])
+    t = time.perf_counter() - t0
+    same = bool(len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"]) and
+                np.array_equal(sel["psi_index"], picks["psi_index"]) and
+                np.array_equal(sel["cx"].astype(np.float32), picks["cx"]) and
+                np.array_equal(sel["cy"].astype(np.float32), picks["cy"]))
+    return t, same
+
+
 def prep_times(templates, d_img, nx, ny, defocus, hip):
     """Micrograph preparation alone (statistics, hot-pixel clamp, tiles and
     their spectra; with phase_flip 1 also the whole-micrograph CTF flip,
@@ -567,6 +609,9 @@ def main():
     ap.add_argument("--io", type=int, default=0,
                     help="also time the host-side formats: MRC / HDF5 reading, score filter, "
                          "lst -> STAR (0 = skip)")
+    ap.add_argument("--small", type=int, default=1,
+                    help="also run the small configuration (1024^2, 64 views, 36 psi) in full "
+                         "on the GPU and, unless --no-cpu, on the FP64 oracle (0 = skip)")
     ap.add_argument("--rocfft", type=int, default=1,
                     help="time rocFFT's inverse 2D transforms of the same size (0 = skip)")
     ap.add_argument("--project", type=int, default=1,
@@ -774,11 +819,23 @@ def main():
         log(f"[bench] norm_type 1: {norm1['sample']}: {norm1['value']:.2f} micrographs/hour")
     hip.free(d_img)
 
+    small = None
+    if args.small:
+        small, small_wl, small_picks = small_config()
+        log(f"[bench] small configuration ({small['maps']} maps): GPU {small['gpu_s']:.3f} s "
+            f"({small['picks']} picks)")
+
     cpu = None
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(templates, image, defocus, nthreads, args.cpu_views)
         log(f"[bench] cpu baseline: {cpu['sample']}")
+        if small is not None:
+            small["cpu_s"], small["identical_picks"] = cpu_baseline_small(small_wl, small_picks,
+                                                                         nthreads)
+            small["cpu_threads"] = nthreads
+            log(f"[bench] small configuration: FP64 oracle {small['cpu_s']:.1f} s on {nthreads} "
+                f"threads, identical picks {small['identical_picks']}")
 
     out = {
         "metric": "micrographs/hour at 5760x4092, 1650 views x72 psi; batched vs plain",
@@ -822,6 +879,7 @@ def main():
         "cli": cli,
         "norm1": norm1,
         "prep": prep,
+        "small": small,
         "io": io,
         "rocfft": rocfft,
         "project": proj,
