@@ -114,3 +114,22 @@ def test_autotune(gpu, full_case):
         got = pk.mapstats().reshape(ref.shape)
     _compare(got, ref, "window 720 after autotune")
     _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
+
+
+def test_full_size_phase_flip(gpu, full_case):
+    """phase_flip = 1 at the full-size shape: the whole-micrograph CTF flip,
+    whitening, weighting and band pass (init_d_image, main.cu:548-586) run on
+    a 5760 x 4092 transform (4092 = 2^2 3 11 31), then per-tile amplitude
+    normalisation and the 88-tile search (8 views, one in-plane step)."""
+    import gisspa_amd
+    tm, image, params, _, _ = full_case
+    params = dict(params, phase_flip=1)
+    p = orc.params(720, **params)
+    ref = orc.pick_norm0(p, orc.ctf(1.8), tm, image, 0, 1, nthreads=NTHREADS)
+    with gisspa_amd.Picker(0, **params) as pk:
+        pk.set_templates(tm, 1.8)
+        pk.set_micrograph(image, 1.8)
+        picks = pk.pick(0, 1)
+        got = pk.mapstats().reshape(ref.shape)
+    _compare(got, ref, "5760x4092 / window 720, phase_flip 1")
+    _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
