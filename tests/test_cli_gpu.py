@@ -184,3 +184,43 @@ def test_cli_hdf_templates_of_mixed_boxes(gpu, tmp_path):
         assert float(m.group(6)) == pytest.approx(float(np.float32(eul[p["view"]][0])), abs=1e-5)
         assert float(m.group(9)) == p["cx"] and float(m.group(10)) == p["cy"]
         assert float(m.group(11)) == pytest.approx(float(p["score"]), rel=1e-5, abs=2e-6)
+
+
+def test_cli_norm_type1(gpu, tmp_path):
+    """norm_type = 1 through ./main (main.cu:1170-1229): the output lines
+    hold the oracle's run records (z-scores within 1e-4; a record within
+    1e-4 of the threshold may be missing on either side)."""
+    c = _cases.tiny(seed=3, l=64, box=48, n_views=4, size=(200, 176), phistep=45.0, overlap=16,
+                    threshold=3.0)
+    c.params["norm_type"] = 1
+    eul = synth.random_eulers(len(c.templates), 13)
+    wl = synth.Workload("cli", c.templates, eul, [(c.image, c.defocus)], dict(c.params), [])
+    cfg = synth.write_workload(wl, str(tmp_path))
+    r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = open(os.path.join(tmp_path, "picks.lst")).read().splitlines()
+    op = orc.params(c.l, **c.params)
+    ref = orc.pick_norm1(op, orc.ctf(c.defocus), c.templates, c.image)
+    thr = c.params["threshold"]
+    rk = {}
+    for s in ref:
+        e3 = 360.0 - orc.psi_value(op.phi_step, int(s["psi_index"]))
+        e3 = e3 - 360 if e3 >= 360 else e3
+        rk[(round(e3, 4), int(s["view"]), float(s["cx"]), float(s["cy"]))] = float(s["score"])
+    gk = {}
+    for ln in lines:
+        m = LINE.match(ln)
+        assert m, ln
+        view = [j for j in range(len(eul))
+                if abs(float(m.group(6)) - float(np.float32(eul[j][0]))) < 1e-5 and
+                abs(float(m.group(7)) - float(np.float32(eul[j][1]))) < 1e-5]
+        assert len(view) == 1, ln
+        gk[(round(float(m.group(8)), 4), view[0], float(m.group(9)), float(m.group(10)))] = \
+            float(m.group(11))
+    print("norm1 CLI lines", len(gk), "oracle", len(rk))
+    assert len(rk) > 10
+    for k in set(rk) ^ set(gk):
+        s = rk.get(k, gk.get(k))
+        assert abs(s - thr) <= 1e-4 * thr + 2e-6, (k, s)
+    for k in set(rk) & set(gk):
+        assert abs(rk[k] - gk[k]) <= 1e-4 * max(1.0, abs(rk[k])) + 2e-6, (k, rk[k], gk[k])

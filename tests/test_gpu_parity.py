@@ -180,3 +180,35 @@ def test_phase_flip_path(gpu, size):
     assert np.array_equal(got["pos"], ref["pos"])
     sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
     assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
+
+
+@pytest.mark.parametrize("phase_flip", [0, 1])
+def test_invert_path(gpu, phase_flip):
+    """invert = 1: the micrograph's contrast inverted after the hot-pixel
+    clamp (init_d_image, main.cu:523-547), with and without the phase flip;
+    tile spectra, per-map scores and picks against the oracle."""
+    import gisspa_amd
+    c = _cases.tiny(seed=11, l=96, threshold=3.5)
+    c.params["invert"] = 1
+    c.params["phase_flip"] = phase_flip
+    op = _oparams(c)
+    ref_tiles = orc.micrograph_prep(op, orc.ctf(c.defocus), c.image)
+    ref = orc.pick_norm0(op, orc.ctf(c.defocus), c.templates, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates(c.templates, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        tiles = pk.tile_spectra()
+        picks = pk.pick()
+        got = pk.mapstats().reshape(ref.shape)
+    H = c.l // 2 + 1
+    rh = np.transpose(ref_tiles, (0, 2, 1))[:, :H, :]
+    err = np.abs(tiles - rh).max() / np.abs(rh).max()
+    print("invert tile spectra rel err", err)
+    assert err < 2e-5
+    rel = np.abs(got["score"] - ref["score"]) / np.maximum(np.abs(ref["score"]), 1.0)
+    print("invert score max rel err", rel.max(), "picks", len(picks))
+    assert rel.max() < 1e-4
+    assert np.array_equal(got["pos"], ref["pos"])
+    sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
+    assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
+    assert np.array_equal(sel["cx"].astype(np.float32), picks["cx"])

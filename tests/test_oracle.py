@@ -291,6 +291,22 @@ def test_micrograph_prep_mean_removal_and_hot_pixels():
             k += 1
 
 
+def test_micrograph_invert_after_the_clamp():
+    """invert = 1 negates the micrograph after the hot-pixel clamp
+    (main.cu:541-545), and every later step of the phase_flip = 0 path is
+    linear: with hot and cold pixels, the tiles are exactly the negated tiles
+    of invert = 0."""
+    l = 32
+    ny, nx = 70, 90
+    img = RNG.standard_normal((ny, nx)).astype(np.float32) + 0.6
+    img[10, 10] = 40.0
+    img[20, 50] = -40.0
+    t0 = orc.micrograph_prep(_tp(l, overlap=8), orc.ctf(1.5), img)
+    t1 = orc.micrograph_prep(_tp(l, overlap=8, invert=1), orc.ctf(1.5), img)
+    assert np.abs(t0).max() > 0
+    assert np.array_equal(t1, -t0)
+
+
 def test_micrograph_hot_pixel_bound_is_integer_truncated():
     """avg + 6 sd = 5.x is truncated to 5: a pixel at 5.5 is replaced."""
     l = 32
