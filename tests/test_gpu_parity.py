@@ -212,3 +212,32 @@ def test_invert_path(gpu, phase_flip):
     sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
     assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
     assert np.array_equal(sel["cx"].astype(np.float32), picks["cx"])
+
+
+def test_window_adjusted_to_the_templates(gpu):
+    """Templates larger than the configured window: the window grows to the
+    smallest multiple of 32 that holds them (adjustPaddingSize,
+    main.cu:158-178; 80 px templates, window 64 -> 96), while the CTF
+    sampling of the template weighting keeps the configured window
+    (ds, main.cu:155 before main.cu:1116).  The overlap key is left out, so
+    tiles use the reference's default stride l + 1 (main.cu:228)."""
+    import gisspa_amd
+    c = _cases.tiny(seed=12, l=64, box=80, n_views=4, size=(352, 320), threshold=3.5)
+    del c.params["overlap"]
+    op = orc.params(96, **c.params)
+    assert op.l == 96 and op.l_config == 64 and op.overlap == -1
+    ref = orc.pick_norm0(op, orc.ctf(c.defocus), c.templates, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates(c.templates, c.defocus)
+        info = pk.info()
+        pk.set_micrograph(c.image, c.defocus)
+        picks = pk.pick()
+        got = pk.mapstats().reshape(ref.shape)
+    assert info.l == 96
+    rel = np.abs(got["score"] - ref["score"]) / np.maximum(np.abs(ref["score"]), 1.0)
+    print("adjusted window: score max rel err", rel.max(), "maps", ref.size, "picks", len(picks))
+    assert rel.max() < 1e-4
+    assert np.array_equal(got["pos"], ref["pos"])
+    sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
+    assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
+    assert np.array_equal(sel["cx"].astype(np.float32), picks["cx"])
