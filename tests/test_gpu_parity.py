@@ -241,3 +241,38 @@ def test_window_adjusted_to_the_templates(gpu):
     sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
     assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
     assert np.array_equal(sel["cx"].astype(np.float32), picks["cx"])
+
+
+@pytest.mark.parametrize("phase_flip", [0, 1])
+def test_astigmatic_ctf(gpu, phase_flip):
+    """Astigmatic CTF (dfdiff 0.12 um, dfang 37 degrees; CTF_AST,
+    GPU_func.cu:292-315) in the template weighting and, with phase_flip 1,
+    in the micrograph's flip and weighting, on a non-square micrograph:
+    template real parts, tile spectra, per-map scores and picks against the
+    oracle."""
+    import gisspa_amd
+    c = _cases.tiny(seed=13, l=96, size=(352, 256), threshold=3.5)
+    c.params["phase_flip"] = phase_flip
+    op = _oparams(c)
+    ctf = orc.ctf(c.defocus, 0.12, 37.0)
+    ref_t = orc.template_prep(op, ctf, c.templates)
+    ref_tiles = orc.micrograph_prep(op, ctf, c.image)
+    ref = orc.pick_norm0(op, ctf, c.templates, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates(c.templates, c.defocus, 0.12, 37.0)
+        treal = pk.template_real()
+        pk.set_micrograph(c.image, c.defocus, 0.12, 37.0)
+        tiles = pk.tile_spectra()
+        picks = pk.pick()
+        got = pk.mapstats().reshape(ref.shape)
+    et = np.abs(treal - ref_t.real).max() / np.abs(ref_t.real).max()
+    H = c.l // 2 + 1
+    rh = np.transpose(ref_tiles, (0, 2, 1))[:, :H, :]
+    es = np.abs(tiles - rh).max() / np.abs(rh).max()
+    rel = np.abs(got["score"] - ref["score"]) / np.maximum(np.abs(ref["score"]), 1.0)
+    print("astigmatic: template", et, "tiles", es, "scores", rel.max(), "picks", len(picks))
+    assert et < 2e-5 and es < 2e-5
+    assert rel.max() < 1e-4
+    assert np.array_equal(got["pos"], ref["pos"])
+    sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
+    assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])

@@ -81,6 +81,26 @@ def test_ctf_matches_textbook():
         assert orc.ctf_ast(p, c, x, y, l, l, 0) == (1.0 if got > 0 else -1.0)
 
 
+def test_ctf_astigmatism_axes():
+    """Astigmatism (dfdiff 0.15 um at dfang 45 degrees): along the 45 degree
+    axis the CTF is the textbook one of defocus - dfdiff, along -45 degrees
+    (90 degrees away) of defocus + dfdiff, and halfway (0 degrees) of the
+    mean defocus; a dropped factor 2 in the angle or a sign error in dfdiff
+    fails."""
+    p = orc.params(256, angpix=1.3, energy=300, cs=2.7, ampconst=0.1, Highres=8, Lowres=100,
+                   phistep=10, diameter=100, kk=3)
+    c = orc.ctf(1.7, 0.15, 45.0)
+    l = 256
+    for k in (5, 17, 40, 90):
+        for dy, df in ((k, 1.7 - 0.15), (-k, 1.7 + 0.15)):
+            s2 = (k * k + dy * dy) / (l * l) / (p.apix ** 2)
+            ref = textbook_ctf(s2, df, 2.7, 300, p.ampconst)
+            assert orc.ctf_ast(p, c, k, l // 2 + dy, l, l, 1) == pytest.approx(ref, abs=1e-6)
+        s2 = (k * k) / (l * l) / (p.apix ** 2)
+        assert orc.ctf_ast(p, c, k, l // 2, l, l, 1) == pytest.approx(
+            textbook_ctf(s2, 1.7, 2.7, 300, p.ampconst), abs=1e-6)
+
+
 def test_ctf_first_zero_and_friedel():
     p = orc.params(512, angpix=1.0, energy=300, cs=0.0, ampconst=0.1, Highres=8, Lowres=100,
                    phistep=10, diameter=100, kk=3)
