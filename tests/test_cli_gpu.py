@@ -224,3 +224,39 @@ def test_cli_norm_type1(gpu, tmp_path):
         assert abs(s - thr) <= 1e-4 * thr + 2e-6, (k, s)
     for k in set(rk) & set(gk):
         assert abs(rk[k] - gk[k]) <= 1e-4 * max(1.0, abs(rk[k])) + 2e-6, (k, rk[k], gk[k])
+
+
+def _write_lsx(path, entries, width=96):
+    """#LSX list (fixed-width records after a header giving the record size,
+    DataReader.cpp:330-347)."""
+    with open(path, "w") as f:
+        f.write("#LSX\n# fixed-width records: every line below has the size given next\n")
+        f.write(f"# {width}\n")
+        for idx, p, df, dfdiff, dfang in entries:
+            rec = f"{idx}\t{p}\tdefocus={df:.6f}\tdfdiff={dfdiff:.6f}\tdfang={dfang:.6f}"
+            assert len(rec) < width
+            f.write(rec.ljust(width - 1) + "\n")
+
+
+def test_cli_lsx_list(gpu, tmp_path):
+    """The same three micrographs listed as #LSX (fixed-width records read
+    by seeking) and as #LST give the same output file."""
+    sizes = [(288, 256), (256, 320), (320, 288)]
+    cases = [_cases.tiny(seed=40 + i, l=96, size=sz, threshold=4.0) for i, sz in enumerate(sizes)]
+    c0 = cases[0]
+    eul = synth.random_eulers(len(c0.templates), 21)
+    mics = [(c.image, 1.4 + 0.2 * i) for i, c in enumerate(cases)]
+    wl = synth.Workload("cli", c0.templates, eul, mics, dict(c0.params), [])
+    cfg = synth.write_workload(wl, str(tmp_path))
+    r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lst_out = open(os.path.join(tmp_path, "picks.lst")).read()
+    assert lst_out.count("\n") > 0
+    _write_lsx(str(tmp_path / "micrographs.lsx"),
+               [(i, f"mic{i:03d}.mrc", df, 0.0, 0.0) for i, (_, df) in enumerate(mics)])
+    text = open(cfg).read().replace("micrographs.lst", "micrographs.lsx").replace(
+        "picks.lst", "picks_lsx.lst")
+    open(cfg, "w").write(text)
+    r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert open(os.path.join(tmp_path, "picks_lsx.lst")).read() == lst_out
