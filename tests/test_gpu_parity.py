@@ -276,3 +276,27 @@ def test_astigmatic_ctf(gpu, phase_flip):
     assert np.array_equal(got["pos"], ref["pos"])
     sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
     assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
+
+
+@pytest.mark.parametrize("l,size,box,overlap", [
+    (96, (80, 72), 32, 24),          # micrograph smaller than the window: one zero-padded tile
+    (1024, (1500, 1100), 128, 256),  # the largest compiled window
+])
+def test_window_edge_cases(gpu, l, size, box, overlap):
+    import gisspa_amd
+    c = _cases.tiny(seed=14, l=l, box=box, n_views=2, size=size, phistep=180.0,
+                    overlap=overlap, threshold=3.0)
+    op = _oparams(c)
+    ref = orc.pick_norm0(op, orc.ctf(c.defocus), c.templates, c.image)
+    with gisspa_amd.Picker(0, **c.params) as pk:
+        pk.set_templates(c.templates, c.defocus)
+        pk.set_micrograph(c.image, c.defocus)
+        picks = pk.pick()
+        got = pk.mapstats().reshape(ref.shape)
+    rel = np.abs(got["score"] - ref["score"]) / np.maximum(np.abs(ref["score"]), 1.0)
+    print(f"window {l}, micrograph {size}: {ref.shape} maps, score max rel err {rel.max():.2e}, "
+          f"picks {len(picks)}")
+    assert rel.max() < 1e-4
+    assert np.array_equal(got["pos"], ref["pos"])
+    sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
+    assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
