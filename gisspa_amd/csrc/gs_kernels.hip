@@ -278,6 +278,35 @@ __global__ __launch_bounds__(WG) void k_rows_c2r(const float2* __restrict__ Gin,
   }
 }
 
+// The same row C2R streamed: each thread reads X[k] and X[M - k] of its
+// inputs straight from the half spectra and pre-processes them on the way
+// into the first transform pass (the executor of the correlation's pass 2),
+// then writes its real pairs; no LDS staging of the spectra.
+template <int L>
+__global__ __launch_bounds__(WG) void k_rows_c2r_stream(const float2* __restrict__ Gin,
+                                                        float* __restrict__ out,
+                                                        const float2* __restrict__ twL,
+                                                        const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int M = G::M, HP = G::HP, RB = G::RB;
+  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  const int b = blockIdx.y;
+  const int y0 = blockIdx.x * RB;
+  const float2* gb = Gin + (long)b * L * HP;
+  auto load = [&](int r, int k) -> float2 {
+    const float2* row = gb + (long)min(y0 + r, L - 1) * HP;
+    float2 w = twL[k];
+    w.y = -w.y;
+    return c2r_pre(row[k], row[M - k], w);
+  };
+  float* ob = out + (long)b * L * L;
+  auto store = [&](int r, int n, float2 z) {
+    const int y = y0 + r;
+    if (y < L) *reinterpret_cast<float2*>(ob + (long)y * L + 2 * n) = z;
+  };
+  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+}
+
 // ---------------------------------------------------------------------------
 // Correlation (compute_corner_CCG GPU_func.cu:584-615, cuFFT inverse
 // main.cu:848, get_peak_and_SUM GPU_func.cu:667-708, host loop main.cu:856-881)
@@ -1212,6 +1241,16 @@ static void rows_c2r_t(const float2* G, int nb, float* out, const float2* twL, c
                        hipStream_t s) {
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
   hipLaunchKernelGGL((k_rows_c2r<L>), grid, dim3(WG), 0, s, G, out, twL, twM);
+}
+template <int L>
+static void rows_c2r_stream_t(const float2* G, int nb, float* out, const float2* twL,
+                              const float2* twM, hipStream_t s) {
+  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  hipLaunchKernelGGL((k_rows_c2r_stream<L>), grid, dim3(WG), 0, s, G, out, twL, twM);
+}
+void launch_rows_c2r_stream(int L, const float2* G, int nb, float* out, const float2* twL,
+                            const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_c2r_stream_t, G, nb, out, twL, twM, s);
 }
 void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* twL,
                      const float2* twM, hipStream_t s) {

@@ -94,6 +94,10 @@ void launch_cols_inv_T(int L, const float2* spec, int nb, float2* G, const float
 // Row C2R of G[b][y][kx] into real images out[b][y][x].
 void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* twL,
                      const float2* twM, hipStream_t s);
+// the same transform with the spectra read and pre-processed on the way into
+// the first pass (no LDS staging); equal to rounding
+void launch_rows_c2r_stream(int L, const float2* G, int nb, float* out, const float2* twL,
+                            const float2* twM, hipStream_t s);
 
 // Correlation pass 1: P = tile[t] * conj(tmpl[J]) (the tile spectra carry
 // (-1)^(kx+ky) sgn), column inverse and C2R pre-processing, written to the

@@ -598,7 +598,7 @@ static void ref_forward_t(const RefArgs& a, hipStream_t s) {
                      a.wtab, a.gtmp, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_norm_m, dim3(small), dim3(64), 0, s, a.part, a.nb, a.count, a.sc,
                      a.pre, NCB, FSLOTS);
-  launch_rows_c2r(L, a.gtmp, a.nb, a.Tun, a.twL, a.twM, s);
+  launch_rows_c2r_stream(L, a.gtmp, a.nb, a.Tun, a.twL, a.twM, s);
   launch_rows_r2c_rotb(L, a.Tun, a.nb, a.cs, a.pre, a.dmask, a.spec, a.rot_part, a.twL, a.twM, s);
   hipLaunchKernelGGL(k_ref_rot_stats, dim3(small), dim3(64), 0, s, a.rot_part,
                      (L + Geo<L>::RB - 1) / Geo<L>::RB, a.nb, (double)L * L, a.sc, a.bscale);
@@ -615,7 +615,7 @@ static void ref_backward_t(const RefArgs& a, hipStream_t s) {
   // d/dThat -> d/dRn (C2R, / L^2 applied in the next kernels)
   hipLaunchKernelGGL((k_ref_grad_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.tiles, a.spec, a.cand,
                      a.sc, a.gtmp, a.twL);
-  launch_rows_c2r(L, a.gtmp, a.nb, a.gRn, a.twL, a.twM, s);
+  launch_rows_c2r_stream(L, a.gtmp, a.nb, a.gRn, a.twL, a.twM, s);
   hipLaunchKernelGGL((k_ref_norm_red<L>), grid, blk, 0, s, a.Tun, a.gRn, a.cs, a.pre, a.sc,
                      a.dmask, a.part);
   hipLaunchKernelGGL(k_ref_norm_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, (double)L * L,
