@@ -300,3 +300,36 @@ def test_window_edge_cases(gpu, l, size, box, overlap):
     assert np.array_equal(got["pos"], ref["pos"])
     sel = orc.select_norm0(op, ref, 0, c.image.shape[1], c.image.shape[0])
     assert len(sel) == len(picks) and np.array_equal(sel["view"], picks["view"])
+
+
+def test_norm_type1_batch_of_micrographs(gpu):
+    """norm_type 1 over two micrographs of different sizes in one tile pool
+    (gs_set_micrographs): the per-pixel statistics are per tile, so each
+    micrograph's records equal the oracle's for that micrograph alone, and
+    records come micrograph by micrograph."""
+    import gisspa_amd
+    c1 = _cases.tiny(seed=3, l=64, box=48, n_views=4, size=(200, 176), phistep=45.0,
+                     overlap=16, threshold=3.0)
+    c2 = _cases.tiny(seed=15, l=64, box=48, n_views=4, size=(176, 232), phistep=45.0,
+                     overlap=16, threshold=3.0)
+    c1.params["norm_type"] = 1
+    op = _oparams(c1)
+    thr = c1.params["threshold"]
+    with gisspa_amd.Picker(0, **c1.params) as pk:
+        pk.set_templates(c1.templates, c1.defocus)
+        pk.set_micrographs([c1.image, c2.image], [c1.defocus, c1.defocus])
+        got = pk.pick()
+    assert np.all(np.diff(got["mic"]) >= 0)
+    key = lambda r: (int(r["psi_index"]), int(r["view"]), float(r["cx"]), float(r["cy"]))
+    for i, c in enumerate((c1, c2)):
+        ref = orc.pick_norm1(op, orc.ctf(c1.defocus), c1.templates, c.image)
+        mine = got[got["mic"] == i]
+        rk = {key(r): float(r["score"]) for r in ref}
+        gk = {key(r): float(r["score"]) for r in mine}
+        print("mic", i, "norm1 records gpu", len(gk), "oracle", len(rk))
+        assert len(rk) > 10
+        for k in set(rk) ^ set(gk):
+            s = rk.get(k, gk.get(k))
+            assert abs(s - thr) <= 1e-4 * thr, (i, k, s)
+        for k in set(rk) & set(gk):
+            assert abs(rk[k] - gk[k]) <= 1e-4 * max(1.0, abs(rk[k])), (i, k)
