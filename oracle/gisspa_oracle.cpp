@@ -27,6 +27,11 @@
  *     order, GPU_func.cu:691, and first block, main.cu:859).
  *   - all sums are sequential FP64 instead of float atomics / float partials.
  *   - l is any even size; the reference's grids silently need l % 32 == 0.
+ *
+ * Pins: every function below is checked on the CPU against code it shares
+ * nothing with (numpy's FFT, the textbook CTF, closed forms, brute force):
+ * tests/test_oracle.py, tests/test_oracle_pins.py, tests/test_dedup_oracle.py,
+ * tests/test_project_oracle.py.
  */
 #include "gisspa_oracle.h"
 
