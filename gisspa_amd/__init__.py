@@ -159,6 +159,7 @@ def load() -> ctypes.CDLL:
     lib.gs_set_batch.argtypes = [vp, ctypes.c_int]
     lib.gs_set_profiling.argtypes = [vp, ctypes.c_int]
     lib.gs_set_tile_block.argtypes = [vp, ctypes.c_int]
+    lib.gs_set_record_capacity.argtypes = [vp, ctypes.c_longlong]
     lib.gs_set_pipeline.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.gs_adjoint.argtypes = [vp, vp, ctypes.c_int, vp]
     lib.gs_refine.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
@@ -444,6 +445,10 @@ class Picker:
 
     def set_tile_block(self, n: int) -> None:
         _check(self.lib.gs_set_tile_block(self.ctx, n), "gs_set_tile_block")
+
+    def set_record_capacity(self, records: int) -> None:
+        """Initial capacity of the device-side pick buffers (gs_set_record_capacity)."""
+        _check(self.lib.gs_set_record_capacity(self.ctx, records), "gs_set_record_capacity")
 
     def autotune(self, psi_index: int = 0) -> GsTune:
         """Time the correlation schedules on the device and keep the fastest

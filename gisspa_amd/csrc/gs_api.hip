@@ -109,6 +109,11 @@ struct Ctx {
   std::vector<hipEvent_t> prof_ev;
   int prof_batches = 0;
   std::vector<gs_particle> picks;  // of the last gs_pick
+  // initial capacity (records) of the device-side pick buffers: selected
+  // maps of norm_type 0 (SelRec, 24 B) and run records of norm_type 1
+  // (N1Rec, 16 B); a run that selects more grows the buffer and selects again
+  long long rec_cap0 = 1ll << 22;
+  long long wait_ticks = 0;  // in-kernel dependency wait budget (wall-clock ticks)
   int stats_psi_begin = 0, stats_npsi = 0;
   gs_timing timing{};
 };
@@ -501,6 +506,20 @@ void drop_graphs(Ctx* c) {
   c->graph_key.clear();
 }
 
+// Profiling event i of the sampled psi, created on first use.
+int prof_record(Ctx* c, size_t i, hipStream_t s) {
+  while (c->prof_ev.size() <= i) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->prof_ev.push_back(e);
+  }
+  HIPCHK(hipEventRecord(c->prof_ev[i], s));
+  return GS_OK;
+}
+
+// a device buffer of at least n bytes
+bool fits(const DevBuf& b, size_t n) { return b.p && b.bytes >= n; }
+
 int ensure_corr_buffers(Ctx* c) {
   const int L = c->L;
   const long per_map = c->geo.gmap * (long)sizeof(float2);
@@ -575,32 +594,39 @@ int ensure_corr_buffers(Ctx* c) {
 // stream (the stream order completes each intermediate before it is read);
 // stats[tile][view] <- scores.
 int correlate_psi(Ctx* c, MapStat* out) {
-  if (!c->psum.p || !c->part.p || !c->maps_blk.p || !c->err.p) return GS_ERR_STATE;
-  if (c->pipeline == GS_PIPE_STREAM ? (!c->Gring.p || !c->counters.p) : (!c->G[0].p || !c->G[1].p))
-    return GS_ERR_STATE;  // ensure_corr_buffers allocates what the schedule uses
   const int L = c->L;
   const int nm = c->T * c->N;
   const int nb = std::min(c->batch, nm);
+  if (nm <= 0 || nb <= 0) return GS_ERR_STATE;
   const int nbatch = (nm + nb - 1) / nb;
   const float lo = c->p.diameter / 4.0f;
   const float hi = (float)L - c->p.diameter / 4.0f;
   const int ncb = corr_ncb(L), np = corr_nparts(L);
+  // every buffer the launches below address, at the size they address
+  // (ensure_corr_buffers allocates what the schedule uses)
+  const size_t per_map = (size_t)c->geo.gmap * sizeof(float2);
+  const size_t half = (size_t)(L / 2 + 1) * L * sizeof(float2);
+  if (!fits(c->tiles, (size_t)c->T * half) || !fits(c->tspec, (size_t)c->N * half) ||
+      !fits(c->maps_blk, (size_t)nm * sizeof(MapRef)) ||
+      !fits(c->part, (size_t)nm * np * sizeof(CorrPartial)) ||
+      !fits(c->psum, (size_t)nm * ncb * sizeof(float2)) || !fits(c->err, sizeof(int)) ||
+      !out)
+    return GS_ERR_STATE;
+  if (c->pipeline == GS_PIPE_STREAM ? (!fits(c->Gring, (size_t)8 * c->ring * per_map) ||
+                                       !fits(c->counters, (size_t)2 * nm * sizeof(int)))
+                                    : (!fits(c->G[0], (size_t)nb * per_map) ||
+                                       !fits(c->G[1], (size_t)nb * per_map)))
+    return GS_ERR_STATE;
   hipStream_t s = c->stream;
   if (c->pipeline == GS_PIPE_STREAM) {
-    if (c->profile_now) {
-      while (c->prof_ev.size() < 2) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        c->prof_ev.push_back(e);
-      }
-      HIPCHK(hipEventRecord(c->prof_ev[0], s));
-    }
+    if (c->profile_now) CHK(prof_record(c, 0, s));
     launch_corr_stream(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps_blk.as<MapRef>(),
                        nm, c->lag, c->ring, c->Gring.as<float2>(), c->psum.as<float2>(),
                        c->counters.as<int>(), lo, hi, c->part.as<CorrPartial>(),
-                       c->err.as<int>(), c->twL.as<float2>(), c->twM.as<float2>(), s);
+                       c->err.as<int>(), c->wait_ticks, c->twL.as<float2>(), c->twM.as<float2>(),
+                       s);
     if (c->profile_now) {
-      HIPCHK(hipEventRecord(c->prof_ev[1], s));
+      CHK(prof_record(c, 1, s));
       c->prof_batches = 1;
     }
     launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
@@ -608,59 +634,45 @@ int correlate_psi(Ctx* c, MapStat* out) {
     return GS_OK;
   }
   if (c->pipeline == GS_PIPE_LANES) {
-    if (c->profile_now) {
-      while (c->prof_ev.size() < 3 * (size_t)nbatch + 2) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        c->prof_ev.push_back(e);
-      }
-      HIPCHK(hipEventRecord(c->prof_ev[3 * nbatch], s));
-    }
+    if (c->profile_now) CHK(prof_record(c, 3 * (size_t)nbatch, s));
     // the lanes start after the rotated templates of this psi are ready
     HIPCHK(hipEventRecord(c->ev_ready, s));
     for (int i = 0; i < Ctx::NLANE; ++i) HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
     for (int k = 0; k < nbatch; ++k) {
       const int m0 = k * nb, cnt = std::min(nb, nm - m0);
       hipStream_t ls = c->lane_stream[k % 2];
-      if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k], ls));
+      if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k, ls));
       launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
                         c->maps_blk.as<MapRef>() + m0, cnt, c->G[k % 2].as<float2>(),
                         c->psum.as<float2>() + (size_t)m0 * ncb, c->twL.as<float2>(), ls,
                         c->var1);
-      if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k + 1], ls));
+      if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k + 1, ls));
       launch_corr_pass2(L, c->G[k % 2].as<float2>(), cnt, lo, hi,
                         c->part.as<CorrPartial>() + (size_t)m0 * np, c->twL.as<float2>(),
                         c->twM.as<float2>(), ls, c->var2);
-      if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * k + 2], ls));
+      if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k + 2, ls));
     }
     if (c->profile_now) c->prof_batches = nbatch;
     for (int i = 0; i < Ctx::NLANE; ++i) {
       HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
       HIPCHK(hipStreamWaitEvent(s, c->ev_lane[i], 0));
     }
-    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[3 * nbatch + 1], s));
+    if (c->profile_now) CHK(prof_record(c, 3 * (size_t)nbatch + 1, s));
     launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
                          c->maps_blk.as<MapRef>(), c->N, out, s);
     return GS_OK;
   }
-  if (c->profile_now) {
-    while (c->prof_ev.size() < 2 * (size_t)(nbatch + 1)) {
-      hipEvent_t e;
-      HIPCHK(hipEventCreate(&e));
-      c->prof_ev.push_back(e);
-    }
-  }
   for (int k = 0; k <= nbatch; ++k) {
     const int m1 = k * nb, cnt1 = k < nbatch ? std::min(nb, nm - m1) : 0;
     const int m2 = (k - 1) * nb, cnt2 = k >= 1 ? std::min(nb, nm - m2) : 0;
-    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[2 * k], s));
+    if (c->profile_now) CHK(prof_record(c, 2 * (size_t)k, s));
     launch_corr_fused(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
                       c->maps_blk.as<MapRef>() + (cnt1 ? m1 : 0), cnt1, c->G[k % 2].as<float2>(),
                       c->psum.as<float2>() + (size_t)(cnt1 ? m1 : 0) * ncb,
                       c->G[(k + 1) % 2].as<float2>(), cnt2, lo, hi,
                       c->part.as<CorrPartial>() + (size_t)(cnt2 ? m2 : 0) * np,
                       c->twL.as<float2>(), c->twM.as<float2>(), s);
-    if (c->profile_now) HIPCHK(hipEventRecord(c->prof_ev[2 * k + 1], s));
+    if (c->profile_now) CHK(prof_record(c, 2 * (size_t)k + 1, s));
   }
   if (c->profile_now) c->prof_batches = nbatch + 1;
   launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
@@ -730,20 +742,23 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   const int nb = c->T * nv1;   // maps of a full batch
   const int ncb = corr_ncb(L);
   const size_t npix = (size_t)c->T * L2;
+  const long kb = norm1_key_bytes(L);
+  if (nm <= 0 || nb <= 0) return GS_ERR_STATE;
   for (int i = 0; i < Ctx::NLANE; ++i) CHK(c->G[i].ensure((size_t)nb * per_map));
   CHK(c->psum.ensure((size_t)Ctx::NLANE * nb * ncb * sizeof(float2)));
   CHK(c->n1_s1.ensure(npix * sizeof(double)));
   CHK(c->n1_s2.ensure(npix * sizeof(double)));
   CHK(c->n1_norm.ensure(npix * 2 * sizeof(float)));
-  CHK(c->n1_keys.ensure((size_t)Ctx::NLANE * nb * norm1_key_bytes(L)));
-  const int cap = 1 << 22;
-  CHK(c->n1_rec.ensure((size_t)cap * sizeof(N1Rec)));
-  CHK(c->sel_count.ensure(sizeof(int)));
+  CHK(c->n1_keys.ensure((size_t)Ctx::NLANE * nb * kb));
+  CHK(c->n1_rec.ensure((size_t)std::max(1ll, c->rec_cap0) * sizeof(N1Rec)));
+  CHK(c->sel_count.ensure(sizeof(unsigned long long)));
+  const size_t half = (size_t)(L / 2 + 1) * L * sizeof(float2);
+  if (!fits(c->maps, (size_t)nm * sizeof(MapRef)) || !fits(c->tiles, (size_t)c->T * half) ||
+      !fits(c->tspec, (size_t)c->N * half) || !fits(c->tdesc, (size_t)c->T * sizeof(TileDesc)))
+    return GS_ERR_STATE;
   hipStream_t s = c->stream;
-  if (!c->maps.p) return GS_ERR_STATE;
   HIPCHK(hipMemsetAsync(c->n1_s1.p, 0, npix * sizeof(double), s));
   HIPCHK(hipMemsetAsync(c->n1_s2.p, 0, npix * sizeof(double), s));
-  HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), s));
   const MapRef* maps = c->maps.as<MapRef>();
   // Batches alternate between the two lane streams, each with its own
   // intermediate, partial sums and run keys, so that pass 1 of one batch runs
@@ -785,43 +800,61 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
   }
   launch_norm1_stats(c->n1_s1.as<double>(), c->n1_s2.as<double>(), (long)npix, (double)c->N * P,
                      c->n1_norm.p, s);
-  const long kb = norm1_key_bytes(L);
-  for (int ip = psi_begin; ip < psi_end; ++ip) {
-    CHK(rotate_templates(c, ip));
-    CHK(lanes_start());
-    for (int m0 = 0, k = 0; m0 < nm; m0 += nb, ++k) {
-      const int cnt = std::min(nb, nm - m0), lane = k % 2;
-      hipStream_t ls = c->lane_stream[lane];
-      void* keys = static_cast<char*>(c->n1_keys.p) + (size_t)lane * nb * kb;
-      pass1(m0, cnt, lane);
-      launch_norm1_zrun(L, c->G[lane].as<float2>(), maps + m0, cnt, c->n1_norm.p, keys,
-                        c->twM.as<float2>(), ls);
-      launch_norm1_runfin(L, keys, cnt, (ip - psi_begin) * nm, c->T, maps + m0,
-                          c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
-                          c->sel_count.as<int>(), c->n1_rec.p, cap, ls);
+  // second pass: every run record is counted, the first `cap` are written;
+  // if more were found the record buffer grows and the pass runs again
+  auto pass2 = [&](long long cap) -> int {
+    HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(unsigned long long), s));
+    for (int ip = psi_begin; ip < psi_end; ++ip) {
+      CHK(rotate_templates(c, ip));
+      CHK(lanes_start());
+      for (int m0 = 0, k = 0; m0 < nm; m0 += nb, ++k) {
+        const int cnt = std::min(nb, nm - m0), lane = k % 2;
+        hipStream_t ls = c->lane_stream[lane];
+        void* keys = static_cast<char*>(c->n1_keys.p) + (size_t)lane * nb * kb;
+        pass1(m0, cnt, lane);
+        launch_norm1_zrun(L, c->G[lane].as<float2>(), maps + m0, cnt, c->n1_norm.p, keys,
+                          c->twM.as<float2>(), ls);
+        launch_norm1_runfin(L, keys, cnt, (long long)(ip - psi_begin) * nm, c->T, maps + m0,
+                            c->tdesc.as<TileDesc>(), c->p.threshold, c->p.diameter,
+                            c->sel_count.as<unsigned long long>(), c->n1_rec.p, cap, ls);
+      }
+      CHK(lanes_join());
     }
-    CHK(lanes_join());
+    return GS_OK;
+  };
+  auto read_count = [&](unsigned long long* n) -> int {
+    HIPCHK(hipMemcpyAsync(n, c->sel_count.p, sizeof(*n), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    return GS_OK;
+  };
+  long long cap = (long long)(c->n1_rec.bytes / sizeof(N1Rec));
+  unsigned long long count = 0;
+  CHK(pass2(cap));
+  CHK(read_count(&count));
+  if (count > (unsigned long long)cap) {
+    cap = (long long)count;
+    CHK(c->n1_rec.ensure((size_t)cap * sizeof(N1Rec)));
+    CHK(pass2(cap));
+    unsigned long long again = 0;
+    CHK(read_count(&again));
+    if (again != count) return GS_ERR_DEVICE;  // the second pass is deterministic
   }
-  int count = 0;
-  HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  HIPCHK(hipGetLastError());
-  if (count > cap) return GS_ERR_CAPACITY;
   std::vector<N1Rec> recs(count);
   if (count) CHK(copy_out(c, recs.data(), c->n1_rec.p, (size_t)count * sizeof(N1Rec)));
-  // canonical order: micrograph, psi, tile, view, block (main.cu:784-819);
-  // r.m = psi offset * nm + view * T + tile
-  const int nblk = (int)((L2 + 1023) / 1024);
+  // canonical order: micrograph, psi, tile, view, run (main.cu:784-819);
+  // r.m = psi offset * nm + view * T + tile, the run is pos / 1024
+  const long long nblk = (L2 + 1023) / 1024;
   auto key = [&](const N1Rec& r) {
-    const int ipl = r.m / nm, mm = r.m % nm;
-    const MapRef mr{mm % c->T, mm / c->T};
-    return (((long long)ipl * c->T + mr.tile) * c->N + mr.view) * nblk + r.block;
+    const long long ipl = r.m / nm, mm = r.m % nm;
+    const long long tile = mm % c->T, view = mm / c->T;
+    return ((ipl * c->T + tile) * c->N + view) * nblk + r.pos / 1024;
   };
   std::sort(recs.begin(), recs.end(), [&](const N1Rec& a, const N1Rec& b) { return key(a) < key(b); });
   picks.resize(count);
-  for (int i = 0; i < count; ++i) {
+  for (size_t i = 0; i < count; ++i) {
     const N1Rec& r = recs[i];
-    const int ip = r.m / nm + psi_begin, mm = r.m % nm;
+    const int ip = (int)(r.m / nm) + psi_begin, mm = (int)(r.m % nm);
     const int t = mm % c->T;
     int mi = 0;
     while (mi + 1 < (int)c->mics.size() && c->mics[mi + 1].tile0 <= t) ++mi;
@@ -1070,6 +1103,14 @@ int gs_create(const gs_params* p, int device, void** out) {
     ok = hipStreamCreateWithFlags(&c->lane_stream[i], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&c->ev_lane[i], hipEventDisableTiming) == hipSuccess;
   ok = ok && hipEventCreateWithFlags(&c->ev_ready, hipEventDisableTiming) == hipSuccess;
+  // in-kernel dependency waits give up after 2 s of the device's wall clock
+  int rate_khz = 0;
+  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+      rate_khz <= 0) {
+    (void)hipGetLastError();
+    rate_khz = 100000;  // the 100 MHz constant clock of CDNA parts
+  }
+  c->wait_ticks = 2000ll * rate_khz;
   if (!ok) {
     (void)hipGetLastError();
     delete c;
@@ -1231,6 +1272,13 @@ int gs_set_tile_block(void* ctx, int tiles) {
   return GS_OK;
 }
 
+int gs_set_record_capacity(void* ctx, long long records) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || records < 0) return GS_ERR_ARG;
+  c->rec_cap0 = records ? records : (1ll << 22);
+  return GS_OK;
+}
+
 int gs_set_batch(void* ctx, int maps_per_batch) {
   Ctx* c = static_cast<Ctx*>(ctx);
   if (!c || maps_per_batch < 0) return GS_ERR_ARG;
@@ -1247,6 +1295,7 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   Ctx* c = static_cast<Ctx*>(ctx);
   if (!c || !n_out) return GS_ERR_ARG;
   *n_out = 0;
+  c->picks.clear();  // a failed pick leaves no stale list for gs_get_picks
   if (c->N == 0 || !c->have_mic) return GS_ERR_STATE;
   if (c->p.norm_type != 0 && c->p.norm_type != 1) return GS_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->device));
@@ -1283,23 +1332,39 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
     c->profile_now = false;
     CHK(r);
   }
-  // device-side selection, then canonical order on the host
+  // device-side selection, then canonical order on the host: every selected
+  // map is counted (64-bit), the first `cap` records are written; a larger
+  // count grows the record buffer and selects again
   const long long nmaps = npsi * per_psi;
-  // room for every map: a low threshold can select most of them
-  const int capsel = (int)std::max<long long>(nmaps, 1);
-  CHK(c->sel.ensure((size_t)capsel * sizeof(SelRec)));
-  CHK(c->sel_count.ensure(sizeof(int)));
-  HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(int), c->stream));
-  launch_select(c->stats.as<MapStat>(), nmaps, c->T, c->N, c->L, c->tdesc.as<TileDesc>(),
-                c->p.threshold, c->p.diameter, c->sel_count.as<int>(), c->sel.as<SelRec>(), capsel,
-                c->stream);
+  CHK(c->sel.ensure((size_t)std::max(1ll, std::min(nmaps, c->rec_cap0)) * sizeof(SelRec)));
+  CHK(c->sel_count.ensure(sizeof(unsigned long long)));
+  auto select = [&](long long cap) -> int {
+    HIPCHK(hipMemsetAsync(c->sel_count.p, 0, sizeof(unsigned long long), c->stream));
+    launch_select(c->stats.as<MapStat>(), nmaps, c->T, c->N, c->L, c->tdesc.as<TileDesc>(),
+                  c->p.threshold, c->p.diameter, c->sel_count.as<unsigned long long>(),
+                  c->sel.as<SelRec>(), cap, c->stream);
+    return GS_OK;
+  };
+  long long capsel = (long long)(c->sel.bytes / sizeof(SelRec));
+  CHK(select(capsel));
   HIPCHK(hipEventRecord(e1, c->stream));
-  int count = 0, err = 0;
-  HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  unsigned long long count = 0;
+  int err = 0;
+  HIPCHK(hipMemcpyAsync(&count, c->sel_count.p, sizeof(count), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(&err, c->err.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipGetLastError());
   if (err) return GS_ERR_DEVICE;  // an in-kernel dependency wait ran out
+  if (count > (unsigned long long)capsel) {
+    if (count > (unsigned long long)nmaps) return GS_ERR_DEVICE;  // one record per map at most
+    capsel = (long long)count;
+    CHK(c->sel.ensure((size_t)capsel * sizeof(SelRec)));
+    CHK(select(capsel));
+    unsigned long long again = 0;
+    HIPCHK(hipMemcpyAsync(&again, c->sel_count.p, sizeof(again), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (again != count) return GS_ERR_DEVICE;
+  }
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   c->timing.pick_total_ms = ms;
@@ -1325,12 +1390,11 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
     c->timing.corr_psi_ms = c->timing.corr_ms;
   }
   if (c->prof_batches) c->timing.corr_sampled_maps = per_psi;
-  if (count > capsel) return GS_ERR_DEVICE;  // cannot happen: one record per map at most
   std::vector<SelRec> recs(count);
   if (count)
     CHK(copy_out(c, recs.data(), c->sel.p, (size_t)count * sizeof(SelRec)));
   c->picks.resize(count);
-  for (int i = 0; i < count; ++i) {
+  for (size_t i = 0; i < count; ++i) {
     const long long m = recs[i].m;
     const int ip = (int)(m / per_psi) + psi_begin;
     const long long rem = m % per_psi;
@@ -1347,13 +1411,13 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   }
   // one micrograph after the other, each in the reference's psi, tile, view order
   std::vector<long long> order(count);
-  for (int i = 0; i < count; ++i) order[i] = i;
+  for (size_t i = 0; i < count; ++i) order[i] = (long long)i;
   std::sort(order.begin(), order.end(), [&](long long a, long long b) {
     if (c->picks[a].mic != c->picks[b].mic) return c->picks[a].mic < c->picks[b].mic;
     return recs[a].m < recs[b].m;
   });
   std::vector<gs_particle> sorted(count);
-  for (int i = 0; i < count; ++i) sorted[i] = c->picks[order[i]];
+  for (size_t i = 0; i < count; ++i) sorted[i] = c->picks[order[i]];
   c->picks.swap(sorted);
   return gs_get_picks(ctx, out, cap, n_out);
 }

@@ -578,15 +578,20 @@ __global__ __launch_bounds__(WG) void k_corr_fused(
   }
 }
 
-// Bounded wait of thread 0 until *ctr >= target (agent-scope loads).  A wait
-// that runs out (about 2^18 polls) sets *err and gives up, and once *err is
-// set every later wait returns at once, so the launch always drains; the host
+// Bounded wait of thread 0 until *ctr >= target.  The counter load is an
+// agent-scope acquire and every thread then runs an agent-scope acquire
+// fence, so the intermediate the producers wrote before their release is
+// visible to the whole workgroup.  A wait that runs out (wall-clock budget
+// of `ticks` counter ticks) sets *err and gives up, and once *err is set
+// every later wait returns at once, so the launch always drains; the host
 // then reports the error instead of results.
-__device__ __forceinline__ void wait_count(const int* ctr, int target, int* err) {
+__device__ __forceinline__ void wait_count(const int* ctr, int target, int* err,
+                                           long long ticks) {
   if (threadIdx.x == 0) {
-    int it = 0;
-    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (++it > (1 << 18) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    const unsigned long long t0 = wall_clock64();
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if ((long long)(wall_clock64() - t0) > ticks ||
+          __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -594,12 +599,14 @@ __device__ __forceinline__ void wait_count(const int* ctr, int target, int* err)
     }
   }
   __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
-// every thread's stores complete (vmcnt(0)), then thread 0 counts them
+// every thread's stores are released at agent scope before the barrier, then
+// thread 0 counts the workgroup (a release increment)
 __device__ __forceinline__ void signal_count(int* ctr) {
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's stores done
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Whole-psi correlation in one launch.  Map m belongs to XCD m % 8: all its
@@ -621,8 +628,8 @@ __global__ __launch_bounds__(WG) void k_corr_stream(
     const float2* __restrict__ tiles, const float2* __restrict__ tspec,
     const MapRef* __restrict__ maps, int nmaps, int lag, int ring, float2* __restrict__ Gring,
     float2* __restrict__ psum, int* __restrict__ ready, int* __restrict__ done, int blo, int bhi,
-    CorrPartial* __restrict__ part, int* __restrict__ err, const float2* __restrict__ twL,
-    const float2* __restrict__ twM) {
+    CorrPartial* __restrict__ part, int* __restrict__ err, long long wait_ticks,
+    const float2* __restrict__ twL, const float2* __restrict__ twM) {
   using G = Geo<L>;
   constexpr int NCB = G::NCB, NCB8 = (NCB + 7) / 8;
   constexpr int NP = (L + G::RB - 1) / G::RB;
@@ -645,14 +652,14 @@ __global__ __launch_bounds__(WG) void k_corr_stream(
   if (kind == 0) {
     const int m = 8 * r + x, cb = k;
     if (m >= nmaps || cb >= NCB) return;
-    if (m >= 8 * ring) wait_count(done + (m - 8 * ring), NP, err);
+    if (m >= 8 * ring) wait_count(done + (m - 8 * ring), NP, err, wait_ticks);
     corr_pass1_body<L, true>(lds, red_f, cb, m, (r % ring) * 8 + x, tiles, tspec, maps, Gring,
                              psum, twL);
     signal_count(ready + m);
   } else {
     const int m = 8 * (r - lag) + x, rb = k;
     if (r < lag || m >= nmaps || rb >= NP) return;
-    wait_count(ready + m, NCB, err);
+    wait_count(ready + m, NCB, err, wait_ticks);
     corr_pass2_body<L, true>(lds, red_f, red_i, m, ((r - lag) % ring) * 8 + x, rb, Gring, blo,
                              bhi, part, NP, twM);
     signal_count(done + m);
@@ -830,10 +837,11 @@ __global__ __launch_bounds__(WG) void k_norm1_zrun(const float2* __restrict__ Gi
 // and centre bounds of the host loop (main.cu:796-819); records carry the
 // map's canonical index mbase + view * ntiles + tile.
 template <int L>
-__global__ void k_norm1_runfin(const unsigned long long* __restrict__ keys, int mbase,
+__global__ void k_norm1_runfin(const unsigned long long* __restrict__ keys, long long mbase,
                                int ntiles, const MapRef* __restrict__ maps,
                                const TileDesc* __restrict__ tdesc, float thres, float dmin,
-                               int* __restrict__ count, N1Rec* __restrict__ out, int cap) {
+                               unsigned long long* __restrict__ count, N1Rec* __restrict__ out,
+                               long long cap) {
   constexpr long L2 = (long)L * L;
   constexpr int NBLK = (int)((L2 + 1023) / 1024);
   constexpr int RB = Geo<L>::RB;
@@ -849,11 +857,11 @@ __global__ void k_norm1_runfin(const unsigned long long* __restrict__ keys, int 
   const TileDesc td = tdesc[maps[m].tile];
   const int cx = td.ox + p % L, cy = td.oy + p / L;
   if (cx >= dmin && cx < td.nx - dmin && cy >= dmin && cy < td.ny - dmin) {
-    const int i = atomicAdd(count, 1);
-    if (i < cap) {
+    const unsigned long long i = atomicAdd(count, 1ull);
+    if (i < (unsigned long long)cap) {
       N1Rec r;
-      r.m = mbase + maps[m].view * ntiles + maps[m].tile;
-      r.block = k; r.pos = p; r.z = v;
+      r.m = mbase + (long long)maps[m].view * ntiles + maps[m].tile;
+      r.pos = p; r.z = v;
       out[i] = r;
     }
   }
@@ -1331,8 +1339,8 @@ static void corr_pass2_t(const float2* G, int nmaps, int blo, int bhi, CorrParti
 template <int L>
 static void corr_stream_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
                           int lag, int ring, float2* Gring, float2* psum, int* ready, int* done,
-                          int blo, int bhi, CorrPartial* part, int* err, const float2* twL,
-                          const float2* twM, hipStream_t s) {
+                          int blo, int bhi, CorrPartial* part, int* err, long long wait_ticks,
+                          const float2* twL, const float2* twM, hipStream_t s) {
   using G = Geo<L>;
   constexpr int NP = (L + G::RB - 1) / G::RB;
   constexpr int SR = 8 * ((G::NCB + 7) / 8) + 8 * ((NP + 7) / 8);
@@ -1342,7 +1350,8 @@ static void corr_stream_t(const float2* tiles, const float2* tspec, const MapRef
   const long rounds = (nmaps + 7) / 8 + lag;
   const long nwg = 8l * SR * rounds;
   hipLaunchKernelGGL((k_corr_stream<L>), dim3((unsigned)nwg), dim3(WG), 0, s, tiles, tspec, maps,
-                     nmaps, lag, ring, Gring, psum, ready, done, blo, bhi, part, err, twL, twM);
+                     nmaps, lag, ring, Gring, psum, ready, done, blo, bhi, part, err, wait_ticks,
+                     twL, twM);
 }
 
 // integer form of get_peak_and_SUM's exclusion: x < lo || x > hi
@@ -1363,12 +1372,12 @@ void launch_fill_int(int* p, long n, int v, hipStream_t s) {
 }
 void launch_corr_stream(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
                         int nmaps, int lag, int ring, float2* Gring, float2* psum, int* counters,
-                        float lo, float hi, CorrPartial* part, int* err, const float2* twL,
-                        const float2* twM, hipStream_t s) {
+                        float lo, float hi, CorrPartial* part, int* err, long long wait_ticks,
+                        const float2* twL, const float2* twM, hipStream_t s) {
   int blo, bhi;
   border_range(L, lo, hi, &blo, &bhi);
   DISPATCH(L, corr_stream_t, tiles, tspec, maps, nmaps, lag, ring, Gring, psum, counters,
-           counters + nmaps, blo, bhi, part, err, twL, twM, s);
+           counters + nmaps, blo, bhi, part, err, wait_ticks, twL, twM, s);
 }
 void launch_corr_fused(int L, const float2* tiles, const float2* tspec, const MapRef* maps1,
                        int nmaps1, float2* G1, float2* psum1, const float2* G2, int nmaps2,
@@ -1479,16 +1488,17 @@ void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, co
   DISPATCH(L, norm1_zrun_t, G, maps, nmaps, norm, static_cast<unsigned long long*>(keys), twM, s);
 }
 template <int L>
-static void norm1_runfin_t(const unsigned long long* keys, int nmaps, int mbase, int ntiles,
-                           const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
-                           int* count, N1Rec* out, int cap, hipStream_t s) {
+static void norm1_runfin_t(const unsigned long long* keys, int nmaps, long long mbase,
+                           int ntiles, const MapRef* maps, const TileDesc* tdesc, float thres,
+                           float dmin, unsigned long long* count, N1Rec* out, long long cap,
+                           hipStream_t s) {
   const int nblk = (int)(((long)L * L + 1023) / 1024);
   hipLaunchKernelGGL(k_norm1_runfin<L>, dim3((nblk + 255) / 256, nmaps), dim3(256), 0, s, keys,
                      mbase, ntiles, maps, tdesc, thres, dmin, count, out, cap);
 }
-void launch_norm1_runfin(int L, const void* keys, int nmaps, int mbase, int ntiles,
+void launch_norm1_runfin(int L, const void* keys, int nmaps, long long mbase, int ntiles,
                          const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
-                         int* count, void* out, int cap, hipStream_t s) {
+                         unsigned long long* count, void* out, long long cap, hipStream_t s) {
   DISPATCH(L, norm1_runfin_t, static_cast<const unsigned long long*>(keys), nmaps, mbase, ntiles,
            maps, tdesc, thres, dmin, count, static_cast<N1Rec*>(out), cap, s);
 }
@@ -1502,7 +1512,8 @@ void launch_norm1_stats(const double* s1, const double* s2, long n, double total
 
 __global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T, int N, int L,
                          const TileDesc* __restrict__ tdesc, float thres, float dmin,
-                         int* __restrict__ count, SelRec* __restrict__ out, int cap) {
+                         unsigned long long* __restrict__ count, SelRec* __restrict__ out,
+                         long long cap) {
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nmaps) return;
   const MapStat s = st[m];
@@ -1512,8 +1523,8 @@ __global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T,
   const float cx = (float)(td.ox + s.pos % L);
   const float cy = (float)(td.oy + s.pos / L);
   if (cx >= dmin && cx < td.nx - dmin && cy >= dmin && cy < td.ny - dmin) {
-    const int i = atomicAdd(count, 1);
-    if (i < cap) {
+    const unsigned long long i = atomicAdd(count, 1ull);
+    if (i < (unsigned long long)cap) {
       SelRec r;
       r.m = m; r.cx = cx; r.cy = cy; r.score = s.score; r.pad = 0;
       out[i] = r;
@@ -1522,7 +1533,8 @@ __global__ void k_select(const MapStat* __restrict__ st, long long nmaps, int T,
 }
 
 void launch_select(const MapStat* st, long long nmaps, int T, int N, int L, const TileDesc* tdesc,
-                   float thres, float dmin, int* count, SelRec* out, int cap, hipStream_t s) {
+                   float thres, float dmin, unsigned long long* count, SelRec* out,
+                   long long cap, hipStream_t s) {
   if (nmaps <= 0) return;
   hipLaunchKernelGGL(k_select, dim3((unsigned)((nmaps + 255) / 256)), dim3(256), 0, s, st, nmaps, T,
                      N, L, tdesc, thres, dmin, count, out, cap);

@@ -121,11 +121,12 @@ void launch_corr_fused(int L, const float2* tiles, const float2* tspec, const Ma
 void launch_fill_int(int* p, long n, int v, hipStream_t s);
 // one launch for all maps of a psi: pass 1 and pass 2 with in-kernel
 // dependencies through a ring of `ring` intermediates (lag < ring);
-// counters: 2 nmaps ints (zeroed by the launcher); *err set if a wait ran out
+// counters: 2 nmaps ints (zeroed by the launcher); *err set if a wait ran
+// out (wait_ticks: budget of one wait in wall-clock counter ticks)
 void launch_corr_stream(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
                         int nmaps, int lag, int ring, float2* Gring, float2* psum, int* counters,
-                        float lo, float hi, CorrPartial* part, int* err, const float2* twL,
-                        const float2* twM, hipStream_t s);
+                        float lo, float hi, CorrPartial* part, int* err, long long wait_ticks,
+                        const float2* twL, const float2* twM, hipStream_t s);
 // multiply tile half spectra by (-1)^(kx+ky) sgn (compute_corner_CCG's factor)
 void launch_checker_sign(int L, float2* spec, int ntiles, hipStream_t s);
 int corr_ncb(int L);
@@ -153,18 +154,24 @@ void launch_norm1_stats(const double* s1, const double* s2, long n, double total
 void launch_norm1_zrun(int L, const float2* G, const MapRef* maps, int nmaps, const void* norm,
                        void* keys, const float2* twM, hipStream_t s);
 long norm1_key_bytes(int L);
-// records {int m, block, pos; float z} of 1024-pixel runs above threshold;
-// m = mbase + view * ntiles + tile (canonical view-major map index)
+// records of 1024-pixel runs above threshold: m = mbase + view * ntiles +
+// tile (canonical view-major map index), pos = y * L + x of the run maximum
+// (its run is pos / 1024), z its z-score.  *count (64-bit) counts every
+// record; the first cap are written, so a caller that reads count > cap
+// grows the buffer and runs again.
 struct N1Rec {
-  int m, block, pos;
+  long long m;
+  int pos;
   float z;
 };
-void launch_norm1_runfin(int L, const void* keys, int nmaps, int mbase, int ntiles,
+void launch_norm1_runfin(int L, const void* keys, int nmaps, long long mbase, int ntiles,
                          const MapRef* maps, const TileDesc* tdesc, float thres, float dmin,
-                         int* count, void* out, int cap, hipStream_t s);
-// threshold + centre-bounds selection of norm_type 0 maps (main.cu:871-891)
+                         unsigned long long* count, void* out, long long cap, hipStream_t s);
+// threshold + centre-bounds selection of norm_type 0 maps (main.cu:871-891);
+// *count counts every selected map, the first cap records are written
 void launch_select(const MapStat* st, long long nmaps, int T, int N, int L, const TileDesc* tdesc,
-                   float thres, float dmin, int* count, SelRec* out, int cap, hipStream_t s);
+                   float thres, float dmin, unsigned long long* count, SelRec* out,
+                   long long cap, hipStream_t s);
 int norm1_record_bytes();
 
 // ---- template preparation ----

@@ -347,6 +347,12 @@ int gs_set_pipeline(void* ctx, int mode, int lag, int ring);
  * 192 MiB, so the block's tile spectra stay in the last-level cache (96 at
  * window 720: one 5760 x 4092 micrograph is one block). */
 int gs_set_tile_block(void* ctx, int tiles);
+/* Initial capacity, in records, of the device-side pick buffers (maps that
+ * pass the threshold for norm_type 0, 24 bytes each; 1024-pixel runs for
+ * norm_type 1, 16 bytes each).  A gs_pick that finds more grows the buffer
+ * to the exact count and selects again, so the capacity changes only memory
+ * and time, never the picks.  0 restores the default (4,194,304). */
+int gs_set_record_capacity(void* ctx, long long records);
 int gs_destroy(void* ctx);
 const char* gs_strerror(int code);
 

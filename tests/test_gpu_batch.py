@@ -207,3 +207,55 @@ def test_tile_block_order_is_invisible(gpu, tile_block):
     p1, s1 = run(tile_block)
     assert np.array_equal(s0, s1)
     assert np.array_equal(p0, p1)
+
+
+def test_launch_schedule_with_profiling(gpu):
+    """The launch-per-batch schedule with per-launch profiling on, in batches
+    of 189 maps (480 maps: two full batches and a ragged one), gives the
+    two-stream schedule's bits; the profiling events of every schedule are
+    created on first use, however many launches a psi takes."""
+    import gisspa_amd
+    c = _cases.tiny(seed=8, l=96, n_views=24)
+    out = {}
+    for mode in (gisspa_amd.GS_PIPE_LANES, gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM):
+        with gisspa_amd.Picker(0, **c.params) as pk:
+            pk.set_pipeline(mode)
+            pk.set_batch(189)
+            pk.set_profiling(True)
+            pk.set_templates(c.templates, c.defocus)
+            pk.set_micrograph(c.image, c.defocus)
+            info = pk.info()
+            assert info.n_tiles * info.n_views == 480
+            out[mode] = (pk.pick(), pk.mapstats(), pk.timing())
+    a = out[gisspa_amd.GS_PIPE_LANES]
+    for mode in (gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM):
+        b = out[mode]
+        assert np.array_equal(a[1], b[1]), mode
+        assert np.array_equal(a[0], b[0]), mode
+        assert b[2].corr_launches >= 1
+    assert a[2].corr_launches == 3 and out[gisspa_amd.GS_PIPE_LAUNCH][2].corr_launches == 4
+
+
+@pytest.mark.parametrize("norm_type", [0, 1])
+def test_record_buffer_grows(gpu, norm_type):
+    """A device-side record buffer smaller than the selection (capacity 1 or
+    7 records) is grown to the exact count and the selection runs again: the
+    picks equal those of the default capacity, also on a second search in the
+    same context."""
+    import gisspa_amd
+    c = _cases.tiny(seed=5, l=96, threshold=3.0 if norm_type else 4.0)
+    c.params["norm_type"] = norm_type
+    runs = {}
+    for cap in (0, 1, 7):
+        with gisspa_amd.Picker(0, **c.params) as pk:
+            if cap:
+                pk.set_record_capacity(cap)
+            pk.set_templates(c.templates, c.defocus)
+            pk.set_micrograph(c.image, c.defocus)
+            runs[cap] = pk.pick()
+            if cap == 7:
+                assert np.array_equal(pk.pick(), runs[cap])
+    print("norm_type", norm_type, "picks", len(runs[0]))
+    assert len(runs[0]) > 7
+    for cap in (1, 7):
+        assert np.array_equal(runs[cap], runs[0]), cap
