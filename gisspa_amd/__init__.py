@@ -167,6 +167,7 @@ def load() -> ctypes.CDLL:
     lib.gs_destroy.argtypes = [vp]
     lib.gs_remove_repeats.argtypes = [ctypes.c_int, vp, ctypes.c_size_t, ctypes.c_double,
                                       ctypes.c_double, vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.gs_remove_repeats_noalloc.argtypes = lib.gs_remove_repeats.argtypes
     lib.gs_project.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
     lib.gs_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(GsTune)]
     lib.gs_set_templates_mixed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), vp,

@@ -27,6 +27,8 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gisspa.h"
@@ -371,24 +373,106 @@ struct StreamGuard {
   }
 };
 
-int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center, double thr,
-                   unsigned char* keep, size_t* n_kept) {
+__global__ void k_dd_fill(int* __restrict__ p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void fill_int(int* p, int n, int v, hipStream_t st) {
+  if (n > 0) k_dd_fill<<<(n + 255) / 256, 256, 0, st>>>(p, n, v);
+}
+
+// Device memory of one call.  Owning: every request is a fresh allocation,
+// freed when the arena goes.  Scratch: requests are 256-byte aligned slices
+// of a caller-free region (nothing is allocated); a request beyond its end
+// is GS_ERR_CAPACITY.
+class DevArena {
+ public:
+  DevArena() = default;
+  DevArena(void* base, size_t cap) : base_(static_cast<char*>(base)), cap_(cap) {}
+  template <class T>
+  int get(size_t count, T** out) {
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (!base_) {
+      owned_.emplace_back(new DevBuf);
+      CHK(owned_.back()->ensure(bytes));
+      *out = static_cast<T*>(owned_.back()->p);
+      return GS_OK;
+    }
+    const size_t at = (used_ + 255) & ~size_t(255);
+    if (at > cap_ || bytes > cap_ - at) return GS_ERR_CAPACITY;
+    *out = reinterpret_cast<T*>(base_ + at);
+    used_ = at + bytes;
+    return GS_OK;
+  }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+  std::vector<std::unique_ptr<DevBuf>> owned_;
+};
+
+// Host staging of one call: std::vector storage, or slices of a static
+// region (the allocation-free entry point, which holds its lock meanwhile).
+class HostArena {
+ public:
+  HostArena() = default;
+  HostArena(void* base, size_t cap) : base_(static_cast<char*>(base)), cap_(cap) {}
+  template <class T>
+  int get(size_t count, T** out) {
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (!base_) {
+      owned_.emplace_back(bytes);
+      *out = reinterpret_cast<T*>(owned_.back().data());
+      return GS_OK;
+    }
+    const size_t at = (used_ + 63) & ~size_t(63);
+    if (at > cap_ || bytes > cap_ - at) return GS_ERR_CAPACITY;
+    *out = reinterpret_cast<T*>(base_ + at);
+    used_ = at + bytes;
+    return GS_OK;
+  }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+  std::vector<std::vector<unsigned char>> owned_;
+};
+
+int check_args(const gs_pick_rec* recs, size_t n_, double center, double thr, unsigned char* keep,
+               size_t* n_kept) {
   if (n_kept) *n_kept = 0;
   if ((!recs || !keep) && n_ > 0) return GS_ERR_ARG;
   if (n_ > (size_t)(INT_MAX / 16)) return GS_ERR_ARG;
   if (!(center == center) || !(thr == thr)) return GS_ERR_ARG;  // NaN
-  const int n = (int)n_;
-  if (n == 0) return GS_OK;
+  return GS_OK;
+}
+
+int select_device(int device) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
     (void)hipGetLastError();
     return GS_ERR_DEVICE;  // as gs_create
   }
   HIPCHK(hipSetDevice(device));
+  return GS_OK;
+}
 
+// The duplicate removal of n (>= 1) picks on stream st, device buffers from
+// `dev`, host staging from `host`; amb_cap0 is the first capacity of the
+// list of undecided pairs (a larger count repeats the counting pass with the
+// exact capacity, within max_amb).
+int remove_repeats_run(DevArena& dev, HostArena& host, hipStream_t st, const gs_pick_rec* recs,
+                       int n, double center, double thr, unsigned char* keep, size_t* n_kept,
+                       int amb_cap0, long long max_amb) {
   // host side: structure of arrays, Euler matrices, group ranges
-  std::vector<double> x(n), y(n), s(n), e((size_t)n * 9);
-  std::vector<int> glo(n), ghi(n);
+  double *x, *y, *s, *e;
+  int *glo, *ghi;
+  CHK(host.get(n, &x));
+  CHK(host.get(n, &y));
+  CHK(host.get(n, &s));
+  CHK(host.get((size_t)n * 9, &e));
+  CHK(host.get(n, &glo));
+  CHK(host.get(n, &ghi));
   for (int i = 0; i < n; ++i) {
     x[i] = recs[i].x;
     y[i] = recs[i].y;
@@ -405,33 +489,30 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
     lo = hi;
   }
 
-  StreamGuard sg;
-  HIPCHK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
-  hipStream_t st = sg.s;
-  DevBuf dx, dy, ds, de, dglo, dghi, dcnt, doff, doff2, dtot, dtotal, dbump, dnbr, damb_n,
-      damb_pair, damb_q0, dkeys, dstate, dopen, dfirst, dkeep;
+  double *dx, *dy, *ds, *de;
+  int *dglo, *dghi, *doff, *damb_n;
   const size_t nd = (size_t)n * sizeof(double), ni = (size_t)n * sizeof(int);
-  CHK(dx.ensure(nd));
-  CHK(dy.ensure(nd));
-  CHK(ds.ensure(nd));
-  CHK(de.ensure(nd * 9));
-  CHK(dglo.ensure(ni));
-  CHK(dghi.ensure(ni));
-  CHK(doff.ensure(ni + sizeof(int)));
-  CHK(damb_n.ensure(sizeof(int)));
-  HIPCHK(hipMemcpyAsync(dx.p, x.data(), nd, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dy.p, y.data(), nd, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ds.p, s.data(), nd, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(de.p, e.data(), nd * 9, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dglo.p, glo.data(), ni, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dghi.p, ghi.data(), ni, hipMemcpyHostToDevice, st));
+  CHK(dev.get(n, &dx));
+  CHK(dev.get(n, &dy));
+  CHK(dev.get(n, &ds));
+  CHK(dev.get((size_t)n * 9, &de));
+  CHK(dev.get(n, &dglo));
+  CHK(dev.get(n, &dghi));
+  CHK(dev.get((size_t)n + 1, &doff));
+  CHK(dev.get(1, &damb_n));
+  HIPCHK(hipMemcpyAsync(dx, x, nd, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dy, y, nd, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ds, s, nd, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(de, e, nd * 9, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dglo, glo, ni, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dghi, ghi, ni, hipMemcpyHostToDevice, st));
 
   DedupArgs a;
-  a.x = dx.as<double>();
-  a.y = dy.as<double>();
-  a.e = de.as<double>();
-  a.glo = dglo.as<int>();
-  a.ghi = dghi.as<int>();
+  a.x = dx;
+  a.y = dy;
+  a.e = de;
+  a.glo = dglo;
+  a.ghi = dghi;
   a.n = n;
   a.center = center;
   // every r2 with sqrt(r2) < center satisfies r2 <= center^2 (1 + 1e-12)
@@ -447,92 +528,97 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
   const int nch = (maxrange + ch - 1) / ch;
   const dim3 grid2(blocks, nch);
   const size_t nslot = (size_t)n * nch;
-  CHK(dcnt.ensure(nslot * sizeof(int)));
-  CHK(doff2.ensure(nslot * sizeof(int)));
-  CHK(dtot.ensure(ni));
-  CHK(dtotal.ensure(sizeof(long long)));
+  int *dcnt, *doff2, *dtot;
+  long long* dtotal;
+  CHK(dev.get(nslot, &dcnt));
+  CHK(dev.get(nslot, &doff2));
+  CHK(dev.get(n, &dtot));
+  CHK(dev.get(1, &dtotal));
 
   // 1. counts and undecided pairs (a second pass if the pair list overflowed)
-  int amb_cap = 1 << 16, amb_n = 0;
+  int amb_cap = amb_cap0, amb_n = 0;
+  int2* damb_pair;
+  double* damb_q0;
   for (;;) {
-    CHK(damb_pair.ensure((size_t)amb_cap * sizeof(int2)));
-    CHK(damb_q0.ensure((size_t)amb_cap * sizeof(double)));
-    HIPCHK(hipMemsetAsync(damb_n.p, 0, sizeof(int), st));
-    k_dd_pairs<false><<<grid2, DWG, 0, st>>>(a, ch, dcnt.as<int>(), nullptr, nullptr, nullptr,
-                                              damb_n.as<int>(), amb_cap, damb_pair.as<int2>(),
-                                              damb_q0.as<double>(), nullptr, 0);
+    CHK(dev.get(amb_cap, &damb_pair));
+    CHK(dev.get(amb_cap, &damb_q0));
+    fill_int(damb_n, 1, 0, st);
+    k_dd_pairs<false><<<grid2, DWG, 0, st>>>(a, ch, dcnt, nullptr, nullptr, nullptr, damb_n,
+                                              amb_cap, damb_pair, damb_q0, nullptr, 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(&amb_n, damb_n.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&amb_n, damb_n, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (amb_n <= amb_cap) break;
+    if (amb_n > max_amb) return GS_ERR_CAPACITY;
     amb_cap = amb_n;
   }
-  std::vector<int2> amb_pair(amb_n);
-  std::vector<double> amb_q0(amb_n);
+  int2* amb_pair;
+  double* amb_q0;
+  CHK(host.get(amb_n, &amb_pair));
+  CHK(host.get(amb_n, &amb_q0));
   if (amb_n > 0) {
-    HIPCHK(hipMemcpyAsync(amb_pair.data(), damb_pair.p, amb_n * sizeof(int2),
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(amb_q0.data(), damb_q0.p, amb_n * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(amb_pair, damb_pair, amb_n * sizeof(int2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(amb_q0, damb_q0, amb_n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   // undecided pairs: the C library's acos decides; a duplicate joins the
   // counts of both picks, in the chunk that holds the other pick
-  std::vector<unsigned long long> keys;
-  std::vector<long> bumps;
+  unsigned long long* keys;
+  long* bumps;
+  CHK(host.get(amb_n, &keys));
+  CHK(host.get((size_t)2 * amb_n, &bumps));
+  int nkeys = 0, nbumps = 0;
   auto chunk_slot = [&](int i, int j) -> long {
     return (long)((j - glo[(i / DWG) * DWG]) / ch) * n + i;
   };
   for (int k = 0; k < amb_n; ++k) {
     if (!euler_close(euler_angle(amb_q0[k]), thr)) continue;
     const int pa = amb_pair[k].x, pb = amb_pair[k].y;
-    keys.push_back((unsigned long long)pa * (unsigned)n + (unsigned)pb);
-    bumps.push_back(chunk_slot(pa, pb));
-    bumps.push_back(chunk_slot(pb, pa));
+    keys[nkeys++] = (unsigned long long)pa * (unsigned)n + (unsigned)pb;
+    bumps[nbumps++] = chunk_slot(pa, pb);
+    bumps[nbumps++] = chunk_slot(pb, pa);
   }
-  std::sort(keys.begin(), keys.end());
-  CHK(dkeys.ensure(std::max<size_t>(1, keys.size()) * sizeof(unsigned long long)));
-  CHK(dbump.ensure(std::max<size_t>(1, bumps.size()) * sizeof(long)));
-  if (!keys.empty()) {
-    HIPCHK(hipMemcpyAsync(dkeys.p, keys.data(), keys.size() * sizeof(unsigned long long),
-                          hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dbump.p, bumps.data(), bumps.size() * sizeof(long),
-                          hipMemcpyHostToDevice, st));
-    const int nb = (int)bumps.size();
-    k_dd_bump<<<(nb + 255) / 256, 256, 0, st>>>(dcnt.as<int>(), dbump.as<long>(), nb);
+  std::sort(keys, keys + nkeys);
+  unsigned long long* dkeys;
+  long* dbump;
+  CHK(dev.get(nkeys, &dkeys));
+  CHK(dev.get(nbumps, &dbump));
+  if (nkeys > 0) {
+    HIPCHK(hipMemcpyAsync(dkeys, keys, nkeys * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                          st));
+    HIPCHK(hipMemcpyAsync(dbump, bumps, nbumps * sizeof(long), hipMemcpyHostToDevice, st));
+    k_dd_bump<<<(nbumps + 255) / 256, 256, 0, st>>>(dcnt, dbump, nbumps);
   }
-  k_dd_chunk_offsets<<<blocks, DWG, 0, st>>>(dcnt.as<int>(), n, nch, doff2.as<int>(),
-                                             dtot.as<int>());
-  k_dd_scan<<<1, SWG, 0, st>>>(dtot.as<int>(), n, doff.as<int>(), dtotal.as<long long>());
+  k_dd_chunk_offsets<<<blocks, DWG, 0, st>>>(dcnt, n, nch, doff2, dtot);
+  k_dd_scan<<<1, SWG, 0, st>>>(dtot, n, doff, dtotal);
   HIPCHK(hipGetLastError());
   long long total = 0;
-  HIPCHK(hipMemcpyAsync(&total, dtotal.p, sizeof(long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&total, dtotal, sizeof(long long), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   // (keeps the host copies of the keys and bumps alive until here)
   if (total > INT_MAX) return GS_ERR_CAPACITY;
-  CHK(dnbr.ensure(std::max<size_t>(1, (size_t)total) * sizeof(int)));
+  int* dnbr;
+  CHK(dev.get((size_t)total, &dnbr));
 
   // 2. neighbour lists
-  k_dd_pairs<true><<<grid2, DWG, 0, st>>>(a, ch, dcnt.as<int>(), doff.as<int>(),
-                                           doff2.as<int>(), dnbr.as<int>(), nullptr, 0, nullptr,
-                                           nullptr, dkeys.as<unsigned long long>(),
-                                           (int)keys.size());
+  k_dd_pairs<true><<<grid2, DWG, 0, st>>>(a, ch, dcnt, doff, doff2, dnbr, nullptr, 0, nullptr,
+                                           nullptr, dkeys, nkeys);
   HIPCHK(hipGetLastError());
 
   // 3. settle the picks that are unmarked at their turn, in rounds
-  CHK(dstate.ensure(ni));
-  CHK(dopen.ensure(kRoundsPerCheck * sizeof(int)));
-  HIPCHK(hipMemsetAsync(dstate.p, 0, ni, st));
+  int *dstate, *dopen;
+  CHK(dev.get(n, &dstate));
+  CHK(dev.get(kRoundsPerCheck, &dopen));
+  fill_int(dstate, n, ST_OPEN, st);
   int rounds = 0;
   for (;;) {
-    HIPCHK(hipMemsetAsync(dopen.p, 0, kRoundsPerCheck * sizeof(int), st));
+    fill_int(dopen, kRoundsPerCheck, 0, st);
     for (int r = 0; r < kRoundsPerCheck; ++r)
-      k_dd_round<<<blocks, DWG, 0, st>>>(doff.as<int>(), dnbr.as<int>(), ds.as<double>(), n,
-                                         dstate.as<int>(), dopen.as<int>() + r);
+      k_dd_round<<<blocks, DWG, 0, st>>>(doff, dnbr, ds, n, dstate, dopen + r);
     HIPCHK(hipGetLastError());
     int open = 0;
-    HIPCHK(hipMemcpyAsync(&open, dopen.as<int>() + kRoundsPerCheck - 1, sizeof(int),
-                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&open, dopen + kRoundsPerCheck - 1, sizeof(int), hipMemcpyDeviceToHost,
+                          st));
     HIPCHK(hipStreamSynchronize(st));
     rounds += kRoundsPerCheck;
     if (open == 0) break;
@@ -540,15 +626,14 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
   }
 
   // 4. final marks
-  CHK(dfirst.ensure(ni));
-  CHK(dkeep.ensure(n));
-  k_dd_first_mark<<<blocks, DWG, 0, st>>>(doff.as<int>(), dnbr.as<int>(), ds.as<double>(), n,
-                                          dstate.as<int>(), dfirst.as<int>());
-  k_dd_keep<<<blocks, DWG, 0, st>>>(doff.as<int>(), dnbr.as<int>(), ds.as<double>(), n,
-                                    dstate.as<int>(), dfirst.as<int>(),
-                                    dkeep.as<unsigned char>());
+  int* dfirst;
+  unsigned char* dkeep;
+  CHK(dev.get(n, &dfirst));
+  CHK(dev.get(n, &dkeep));
+  k_dd_first_mark<<<blocks, DWG, 0, st>>>(doff, dnbr, ds, n, dstate, dfirst);
+  k_dd_keep<<<blocks, DWG, 0, st>>>(doff, dnbr, ds, n, dstate, dfirst, dkeep);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(keep, dkeep.p, n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(keep, dkeep, n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (n_kept) {
     size_t kept = 0;
@@ -558,10 +643,67 @@ int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center
   return GS_OK;
 }
 
+int remove_repeats(int device, const gs_pick_rec* recs, size_t n_, double center, double thr,
+                   unsigned char* keep, size_t* n_kept) {
+  CHK(check_args(recs, n_, center, thr, keep, n_kept));
+  if (n_ == 0) return GS_OK;
+  CHK(select_device(device));
+  StreamGuard sg;
+  HIPCHK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+  DevArena dev;
+  HostArena host;
+  return remove_repeats_run(dev, host, sg.s, recs, (int)n_, center, thr, keep, n_kept, 1 << 16,
+                            LLONG_MAX);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: every device buffer of the call is a slice of
+// `scratch`, one static array per device (each GPU the library runs on holds
+// it until the process ends); the host staging is a static region.  Calls
+// are serialised by a lock (they share the staging); calls from other
+// processes on the same GPU have their own copy of the array.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)GS_NOALLOC_SCRATCH_MIB << 20;
+constexpr size_t kHostStageBytes = (size_t)12 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+alignas(64) static unsigned char host_stage[kHostStageBytes];
+std::mutex noalloc_lock;
+constexpr int kMaxDevices = 64;
+void* scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+
+int remove_repeats_noalloc(int device, const gs_pick_rec* recs, size_t n_, double center,
+                           double thr, unsigned char* keep, size_t* n_kept) {
+  CHK(check_args(recs, n_, center, thr, keep, n_kept));
+  if (n_ > (size_t)GS_NOALLOC_MAX_PICKS) return GS_ERR_CAPACITY;
+  if (n_ == 0) return GS_OK;
+  if (device >= kMaxDevices) return GS_ERR_DEVICE;
+  CHK(select_device(device));
+  std::lock_guard<std::mutex> g(noalloc_lock);
+  if (!scratch_ptr[device]) {
+    // the array's name is a host-side stand-in: its device address on the
+    // current device comes from the runtime, once per device
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return GS_ERR_DEVICE;
+    scratch_ptr[device] = p;
+  }
+  DevArena dev(scratch_ptr[device], kScratchBytes);
+  HostArena host(host_stage, kHostStageBytes);
+  // the per-thread default stream: no stream is created
+  return remove_repeats_run(dev, host, hipStreamPerThread, recs, (int)n_, center, thr, keep,
+                            n_kept, GS_NOALLOC_MAX_UNDECIDED, GS_NOALLOC_MAX_UNDECIDED);
+}
+
 }  // namespace
 }  // namespace gs
 
 extern "C" int gs_remove_repeats(int device, const gs_pick_rec* recs, size_t n, double center,
                                  double euler_thres, unsigned char* keep, size_t* n_kept) {
   return gs::remove_repeats(device, recs, n, center, euler_thres, keep, n_kept);
+}
+
+extern "C" int gs_remove_repeats_noalloc(int device, const gs_pick_rec* recs, size_t n,
+                                         double center, double euler_thres, unsigned char* keep,
+                                         size_t* n_kept) {
+  return gs::remove_repeats_noalloc(device, recs, n, center, euler_thres, keep, n_kept);
 }

@@ -274,6 +274,28 @@ typedef struct gs_pick_rec {
 int gs_remove_repeats(int device, const gs_pick_rec* recs, size_t n, double center,
                       double euler_thres, unsigned char* keep, size_t* n_kept);
 
+/* gs_remove_repeats without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and the same keep flags as
+ * gs_remove_repeats; every device buffer of the call is a slice of one
+ * static device array of GS_NOALLOC_SCRATCH_MIB MiB per device (held from
+ * the library's first use of that device until the process ends), host
+ * staging a static 12 MiB region, work on the calling thread's default
+ * stream.  Limits: n <= GS_NOALLOC_MAX_PICKS picks and at most
+ * GS_NOALLOC_MAX_UNDECIDED pairs whose orientation test needs the host's
+ * acos; the neighbour lists must fit what the per-pick buffers leave of the
+ * array (about 3.6 million duplicate pairs at the maximum n, more for
+ * fewer picks).  Beyond a limit the call returns GS_ERR_CAPACITY and keep is
+ * unspecified.  All calls on one device share its array, which keeps the
+ * last call's contents (each call writes every part it reads first); calls
+ * that overlapped on one device would race, so calls in one process are
+ * serialised by a lock and return only after their device work is done. */
+#define GS_NOALLOC_SCRATCH_MIB 32
+#define GS_NOALLOC_MAX_PICKS 65536
+#define GS_NOALLOC_MAX_UNDECIDED 65536
+int gs_remove_repeats_noalloc(int device, const gs_pick_rec* recs, size_t n, double center,
+                              double euler_thres, unsigned char* keep, size_t* n_kept);
+
 /* Projections of a density map at Euler angles: the templates of a picking
  * run, made by EMAN's project3d in the reference's protocol (step 5; not
  * part of the reference's code).  vol: box^3 float32 [z][y][x]; euler:
