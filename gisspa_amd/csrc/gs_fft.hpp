@@ -5,8 +5,10 @@
 // R inputs of its butterflies (stride N/R, conflict-free ds_read_b64), applies
 // the twiddles from an LDS table, evaluates an R-point DFT in registers, then
 // after a barrier writes the Stockham autosort positions.  Radix DFTs are
-// built at compile time from 2/3/4/5/7 kernels by Cooley-Tukey composition,
-// so any N = 2^a 3^b 5^c 7^d works with the same code.
+// built at compile time: hand-written 2/3/4/5-point kernels, direct
+// evaluation for other primes (7, 11, ..., 31), Cooley-Tukey for prime
+// powers and Good-Thomas for coprime splits, so any N that factors into
+// radices of at most 32 works with the same code.
 //
 // Sign convention: DIR = -1 forward (exp(-2 pi i k n / N)), DIR = +1 inverse;
 // transforms are unnormalised.
@@ -156,25 +158,28 @@ struct Dft<5, DIR> {
   }
 };
 
-// generic prime radix: direct O(R^2) evaluation with compile-time constants
-template <int R, int DIR, int Q, int RR>
-struct DirectTerm {
-  __device__ __forceinline__ static float2 acc(const float2* v) {
-    float2 t = twc<R, (Q * RR) % R, DIR>(v[RR]);
-    if constexpr (RR + 1 < R) {
-      float2 u = DirectTerm<R, DIR, Q, RR + 1>::acc(v);
-      return cadd(t, u);
-    } else {
-      return t;
+// Odd prime radix R: X_k = v_0 + sum_j v_j w^(jk), w = exp(DIR 2 pi i / R).
+// Inputs j and R - j share their cosines and sines: with a_j = v_j + v_(R-j),
+// b_j = v_j - v_(R-j) (j = 1 .. (R-1)/2),
+//   P_k = v_0 + sum_j a_j cos(2 pi jk / R),  Q_k = sum_j b_j DIR sin(2 pi jk / R),
+//   X_k = P_k + i Q_k,  X_(R-k) = P_k - i Q_k,
+// about a quarter of the multiplications of the plain sum; unrolled loops over
+// constexpr tables, so the compiler sees straight-line code with constants.
+template <int R, int DIR>
+struct PrimeTab {
+  struct T {
+    float c[R], s[R];
+  };
+  static constexpr T make() {
+    T t{};
+    for (int m = 0; m < R; ++m) {
+      const double ang = 2.0 * kPi * (double)m / (double)R;
+      t.c[m] = (float)ct_cos(ang);
+      t.s[m] = (float)(DIR * ct_sin(ang));
     }
+    return t;
   }
-};
-template <int R, int DIR, int Q>
-struct DirectOut {
-  __device__ __forceinline__ static void run(const float2* v, float2* o) {
-    o[Q] = DirectTerm<R, DIR, Q, 0>::acc(v);
-    if constexpr (Q + 1 < R) DirectOut<R, DIR, Q + 1>::run(v, o);
-  }
+  static constexpr T tab = make();
 };
 
 // largest power of R's smallest prime that divides R
@@ -200,9 +205,35 @@ struct DftImpl;
 
 template <int R, int DIR>
 struct DftImpl<R, DIR, 0> {
+  static constexpr int HR = (R - 1) / 2;
   __device__ __forceinline__ static void run(float2* v) {
+    constexpr auto T = PrimeTab<R, DIR>::tab;
+    float2 a[HR], b[HR];
+#pragma unroll
+    for (int j = 0; j < HR; ++j) {
+      a[j] = cadd(v[j + 1], v[R - 1 - j]);
+      b[j] = csub(v[j + 1], v[R - 1 - j]);
+    }
+    float2 x0 = v[0];
+#pragma unroll
+    for (int j = 0; j < HR; ++j) x0 = cadd(x0, a[j]);
     float2 o[R];
-    DirectOut<R, DIR, 0>::run(v, o);
+    o[0] = x0;
+#pragma unroll
+    for (int k = 1; k <= HR; ++k) {
+      float2 p = v[0], q = make_float2(0.0f, 0.0f);
+#pragma unroll
+      for (int j = 0; j < HR; ++j) {
+        const int m = ((j + 1) * k) % R;
+        p.x += a[j].x * T.c[m];
+        p.y += a[j].y * T.c[m];
+        q.x += b[j].x * T.s[m];
+        q.y += b[j].y * T.s[m];
+      }
+      // i Q = (-Q.y, Q.x)
+      o[k] = make_float2(p.x - q.y, p.y + q.x);
+      o[R - k] = make_float2(p.x + q.y, p.y - q.x);
+    }
 #pragma unroll
     for (int i = 0; i < R; ++i) v[i] = o[i];
   }
@@ -325,9 +356,8 @@ constexpr double plan_cost(int N, int NB, int WG, const int* rad, int n) {
   return c;
 }
 
-// Fewest-work factorisation of N into at most three radices <= 32 (each a
-// product of 2, 3, 5, 7 handled by the register DFTs); ties keep the first
-// found (larger leading radix).
+// Fewest-work factorisation of N into at most three radices <= 32; ties keep
+// the first found (larger leading radix).
 constexpr Plan make_plan(int N, int NB = 1, int WG = 256) {
   Plan best{0, {0}};
   double bc = 1e300;
@@ -450,12 +480,12 @@ constexpr Plan variant_plan(int N, int NB, int WG, int MAPL, int V) {
   return p;
 }
 
+// a length the executor runs: its plan's radices multiply back to it
 constexpr bool plan_ok(int n) {
-  while (n % 2 == 0) n /= 2;
-  while (n % 3 == 0) n /= 3;
-  while (n % 5 == 0) n /= 5;
-  while (n % 7 == 0) n /= 7;
-  return n == 1;
+  const Plan p = make_plan(n);
+  int prod = 1;
+  for (int i = 0; i < p.count; ++i) prod *= p.r[i];
+  return p.count > 0 && prod == n;
 }
 
 // ---------------------------------------------------------------------------
@@ -519,7 +549,7 @@ __device__ __forceinline__ void lds_passes(float2* lds, const float2* tw) {
 // the writes that filled lds/tw and this call (the caller's responsibility).
 template <int N, int NB, int WG, int DIR, int STRIDE>
 __device__ __forceinline__ void lds_fft(float2* lds, const float2* tw) {
-  static_assert(plan_ok(N), "FFT length must factor into 2, 3, 5, 7");
+  static_assert(plan_ok(N), "FFT length must factor into radices of at most 32");
   lds_passes<N, 0, 1, NB, WG, DIR, STRIDE>(lds, tw);
 }
 
@@ -625,7 +655,7 @@ template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int V = 0, class L
           class StoreF>
 __device__ __forceinline__ void fft_exec(float2* lds, const float2* tw, LoadF& load,
                                          StoreF& store) {
-  static_assert(plan_ok(N), "FFT length must factor into 2, 3, 5, 7");
+  static_assert(plan_ok(N), "FFT length must factor into radices of at most 32");
   constexpr Plan P = variant_plan(N, NB, WG, MAPL, V);
   static_assert(P.count != 1 || MAP0 == MAPL, "single-pass plans use one mapping");
   constexpr int MAPL_EFF = MAPL;

@@ -7,6 +7,7 @@
 #include <cmath>
 
 #include "gs_device.hpp"
+#include "gs_sizes.hpp"
 #include "gs_fft.hpp"
 #include "gs_kernels.hpp"
 
@@ -1094,7 +1095,6 @@ __global__ void k_img_mean(const double* __restrict__ partials, int nblocks, lon
 // ---------------------------------------------------------------------------
 // host launchers, dispatched over the supported tile sizes
 // ---------------------------------------------------------------------------
-#define GS_SIZES(X) X(64) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(448) X(512) X(576) X(640) X(720) X(768) X(960) X(1024)
 
 bool geometry_supported(int L) {
   switch (L) {
@@ -1124,26 +1124,7 @@ Geometry geometry_of(int L) {
   return g;
 }
 
-#define DISPATCH(L, FN, ...)                 \
-  switch (L) {                               \
-    case 64: FN<64>(__VA_ARGS__); break;     \
-    case 96: FN<96>(__VA_ARGS__); break;     \
-    case 128: FN<128>(__VA_ARGS__); break;   \
-    case 160: FN<160>(__VA_ARGS__); break;   \
-    case 192: FN<192>(__VA_ARGS__); break;   \
-    case 256: FN<256>(__VA_ARGS__); break;   \
-    case 320: FN<320>(__VA_ARGS__); break;   \
-    case 384: FN<384>(__VA_ARGS__); break;   \
-    case 448: FN<448>(__VA_ARGS__); break;   \
-    case 512: FN<512>(__VA_ARGS__); break;   \
-    case 576: FN<576>(__VA_ARGS__); break;   \
-    case 640: FN<640>(__VA_ARGS__); break;   \
-    case 720: FN<720>(__VA_ARGS__); break;   \
-    case 768: FN<768>(__VA_ARGS__); break;   \
-    case 960: FN<960>(__VA_ARGS__); break;   \
-    case 1024: FN<1024>(__VA_ARGS__); break; \
-    default: break;                          \
-  }
+#define DISPATCH(L, FN, ...) GS_DISPATCH(L, FN, __VA_ARGS__)
 
 template <int L>
 static void rows_r2c_pad_t(const float* src, int bx, int by, const float* scale, int nb,

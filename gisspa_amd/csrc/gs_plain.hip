@@ -12,6 +12,7 @@
 
 #include "gs_fft.hpp"
 #include "gs_kernels.hpp"
+#include "gs_sizes.hpp"
 
 namespace gs {
 
@@ -194,26 +195,7 @@ __global__ __launch_bounds__(256) void k_plain_peak(const float2* __restrict__ c
 }
 
 // ---------------------------------------------------------------------------
-#define PDISPATCH(L, FN, ...)                \
-  switch (L) {                               \
-    case 64: FN<64>(__VA_ARGS__); break;     \
-    case 96: FN<96>(__VA_ARGS__); break;     \
-    case 128: FN<128>(__VA_ARGS__); break;   \
-    case 160: FN<160>(__VA_ARGS__); break;   \
-    case 192: FN<192>(__VA_ARGS__); break;   \
-    case 256: FN<256>(__VA_ARGS__); break;   \
-    case 320: FN<320>(__VA_ARGS__); break;   \
-    case 384: FN<384>(__VA_ARGS__); break;   \
-    case 448: FN<448>(__VA_ARGS__); break;   \
-    case 512: FN<512>(__VA_ARGS__); break;   \
-    case 576: FN<576>(__VA_ARGS__); break;   \
-    case 640: FN<640>(__VA_ARGS__); break;   \
-    case 720: FN<720>(__VA_ARGS__); break;   \
-    case 768: FN<768>(__VA_ARGS__); break;   \
-    case 960: FN<960>(__VA_ARGS__); break;   \
-    case 1024: FN<1024>(__VA_ARGS__); break; \
-    default: break;                          \
-  }
+#define PDISPATCH(L, FN, ...) GS_DISPATCH(L, FN, __VA_ARGS__)
 
 int plain_blocks(int L) { return (int)(((long)L * L + 1023) / 1024); }
 int plain_part_bytes() { return (int)sizeof(PeakPart); }

@@ -19,6 +19,7 @@
 // touches each source pixel, so no atomics) and its psi derivative, R2C to
 // d/dA, and the derivative of the CTF weighting.  All sums are FP64.
 #include "gs_device.hpp"
+#include "gs_sizes.hpp"
 
 namespace gs {
 
@@ -568,26 +569,7 @@ __global__ void k_ref_out(const double* __restrict__ part, int nb, RefScal* __re
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-#define RDISPATCH(L, FN, ...)                \
-  switch (L) {                               \
-    case 64: FN<64>(__VA_ARGS__); break;     \
-    case 96: FN<96>(__VA_ARGS__); break;     \
-    case 128: FN<128>(__VA_ARGS__); break;   \
-    case 160: FN<160>(__VA_ARGS__); break;   \
-    case 192: FN<192>(__VA_ARGS__); break;   \
-    case 256: FN<256>(__VA_ARGS__); break;   \
-    case 320: FN<320>(__VA_ARGS__); break;   \
-    case 384: FN<384>(__VA_ARGS__); break;   \
-    case 448: FN<448>(__VA_ARGS__); break;   \
-    case 512: FN<512>(__VA_ARGS__); break;   \
-    case 576: FN<576>(__VA_ARGS__); break;   \
-    case 640: FN<640>(__VA_ARGS__); break;   \
-    case 720: FN<720>(__VA_ARGS__); break;   \
-    case 768: FN<768>(__VA_ARGS__); break;   \
-    case 960: FN<960>(__VA_ARGS__); break;   \
-    case 1024: FN<1024>(__VA_ARGS__); break; \
-    default: break;                          \
-  }
+#define RDISPATCH(L, FN, ...) GS_DISPATCH(L, FN, __VA_ARGS__)
 
 template <int L>
 static void ref_forward_t(const RefArgs& a, hipStream_t s) {

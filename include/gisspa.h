@@ -161,7 +161,9 @@ int gs_create(const gs_params* p, int device, void** ctx);
  * bx x by floats, x fastest.  tctf: CTF used for the |CTF| weighting --
  * the reference uses the first micrograph's values (main.cu:1106, 1133).
  * If bx or by exceeds window_size the window is enlarged to the next
- * multiple of 32 (adjustPaddingSize, main.cu:158-178). */
+ * multiple of 32 (adjustPaddingSize, main.cu:158-178).  Windows compiled:
+ * every multiple of 32 from 64 to 1024, and 720; any other size returns
+ * GS_ERR_UNSUPPORTED (the reference's grids assume a multiple of 32). */
 int gs_set_templates(void* ctx, const float* host, int n, int bx, int by, const gs_ctf* tctf);
 /* Templates of different boxes in one set: template j is by[j] x bx[j]
  * floats at host[j].  The reference reads every template with its own
