@@ -89,14 +89,23 @@ class Hip:
         self.lib.hipFree(ctypes.c_void_p(p))
 
 
+REF_EULERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                          "euler_C1_step5.npy")
+
+
+def reference_views() -> np.ndarray:
+    """(alt, az) of the 1649 views of the reference's Euler_file/euler_C1_step5.lst."""
+    return np.load(REF_EULERS)[:, :2].astype(np.float64)
+
+
 def full_workload(cache_dir: str):
     from gisspa_amd import synth
-    path = os.path.join(cache_dir, "gisspa_full_seed0.npz")
+    path = os.path.join(cache_dir, "gisspa_full_c1step5_seed0.npz")
     if os.path.exists(path):
         d = np.load(path)
         return d["templates"], d["image"], float(d["defocus"])
     t = time.time()
-    wl = synth.full_workload(seed=0)
+    wl = synth.full_workload(seed=0, eulers=reference_views())
     img, df = wl.micrographs[0]
     log(f"[bench] generated full workload in {time.time() - t:.1f}s")
     try:
@@ -120,9 +129,14 @@ def cpu_baseline(templates, image, defocus, nthreads: int, n_views: int):
     from oracle import oracle as orc
     p = orc.params(720, **FULL_PARAMS)
     c = orc.ctf(defocus)
+    orc.set_threads(nthreads)
     t0 = time.time()
     tiles = orc.micrograph_prep(p, c, image)
     t_prep = time.time() - t0
+    orc.set_threads(1)
+    t0 = time.time()
+    orc.micrograph_prep(p, c, image)
+    t_prep1 = time.time() - t0
     treal = orc.template_prep(p, c, np.ascontiguousarray(templates[:n_views]))
     t0 = time.time()
     spec = orc.rotate_normalize(p, treal, 0.0)
@@ -134,14 +148,16 @@ def cpu_baseline(templates, image, defocus, nthreads: int, n_views: int):
     t0 = time.time()
     orc.correlate_many(p, tiles, spec[:1], 1)
     t_map1 = (time.time() - t0) / tiles.shape[0]
+    orc.set_threads(nthreads)
     T = tiles.shape[0]
     N = templates.shape[0]
-    sec = t_prep + 72 * (N / n_views) * (t_rot + t_maps)
-    sec1 = t_prep + 72 * N * (t_rot / n_views + T * t_map1)
+    sec = t_prep + 72 * (N / n_views) * (t_rot / nthreads + t_maps)
+    sec1 = t_prep1 + 72 * N * (t_rot / n_views + T * t_map1)
     return dict(value=3600.0 / sec, unit="micrographs/hour", cores=nthreads, kind="port",
                 single_thread_value=3600.0 / sec1,
-                sample=f"FP64 oracle on the host: micrograph prep {t_prep:.1f}s (1 thread), "
-                       f"rotation of {n_views} views {t_rot:.2f}s (1 thread), "
+                sample=f"FP64 oracle on the host: micrograph prep {t_prep:.1f}s ({nthreads} "
+                       f"threads; {t_prep1:.1f}s on 1), rotation of {n_views} views "
+                       f"{t_rot:.2f}s (1 thread, divided by {nthreads} for the threaded rate), "
                        f"{n_views} views x {T} tiles = {n_views * T} CC maps {t_maps:.1f}s "
                        f"({nthreads} threads), 1 view x {T} tiles {T * t_map1:.1f}s "
                        f"(1 thread); extrapolated to {N} views x 72 psi")
@@ -357,32 +373,48 @@ def project_rate(n_views: int, no_cpu: bool, box: int = 180):
     return out
 
 
-def batched_rate(templates, n_mics: int, npsi: int, seed: int = 1, tile_block: int = 0,
-                 autotune: bool = False):
-    """Many micrographs in one tile pool (gs_set_micrographs): n_mics images
-    of sizes drawn from 3838x3710, 4096x4096, 5760x4092, each with its own
-    defocus (0.8-2.5 um) and 200-400 particles, templates of the 1649 views in
-    mixed boxes of 128, 180 and 256 px (cropped / padded from the 180 px
-    projections); npsi in-plane steps timed, extrapolated to 72."""
-    import gisspa_amd
+BATCH_SIZES = [(3838, 3710), (4096, 4096), (5760, 4092)]
+
+
+def _batched_micrograph(args):
+    i, seed, nx, ny, df, npart = args
     from gisspa_amd import synth
-    rng = np.random.default_rng(seed)
-    sizes = [(3838, 3710), (4096, 4096), (5760, 4092)]
     dens = synth.blob_density(60, 180, 1)
-    views = synth.sphere_eulers(templates.shape[0])
-    imgs, dfs, shapes = [], [], []
-    t = time.time()
+    mg = synth.make_micrograph(nx, ny, dens, reference_views(), npart, 0.05, seed * 100 + i,
+                               apix=1.0, defocus_um=df, ampconst=0.1, box=180, psi_step=5.0)
+    return mg.image
+
+
+def batched_inputs(n_mics: int, seed: int = 1):
+    """n_mics micrographs of sizes drawn from 3838x3710, 4096x4096 and
+    5760x4092, each with its own defocus (0.8-2.5 um) and 200-400 particles
+    at the reference's 1649 views; made in worker processes (spawned, so
+    they never touch the GPU) before the device is used."""
+    import multiprocessing as mp
+    rng = np.random.default_rng(seed)
+    jobs = []
     for i in range(n_mics):
-        nx, ny = sizes[int(rng.integers(0, 3))]
-        df = float(rng.uniform(0.8, 2.5))
-        npart = int(rng.integers(200, 401))
-        mg = synth.make_micrograph(nx, ny, dens, views, npart, 0.05, seed * 100 + i, apix=1.0,
-                                   defocus_um=df, ampconst=0.1, box=180, psi_step=5.0)
-        imgs.append(mg.image)
-        dfs.append(df)
-        shapes.append(f"{nx}x{ny}")
-        log(f"[bench] batched: micrograph {i} {nx}x{ny} defocus {df:.2f} um, {npart} particles")
+        nx, ny = BATCH_SIZES[int(rng.integers(0, 3))]
+        jobs.append((i, seed, nx, ny, float(rng.uniform(0.8, 2.5)), int(rng.integers(200, 401))))
+    t = time.time()
+    with mp.get_context("spawn").Pool(min(16, os.cpu_count() or 1, n_mics)) as pool:
+        imgs = pool.map(_batched_micrograph, jobs)
     log(f"[bench] batched: generated {n_mics} micrographs in {time.time() - t:.1f}s")
+    return imgs, [j[4] for j in jobs], [f"{j[2]}x{j[3]}" for j in jobs]
+
+
+def batched_rate(templates, inputs, npsi: int, refine_per_mic: int, refine_iters: int,
+                 tile_block: int = 0, autotune: bool = False):
+    """Many micrographs in one tile pool (gs_set_micrographs): the batch of
+    batched_inputs, templates of the 1649 views in mixed boxes of 128, 180 and
+    256 px (cropped / padded from the 180 px projections).  Timed after one
+    warm-up psi: npsi in-plane steps, then the local refinement of the
+    top refine_per_mic picks per micrograph of those steps (perturbed by
+    +-2.5 deg, +-1 px, +-200 A).  Rates: the picker alone (the steps
+    extrapolated to 72) and picking + refinement per batch."""
+    import gisspa_amd
+    imgs, dfs, shapes = inputs
+    n_mics = len(imgs)
     mixed = []
     for j, tm in enumerate(templates):
         b = (128, 180, 256)[j % 3]
@@ -404,17 +436,44 @@ def batched_rate(templates, n_mics: int, npsi: int, seed: int = 1, tile_block: i
     info = pk.info()
     tune = pk.autotune(0) if autotune else None
     pk.pick(0, 1)                      # warm-up
+    rng = np.random.default_rng(11)
     t = time.time()
-    picks = pk.pick(0, npsi)
-    t_psi = (time.time() - t) / npsi
+    picks = pk.pick(1, 1 + npsi)
+    t_pick = time.time() - t
+    # the best picks of each micrograph, at the picked grid parameters,
+    # perturbed as the differentiation workload's candidates
+    sel = np.concatenate([np.flatnonzero(picks["mic"] == m)[
+        np.argsort(-picks["score"][picks["mic"] == m])[:refine_per_mic]] for m in range(n_mics)])
+    top = picks[np.sort(sel)]
+    peaks = gisspa_amd.peaks_from_picks(top, FULL_PARAMS["phistep"], 0.0)
+    peaks["defocus"] = np.asarray(dfs, np.float32)[top["mic"]]
+    peaks["psi"] += rng.uniform(-2.5, 2.5, len(peaks)).astype(np.float32)
+    peaks["dx"] += rng.uniform(-1, 1, len(peaks)).astype(np.float32)
+    peaks["dy"] += rng.uniform(-1, 1, len(peaks)).astype(np.float32)
+    peaks["defocus"] += rng.uniform(-0.02, 0.02, len(peaks)).astype(np.float32)
+    _, fin = pk.refine(peaks, refine_iters)
+    t_all = time.time() - t
+    t_refine = t_all - t_pick
     pk.close()
+    t_psi = t_pick / npsi
     sec = t_prep + 72 * t_psi
+    sec_refined = sec + t_refine
+    full_equiv = info.n_tiles / n_mics / 88.0   # mean tiles per micrograph over 88
     return {"micrographs": n_mics, "sizes": shapes, "tiles": info.n_tiles,
             "maps_per_psi": info.n_tiles * templates.shape[0], "prep_s": t_prep,
-            "psi_s": t_psi, "picks_timed": int(len(picks)), "value": n_mics * 3600.0 / sec,
+            "psi_s": t_psi, "picks_timed": int(len(picks)),
+            "value": n_mics * 3600.0 / sec,
+            "refined_value": n_mics * 3600.0 / sec_refined,
+            "refine_candidates": int(len(peaks)), "refine_s": t_refine,
+            "mean_score_refined": float(fin["score"].mean()),
+            "full_size_equivalent_value": n_mics * 3600.0 / sec * full_equiv,
+            "full_size_equivalent_refined_value": n_mics * 3600.0 / sec_refined * full_equiv,
             "tile_block": tune.tile_block if tune else tile_block,
             "unit": "micrographs/hour",
-            "sample": f"{npsi} psi timed (after one warm-up psi), extrapolated to 72"}
+            "sample": f"{npsi} psi timed after one warm-up psi, extrapolated to 72; refinement "
+                      f"of the top {refine_per_mic} picks per micrograph ({len(peaks)} "
+                      f"candidates, up to {refine_iters} steps) timed with the picking; "
+                      f"full-size equivalent = rate x mean tiles per micrograph / 88"}
 
 
 def rocfft_compare(L: int, batch: int = 189, reps: int = 5):
@@ -597,9 +656,13 @@ def main():
                     help="also run the combined stress shape on N 5760x4092 micrographs "
                          "(4600 views of 256 px, 120 psi, refinement of the top 50,000 picks; "
                          "0 = skip)")
-    ap.add_argument("--batched", type=int, default=0, metavar="N",
+    ap.add_argument("--batched", type=int, default=16, metavar="N",
                     help="also time N micrographs of uneven sizes in one tile pool with mixed "
-                         "template boxes (0 = skip)")
+                         "template boxes, picking and refinement (0 = skip)")
+    ap.add_argument("--batched-psi", type=int, default=4,
+                    help="in-plane steps timed in the batched leg (after one warm-up step)")
+    ap.add_argument("--batched-refine", type=int, default=500, metavar="N",
+                    help="picks refined per micrograph in the batched leg")
     ap.add_argument("--cli", type=int, default=0, metavar="N",
                     help="also run the ./main command line on N full-size micrographs, once per "
                          "micrograph_batch of 1 and 4 (0 = skip)")
@@ -624,6 +687,9 @@ def main():
         print(f"bench.py: this build runs the picker on one GPU; --gpus {args.gpus} is not "
               f"supported (shard micrographs across processes instead)", file=sys.stderr)
         sys.exit(2)
+
+    # the batched leg's micrographs, made before the device is used
+    batched_in = batched_inputs(args.batched) if args.batched else None
 
     import gisspa_amd
     FULL_PARAMS["threshold"] = args.threshold
@@ -771,10 +837,16 @@ def main():
 
     batched = None
     if args.batched:
-        batched = batched_rate(templates, args.batched, max(1, min(npsi, 4)),
+        batched = batched_rate(templates, batched_in, max(1, args.batched_psi),
+                               args.batched_refine, args.refine_iters,
                                tile_block=args.tile_block, autotune=bool(args.autotune))
+        batched_in = None
         log(f"[bench] batched: {batched['micrographs']} micrographs ({batched['tiles']} tiles), "
-            f"{batched['psi_s']:.2f}s per psi, {batched['value']:.1f} micrographs/hour")
+            f"{batched['psi_s']:.2f}s per psi, {batched['value']:.1f} micrographs/hour picked, "
+            f"{batched['refined_value']:.1f} picked + refined ({batched['refine_candidates']} "
+            f"candidates in {batched['refine_s']:.2f}s); full-size equivalent "
+            f"{batched['full_size_equivalent_value']:.1f} / "
+            f"{batched['full_size_equivalent_refined_value']:.1f}")
 
     rocfft = None
     if args.rocfft:
@@ -885,6 +957,11 @@ def main():
         "project": proj,
         "plain": plain,
         "speedup_vs_plain": (mph / plain["value"]) if plain else None,
+        # picking plus the local refinement of the top picks (the refine leg's
+        # candidates) per micrograph
+        "refined_value": (3600.0 / (sec_per_mic + refine["seconds"])) if refine else None,
+        "refined_speedup_vs_plain": (3600.0 / (sec_per_mic + refine["seconds"]) / plain["value"])
+        if (refine and plain) else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

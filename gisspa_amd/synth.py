@@ -270,11 +270,13 @@ def small_workload(seed: int = 0, n_views: int = 64, box: int = 128, size: int =
 def full_workload(seed: int = 0, n_views: int = 1649, box: int = 180, nx: int = 5760,
                   ny: int = 4092, n_particles: int = 300, snr: float = 0.05,
                   window: int = 720, overlap: int = 180, phistep: float = 5.0,
-                  defocus: float = 1.8, threshold: float = 7.0) -> Workload:
+                  defocus: float = 1.8, threshold: float = 7.0,
+                  eulers: np.ndarray | None = None) -> Workload:
     """Plain method, full size: 5760x4092, 180^2 templates of a 60-blob 180^3
-    density over a 1649-view ~5 degree sphere, 72 psi, 1.8 um."""
+    density over a 1649-view ~5 degree sphere (the given (alt, az) rows, or a
+    Fibonacci lattice of n_views), 72 psi, 1.8 um."""
     dens = blob_density(60, box, seed + 1)
-    eul = sphere_eulers(n_views)
+    eul = sphere_eulers(n_views) if eulers is None else np.asarray(eulers, np.float64)[:, :2]
     tm = project_templates(dens, box, eul)
     mg = make_micrograph(nx, ny, dens, eul, n_particles, snr, seed, apix=1.0, defocus_um=defocus,
                          ampconst=0.1, box=box, psi_step=phistep)

@@ -168,12 +168,20 @@ extern "C" int or_fft1d(double* data, int n, int sign) {
   return 0;
 }
 
+/* Rows, then columns; each 1D transform is independent, so large images
+ * (whole micrographs) split them over threads with identical results. */
 static void fft2d(cd* d, int nx, int ny, int sign) {
   auto px = get_plan(nx);
   auto py = get_plan(ny);
-  std::vector<cd> a, b;
-  for (int y = 0; y < ny; ++y) fft1d_strided(d + (long)y * nx, nx, 1, sign, *px, a, b);
-  for (int x = 0; x < nx; ++x) fft1d_strided(d + x, ny, nx, sign, *py, a, b);
+  const bool par = (long)nx * ny >= (1L << 20);
+#pragma omp parallel if (par)
+  {
+    std::vector<cd> a, b;
+#pragma omp for schedule(static)
+    for (int y = 0; y < ny; ++y) fft1d_strided(d + (long)y * nx, nx, 1, sign, *px, a, b);
+#pragma omp for schedule(static)
+    for (int x = 0; x < nx; ++x) fft1d_strided(d + x, ny, nx, sign, *py, a, b);
+  }
 }
 
 extern "C" int or_fft2d(double* data, int nx, int ny, int sign) {
@@ -506,6 +514,7 @@ extern "C" int or_micrograph_prep(const or_params* p, const or_ctf* c, const flo
     fft2d(F.data(), nx, ny, -1);
     for (long i = 0; i < NP; ++i) F[i] /= (double)NP;
     /* do_phase_flip, GPU_func.cu:785-796 */
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) F[(long)y * nx + x] *= or_ctf_ast(x, (y + ny / 2) % ny, nx, ny, p, c, 0);
     /* SQRSum_by_circle mode 1, GPU_func.cu:98-128 */
@@ -515,6 +524,7 @@ extern "C" int or_micrograph_prep(const or_params* p, const or_ctf* c, const flo
         int r = radial_bin(x, y, nx, ny);
         if (r < std::max(nx, ny) / 2 && r >= 0) { ra[r] += std::norm(F[(long)y * nx + x]); rb[r] += 1; }
       }
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x)
         F[(long)y * nx + x] *= micrograph_weight(x, y, nx, ny, ra.data(), rb.data(), p, c);
@@ -533,6 +543,7 @@ extern "C" int or_micrograph_prep(const or_params* p, const or_ctf* c, const flo
   }
   /* split_IMG, GPU_func.cu:564-581 ; tile FFT main.cu:758-769 */
   cd* out = reinterpret_cast<cd*>(tile_spec);
+#pragma omp parallel for collapse(2) schedule(dynamic)
   for (int iy = 0; iy < byn; ++iy)
     for (int ix = 0; ix < bxn; ++ix) {
       cd* t = out + (long)(ix + iy * bxn) * L2;
@@ -628,6 +639,12 @@ static void set_threads(int nthreads) {
 #else
   (void)nthreads;
 #endif
+}
+
+/* Threads of the oracle's parallel loops from now on (0: leave as is). */
+extern "C" int or_set_threads(int nthreads) {
+  set_threads(nthreads);
+  return 0;
 }
 
 extern "C" int or_pick_norm0(const or_params* p, const or_ctf* c, const float* tmpl, int ntmpl,

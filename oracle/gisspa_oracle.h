@@ -117,6 +117,9 @@ int or_remove_repeats(int n, const int* group, const double* x, const double* y,
  * from euler[3j..3j+2] = (alt, az, phi) degrees; zero outside the box. */
 int or_project(const float* vol, int n, const double* euler, int nviews, double* out);
 
+/* Threads of the oracle's parallel loops from now on (0 leaves them as is). */
+int or_set_threads(int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ def load() -> ctypes.CDLL:
         lib.or_euler_distance.argtypes = [d] * 6 + [vp]
         lib.or_remove_repeats.argtypes = [ctypes.c_int] + [vp] * 7 + [d, d, vp]
         lib.or_project.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp]
+        lib.or_set_threads.argtypes = [ctypes.c_int]
         _lib = lib
     return _lib
 
@@ -184,6 +185,11 @@ def refine_grad_fd(p: OrParams, c0: OrCtf, white, tile, x0, y0, theta, h=1e-3) -
         g[i] = (refine_score(p, c0, white, tile, x0, y0, tp)[0] -
                 refine_score(p, c0, white, tile, x0, y0, tm_)[0]) / (2 * h)
     return g
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (whole-image FFTs, tiles)."""
+    load().or_set_threads(int(n))
 
 
 def num_psi(phi_step: float) -> int:

@@ -35,3 +35,12 @@ def tiny(seed: int = 0, l: int = 128, box: int = 64, n_views: int = 6, size=(320
                   diameter=0.75 * box, threshold=threshold, window_size=l, overlap=overlap,
                   ampconst=0.1)
     return Case(f"tiny{seed}_l{l}", tm, mg.image, 1.5, params)
+
+
+def c1_step5_views() -> np.ndarray:
+    """(alt, az) of the 1649 views of the reference's euler_C1_step5.lst
+    (tests/golden/euler_C1_step5.npy, written by make_euler_fixture.py)."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "euler_C1_step5.npy")
+    return np.load(path)[:, :2].astype(np.float64)

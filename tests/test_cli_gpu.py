@@ -260,3 +260,39 @@ def test_cli_lsx_list(gpu, tmp_path):
     r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     assert open(os.path.join(tmp_path, "picks_lsx.lst")).read() == lst_out
+
+
+def test_cli_reads_the_reference_euler_list(gpu, tmp_path):
+    """The Euler file in the reference's own text layout (right-aligned
+    8-column values split by tabs, as Euler_file/euler_C1_step5.lst, each
+    value the shortest text that reads back as the same float): twelve of
+    its lines, with
+    templates projected at those views; every output line carries the
+    file's alt and az (as parsed by sscanf %f, DataReader.cpp:267-284) and
+    the picks are the oracle's."""
+    views = _cases.c1_step5_views()[::137][:12]
+    c = _cases.tiny(seed=4, l=128, n_views=12, phistep=60.0, threshold=4.0)
+    dens = synth.blob_density(20, 64, 15, sigma_range=(1.5, 4.0))
+    tm = synth.project_templates(dens, 64, views)
+    wl = synth.Workload("cli-euler", tm, views, [(c.image, c.defocus)], dict(c.params), [])
+    cfg = synth.write_workload(wl, str(tmp_path))
+    short = lambda v: np.format_float_positional(np.float32(v), trim="-")
+    with open(os.path.join(tmp_path, "euler.lst"), "w") as f:
+        for alt, az in views:
+            f.write("%8s\t%8s\t%8s\n" % (short(alt), short(az), "0"))
+    r = subprocess.run([MAIN_PATH, cfg], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    lines = open(os.path.join(tmp_path, "picks.lst")).read().splitlines()
+    op = orc.params(c.l, **c.params)
+    st = orc.pick_norm0(op, orc.ctf(c.defocus), tm, c.image)
+    sel = orc.select_norm0(op, st, 0, c.image.shape[1], c.image.shape[0])
+    assert len(lines) == len(sel) > 0
+    for ln, s in zip(lines, sel):
+        m = LINE.match(ln)
+        assert m, ln
+        alt, az = views[s["view"]]
+        assert float(m.group(6)) == pytest.approx(float(np.float32(alt)), abs=1e-5)
+        assert float(m.group(7)) == pytest.approx(float(np.float32(az)), abs=1e-5)
+        assert float(m.group(9)) == s["cx"] and float(m.group(10)) == s["cy"]
+        assert float(m.group(11)) == pytest.approx(s["score"], rel=1e-4, abs=2e-6)

@@ -3,8 +3,9 @@
     particles, 64 views of a 128^3 blob density, window 512 / overlap 128
     (9 tiles), 10 degree in-plane steps (the first 4 of 36);
   * the full-size shape of the benchmark: a 5760 x 4092 micrograph, window
-    720 / overlap 180 (88 tiles), 180 px templates (8 of the 1649 views),
-    one in-plane step, batches of 100 maps (the last one ragged).
+    720 / overlap 180 (88 tiles), 180 px templates at 16 of the 1649 views of
+    the reference's euler_C1_step5.lst (particles embedded at all 1649), the
+    first two in-plane steps, batches of 100 maps (the last one ragged).
 Same tolerances as test_gpu_parity.py: scores within 1e-4 * max(|oracle|, 1),
 identical peak positions and identical pick lists."""
 import os
@@ -14,6 +15,7 @@ import pytest
 
 from gisspa_amd import synth
 from oracle import oracle as orc
+from tests import _cases
 
 pytestmark = pytest.mark.gpu
 
@@ -58,14 +60,15 @@ def test_small_configuration(gpu):
 @pytest.fixture(scope="module")
 def full_case():
     dens = synth.blob_density(60, 180, 1)
-    views = synth.sphere_eulers(1649)
-    tm = synth.project_templates(dens, 180, views[::206][:8])
+    views = _cases.c1_step5_views()
+    assert len(views) == 1649
+    tm = synth.project_templates(dens, 180, views[::103][:16])
     mg = synth.make_micrograph(5760, 4092, dens, views, 300, 0.05, 0, apix=1.0,
                                defocus_um=1.8, ampconst=0.1, box=180, psi_step=5.0)
     params = dict(angpix=1.0, phistep=5.0, kk=3, energy=300, cs=2.7, Highres=8, Lowres=100,
                   diameter=135.0, threshold=7.0, window_size=720, overlap=180, ampconst=0.1)
     p = orc.params(720, **params)
-    ref = orc.pick_norm0(p, orc.ctf(1.8), tm, mg.image, 0, 1, nthreads=NTHREADS)
+    ref = orc.pick_norm0(p, orc.ctf(1.8), tm, mg.image, 0, 2, nthreads=NTHREADS)
     return tm, mg.image, params, p, ref
 
 
@@ -76,9 +79,9 @@ def test_full_size_shape(gpu, full_case):
         pk.set_batch(100)
         pk.set_templates(tm, 1.8)
         pk.set_micrograph(image, 1.8)
-        picks = pk.pick(0, 1)
+        picks = pk.pick(0, 2)
         got = pk.mapstats().reshape(ref.shape)
-    assert ref.shape == (1, 88, 8)
+    assert ref.shape == (2, 88, 16)
     _compare(got, ref, "5760x4092 / window 720")
     _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
 
@@ -93,7 +96,7 @@ def test_fft_schedule_variants(gpu, full_case, v1, v2):
         pk.set_fft_variants(v1, v2)
         pk.set_templates(tm, 1.8)
         pk.set_micrograph(image, 1.8)
-        picks = pk.pick(0, 1)
+        picks = pk.pick(0, 2)
         got = pk.mapstats().reshape(ref.shape)
     _compare(got, ref, f"window 720, FFT variants {v1}/{v2}")
     _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
@@ -110,7 +113,7 @@ def test_autotune(gpu, full_case):
               list(t.pass1_us), list(t.pass2_us))
         assert 0 <= t.pass1_variant < 4 and 0 <= t.pass2_variant < 4
         assert t.candidates >= 6 and t.psi_ms > 0 and min(t.pass1_us) > 0
-        picks = pk.pick(0, 1)
+        picks = pk.pick(0, 2)
         got = pk.mapstats().reshape(ref.shape)
     _compare(got, ref, "window 720 after autotune")
     _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
