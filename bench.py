@@ -253,9 +253,11 @@ def norm1_rate(templates, d_img, nx, ny, defocus, n_views: int):
     """norm_type 1 (per-pixel normalisation over all views x psi,
     main.cu:1170-1229) on the same micrograph with the first n_views views:
     both passes over all 72 psi, wall clock, scaled to all views (the work is
-    linear in the number of views)."""
+    linear in the number of views).  The threshold is the reference's default
+    (7): z-scores of the per-pixel normalisation are not on the scale of the
+    norm_type 0 scores the headline run's threshold is set for."""
     import gisspa_amd
-    pk = gisspa_amd.Picker(0, **dict(FULL_PARAMS, norm_type=1))
+    pk = gisspa_amd.Picker(0, **dict(FULL_PARAMS, norm_type=1, threshold=7.0))
     pk.set_templates(np.ascontiguousarray(templates[:n_views]), defocus)
     pk.set_micrograph_device(d_img, nx, ny, defocus)
     t0 = time.time()

@@ -63,7 +63,7 @@ struct Ctx {
   DevBuf r_wtab;  // refinement weighting table, valid with white_ok
   // refinement batch buffers
   int ref_cap = 0;
-  DevBuf r_cand, r_cs, r_A, r_gtmp, r_Tun, r_spec, r_rot, r_bscale, r_pre, r_gRn, r_gTb, r_part,
+  DevBuf r_cand, r_cs, r_A, r_gtmp, r_Tun, r_spec, r_rot, r_bscale, r_pre, r_gTb, r_gT, r_part,
       r_sc, r_out;
   double half_count = 0;  // pixels of the x <= l/2 half plane inside radius l/2
   // micrograph batch: tile pool of every micrograph, tile descriptors
@@ -967,8 +967,8 @@ int ref_ensure(Ctx* c, int nb) {
   CHK(c->r_rot.ensure(nb * nrb * 2 * sizeof(double)));
   CHK(c->r_bscale.ensure(nb * sizeof(float)));
   CHK(c->r_pre.ensure(nb * sizeof(float)));
-  CHK(c->r_gRn.ensure(nb * L * L * sizeof(float)));
   CHK(c->r_gTb.ensure(nb * L * L * sizeof(float)));
+  CHK(c->r_gT.ensure(nb * L * L * sizeof(float)));
   CHK(c->r_part.ensure(nb * (size_t)ref_parts() * 4 * sizeof(double)));
   CHK(c->r_sc.ensure(nb * sizeof(RefScal)));
   CHK(c->r_out.ensure(nb * 5 * sizeof(float)));
@@ -1027,8 +1027,8 @@ int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out) {
     a.rot_part = c->r_rot.as<double>();
     a.bscale = c->r_bscale.as<float>();
     a.pre = c->r_pre.as<float>();
-    a.gRn = c->r_gRn.as<float>();
     a.gTb = c->r_gTb.as<float>();
+    a.gT = c->r_gT.as<float>();
     a.part = c->r_part.as<double>();
     a.sc = c->r_sc.as<RefScal>();
     a.out = c->r_out.as<float>();

@@ -157,6 +157,15 @@ __device__ __forceinline__ float ctf_ast_f(int x1, int y1, int nx, int ny, const
 
 
 // The refinement's rotation sample at output pixel (x, y): bilinear value of
+// C2R pre-processing of a half-spectrum row:
+// Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) e^{+2 pi i k / L}
+__device__ __forceinline__ float2 c2r_pre(float2 xk, float2 xmk, float2 wk_conj) {
+  float2 s = make_float2(xk.x + xmk.x, xk.y - xmk.y);
+  float2 d = make_float2(xk.x - xmk.x, xk.y + xmk.y);
+  float2 t = cmul(d, wk_conj);
+  return make_float2(s.x - t.y, s.y + t.x);
+}
+
 // src at the rotation of (x - L/2, y - L/2) by cs = (cos, sin), coordinates
 // in double precision, the boundary rules of rotate_subIMG
 // (GPU_func.cu:496-560); also the slopes d/dx2, d/dy2 and the mask factor of
@@ -196,6 +205,54 @@ __device__ __forceinline__ RotSampleD rot_sample_d(const float* src, int x, int 
   o.dty = (a3 - a0) * tt + (a2 - a1) * t;
   return o;
 }
+// Bilinear adjoint as a gather: source pixel q = (qx, qy) collects g of every
+// output pixel whose interpolation cell (rot_sample_d's arithmetic, boundary
+// rules included) has q as a corner, times that corner's weight.  The
+// outputs whose cell can touch q lie within 2 pixels of the inverse rotation
+// of q: x - c = R^T (q - c).  An output pixel samples q's cells only if its
+// rotated position lies within 1 of q on both axes, so within sqrt(2) of
+// R^T (q - c) on each: the 4 x 4 window floor(c) - 1 .. floor(c) + 2 holds
+// all of them (0.41 px to spare for the float rounding of c); the others
+// carry weight 0.  Pixels beyond the mask's reach receive nothing.
+template <int L>
+__device__ __forceinline__ float rot_adj_gather(const float* g, int qx, int qy, double2 cs,
+                                                float dmask) {
+  const long i = (long)qy * L + qx;
+  const float reach = dmask * 0.5f + 12.0f + 3.0f;
+  const float qX = (float)(qx - L / 2), qY = (float)(qy - L / 2);
+  float acc = 0.0f;
+  if (qX * qX + qY * qY > reach * reach) return acc;
+  const float cX = (float)(cs.x * qX - cs.y * qY), cY = (float)(cs.y * qX + cs.x * qY);
+  const int x0 = (int)floorf(cX) + L / 2 - 1, y0 = (int)floorf(cY) + L / 2 - 1;
+  for (int dy = 0; dy < 4; ++dy) {
+    const int y = y0 + dy;
+    if (y < 0 || y >= L) continue;
+    for (int dx = 0; dx < 4; ++dx) {
+      const int x = x0 + dx;
+      if (x < 0 || x >= L) continue;
+      const float gv = g[(long)y * L + x];
+      if (gv == 0.0f) continue;
+      const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
+      const double x2 = cs.x * xf + cs.y * yf + (double)(L / 2);
+      const double y2 = -cs.y * xf + cs.x * yf + (double)(L / 2);
+      if (x2 < 0 || x2 > L - 1.0 || y2 < 0 || y2 > L - 1.0) continue;
+      const int ii = (int)floor(x2), jj = (int)floor(y2);
+      long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
+      if (ii == L - 1) { k1--; k2--; }
+      if (jj == L - 1) { k2 -= L; k3 -= L; }
+      const float t = (float)(x2 - ii), u = (float)(y2 - jj);
+      const float tt = 1.0f - t, uu = 1.0f - u;
+      float wq = 0.0f;
+      if (k0 == i) wq += tt * uu;
+      if (k1 == i) wq += t * uu;
+      if (k2 == i) wq += t * u;
+      if (k3 == i) wq += tt * u;
+      acc += gv * wq;
+    }
+  }
+  return acc;
+}
+
 template <int L>
 __device__ __forceinline__ float rot_value_d(const float* src, int x, int y, double2 cs,
                                              float dmask, float pre) {

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace gs {
 
@@ -578,6 +579,16 @@ __host__ __device__ constexpr int exec_stride() {
 
 enum { MAP_J = 0, MAP_B = 1 };
 
+// f(std::integral_constant<int, I>) for I = 0 .. N-1, in order
+template <class F, int... I>
+__device__ __forceinline__ void unroll_seq(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void unroll(F&& f) {
+  unroll_seq(f, std::make_integer_sequence<int, N>{});
+}
+
 template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int PASS, int NS, int V, class LoadF,
           class StoreF>
 __device__ __forceinline__ void fft_exec_pass(float2* lds, const float2* tw, LoadF& load,
@@ -601,10 +612,18 @@ __device__ __forceinline__ void fft_exec_pass(float2* lds, const float2* tw, Loa
     if (TOT % WG == 0 || g < TOT) {
       const int b = (MAPP == MAP_J) ? g / M : g % NB;
       const int j = (MAPP == MAP_J) ? g % M : g / NB;
+      if constexpr (FIRST &&
+                    std::is_invocable_v<LoadF&, int, int, std::integral_constant<int, 0>>) {
+        // affine form: load(b, j, q) supplies element j + q * M, q known at
+        // compile time, so the functor can keep per-thread bases and fold q
+        // into constants
+        unroll<R>([&](auto qc) { v[p][decltype(qc)::value] = load(b, j, qc); });
+      } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if constexpr (FIRST) v[p][r] = load(b, j + r * M);
-        else v[p][r] = lds[b * STRIDE + pidx(j + r * M)];
+        for (int r = 0; r < R; ++r) {
+          if constexpr (FIRST) v[p][r] = load(b, j + r * M);
+          else v[p][r] = lds[b * STRIDE + pidx(j + r * M)];
+        }
       }
       if constexpr (NS > 1) {
         const int k = j % NS;

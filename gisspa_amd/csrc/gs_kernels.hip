@@ -24,13 +24,6 @@ __device__ __forceinline__ float2 r2c_post(float2 zk, float2 zmk, float2 wk) {
   float2 o = make_float2(0.5f * d.y, -0.5f * d.x);       // d / (2i)
   return cadd(e, cmul(wk, o));
 }
-// Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) e^{+2 pi i k / L}
-__device__ __forceinline__ float2 c2r_pre(float2 xk, float2 xmk, float2 wk_conj) {
-  float2 s = make_float2(xk.x + xmk.x, xk.y - xmk.y);
-  float2 d = make_float2(xk.x - xmk.x, xk.y + xmk.y);
-  float2 t = cmul(d, wk_conj);
-  return make_float2(s.x - t.y, s.y + t.x);
-}
 
 // ---------------------------------------------------------------------------
 // Row R2C: rows y0..y0+RB-1 of image b -> columns [b][kx][y] (transposed)
@@ -74,6 +67,7 @@ __device__ __forceinline__ float row_value(const RowSrc& s, int b, int y, int x,
     return (float)((double)s.img[(long)gy * s.nx + gx] - (double)aux);
   } else if constexpr (SRC == SRC_IMG) {
     return s.imgs[((long)b * L + y) * L + x];
+
   } else if constexpr (SRC == SRC_ROTB) {
     // the refinement's rotation: continuous angle, double-precision
     // coordinates (the cells of the FP64 objective), same mask
@@ -458,22 +452,6 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
   constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
   const int y0 = rb * RB;
   const float2* gb = Gin + (long)gm * NCB * L * CB;
-  // Z'[k] of row y, already pre-processed by pass 1 (rows past L only exist
-  // when RB does not divide L; they are read clamped and never stored)
-  auto load = [&](int r, int k) -> float2 {
-    const float2* q = gb + pair_slot<L>(k, min(y0 + r, L - 1));
-    if constexpr (COH) return load_coh(q);
-    // plain loads: the two halves of a 64-byte slot row are read by
-    // different loads, and a cached line serves both
-    else return *q;
-  };
-  // z'[n] = (c[y][2n], c[y][2n+1]).  Border pixels (x or y outside
-  // [blo, bhi], the integer form of get_peak_and_SUM's d/4 test) count as 0.
-  // Each thread visits its pixels in increasing linear index (round, then
-  // output slot, then x), so a strict '>' keeps the lowest index among equal
-  // values; the running maximum records a compile-time code of the winning
-  // (round, slot, half) and the position is decoded once at the end.  The
-  // border tests are uniform: rows per workgroup, columns per output slot.
   constexpr Plan PL = variant_plan(M, RB, WG, MAP_J, V);
   constexpr int RL = PL.r[PL.count - 1];  // radix of the last pass
   constexpr int NSL = M / RL;             // its butterflies per transform
@@ -487,11 +465,57 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
     }
     return;
   }
+  // Z'[k] of row y, already pre-processed by pass 1 (rows past L only exist
+  // when RB does not divide L; they are read clamped and never stored)
+  auto load = [&](int r, int k) -> float2 {
+    const float2* q = gb + pair_slot<L>(k, min(y0 + r, L - 1));
+    if constexpr (COH) return load_coh(q);
+    // plain loads: the two halves of a 64-byte slot row are read by
+    // different loads, and a cached line serves both
+    else return *q;
+  };
+  // The first radix pass reads elements k = j + q M1 (q = 0 .. R0-1) of a
+  // row.  With M1 and M multiples of CB/2, the slot of k moves by a fixed
+  // C1 bytes per step of q: forward for columns k <= M/2, backward for their
+  // partners k > M/2.  So each thread computes two byte offsets once (q = 0
+  // and q = R0-1) and every load is a buffer load whose step is a constant
+  // scalar offset; only the one q whose columns straddle M/2 selects per
+  // lane.  Reads past the map's intermediate return 0 (buffer range check).
+  constexpr int R0 = PL.r[0], M1 = M / R0, HCB = CB / 2;
+  constexpr bool AFFINE = !COH && PL.count > 1 && M1 % HCB == 0 && M % HCB == 0;
+  constexpr int C1 = (M1 / HCB) * L * CB * 8;
+  const __amdgpu_buffer_rsrc_t grs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float2*>(gb), 0, NCB * L * CB * 8, 0x00020000);
+  int voff_lo = 0, voff_hi = 0;
+  auto load_aff = [&](int r, int j, auto qc) -> float2 {
+    constexpr int q = decltype(qc)::value;
+    constexpr int KLO = q * M1, KHI = q * M1 + M1 - 1;
+    if constexpr (q == 0) {
+      const int y = min(y0 + r, L - 1);
+      voff_lo = pair_slot<L>(j, y) * 8;
+      voff_hi = pair_slot<L>(j + (R0 - 1) * M1, y) * 8;
+    }
+    int vo, so;
+    if constexpr (KHI <= M / 2) {
+      vo = voff_lo;
+      so = q * C1;
+    } else if constexpr (KLO > M / 2) {
+      vo = voff_hi;
+      so = (R0 - 1 - q) * C1;
+    } else {
+      vo = (j + KLO <= M / 2) ? voff_lo + q * C1 : voff_hi + (R0 - 1 - q) * C1;
+      so = 0;
+    }
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(grs, vo, so, 0));
+  };
+  // z'[n] = (c[y][2n], c[y][2n+1]).  Border pixels (x or y outside
+  // [blo, bhi], the integer form of get_peak_and_SUM's d/4 test) count as 0.
+  // The border tests are uniform: rows per workgroup, columns per output
+  // slot.  Each thread visits its pixels in increasing linear index (round,
+  // then output slot, then x).
   const unsigned brange = (unsigned)(bhi - blo);
   const bool rows_in = y0 >= blo && y0 + RB - 1 <= bhi;
-  float vmax = 0.0f;
-  int vcode = -1;
-  auto store = [&](int r, int n, float2 z, int pr, int slot) {
+  auto masked = [&](int r, int n, float2 z, int slot) -> float2 {
     float a = z.x, b = z.y;
     if (!rows_in) {
       if ((unsigned)(y0 + r - blo) > brange) { a = 0.0f; b = 0.0f; }
@@ -502,28 +526,96 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
       a = ((unsigned)(x0 - blo) <= brange) ? a : 0.0f;
       b = ((unsigned)(x0 + 1 - blo) <= brange) ? b : 0.0f;
     }
-    const int code = (pr * RL + slot) * 2;
-    if (a > vmax) { vmax = a; vcode = code; }
-    if (b > vmax) { vmax = b; vcode = code + 1; }
+    return make_float2(a, b);
   };
-  fft_exec<M, RB, WG, +1, MAP_J, MAP_J, V>(lds, twM, load, store);
   int pos = INT_MAX;
-  if (vcode >= 0) {
-    const int pr = vcode / (2 * RL), slot = (vcode >> 1) % RL, half = vcode & 1;
-    const int g = threadIdx.x + pr * WG;
-    const int row = g / NSL, j = g % NSL;
-    pos = (y0 + row) * L + 2 * (j + slot * NSL) + half;
+  float bmax = 0.0f;
+  if constexpr (AFFINE) {
+    // The maximum is taken with plain max operations; its position is found
+    // afterwards, only by the threads that hold the workgroup's maximum:
+    // their first equal pixel in visiting order is their lowest index.
+    constexpr int PERL = (RB * NSL + WG - 1) / WG;  // rounds of the last pass
+    float vals[PERL][RL][2];
+#pragma unroll
+    for (int pr = 0; pr < PERL; ++pr)
+#pragma unroll
+      for (int sl = 0; sl < RL; ++sl) vals[pr][sl][0] = vals[pr][sl][1] = 0.0f;
+    auto store = [&](int r, int n, float2 z, int pr, int slot) {
+      const float2 v = masked(r, n, z, slot);
+      vals[pr][slot][0] = v.x;
+      vals[pr][slot][1] = v.y;
+    };
+    fft_exec<M, RB, WG, +1, MAP_J, MAP_J, V>(lds, twM, load_aff, store);
+    float vmax = 0.0f;
+#pragma unroll
+    for (int pr = 0; pr < PERL; ++pr)
+#pragma unroll
+      for (int sl = 0; sl < RL; ++sl)
+        vmax = fmaxf(vmax, fmaxf(vals[pr][sl][0], vals[pr][sl][1]));
+    float wm = vmax;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o, 64));
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red_v[w] = wm;
+    __syncthreads();
+    bmax = red_v[0];
+#pragma unroll
+    for (int i = 1; i < WG / 64; ++i) bmax = fmaxf(bmax, red_v[i]);
+    if (bmax > 0.0f && vmax == bmax) {
+      int code = 0;
+      // last to first, so the first equal pixel's code remains
+#pragma unroll
+      for (int pr = PERL - 1; pr >= 0; --pr)
+#pragma unroll
+        for (int sl = RL - 1; sl >= 0; --sl)
+#pragma unroll
+          for (int h = 1; h >= 0; --h)
+            if (vals[pr][sl][h] == bmax) code = (pr * RL + sl) * 2 + h;
+      const int pr = code / (2 * RL), slot = (code >> 1) % RL, half = code & 1;
+      const int g = threadIdx.x + pr * WG;
+      const int row = g / NSL, j = g % NSL;
+      pos = (y0 + row) * L + 2 * (j + slot * NSL) + half;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pos = min(pos, __shfl_xor(pos, o, 64));
+    if (lane == 0) red_p[w] = pos;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 0; i < WG / 64; ++i) pos = min(pos, red_p[i]);
+    }
+  } else {
+    // the running maximum records a compile-time code of the winning
+    // (round, slot, half); a strict '>' keeps the lowest index among equal
+    // values, and the position is decoded once at the end
+    float vmax = 0.0f;
+    int vcode = -1;
+    auto store = [&](int r, int n, float2 z, int pr, int slot) {
+      const float2 v = masked(r, n, z, slot);
+      const int code = (pr * RL + slot) * 2;
+      if (v.x > vmax) { vmax = v.x; vcode = code; }
+      if (v.y > vmax) { vmax = v.y; vcode = code + 1; }
+    };
+    fft_exec<M, RB, WG, +1, MAP_J, MAP_J, V>(lds, twM, load, store);
+    if (vcode >= 0) {
+      const int pr = vcode / (2 * RL), slot = (vcode >> 1) % RL, half = vcode & 1;
+      const int g = threadIdx.x + pr * WG;
+      const int row = g / NSL, j = g % NSL;
+      pos = (y0 + row) * L + 2 * (j + slot * NSL) + half;
+    }
+    wave_max_idx(vmax, pos);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red_v[w] = vmax; red_p[w] = pos; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      bmax = red_v[0];
+      pos = red_p[0];
+      for (int i = 1; i < WG / 64; ++i) vmax_combine(bmax, pos, red_v[i], red_p[i]);
+    }
   }
-  wave_max_idx(vmax, pos);
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) { red_v[w] = vmax; red_p[w] = pos; }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    float v = red_v[0];
-    int p = red_p[0];
-    for (int i = 1; i < WG / 64; ++i) vmax_combine(v, p, red_v[i], red_p[i]);
     CorrPartial cp;
-    cp.vmax = v; cp.pos = p; cp.s1 = 0.0f; cp.s2 = 0.0f;
+    cp.vmax = bmax; cp.pos = pos; cp.s1 = 0.0f; cp.s2 = 0.0f;
     part[(long)m * nparts + rb] = cp;
   }
 }

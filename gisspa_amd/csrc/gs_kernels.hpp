@@ -81,6 +81,7 @@ void launch_rows_r2c_rotb(int L, const float* treal, int nb, const double2* cs_b
 // Real rows -> half spectrum of real images imgs[b][L][L].
 void launch_rows_r2c_img(int L, const float* imgs, int nb, float2* out, const float2* twL,
                          const float2* twM, hipStream_t s);
+
 // In-place column transform of [b][kx][y]; multiplies by scale (global) or
 // per-b scale from `bscale` (may be null); zero_dc zeroes element (0,0).
 void launch_cols_fwd(int L, float2* spec, int nb, float scale, const float* bscale, int zero_dc,
@@ -266,8 +267,8 @@ struct RefArgs {
   double* rot_part;     // [nb][row blocks][2]
   float* bscale;        // [nb]
   float* pre;           // [nb]
-  float* gRn;           // [nb][L][L]
   float* gTb;           // [nb][L][L]
+  float* gT;            // [nb][L][L]
   double* part;         // [nb][ref_parts()][4]
   RefScal* sc;          // [nb]
   float* out;           // [nb][5]: score, d psi, d dx, d dy, d defocus

@@ -257,18 +257,25 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
 // and kx = L/2 columns are Hermitian in ky, so those two columns are loaded
 // as (G(ky) + conj G(-ky)) / 2.  The inverse column transform then writes the
 // intermediate of the C2R that yields d/dRn.
+// The normalisation adjoint needs sum_x g Rn with g = d/dRn; by Parseval it
+// is sum_half w Re(G conj(That)) (w = 2 between the columns 0 and L/2, 1 on
+// them), accumulated here from the values the C2R receives; sum_x g is
+// Re G(0, 0) = 0, as the DC carries no gradient.
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__ tiles,
                                                       const float2* __restrict__ spec,
                                                       const RefCand* __restrict__ cand,
                                                       const RefScal* __restrict__ sc,
                                                       float2* __restrict__ Gout,
+                                                      double* __restrict__ part,
                                                       const float2* __restrict__ twL) {
   using Gm = Geo<L>;
   constexpr int H = Gm::H, HP = Gm::HP, CB = Gm::CB;
+  static_assert(ncol_blocks<L>() <= FSLOTS, "partial slots");
   __shared__ float2 lds[CB * L];
   __shared__ float2 tw[L];
   __shared__ float2 ex[H], ey[L];
+  __shared__ double red[WG / 64];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   const RefCand cd = cand[b];
   load_twiddles<L, WG>(tw, twL);
@@ -278,32 +285,56 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
   const double sg = sc[b].sigma;
   const float is = sg > 0 ? (float)(1.0 / sg) : 0.0f;
   const float cs3 = sg > 0 ? (float)(sc[b].c / (sg * sg * sg)) : 0.0f;
-  auto grad = [&](int kx, int ky) -> float2 {
+  // G = g I; Re(G conj(That)) = Re(g P) comes with it (the Parseval term)
+  auto grad = [&](int kx, int ky, float* gp) -> float2 {
     if (kx == 0 && ky == 0) return make_float2(0.f, 0.f);
     const long i = (long)kx * L + ky;
     const float2 iv = I[i];
     const float2 P = cmulc(iv, T[i]);
     const float2 e = cmul(ex[kx], ey[ky]);
     const float2 g = make_float2(e.x * is - cs3 * P.x, e.y * is + cs3 * P.y);
+    if (gp) *gp = fmaf(g.x, P.x, -g.y * P.y);
     return cmul(g, iv);
   };
   const int ncol = min(CB, H - kx0);
+  // a thread's ~CB L / WG terms in FP32, the block in FP64; on the columns
+  // kx = 0 and L/2 the symmetrised G gives the same sum over ky as G itself
+  // (That is Hermitian there)
+  float accf = 0.0f;
   for (int g = threadIdx.x; g < CB * L; g += WG) {
     const int c = g / L, ky = g - c * L;
     float2 gv = make_float2(0.f, 0.f);
     if (c < ncol) {
       const int kx = kx0 + c;
-      gv = grad(kx, ky);
-      if (kx == 0 || kx == L / 2) {
-        const float2 gm = grad(kx, (L - ky) % L);
+      float gp = 0.0f;
+      gv = grad(kx, ky, &gp);
+      const bool edge = kx == 0 || kx == L / 2;
+      if (edge) {
+        const float2 gm = grad(kx, (L - ky) % L, nullptr);
         gv = make_float2(0.5f * (gv.x + gm.x), 0.5f * (gv.y - gm.y));
       }
+      accf = fmaf(edge ? 1.0f : 2.0f, gp, accf);
     }
     lds[g] = gv;
   }
+  double acc[1] = {(double)accf};
+  block_sum_n<1>(acc, red);  // (its barriers also order the LDS fill)
+  if (threadIdx.x == 0) part[(long)b * FSLOTS + blockIdx.x] = acc[0];
   __syncthreads();
   lds_fft<L, CB, WG, +1, L>(lds, tw);
   cols_inv_store<L>(lds, Gout + (long)b * L * HP, kx0, ncol);
+}
+
+// mean of g = d/dRn (0: no gradient at DC) and of g Rn (the column partials
+// of k_ref_grad_cols over L^2)
+__global__ void k_ref_norm_fin(const double* __restrict__ part, int nb, int ncb, double L2,
+                               RefScal* __restrict__ sc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double gr = 0;
+  for (int i = 0; i < ncb; ++i) gr += part[(long)b * FSLOTS + i];
+  sc[b].gmean = 0.0;
+  sc[b].grn = gr / L2;
 }
 
 // Backward: forward column transform of the row R2C of d/dT (d/dA), reduced
@@ -396,85 +427,65 @@ __global__ void k_ref_score_fin(const double* __restrict__ part, int nb, int L,
 // ---------------------------------------------------------------------------
 // backward
 // ---------------------------------------------------------------------------
-// sums of g = d/dRn and of g Rn over the pixels (normalisation adjoint)
+// Row C2R of the gradient intermediate (g L^2 = d/dRn L^2, as
+// k_rows_c2r_stream) whose store applies the normalisation and mask adjoint
+// per pixel: d/dRm = (g - mean g - Rn mean(g Rn)) / sd, d/d(Tb p) = d/dRm
+// mask, written as d/dTb; it also accumulates the sums for d/dpsi (bilinear
+// slopes times d(x2, y2)/dpsi) and d/dm, one partial per row block.
 template <int L>
-__global__ __launch_bounds__(WG) void k_ref_norm_red(const float* __restrict__ Tun,
-                                                     const float* __restrict__ gRn,
-                                                     const double2* __restrict__ cs_b,
-                                                     const float* __restrict__ pre,
-                                                     const RefScal* __restrict__ sc, float dmask,
-                                                     double* __restrict__ part) {
+__global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(const float2* __restrict__ Gin,
+                                                          const float* __restrict__ Tun,
+                                                          const double2* __restrict__ cs_b,
+                                                          const float* __restrict__ pre,
+                                                          const RefScal* __restrict__ sc,
+                                                          float dmask, float* __restrict__ gTb,
+                                                          double* __restrict__ part,
+                                                          const float2* __restrict__ twL,
+                                                          const float2* __restrict__ twM) {
+  using G = Geo<L>;
+  constexpr int M = G::M, HP = G::HP, RB = G::RB;
   constexpr long L2 = (long)L * L;
+  static_assert((L + RB - 1) / RB <= RPARTS, "partial slots");
+  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
-  const double2 cs = cs_b[b];
-  const float p = pre[b];
-  const double mu = sc[b].mu, isd = 1.0 / sc[b].sd;
-  const float* src = Tun + b * L2;
-  double acc[2] = {0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
-    const int y = (int)(i / L), x = (int)(i - (long)y * L);
-    const RotSampleD rs = rot_sample_d<L>(src, x, y, cs, dmask);
-    const double rn = ((double)(rs.tb * p * rs.mask) - mu) * isd;
-    const double g = (double)gRn[b * L2 + i] / (double)L2;
-    acc[0] += g;
-    acc[1] += g * rn;
-  }
-  block_sum_n<2>(acc, red);
-  if (threadIdx.x == 0) {
-    part[((long)b * RPARTS + blockIdx.x) * 2] = acc[0];
-    part[((long)b * RPARTS + blockIdx.x) * 2 + 1] = acc[1];
-  }
-}
-
-__global__ void k_ref_norm_fin(const double* __restrict__ part, int nb, double L2,
-                               RefScal* __restrict__ sc) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  double g = 0, gr = 0;
-  for (int i = 0; i < RPARTS; ++i) {
-    g += part[((long)b * RPARTS + i) * 2];
-    gr += part[((long)b * RPARTS + i) * 2 + 1];
-  }
-  sc[b].gmean = g / L2;
-  sc[b].grn = gr / L2;
-}
-
-// d/dTb per output pixel (written to gTb), and the sums for d/dpsi and d/dm
-template <int L>
-__global__ __launch_bounds__(WG) void k_ref_rot_grad(const float* __restrict__ Tun,
-                                                     const float* __restrict__ gRn,
-                                                     const double2* __restrict__ cs_b,
-                                                     const float* __restrict__ pre,
-                                                     const RefScal* __restrict__ sc, float dmask,
-                                                     float* __restrict__ gTb,
-                                                     double* __restrict__ part) {
-  constexpr long L2 = (long)L * L;
-  __shared__ double red[2 * (WG / 64)];
-  const int b = blockIdx.y;
+  const int y0 = blockIdx.x * RB;
+  const float2* gb = Gin + (long)b * L * HP;
   const double2 cs = cs_b[b];
   const float p = pre[b];
   const double mu = sc[b].mu, isd = 1.0 / sc[b].sd, gm = sc[b].gmean, grn = sc[b].grn;
   const float* src = Tun + b * L2;
   const float kdeg = kRefPi / 180.0f;
+  auto load = [&](int r, int k) -> float2 {
+    const float2* row = gb + (long)min(y0 + r, L - 1) * HP;
+    float2 w = twL[k];
+    w.y = -w.y;
+    return c2r_pre(row[k], row[M - k], w);
+  };
+  float* ob = gTb + b * L2;
   double acc[2] = {0, 0};
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
-    const int y = (int)(i / L), x = (int)(i - (long)y * L);
+  auto pixel = [&](int x, int y, float gval) -> float {
     const RotSampleD rs = rot_sample_d<L>(src, x, y, cs, dmask);
-    float out = 0.0f;
-    if (rs.in) {
-      const double rn = ((double)(rs.tb * p * rs.mask) - mu) * isd;
-      const double g = (double)gRn[b * L2 + i] / (double)L2;
-      const double grm = (g - gm - rn * grn) * isd;  // d/dRm
-      const double gv = grm * rs.mask;                // d/d(Tb p)
-      out = (float)(gv * p);                          // d/dTb
-      const float dx2 = (float)(-cs.y * rs.X + cs.x * rs.Y) * kdeg;
-      const float dy2 = (float)(-cs.x * rs.X - cs.y * rs.Y) * kdeg;
-      acc[0] += (double)out * (double)(rs.dtx * dx2 + rs.dty * dy2);
-      acc[1] += gv * (double)rs.tb;
+    if (!rs.in) return 0.0f;
+    const double rn = ((double)(rs.tb * p * rs.mask) - mu) * isd;
+    const double g = (double)gval / (double)L2;
+    const double grm = (g - gm - rn * grn) * isd;  // d/dRm
+    const double gv = grm * rs.mask;                // d/d(Tb p)
+    const float out = (float)(gv * p);              // d/dTb
+    const float dx2 = (float)(-cs.y * rs.X + cs.x * rs.Y) * kdeg;
+    const float dy2 = (float)(-cs.x * rs.X - cs.y * rs.Y) * kdeg;
+    acc[0] += (double)out * (double)(rs.dtx * dx2 + rs.dty * dy2);
+    acc[1] += gv * (double)rs.tb;
+    return out;
+  };
+  auto store = [&](int r, int n, float2 z) {
+    const int y = y0 + r;
+    if (y < L) {
+      const float a = pixel(2 * n, y, z.x), c = pixel(2 * n + 1, y, z.y);
+      *reinterpret_cast<float2*>(ob + (long)y * L + 2 * n) = make_float2(a, c);
     }
-    gTb[b * L2 + i] = out;
-  }
+  };
+  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
   block_sum_n<2>(acc, red);
   if (threadIdx.x == 0) {
     part[((long)b * RPARTS + blockIdx.x) * 2] = acc[0];
@@ -482,11 +493,12 @@ __global__ __launch_bounds__(WG) void k_ref_rot_grad(const float* __restrict__ T
   }
 }
 
-__global__ void k_ref_rot_fin(const double* __restrict__ part, int nb, RefScal* __restrict__ sc) {
+__global__ void k_ref_rot_fin(const double* __restrict__ part, int nb, int np,
+                              RefScal* __restrict__ sc) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double dp = 0, dpre = 0;
-  for (int i = 0; i < RPARTS; ++i) {
+  for (int i = 0; i < np; ++i) {
     dp += part[((long)b * RPARTS + i) * 2];
     dpre += part[((long)b * RPARTS + i) * 2 + 1];
   }
@@ -495,11 +507,7 @@ __global__ void k_ref_rot_fin(const double* __restrict__ part, int nb, RefScal* 
   sc[b].dmacc = -dpre / (sc[b].m * sc[b].m);
 }
 
-// Bilinear adjoint as a gather: source pixel q collects d/dTb of every output
-// pixel whose interpolation cell (the forward's exact arithmetic, boundary
-// rules included) has q as a corner, times that corner's weight.  The
-// outputs whose cell can touch q lie within 2 pixels of the inverse rotation
-// of q.
+// the bilinear adjoint of d/dTb (rot_adj_gather) for every source pixel
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gTb,
                                                     const double2* __restrict__ cs_b, float dmask,
@@ -508,47 +516,9 @@ __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gT
   const int b = blockIdx.y;
   const double2 cs = cs_b[b];
   const float* gsrc = gTb + b * L2;
-  const float reach = dmask * 0.5f + 12.0f + 3.0f;
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
     const int qy = (int)(i / L), qx = (int)(i - (long)qy * L);
-    const float qX = (float)(qx - L / 2), qY = (float)(qy - L / 2);
-    float acc = 0.0f;
-    if (qX * qX + qY * qY <= reach * reach) {
-      // inverse rotation: x - c = R^T (q - c).  An output pixel samples q's
-      // cells only if its rotated position lies within 1 of q on both axes,
-      // so within sqrt(2) of R^T (q - c) on each: the 4 x 4 window
-      // floor(c) - 1 .. floor(c) + 2 holds all of them (0.41 px to spare for
-      // the float rounding of c); the others carry weight 0.
-      const float cX = (float)(cs.x * qX - cs.y * qY), cY = (float)(cs.y * qX + cs.x * qY);
-      const int x0 = (int)floorf(cX) + L / 2 - 1, y0 = (int)floorf(cY) + L / 2 - 1;
-      for (int dy = 0; dy < 4; ++dy) {
-        const int y = y0 + dy;
-        if (y < 0 || y >= L) continue;
-        for (int dx = 0; dx < 4; ++dx) {
-          const int x = x0 + dx;
-          if (x < 0 || x >= L) continue;
-          const float g = gsrc[(long)y * L + x];
-          if (g == 0.0f) continue;
-          const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
-          const double x2 = cs.x * xf + cs.y * yf + (double)(L / 2);
-          const double y2 = -cs.y * xf + cs.x * yf + (double)(L / 2);
-          if (x2 < 0 || x2 > L - 1.0 || y2 < 0 || y2 > L - 1.0) continue;
-          const int ii = (int)floor(x2), jj = (int)floor(y2);
-          long k0 = ii + (long)jj * L, k1 = k0 + 1, k2 = k0 + L + 1, k3 = k0 + L;
-          if (ii == L - 1) { k1--; k2--; }
-          if (jj == L - 1) { k2 -= L; k3 -= L; }
-          const float t = (float)(x2 - ii), u = (float)(y2 - jj);
-          const float tt = 1.0f - t, uu = 1.0f - u;
-          float wq = 0.0f;
-          if (k0 == i) wq += tt * uu;
-          if (k1 == i) wq += t * uu;
-          if (k2 == i) wq += t * u;
-          if (k3 == i) wq += tt * u;
-          acc += g * wq;
-        }
-      }
-    }
-    gT[b * L2 + i] = acc;
+    gT[b * L2 + i] = rot_adj_gather<L>(gsrc, qx, qy, cs, dmask);
   }
 }
 
@@ -594,20 +564,23 @@ static void ref_backward_t(const RefArgs& a, hipStream_t s) {
   const dim3 grid(RPARTS, a.nb), blk(WG);
   const int small = (a.nb + 63) / 64;
   constexpr int NCB = ncol_blocks<L>();
-  // d/dThat -> d/dRn (C2R, / L^2 applied in the next kernels)
+  // d/dThat (with the Parseval sum of the normalisation adjoint) -> the
+  // transposed intermediate of the C2R
   hipLaunchKernelGGL((k_ref_grad_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.tiles, a.spec, a.cand,
-                     a.sc, a.gtmp, a.twL);
-  launch_rows_c2r_stream(L, a.gtmp, a.nb, a.gRn, a.twL, a.twM, s);
-  hipLaunchKernelGGL((k_ref_norm_red<L>), grid, blk, 0, s, a.Tun, a.gRn, a.cs, a.pre, a.sc,
-                     a.dmask, a.part);
-  hipLaunchKernelGGL(k_ref_norm_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, (double)L * L,
-                     a.sc);
-  hipLaunchKernelGGL((k_ref_rot_grad<L>), grid, blk, 0, s, a.Tun, a.gRn, a.cs, a.pre, a.sc,
-                     a.dmask, a.gTb, a.part);
-  hipLaunchKernelGGL(k_ref_rot_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc);
-  hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.gRn);
-  // d/dT (in gRn) -> d/dA: unnormalised R2C
-  launch_rows_r2c_img(L, a.gRn, a.nb, a.A, a.twL, a.twM, s);
+                     a.sc, a.gtmp, a.part, a.twL);
+  hipLaunchKernelGGL(k_ref_norm_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, NCB,
+                     (double)L * L, a.sc);
+  // row C2R -> d/dRn -> d/dTb per pixel, d/dpsi and d/dm sums
+  constexpr int NRB = (L + Geo<L>::RB - 1) / Geo<L>::RB;
+  hipLaunchKernelGGL((k_ref_rows_c2r_grad<L>), dim3(NRB, a.nb), blk, 0, s, a.gtmp, a.Tun, a.cs,
+                     a.pre, a.sc, a.dmask, a.gTb, a.part, a.twL, a.twM);
+  hipLaunchKernelGGL(k_ref_rot_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, NRB, a.sc);
+  // d/dT: the bilinear adjoint of d/dTb, gathered per source pixel (a
+  // separate pass: gathered inside the row transform's loads it took 267 us
+  // per batch of 64 at window 720 against 209 for the two kernels)
+  hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.gT);
+  // d/dT -> d/dA: unnormalised R2C
+  launch_rows_r2c_img(L, a.gT, a.nb, a.A, a.twL, a.twM, s);
   hipLaunchKernelGGL((k_ref_cols_df<L>), dim3(NCB, a.nb), blk, 0, s, a.A, a.white, a.cand, a.w,
                      a.wtab, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.out, NCB,

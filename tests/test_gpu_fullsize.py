@@ -1,7 +1,7 @@
 """GPU parity at the benchmark shapes, against the FP64 oracle:
-  * the plain-method small configuration: a 1024 x 1024 micrograph with 20
-    particles, 64 views of a 128^3 blob density, window 512 / overlap 128
-    (9 tiles), 10 degree in-plane steps (the first 4 of 36);
+  * the plain-method small configuration in full: a 1024 x 1024 micrograph
+    with 20 particles, 64 views of a 128^3 blob density, window 512 / overlap
+    128 (9 tiles), all 36 in-plane steps of 10 degrees (20,736 maps);
   * the full-size shape of the benchmark: a 5760 x 4092 micrograph, window
     720 / overlap 180 (88 tiles), 180 px templates at 16 of the 1649 views of
     the reference's euler_C1_step5.lst (particles embedded at all 1649), the
@@ -47,11 +47,12 @@ def test_small_configuration(gpu):
     wl = synth.small_workload(seed=0)
     img, df = wl.micrographs[0]
     p = orc.params(512, **wl.params)
-    ref = orc.pick_norm0(p, orc.ctf(df), wl.templates, img, 0, 4, nthreads=NTHREADS)
+    ref = orc.pick_norm0(p, orc.ctf(df), wl.templates, img, nthreads=NTHREADS)
+    assert ref.shape == (36, 9, 64)
     with gisspa_amd.Picker(0, **wl.params) as pk:
         pk.set_templates(wl.templates, df)
         pk.set_micrograph(img, df)
-        picks = pk.pick(0, 4)
+        picks = pk.pick()
         got = pk.mapstats().reshape(ref.shape)
     _compare(got, ref, "1024^2 / window 512")
     _compare_picks(picks, orc.select_norm0(p, ref, 0, img.shape[1], img.shape[0]))
