@@ -681,6 +681,143 @@ __device__ __forceinline__ void fft_exec(float2* lds, const float2* tw, LoadF& l
   fft_exec_pass<N, NB, WG, DIR, MAP0, MAPL_EFF, 0, 1, V>(lds, tw, load, store);
 }
 
+// ---------------------------------------------------------------------------
+// Split-exchange executor: the same passes, thread mapping and arithmetic as
+// fft_exec, but LDS holds one float per element, so a workgroup needs half
+// the LDS and more workgroups fit on a CU.  Between two passes the real parts
+// go through LDS first, then (after every thread has read them) the
+// imaginary parts, with three barriers instead of one.
+// ---------------------------------------------------------------------------
+struct NoPending {
+  __device__ __forceinline__ void operator()() const {}
+};
+
+template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int PASS, int NS, int V, class LoadF,
+          class StoreF, class PendF>
+__device__ __forceinline__ void fft_exec_pass_split(float* lds, const float2* tw, LoadF& load,
+                                                    StoreF& store, PendF& pend_im) {
+  constexpr Plan P = variant_plan(N, NB, WG, MAPL, V);
+  constexpr int R = P.r[PASS];
+  constexpr bool FIRST = PASS == 0, LAST = PASS == P.count - 1;
+  constexpr int M = N / R;
+  constexpr int TOT = NB * M;
+  constexpr int PER = (TOT + WG - 1) / WG;
+  constexpr int TWS = N / (NS * R);
+  constexpr int STRIDE = padded_len_sm<P.pad_shift, P.pad_mult>(N);
+  auto pidx = [](int i) { return pidx_sm<P.pad_shift, P.pad_mult>(i); };
+  constexpr int MAPP = FIRST ? MAP0 : (LAST ? MAPL : MAP_J);
+  float2 v[PER][R];
+  const int tid = threadIdx.x;
+  if constexpr (FIRST) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int g = tid + p * WG;
+      if (TOT % WG == 0 || g < TOT) {
+        const int b = (MAPP == MAP_J) ? g / M : g % NB;
+        const int j = (MAPP == MAP_J) ? g % M : g / NB;
+        if constexpr (std::is_invocable_v<LoadF&, int, int, std::integral_constant<int, 0>>) {
+          unroll<R>([&](auto qc) { v[p][decltype(qc)::value] = load(b, j, qc); });
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[p][r] = load(b, j + r * M);
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int g = tid + p * WG;
+      if (TOT % WG == 0 || g < TOT) {
+        const int b = (MAPP == MAP_J) ? g / M : g % NB;
+        const int j = (MAPP == MAP_J) ? g % M : g / NB;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[p][r].x = lds[b * STRIDE + pidx(j + r * M)];
+      }
+    }
+    __syncthreads();  // real parts read before the imaginary parts overwrite them
+    pend_im();
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int g = tid + p * WG;
+      if (TOT % WG == 0 || g < TOT) {
+        const int b = (MAPP == MAP_J) ? g / M : g % NB;
+        const int j = (MAPP == MAP_J) ? g % M : g / NB;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[p][r].y = lds[b * STRIDE + pidx(j + r * M)];
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int g = tid + p * WG;
+    if (TOT % WG == 0 || g < TOT) {
+      const int j = (MAPP == MAP_J) ? g % M : g / NB;
+      if constexpr (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[k * r * TWS];
+          if constexpr (DIR > 0) w.y = -w.y;
+          v[p][r] = cmul(v[p][r], w);
+        }
+      }
+      Dft<R, DIR>::run(v[p]);
+    }
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int g = tid + p * WG;
+      if (TOT % WG == 0 || g < TOT) {
+        const int b = (MAPP == MAP_J) ? g / M : g % NB;
+        const int j = (MAPP == MAP_J) ? g % M : g / NB;
+        const int k = j % NS;
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if constexpr (std::is_invocable_v<StoreF&, int, int, float2, int, int>)
+            store(b, base + r * NS, v[p][r], p, r);
+          else
+            store(b, base + r * NS, v[p][r]);
+        }
+      }
+    }
+  } else {
+    if constexpr (!FIRST) __syncthreads();  // this pass's reads done before its writes
+    auto put = [&](auto comp) {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        const int g = tid + p * WG;
+        if (TOT % WG == 0 || g < TOT) {
+          const int b = (MAPP == MAP_J) ? g / M : g % NB;
+          const int j = (MAPP == MAP_J) ? g % M : g / NB;
+          const int k = j % NS;
+          const int base = (j / NS) * NS * R + k;
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            lds[b * STRIDE + pidx(base + r * NS)] = decltype(comp)::value ? v[p][r].y : v[p][r].x;
+        }
+      }
+    };
+    put(std::integral_constant<int, 0>{});
+    __syncthreads();
+    auto wim = [&]() { put(std::integral_constant<int, 1>{}); };
+    fft_exec_pass_split<N, NB, WG, DIR, MAP0, MAPL, PASS + 1, NS * R, V>(lds, tw, load, store, wim);
+  }
+}
+
+// fft_exec with the split exchange; lds holds exec_stride<...>() * NB floats
+template <int N, int NB, int WG, int DIR, int MAP0, int MAPL, int V = 0, class LoadF, class StoreF>
+__device__ __forceinline__ void fft_exec_split(float* lds, const float2* tw, LoadF& load,
+                                               StoreF& store) {
+  static_assert(plan_ok(N), "FFT length must factor into radices of at most 32");
+  constexpr Plan P = variant_plan(N, NB, WG, MAPL, V);
+  static_assert(P.count != 1 || MAP0 == MAPL, "single-pass plans use one mapping");
+  NoPending none;
+  fft_exec_pass_split<N, NB, WG, DIR, MAP0, MAPL, 0, 1, V>(lds, tw, load, store, none);
+}
+
 // fill an LDS twiddle table exp(-2 pi i m / N) from a global table
 template <int N, int WG>
 __device__ __forceinline__ void load_twiddles(float2* tw, const float2* __restrict__ gtw) {

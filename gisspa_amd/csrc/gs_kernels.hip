@@ -306,8 +306,10 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stream(const float2* __restrict
 // Correlation (compute_corner_CCG GPU_func.cu:584-615, cuFFT inverse
 // main.cu:848, get_peak_and_SUM GPU_func.cu:667-708, host loop main.cu:856-881)
 // ---------------------------------------------------------------------------
+// pass 1 exchanges real and imaginary parts in turn (fft_exec_split): one
+// float per element, in float2 units
 template <int L>
-constexpr int corr_lds1() { return Geo<L>::CB * exec_stride<L, Geo<L>::CB, WG, MAP_B>(); }
+constexpr int corr_lds1() { return (Geo<L>::CB * exec_stride<L, Geo<L>::CB, WG, MAP_B>() + 1) / 2; }
 template <int L>
 constexpr int corr_lds2() { return Geo<L>::RB * exec_stride<L / 2, Geo<L>::RB, WG, MAP_J>(); }
 
@@ -365,7 +367,8 @@ __device__ __forceinline__ void corr_pass1_body(float2* lds, float* red_q, int c
     if constexpr (COH) store_coh(gb + y * CB + c, c2r_pre(v, pv, wk));
     else store_nt(gb + y * CB + c, c2r_pre(v, pv, wk));
   };
-  fft_exec<L, CB, WG, +1, MAP_J, MAP_B, V>(lds, twL, load, store);
+  // half the LDS of a float2 exchange: four workgroups per CU instead of three
+  fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B, V>(reinterpret_cast<float*>(lds), twL, load, store);
   sumsq = wave_sum_f(sumsq);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) red_q[wv] = sumsq;
