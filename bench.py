@@ -636,8 +636,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="CC maps per launch (0 = library default)")
     ap.add_argument("--graphs", action="store_true", help="replay per-psi HIP graphs")
     ap.add_argument("--tile-block", type=int, default=0, help="tiles per block (0 = default)")
-    ap.add_argument("--pipeline", type=int, default=2,
-                    help="0: one launch per psi, 1: one launch per batch, 2: two streams")
+    ap.add_argument("--pipeline", type=int, default=3,
+                    help="0: one launch per psi, 1: one launch per batch, 2: two streams, "
+                         "3: pass-1 / pass-2 launches on one stream")
     ap.add_argument("--lag", type=int, default=0)
     ap.add_argument("--ring", type=int, default=0)
     ap.add_argument("--threshold", type=float, default=15.0)
@@ -722,6 +723,7 @@ def main():
     log(f"[bench] template prep {time.time() - t:.2f}s")
     npsi = 72 if args.psi_limit <= 0 else min(72, args.psi_limit)
     tune = None
+    pipeline = args.pipeline
     if args.autotune:
         t = time.time()
         pk.set_micrograph_device(d_img, nx, ny, defocus)
@@ -734,6 +736,7 @@ def main():
             f"{tr.pass2_variant}, batch {tr.batch}, pipeline {tr.pipeline}, psi {tr.psi_ms:.1f} ms;"
             f" pass 1 us/map {[round(v, 3) for v in tr.pass1_us]}, pass 2 "
             f"{[round(v, 3) for v in tr.pass2_us]}")
+        pipeline = tr.pipeline
 
     def step():
         pk.set_micrograph_device(d_img, nx, ny, defocus)
@@ -817,16 +820,17 @@ def main():
     # spectrum read 8 L H + intermediate written 8 L H) + pass 2 (intermediate
     # read 8 L H).
     mpl = nmaps / max(nl, 1)                   # CC maps per launch
-    if args.pipeline == 2:                     # separate pass-1 / pass-2 launches
+    if pipeline in (2, 3):                     # separate pass-1 / pass-2 launches
         t1, t2 = tms / max(nl, 1), tms2 / max(nl, 1)
         a1 = 16.0 * L * H * mpl / (t1 * 1e-3) / 1e9
         a2 = 8.0 * L * H * mpl / (t2 * 1e-3) / 1e9
         kern, t_launch, ach, bpm = (("k_corr_pass1", t1, a1, 16.0 * L * H) if t1 >= t2 else
                                     ("k_corr_pass2", t2, a2, 8.0 * L * H))
         log(f"[bench] k_corr_pass1 {t1:.4f} ms/launch ({a1:.0f} GB/s), k_corr_pass2 {t2:.4f} "
-            f"ms/launch ({a2:.0f} GB/s), {mpl:.1f} maps/launch (two streams overlap)")
+            f"ms/launch ({a2:.0f} GB/s), {mpl:.1f} maps/launch"
+            + (" (two streams overlap)" if pipeline == 2 else " (one stream)"))
     else:
-        kern, t_launch, bpm = ("k_corr_stream" if args.pipeline == 0 else "k_corr_fused",
+        kern, t_launch, bpm = ("k_corr_stream" if pipeline == 0 else "k_corr_fused",
                                tms / max(nl, 1), 24.0 * L * H)
         ach = bpm * mpl / (t_launch * 1e-3) / 1e9
         log(f"[bench] {kern} {t_launch:.4f} ms/launch, {mpl:.1f} maps/launch, "

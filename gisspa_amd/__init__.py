@@ -111,6 +111,7 @@ GS_MODE_PLAIN = 1
 GS_PIPE_STREAM = 0
 GS_PIPE_LAUNCH = 1
 GS_PIPE_LANES = 2
+GS_PIPE_SERIAL = 3
 
 _lib = None
 

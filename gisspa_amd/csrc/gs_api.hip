@@ -82,8 +82,12 @@ struct Ctx {
   // GS_PIPE_STREAM: one launch per psi; each XCD runs pass 2 of its map
   // `lag` rounds back beside pass 1 of its current map, intermediates in a
   // ring of `ring` slots per XCD
-  int pipeline = GS_PIPE_LANES, lag = 2, ring = 4;  // lag, ring: rounds of 8 maps
-  // FFT schedule variant of each correlation pass (GS_PIPE_LANES launches)
+  // default GS_PIPE_SERIAL: a pass-1 launch fills the GPU, so the two-lane
+  // overlap measured no faster than one stream (179.3 vs 179.4 ms per psi of
+  // the full-size workload), and one stream needs one intermediate
+  int pipeline = GS_PIPE_SERIAL, lag = 2, ring = 4;  // lag, ring: rounds of 8 maps
+  // FFT schedule variant of each correlation pass (GS_PIPE_LANES and
+  // GS_PIPE_SERIAL launches)
   int var1 = 0, var2 = 0;
   DevBuf Gring, counters, err;
   // GS_PIPE_LANES: two streams, each running pass 1 then pass 2 of every
@@ -98,6 +102,7 @@ struct Ctx {
   int maps_T = -1, maps_N = -1, maps_tb = -1, maps_nv = -1;
   int n1_nv = 1;   // views per norm_type 1 batch (maps order)
   bool batch_set = false;   // gs_set_batch chose the batch size
+  bool pipeline_set = false;  // gs_set_pipeline chose the schedule (gs_autotune keeps it)
   DevBuf n1_s1, n1_s2, n1_norm, n1_keys, n1_rec;  // norm_type 1
   // plain mode buffers
   DevBuf pl_tspec, pl_tiles, pl_ccg, pl_part, pl_rotpart, pl_mean, pl_var;
@@ -531,12 +536,12 @@ int ensure_corr_buffers(Ctx* c) {
   const int nb = std::min(c->batch, c->T * c->N);
   const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
   // G[0] serves norm_type 1 and the kernel timing in every schedule; G[1]
-  // only the launch-per-batch schedule
+  // the launch-per-batch and two-lane schedules
   CHK(c->G[0].ensure((size_t)nb * per_map));
   if (c->pipeline == GS_PIPE_STREAM) {
     CHK(c->Gring.ensure((size_t)8 * c->ring * per_map));
     CHK(c->counters.ensure((size_t)2 * c->T * c->N * sizeof(int)));
-  } else {
+  } else if (c->pipeline != GS_PIPE_SERIAL) {
     CHK(c->G[1].ensure((size_t)nb * per_map));
   }
   CHK(c->err.ensure(sizeof(int)));
@@ -612,10 +617,11 @@ int correlate_psi(Ctx* c, MapStat* out) {
       !fits(c->psum, (size_t)nm * ncb * sizeof(float2)) || !fits(c->err, sizeof(int)) ||
       !out)
     return GS_ERR_STATE;
+  const bool serial = c->pipeline == GS_PIPE_SERIAL;
   if (c->pipeline == GS_PIPE_STREAM ? (!fits(c->Gring, (size_t)8 * c->ring * per_map) ||
                                        !fits(c->counters, (size_t)2 * nm * sizeof(int)))
                                     : (!fits(c->G[0], (size_t)nb * per_map) ||
-                                       !fits(c->G[1], (size_t)nb * per_map)))
+                                       (!serial && !fits(c->G[1], (size_t)nb * per_map))))
     return GS_ERR_STATE;
   hipStream_t s = c->stream;
   if (c->pipeline == GS_PIPE_STREAM) {
@@ -633,29 +639,35 @@ int correlate_psi(Ctx* c, MapStat* out) {
                          c->maps_blk.as<MapRef>(), c->N, out, s);
     return GS_OK;
   }
-  if (c->pipeline == GS_PIPE_LANES) {
+  if (c->pipeline == GS_PIPE_LANES || serial) {
     if (c->profile_now) CHK(prof_record(c, 3 * (size_t)nbatch, s));
     // the lanes start after the rotated templates of this psi are ready
-    HIPCHK(hipEventRecord(c->ev_ready, s));
-    for (int i = 0; i < Ctx::NLANE; ++i) HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
+    if (!serial) {
+      HIPCHK(hipEventRecord(c->ev_ready, s));
+      for (int i = 0; i < Ctx::NLANE; ++i)
+        HIPCHK(hipStreamWaitEvent(c->lane_stream[i], c->ev_ready, 0));
+    }
     for (int k = 0; k < nbatch; ++k) {
       const int m0 = k * nb, cnt = std::min(nb, nm - m0);
-      hipStream_t ls = c->lane_stream[k % 2];
+      const int lane = serial ? 0 : k % 2;
+      hipStream_t ls = serial ? s : c->lane_stream[lane];
       if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k, ls));
       launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
-                        c->maps_blk.as<MapRef>() + m0, cnt, c->G[k % 2].as<float2>(),
+                        c->maps_blk.as<MapRef>() + m0, cnt, c->G[lane].as<float2>(),
                         c->psum.as<float2>() + (size_t)m0 * ncb, c->twL.as<float2>(), ls,
                         c->var1);
       if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k + 1, ls));
-      launch_corr_pass2(L, c->G[k % 2].as<float2>(), cnt, lo, hi,
+      launch_corr_pass2(L, c->G[lane].as<float2>(), cnt, lo, hi,
                         c->part.as<CorrPartial>() + (size_t)m0 * np, c->twL.as<float2>(),
                         c->twM.as<float2>(), ls, c->var2);
       if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k + 2, ls));
     }
     if (c->profile_now) c->prof_batches = nbatch;
-    for (int i = 0; i < Ctx::NLANE; ++i) {
-      HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
-      HIPCHK(hipStreamWaitEvent(s, c->ev_lane[i], 0));
+    if (!serial) {
+      for (int i = 0; i < Ctx::NLANE; ++i) {
+        HIPCHK(hipEventRecord(c->ev_lane[i], c->lane_stream[i]));
+        HIPCHK(hipStreamWaitEvent(s, c->ev_lane[i], 0));
+      }
     }
     if (c->profile_now) CHK(prof_record(c, 3 * (size_t)nbatch + 1, s));
     launch_corr_finalize(L, c->part.as<CorrPartial>(), c->psum.as<float2>(), nm,
@@ -1248,13 +1260,14 @@ int gs_set_graphs(void* ctx, int enable) {
 
 int gs_set_pipeline(void* ctx, int mode, int lag, int ring) {
   Ctx* c = static_cast<Ctx*>(ctx);
-  if (!c || mode < GS_PIPE_STREAM || mode > GS_PIPE_LANES) return GS_ERR_ARG;
+  if (!c || mode < GS_PIPE_STREAM || mode > GS_PIPE_SERIAL) return GS_ERR_ARG;
   if (lag <= 0) lag = 2;
   if (ring <= 0) ring = 2 * lag;
   if (ring <= lag) return GS_ERR_ARG;
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipDeviceSynchronize());
   c->pipeline = mode;
+  c->pipeline_set = true;
   c->lag = lag;
   c->ring = ring;
   drop_graphs(c);
@@ -1370,7 +1383,7 @@ int gs_pick(void* ctx, int psi_begin, int psi_end, gs_particle* out, size_t cap,
   c->timing.pick_total_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  const int evs = c->pipeline == GS_PIPE_LANES ? 3 : 2;
+  const int evs = (c->pipeline == GS_PIPE_LANES || c->pipeline == GS_PIPE_SERIAL) ? 3 : 2;
   for (int ib = 0; ib < c->prof_batches; ++ib) {
     float a = 0, b = 0;
     HIPCHK(hipEventElapsedTime(&a, c->prof_ev[evs * ib], c->prof_ev[evs * ib + 1]));
@@ -1705,12 +1718,15 @@ int gs_autotune(void* ctx, int psi_index, gs_tune* out) {
   }
   const int b0 = c->batch;
   const int batches[3] = {std::max(1, b0 / 2), b0, 2 * b0};
-  const int pipes[2] = {GS_PIPE_LANES, GS_PIPE_LAUNCH};
+  // a schedule chosen with gs_set_pipeline stays; otherwise the launch
+  // schedules are candidates
   const int p0 = c->pipeline;
+  const int npipe = c->pipeline_set ? 1 : 3;
+  const int pipes[3] = {c->pipeline_set ? p0 : GS_PIPE_SERIAL, GS_PIPE_LANES, GS_PIPE_LAUNCH};
   double best_ms = 1e300, cur_ms = 1e300;
   int best_b = b0, best_p = p0;
   for (int bi = 0; bi < 3; ++bi)
-    for (int pi = 0; pi < 2; ++pi) {
+    for (int pi = 0; pi < npipe; ++pi) {
       c->batch = batches[bi];
       c->pipeline = pipes[pi];
       double ms = 1e300;

@@ -329,10 +329,10 @@ typedef struct gs_tune {
  * of each correlation pass among the compiled variants (radix orders; timed
  * alone on one batch of in-plane step psi_index), then the tile block (the
  * one in use or all tiles, when the tiles span more than one block), the
- * batch size (half, the current, double) and the pipeline (GS_PIPE_LANES,
- * GS_PIPE_LAUNCH), timed on the whole correlation of that step (best of two
- * runs each; the schedule in use stays unless another is more than 1 %
- * faster).  Applies
+ * batch size (half, the current, double) and, unless gs_set_pipeline chose
+ * one, the pipeline (GS_PIPE_SERIAL, GS_PIPE_LANES, GS_PIPE_LAUNCH), timed
+ * on the whole correlation of that step (best of two runs each; the
+ * schedule in use stays unless another is more than 1 % faster).  Applies
  * the choice to the context and reports it in out (may be null).  Needs templates and a
  * micrograph; replaces the last gs_pick's map statistics (its picks are
  * kept).  All schedules give the same scores within FP32 rounding (the
@@ -361,10 +361,12 @@ int gs_set_batch(void* ctx, int maps_per_batch);
  *   GS_PIPE_LAUNCH: one launch per batch (gs_set_batch), pass 1 of batch k
  *     beside pass 2 of batch k - 1;
  *   GS_PIPE_LANES: separate pass-1 and pass-2 launches per batch on two
- *     streams, alternate batches per stream (default).
+ *     streams, alternate batches per stream;
+ *   GS_PIPE_SERIAL: the same launches on the context stream only (default;
+ *     one intermediate; each launch runs alone, so its duration is its own).
  * lag, ring <= 0 select the defaults (2, 2 lag); ring must exceed lag.
  * All schedules give bit-identical results. */
-enum { GS_PIPE_STREAM = 0, GS_PIPE_LAUNCH = 1, GS_PIPE_LANES = 2 };
+enum { GS_PIPE_STREAM = 0, GS_PIPE_LAUNCH = 1, GS_PIPE_LANES = 2, GS_PIPE_SERIAL = 3 };
 int gs_set_pipeline(void* ctx, int mode, int lag, int ring);
 /* Tiles per block of the norm_type 0 processing order: all views of a block
  * before the next block.  0 (the default) takes as many tiles as fit in

@@ -134,11 +134,13 @@ def test_graph_replay_is_bit_exact(gpu):
 def test_pipelines_agree(gpu, lag, ring):
     """The one-launch-per-psi schedule (in-kernel dependencies through a ring
     of intermediates; small rings force slot reuse) and the two-stream
-    schedule give the same bits as the launch-per-batch schedule."""
+    schedule and the one-stream schedule give the same bits as the
+    launch-per-batch schedule."""
     import gisspa_amd
     c = _cases.tiny(seed=8, l=96)
     out = {}
-    for mode in (gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM, gisspa_amd.GS_PIPE_LANES):
+    for mode in (gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM, gisspa_amd.GS_PIPE_LANES,
+                 gisspa_amd.GS_PIPE_SERIAL):
         with gisspa_amd.Picker(0, **c.params) as pk:
             pk.set_pipeline(mode, lag, ring)
             pk.set_batch(5)
@@ -146,7 +148,8 @@ def test_pipelines_agree(gpu, lag, ring):
             pk.set_micrograph(c.image, c.defocus)
             out[mode] = (pk.pick(), pk.mapstats())
     a = out[gisspa_amd.GS_PIPE_LAUNCH]
-    for b in (out[gisspa_amd.GS_PIPE_STREAM], out[gisspa_amd.GS_PIPE_LANES]):
+    for b in (out[gisspa_amd.GS_PIPE_STREAM], out[gisspa_amd.GS_PIPE_LANES],
+              out[gisspa_amd.GS_PIPE_SERIAL]):
         assert np.array_equal(a[1], b[1])
         assert np.array_equal(a[0], b[0])
 
@@ -217,7 +220,8 @@ def test_launch_schedule_with_profiling(gpu):
     import gisspa_amd
     c = _cases.tiny(seed=8, l=96, n_views=24)
     out = {}
-    for mode in (gisspa_amd.GS_PIPE_LANES, gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM):
+    for mode in (gisspa_amd.GS_PIPE_LANES, gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM,
+                 gisspa_amd.GS_PIPE_SERIAL):
         with gisspa_amd.Picker(0, **c.params) as pk:
             pk.set_pipeline(mode)
             pk.set_batch(189)
@@ -228,12 +232,13 @@ def test_launch_schedule_with_profiling(gpu):
             assert info.n_tiles * info.n_views == 480
             out[mode] = (pk.pick(), pk.mapstats(), pk.timing())
     a = out[gisspa_amd.GS_PIPE_LANES]
-    for mode in (gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM):
+    for mode in (gisspa_amd.GS_PIPE_LAUNCH, gisspa_amd.GS_PIPE_STREAM, gisspa_amd.GS_PIPE_SERIAL):
         b = out[mode]
         assert np.array_equal(a[1], b[1]), mode
         assert np.array_equal(a[0], b[0]), mode
         assert b[2].corr_launches >= 1
     assert a[2].corr_launches == 3 and out[gisspa_amd.GS_PIPE_LAUNCH][2].corr_launches == 4
+    assert out[gisspa_amd.GS_PIPE_SERIAL][2].corr_launches == 3
 
 
 @pytest.mark.parametrize("norm_type", [0, 1])
