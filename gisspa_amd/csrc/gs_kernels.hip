@@ -306,12 +306,12 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stream(const float2* __restrict
 // Correlation (compute_corner_CCG GPU_func.cu:584-615, cuFFT inverse
 // main.cu:848, get_peak_and_SUM GPU_func.cu:667-708, host loop main.cu:856-881)
 // ---------------------------------------------------------------------------
-// pass 1 exchanges real and imaginary parts in turn (fft_exec_split): one
-// float per element, in float2 units
+// both passes exchange real and imaginary parts in turn (fft_exec_split):
+// one float per element, sizes in float2 units
 template <int L>
-constexpr int corr_lds1() { return (Geo<L>::CB * exec_stride<L, Geo<L>::CB, WG, MAP_B>() + 1) / 2; }
+constexpr int corr_lds1() { return (Geo<L>::CB * exec_stride_split<L, Geo<L>::CB, WG, MAP_B>() + 1) / 2; }
 template <int L>
-constexpr int corr_lds2() { return Geo<L>::RB * exec_stride<L / 2, Geo<L>::RB, WG, MAP_J>(); }
+constexpr int corr_lds2() { return (Geo<L>::RB * exec_stride_split<L / 2, Geo<L>::RB, WG, MAP_J>() + 1) / 2; }
 
 // Pass 1 of map m, column block cb: P = tile * conj(template) on the paired
 // columns, inverse column transforms, C2R pre-processing, blocked store.
@@ -548,7 +548,7 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
       vals[pr][slot][0] = v.x;
       vals[pr][slot][1] = v.y;
     };
-    fft_exec<M, RB, WG, +1, MAP_J, MAP_J, V>(lds, twM, load_aff, store);
+    fft_exec_split<M, RB, WG, +1, MAP_J, MAP_J, V>(reinterpret_cast<float*>(lds), twM, load_aff, store);
     float vmax = 0.0f;
 #pragma unroll
     for (int pr = 0; pr < PERL; ++pr)
@@ -599,7 +599,7 @@ __device__ __forceinline__ void corr_pass2_body(float2* lds, float* red_v, int* 
       if (v.x > vmax) { vmax = v.x; vcode = code; }
       if (v.y > vmax) { vmax = v.y; vcode = code + 1; }
     };
-    fft_exec<M, RB, WG, +1, MAP_J, MAP_J, V>(lds, twM, load, store);
+    fft_exec_split<M, RB, WG, +1, MAP_J, MAP_J, V>(reinterpret_cast<float*>(lds), twM, load, store);
     if (vcode >= 0) {
       const int pr = vcode / (2 * RL), slot = (vcode >> 1) % RL, half = vcode & 1;
       const int g = threadIdx.x + pr * WG;
@@ -888,7 +888,9 @@ __global__ __launch_bounds__(WG) void k_norm1_zrun(const float2* __restrict__ Gi
   constexpr int NBLK = (int)((L2 + 1023) / 1024);
   constexpr int NRUN = RB * L / 1024 + 2;
   static_assert(RB * L >= 1024, "a run spans at most two row blocks");
-  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  // real and imaginary parts exchanged in turn (half the LDS of a float2
+  // exchange), like the correlation's pass 2
+  __shared__ float lds[RB * exec_stride_split<M, RB, WG, MAP_J>()];
   __shared__ unsigned long long rk[NRUN];
   const int m = blockIdx.y;
   const int y0 = blockIdx.x * RB;
@@ -919,7 +921,7 @@ __global__ __launch_bounds__(WG) void k_norm1_zrun(const float2* __restrict__ Gi
     visit((z.x * inv - q.x) * q.z, pix);
     visit((z.y * inv - q.y) * q.w, pix + 1);
   };
-  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+  fft_exec_split<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
   if (cur >= 0) atomicMax(&rk[cur - k0], run_key(cv, cp));
   __syncthreads();
   for (int i = threadIdx.x; i <= k1 - k0; i += WG) {
