@@ -1572,9 +1572,12 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
   CHK(ref_eval(c, rc.data(), cs.data(), n, fc.data()));
   c->timing.refine_evals = n;
   std::vector<double> step(n, 0.5);
-  // a candidate whose step has shrunk below kMinStep (0.01 degree / pixel /
-  // 0.001 um) has converged and leaves the evaluated set
-  constexpr double kMinStep = 0.01;
+  // a candidate whose step has shrunk below kMinStep (0.05 degree / pixel /
+  // 0.005 um) has converged and leaves the evaluated set.  On the top 10,000
+  // picks of the full-size micrograph (20 steps): 0.01 -> 198,261
+  // evaluations, mean objective 19.9422; 0.05 -> 159,967, 19.9417; 0.1 ->
+  // 134,883, 19.9400 (start 18.3470)
+  constexpr double kMinStep = 0.05;
   std::vector<int> act(n);
   for (int i = 0; i < n; ++i) act[i] = i;
   std::vector<gs_peak> tact;

@@ -176,27 +176,26 @@ template <int L>
 __global__ __launch_bounds__(WG) void k_cols_fwd(float2* __restrict__ spec, float scale,
                                                  const float* __restrict__ bscale, int zero_dc,
                                                  const float2* __restrict__ twL) {
+  // in place: every input of the workgroup's columns is read in the first
+  // radix pass, before the barrier that precedes the last pass's stores
   using G = Geo<L>;
   constexpr int H = G::H, CB = G::CB;
-  __shared__ float2 lds[CB * L];
-  __shared__ float2 tw[L];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
   const int b = blockIdx.y;
   const int kx0 = blockIdx.x * CB;
-  load_twiddles<L, WG>(tw, twL);
   float2* base = spec + ((long)b * H + kx0) * L;
   const int ncol = min(CB, H - kx0);
-  for (int g = threadIdx.x; g < CB * L; g += WG) {
-    const int c = g / L;
-    lds[g] = (c < ncol) ? base[g] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  lds_fft<L, CB, WG, -1, L>(lds, tw);
   const float sc = bscale ? bscale[b] : scale;
-  for (int g = threadIdx.x; g < ncol * L; g += WG) {
-    float2 v = cscale(lds[g], sc);
-    if (zero_dc && kx0 == 0 && g == 0) v = make_float2(0.f, 0.f);
-    base[g] = v;
-  }
+  auto load = [&](int c, int y) -> float2 {
+    return c < ncol ? base[(long)c * L + y] : make_float2(0.f, 0.f);
+  };
+  auto store = [&](int c, int y, float2 v) {
+    if (c >= ncol) return;
+    v = cscale(v, sc);
+    if (zero_dc && kx0 == 0 && c == 0 && y == 0) v = make_float2(0.f, 0.f);
+    base[(long)c * L + y] = v;
+  };
+  fft_exec_split<L, CB, WG, -1, MAP_J, MAP_B>(lds, twL, load, store);
 }
 
 // inverse columns, transposed write to G[b][y][kx] (pitch HP)

@@ -20,6 +20,7 @@
 // d/dA, and the derivative of the CTF weighting.  All sums are FP64.
 #include "gs_device.hpp"
 #include "gs_sizes.hpp"
+#include "gs_fft.hpp"
 
 namespace gs {
 
@@ -189,17 +190,6 @@ constexpr int FSLOTS = 2 * RPARTS;
 template <int L>
 constexpr int ncol_blocks() { return (Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB; }
 
-// inverse columns of the transposed store of k_cols_inv_T
-template <int L>
-__device__ __forceinline__ void cols_inv_store(const float2* lds, float2* __restrict__ gb,
-                                               int kx0, int ncol) {
-  constexpr int HP = Geo<L>::HP, CB = Geo<L>::CB;
-  for (int g = threadIdx.x; g < CB * L; g += WG) {
-    const int y = g / CB, c = g - y * CB;
-    if (c < ncol) gb[(long)y * HP + kx0 + c] = lds[c * L + y];
-  }
-}
-
 // Forward weighting fused into the inverse column pass: A = f(k; defocus) B_j
 // is formed while loading (never stored), with the partials of sum |A|^2 and
 // sum |B|^2 f f' (over the radial bins the normaliser counts), then the
@@ -215,39 +205,36 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
   using Gm = Geo<L>;
   constexpr int H = Gm::H, HP = Gm::HP, CB = Gm::CB;
   static_assert(ncol_blocks<L>() <= FSLOTS, "partial slots");
-  __shared__ float2 lds[CB * L];
-  __shared__ float2 tw[L];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   const RefCand cd = cand[b];
-  load_twiddles<L, WG>(tw, twL);
   const float2* Bj = white + (long)cd.view * H * L;
   const int ncol = min(CB, H - kx0);
   double acc[2] = {0, 0};
-  for (int g = threadIdx.x; g < CB * L; g += WG) {
-    const int c = g / L, ky = g - c * L;
-    float2 v = make_float2(0.f, 0.f);
-    if (c < ncol) {
-      const int kx = kx0 + c;
-      const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0);
-      const float2 bv = Bj[(long)kx * L + ky];
-      v = cscale(bv, wd.f);
-      if (wd.rbin >= 0) {  // inside the |CTF| rings
-        const double b2 = (double)bv.x * bv.x + (double)bv.y * bv.y;
-        acc[0] += b2 * wd.f * wd.f;
-        acc[1] += b2 * wd.f * wd.fd;
-      }
+  // A = f B formed in the first radix pass's loads, with the partial sums
+  auto load = [&](int c, int ky) -> float2 {
+    if (c >= ncol) return make_float2(0.f, 0.f);
+    const int kx = kx0 + c;
+    const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0);
+    const float2 bv = Bj[(long)kx * L + ky];
+    if (wd.rbin >= 0) {  // inside the |CTF| rings
+      const double b2 = (double)bv.x * bv.x + (double)bv.y * bv.y;
+      acc[0] += b2 * wd.f * wd.f;
+      acc[1] += b2 * wd.f * wd.fd;
     }
-    lds[g] = v;
-  }
-  block_sum_n<2>(acc, red);  // (its barriers also order the LDS fill)
+    return cscale(bv, wd.f);
+  };
+  float2* gb = Gout + (long)b * L * HP;
+  auto store = [&](int c, int y, float2 v) {
+    if (c < ncol) gb[(long)y * HP + kx0 + c] = v;
+  };
+  fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
+  block_sum_n<2>(acc, red);
   if (threadIdx.x == 0) {
     part[((long)b * FSLOTS + blockIdx.x) * 2] = acc[0];
     part[((long)b * FSLOTS + blockIdx.x) * 2 + 1] = acc[1];
   }
-  __syncthreads();
-  lds_fft<L, CB, WG, +1, L>(lds, tw);
-  cols_inv_store<L>(lds, Gout + (long)b * L * HP, kx0, ncol);
 }
 
 // Backward: d score / d That(k) = g I with g = E / sigma - (c / sigma^3)
@@ -272,14 +259,12 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
   using Gm = Geo<L>;
   constexpr int H = Gm::H, HP = Gm::HP, CB = Gm::CB;
   static_assert(ncol_blocks<L>() <= FSLOTS, "partial slots");
-  __shared__ float2 lds[CB * L];
-  __shared__ float2 tw[L];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
   __shared__ float2 ex[H], ey[L];
   __shared__ double red[WG / 64];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   const RefCand cd = cand[b];
-  load_twiddles<L, WG>(tw, twL);
-  phase_tables<L>(ex, ey, (double)cd.x0 + cd.dx, (double)cd.y0 + cd.dy);
+  phase_tables<L>(ex, ey, (double)cd.x0 + cd.dx, (double)cd.y0 + cd.dy);  // (ends in a barrier)
   const float2* I = tiles + (long)cd.ptile * H * L;
   const float2* T = spec + (long)b * H * L;
   const double sg = sc[b].sigma;
@@ -301,28 +286,27 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
   // kx = 0 and L/2 the symmetrised G gives the same sum over ky as G itself
   // (That is Hermitian there)
   float accf = 0.0f;
-  for (int g = threadIdx.x; g < CB * L; g += WG) {
-    const int c = g / L, ky = g - c * L;
-    float2 gv = make_float2(0.f, 0.f);
-    if (c < ncol) {
-      const int kx = kx0 + c;
-      float gp = 0.0f;
-      gv = grad(kx, ky, &gp);
-      const bool edge = kx == 0 || kx == L / 2;
-      if (edge) {
-        const float2 gm = grad(kx, (L - ky) % L, nullptr);
-        gv = make_float2(0.5f * (gv.x + gm.x), 0.5f * (gv.y - gm.y));
-      }
-      accf = fmaf(edge ? 1.0f : 2.0f, gp, accf);
+  auto load = [&](int c, int ky) -> float2 {
+    if (c >= ncol) return make_float2(0.f, 0.f);
+    const int kx = kx0 + c;
+    float gp = 0.0f;
+    float2 gv = grad(kx, ky, &gp);
+    const bool edge = kx == 0 || kx == L / 2;
+    if (edge) {
+      const float2 gm = grad(kx, (L - ky) % L, nullptr);
+      gv = make_float2(0.5f * (gv.x + gm.x), 0.5f * (gv.y - gm.y));
     }
-    lds[g] = gv;
-  }
+    accf = fmaf(edge ? 1.0f : 2.0f, gp, accf);
+    return gv;
+  };
+  float2* gb = Gout + (long)b * L * HP;
+  auto store = [&](int c, int y, float2 v) {
+    if (c < ncol) gb[(long)y * HP + kx0 + c] = v;
+  };
+  fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
   double acc[1] = {(double)accf};
-  block_sum_n<1>(acc, red);  // (its barriers also order the LDS fill)
+  block_sum_n<1>(acc, red);
   if (threadIdx.x == 0) part[(long)b * FSLOTS + blockIdx.x] = acc[0];
-  __syncthreads();
-  lds_fft<L, CB, WG, +1, L>(lds, tw);
-  cols_inv_store<L>(lds, Gout + (long)b * L * HP, kx0, ncol);
 }
 
 // mean of g = d/dRn (0: no gradient at DC) and of g Rn (the column partials
@@ -350,30 +334,28 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
                                                     const float2* __restrict__ twL) {
   using Gm = Geo<L>;
   constexpr int H = Gm::H, CB = Gm::CB;
-  __shared__ float2 lds[CB * L];
-  __shared__ float2 tw[L];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
   __shared__ double red[1 * (WG / 64)];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   const RefCand cd = cand[b];
-  load_twiddles<L, WG>(tw, twL);
   const float2* base = rows + ((long)b * H + kx0) * L;
-  const int ncol = min(CB, H - kx0);
-  for (int g = threadIdx.x; g < CB * L; g += WG) {
-    const int c = g / L;
-    lds[g] = (c < ncol) ? base[g] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  lds_fft<L, CB, WG, -1, L>(lds, tw);
   const float2* Bj = white + (long)cd.view * H * L;
+  const int ncol = min(CB, H - kx0);
+  auto load = [&](int c, int ky) -> float2 {
+    return c < ncol ? base[(long)c * L + ky] : make_float2(0.f, 0.f);
+  };
+  // the last radix pass hands each output d/dA(kx, ky) to the reduction
   double acc[1] = {0};
-  for (int g = threadIdx.x; g < ncol * L; g += WG) {
-    const int c = g / L, ky = g - c * L, kx = kx0 + c;
+  auto store = [&](int c, int ky, float2 ga) {
+    if (c >= ncol) return;
+    const int kx = kx0 + c;
     const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0);
-    if (wd.fd == 0.0f) continue;
-    const float2 ga = lds[g], bv = Bj[(long)kx * L + ky];
+    if (wd.fd == 0.0f) return;
+    const float2 bv = Bj[(long)kx * L + ky];
     const float wh = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
     acc[0] += (double)(wh * wd.fd * (ga.x * bv.x + ga.y * bv.y));
-  }
+  };
+  fft_exec_split<L, CB, WG, -1, MAP_J, MAP_B>(lds, twL, load, store);
   block_sum_n<1>(acc, red);
   if (threadIdx.x == 0) part[(long)b * FSLOTS + blockIdx.x] = acc[0];
 }
