@@ -692,15 +692,26 @@ struct NoPending {
   __device__ __forceinline__ void operator()() const {}
 };
 
-// LDS padding of the split exchange where it differs from the float2
-// exchange's (one float per element changes the bank pattern), chosen by
-// timing: the row transforms of the full-size window's correlation pass 2
-// (360 points, 16 rows) run 0.517 instead of 0.583 us per map with
-// (shift 3, mult 2) against the float2 table's (1, 0).
+// LDS layout of the split exchange where it differs from the float2
+// exchange's (one float per element changes the bank pattern).
+//  * shift, mult: padding of the generic layout (element i at
+//    i + (i >> shift) mult);
+//  * pc > 0, for a two-pass plan R0 x R1: element i = q R0 + s lives at
+//    q (R0 + pc) + s.  The first pass writes the R0 outputs of its butterfly
+//    q there and the second pass reads elements s + r R0 at r (R0 + pc) + s,
+//    so every address is a per-thread base plus a compile-time offset (no
+//    per-element index arithmetic), and consecutive butterflies land pc
+//    banks apart.
+// Chosen by timing at the full-size window: pass 2's rows (360 points, 16
+// rows) ran 0.517 us per map with (shift 3, mult 2) and 0.583 with the
+// float2 table's (1, 0); the two-pass layout (pc = 1) then took 20 % of pass
+// 2's and 13 % of pass 1's VALU instructions and 38 % of their LDS
+// instructions away (bank conflicts -72 % / -54 %), pass 1 0.682 instead of
+// 0.704 us per map, pass 2 unchanged (it waits on its loads).
 struct SplitPad {
-  int n, nb, mapl, shift, mult;
+  int n, nb, mapl, shift, mult, pc;
 };
-constexpr SplitPad kSplitPad[] = {{360, 16, 0, 3, 2}};
+constexpr SplitPad kSplitPad[] = {{360, 16, 0, 3, 2, 1}, {720, 8, 1, 6, 3, 1}};
 constexpr Plan split_plan(int N, int NB, int WG, int MAPL, int V) {
   Plan p = variant_plan(N, NB, WG, MAPL, V);
   for (const SplitPad& t : kSplitPad)
@@ -710,10 +721,20 @@ constexpr Plan split_plan(int N, int NB, int WG, int MAPL, int V) {
     }
   return p;
 }
+// pc of the two-pass layout above, 0 where the generic layout applies
+constexpr int split_pair_pad(int N, int NB, int WG, int MAPL, int V) {
+  const Plan p = variant_plan(N, NB, WG, MAPL, V);
+  if (p.count != 2) return 0;
+  for (const SplitPad& t : kSplitPad)
+    if (t.n == N && t.nb == NB && t.mapl == MAPL) return t.pc;
+  return 0;
+}
 // floats per transform (padding included) the split executor needs
 template <int N, int NB, int WG, int MAPL, int V = 0>
 __host__ __device__ constexpr int exec_stride_split() {
   constexpr Plan P = split_plan(N, NB, WG, MAPL, V);
+  constexpr int PC = split_pair_pad(N, NB, WG, MAPL, V);
+  if constexpr (PC > 0) return P.r[1] * (P.r[0] + PC);
   return padded_len_sm<P.pad_shift, P.pad_mult>(N);
 }
 
@@ -728,8 +749,20 @@ __device__ __forceinline__ void fft_exec_pass_split(float* lds, const float2* tw
   constexpr int TOT = NB * M;
   constexpr int PER = (TOT + WG - 1) / WG;
   constexpr int TWS = N / (NS * R);
-  constexpr int STRIDE = padded_len_sm<P.pad_shift, P.pad_mult>(N);
+  constexpr int STRIDE = exec_stride_split<N, NB, WG, MAPL, V>();
+  constexpr int PC = split_pair_pad(N, NB, WG, MAPL, V);
   auto pidx = [](int i) { return pidx_sm<P.pad_shift, P.pad_mult>(i); };
+  // LDS address of input element j + r M of a later pass and of output
+  // element base + r NS of an earlier one (with the two-pass layout, the
+  // second pass's M and the first pass's R are both R0, and NS = 1)
+  auto raddr = [&](int b, int j, int r) {
+    if constexpr (PC > 0) return b * STRIDE + r * (P.r[0] + PC) + j;
+    else return b * STRIDE + pidx(j + r * M);
+  };
+  auto waddr = [&](int b, int j, int base, int r) {
+    if constexpr (PC > 0) return b * STRIDE + j * (P.r[0] + PC) + r;
+    else return b * STRIDE + pidx(base + r * NS);
+  };
   constexpr int MAPP = FIRST ? MAP0 : (LAST ? MAPL : MAP_J);
   float2 v[PER][R];
   const int tid = threadIdx.x;
@@ -756,7 +789,7 @@ __device__ __forceinline__ void fft_exec_pass_split(float* lds, const float2* tw
         const int b = (MAPP == MAP_J) ? g / M : g % NB;
         const int j = (MAPP == MAP_J) ? g % M : g / NB;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[p][r].x = lds[b * STRIDE + pidx(j + r * M)];
+        for (int r = 0; r < R; ++r) v[p][r].x = lds[raddr(b, j, r)];
       }
     }
     __syncthreads();  // real parts read before the imaginary parts overwrite them
@@ -769,7 +802,7 @@ __device__ __forceinline__ void fft_exec_pass_split(float* lds, const float2* tw
         const int b = (MAPP == MAP_J) ? g / M : g % NB;
         const int j = (MAPP == MAP_J) ? g % M : g / NB;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[p][r].y = lds[b * STRIDE + pidx(j + r * M)];
+        for (int r = 0; r < R; ++r) v[p][r].y = lds[raddr(b, j, r)];
       }
     }
   }
@@ -821,7 +854,7 @@ __device__ __forceinline__ void fft_exec_pass_split(float* lds, const float2* tw
           const int base = (j / NS) * NS * R + k;
 #pragma unroll
           for (int r = 0; r < R; ++r)
-            lds[b * STRIDE + pidx(base + r * NS)] = decltype(comp)::value ? v[p][r].y : v[p][r].x;
+            lds[waddr(b, j, base, r)] = decltype(comp)::value ? v[p][r].y : v[p][r].x;
         }
       }
     };

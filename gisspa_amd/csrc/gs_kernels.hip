@@ -306,11 +306,16 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stream(const float2* __restrict
 // main.cu:848, get_peak_and_SUM GPU_func.cu:667-708, host loop main.cu:856-881)
 // ---------------------------------------------------------------------------
 // both passes exchange real and imaginary parts in turn (fft_exec_split):
-// one float per element, sizes in float2 units
-template <int L>
-constexpr int corr_lds1() { return (Geo<L>::CB * exec_stride_split<L, Geo<L>::CB, WG, MAP_B>() + 1) / 2; }
-template <int L>
-constexpr int corr_lds2() { return (Geo<L>::RB * exec_stride_split<L / 2, Geo<L>::RB, WG, MAP_J>() + 1) / 2; }
+// one float per element, sizes in float2 units.  The layout depends on the
+// FFT schedule variant V (its radices), so each kernel sizes LDS for its own.
+template <int L, int V = 0>
+constexpr int corr_lds1() {
+  return (Geo<L>::CB * exec_stride_split<L, Geo<L>::CB, WG, MAP_B, V>() + 1) / 2;
+}
+template <int L, int V = 0>
+constexpr int corr_lds2() {
+  return (Geo<L>::RB * exec_stride_split<L / 2, Geo<L>::RB, WG, MAP_J, V>() + 1) / 2;
+}
 
 // Pass 1 of map m, column block cb: P = tile * conj(template) on the paired
 // columns, inverse column transforms, C2R pre-processing, blocked store.
@@ -399,7 +404,7 @@ __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ ti
                                                    float2* __restrict__ Gout,
                                                    float2* __restrict__ psum,
                                                    const float2* __restrict__ twL) {
-  __shared__ float2 lds[corr_lds1<L>()];
+  __shared__ float2 lds[corr_lds1<L, V>()];
   __shared__ float red_q[WG / 64];
   int cb, m;
   if (!corr_pass1_item<L>(blockIdx.x, nmaps, &cb, &m)) return;
@@ -626,7 +631,7 @@ template <int L, int V>
 __global__ __launch_bounds__(WG) void k_corr_pass2(const float2* __restrict__ Gin, int blo,
                                                    int bhi, CorrPartial* __restrict__ part,
                                                    const float2* __restrict__ twM) {
-  __shared__ float2 lds[corr_lds2<L>()];
+  __shared__ float2 lds[corr_lds2<L, V>()];
   __shared__ float red_v[WG / 64];
   __shared__ int red_p[WG / 64];
   corr_pass2_body<L, false, V>(lds, red_v, red_p, blockIdx.y, blockIdx.y, blockIdx.x, Gin, blo,
