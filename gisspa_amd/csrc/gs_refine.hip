@@ -38,10 +38,13 @@ struct WeightD {
 // defocus (um, positive underfocus: df_ast = -defocus + dfdiff cos(...)),
 // split into a defocus-independent table per half-spectrum cell, so the
 // refinement kernels evaluate the weight of a candidate's defocus with one
-// sincos: with the template CTF's dfdiff / dfang fixed, gamma = g0 + g1
-// defocus (um), and g1 is also d gamma / d defocus.  Entry = (flp, ncurve,
-// g0, g1); ncurve < 0 marks cells outside the |CTF| rings (weight = flp,
-// slope 0, not in the normaliser's sums).
+// sincos: with the template CTF's dfdiff / dfang fixed, gamma = 2 pi (g0 + g1
+// defocus (um)), and 2 pi g1 is d gamma / d defocus.  The table holds the
+// phase in revolutions, so the kernels reduce it by its nearest integer
+// (exact) and take sin / cos of pi times twice the remainder, with no
+// large-argument reduction.  Entry = (flp, ncurve, g0, g1); ncurve < 0 marks
+// cells outside the |CTF| rings (weight = flp, slope 0, not in the
+// normaliser's sums).
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_wtable(WeightArgs w, float4* __restrict__ tab) {
   constexpr int H = L / 2 + 1;
@@ -72,10 +75,9 @@ __global__ __launch_bounds__(WG) void k_ref_wtable(WeightArgs w, float4* __restr
     const float ag = atan2f((float)(y1 - L / 2), (float)x1);
     // df_ast = -defocus + dfdiff cos(2 (dfang - ag)) (dfu + dfv = -2 defocus)
     const float ast = w.dfdiff * cosf(2 * (w.dfang * kRefPi / 180 - ag));
-    const float g1 = 2 * kRefPi * 5000.0f * w.lambda * sc;
-    t.z = -2 * kRefPi * (w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * sc * sc +
-                         ast * 5000.0f * w.lambda * sc);
-    t.w = g1;
+    // gamma / (2 pi), revolutions
+    t.z = -(w.cs * 2.5e6f * w.lambda * w.lambda * w.lambda * sc * sc + ast * 5000.0f * w.lambda * sc);
+    t.w = 5000.0f * w.lambda * sc;
     const float ss = rr * w.ds;
     const float signal = expf(w.bf * ss * ss + w.bf2 * ss + w.bf3) / (w.kk + 1);
     t.y = expf(w.a * ss * ss + w.b * ss + w.b2) / signal;
@@ -83,8 +85,10 @@ __global__ __launch_bounds__(WG) void k_ref_wtable(WeightArgs w, float4* __restr
   tab[i] = t;
 }
 
-// weight_deriv of one cell at a candidate's defocus from its table entry
-__device__ __forceinline__ WeightD weight_tab(float4 t, float defocus, const WeightArgs& w) {
+// weight_deriv of one cell at a candidate's defocus from its table entry;
+// sa = sqrt(1 - ampconst^2)
+__device__ __forceinline__ WeightD weight_tab(float4 t, float defocus, const WeightArgs& w,
+                                              float sa) {
   WeightD o;
   o.rbin = t.y < 0.0f ? -1 : 0;  // only the in-range test is used
   if (t.y < 0.0f) {
@@ -92,15 +96,15 @@ __device__ __forceinline__ WeightD weight_tab(float4 t, float defocus, const Wei
     o.fd = 0.0f;
     return o;
   }
-  const float gamma = t.z + t.w * defocus;
+  const float rev = t.z + t.w * defocus;  // gamma / (2 pi)
+  const float fr = rev - rintf(rev);       // exact, |fr| <= 1/2
   float sg, cg;
-  sincosf(gamma, &sg, &cg);
-  const float sa = sqrtf(1.0f - w.ampconst * w.ampconst);
+  sincospif(2.0f * fr, &sg, &cg);
   const float v = sa * sg + w.ampconst * cg;
-  const float dv = (sa * cg - w.ampconst * sg) * t.w;
+  const float dv = (sa * cg - w.ampconst * sg) * (2.0f * kRefPi * t.w);
   const float vv = fabsf(v), dvv = v >= 0 ? dv : -dv;
   const float q = t.y + w.kk * vv * vv;
-  const float rq = 1.0f / sqrtf(q);
+  const float rq = rsqrtf(q);
   o.f = t.x * vv * rq;
   o.fd = t.x * t.y * rq * rq * rq * dvv;
   return o;
@@ -211,12 +215,13 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
   const RefCand cd = cand[b];
   const float2* Bj = white + (long)cd.view * H * L;
   const int ncol = min(CB, H - kx0);
+  const float sa = sqrtf(1.0f - w0.ampconst * w0.ampconst);
   double acc[2] = {0, 0};
   // A = f B formed in the first radix pass's loads, with the partial sums
   auto load = [&](int c, int ky) -> float2 {
     if (c >= ncol) return make_float2(0.f, 0.f);
     const int kx = kx0 + c;
-    const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0);
+    const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0, sa);
     const float2 bv = Bj[(long)kx * L + ky];
     if (wd.rbin >= 0) {  // inside the |CTF| rings
       const double b2 = (double)bv.x * bv.x + (double)bv.y * bv.y;
@@ -341,6 +346,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
   const float2* base = rows + ((long)b * H + kx0) * L;
   const float2* Bj = white + (long)cd.view * H * L;
   const int ncol = min(CB, H - kx0);
+  const float sa = sqrtf(1.0f - w0.ampconst * w0.ampconst);
   auto load = [&](int c, int ky) -> float2 {
     return c < ncol ? base[(long)c * L + ky] : make_float2(0.f, 0.f);
   };
@@ -349,7 +355,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
   auto store = [&](int c, int ky, float2 ga) {
     if (c >= ncol) return;
     const int kx = kx0 + c;
-    const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0);
+    const WeightD wd = weight_tab(wtab[(long)kx * L + ky], cd.df, w0, sa);
     if (wd.fd == 0.0f) return;
     const float2 bv = Bj[(long)kx * L + ky];
     const float wh = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
