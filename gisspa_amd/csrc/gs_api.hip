@@ -623,6 +623,8 @@ int correlate_psi(Ctx* c, MapStat* out) {
                                     : (!fits(c->G[0], (size_t)nb * per_map) ||
                                        (!serial && !fits(c->G[1], (size_t)nb * per_map))))
     return GS_ERR_STATE;
+  int ylo, yhi;  // rows of the intermediate the peak search reads
+  corr_store_rows(L, lo, hi, &ylo, &yhi);
   hipStream_t s = c->stream;
   if (c->pipeline == GS_PIPE_STREAM) {
     if (c->profile_now) CHK(prof_record(c, 0, s));
@@ -653,7 +655,7 @@ int correlate_psi(Ctx* c, MapStat* out) {
       hipStream_t ls = serial ? s : c->lane_stream[lane];
       if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k, ls));
       launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
-                        c->maps_blk.as<MapRef>() + m0, cnt, c->G[lane].as<float2>(),
+                        c->maps_blk.as<MapRef>() + m0, cnt, ylo, yhi, c->G[lane].as<float2>(),
                         c->psum.as<float2>() + (size_t)m0 * ncb, c->twL.as<float2>(), ls,
                         c->var1);
       if (c->profile_now) CHK(prof_record(c, 3 * (size_t)k + 1, ls));
@@ -792,7 +794,7 @@ int pick_norm1(Ctx* c, int psi_begin, int psi_end, std::vector<gs_particle>& pic
     return GS_OK;
   };
   auto pass1 = [&](int m0, int cnt, int lane) {
-    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), maps + m0, cnt,
+    launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), maps + m0, cnt, 0, L - 1,
                       c->G[lane].as<float2>(), c->psum.as<float2>() + (size_t)lane * nb * ncb,
                       c->twL.as<float2>(), c->lane_stream[lane]);
   };
@@ -1514,9 +1516,11 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
   for (auto& x : e) HIPCHK(hipEventCreate(&x));
   HIPCHK(hipEventRecord(e[0], c->stream));
   CHK(c->stats.ensure((size_t)nb * sizeof(MapStat)));
+  int ylo, yhi;
+  corr_store_rows(L, lo, hi, &ylo, &yhi);
   for (int i = 0; i < reps; ++i)
     launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(), c->maps_blk.as<MapRef>(), nb,
-                      c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(),
+                      ylo, yhi, c->G[0].as<float2>(), c->psum.as<float2>(), c->twL.as<float2>(),
                       c->stream, c->var1);
   HIPCHK(hipEventRecord(e[1], c->stream));
   for (int i = 0; i < reps; ++i)
@@ -1656,6 +1660,8 @@ int gs_autotune(void* ctx, int psi_index, gs_tune* out) {
   CHK(rotate_templates(c, psi_index));
   const int nb = std::min(c->batch, c->T * c->N);
   const float lo = c->p.diameter / 4.0f, hi = (float)L - c->p.diameter / 4.0f;
+  int ylo, yhi;
+  corr_store_rows(L, lo, hi, &ylo, &yhi);
   hipEvent_t e[2];
   for (auto& x : e) HIPCHK(hipEventCreate(&x));
   auto timed = [&](bool pass1, int v, double* us) -> int {
@@ -1663,7 +1669,7 @@ int gs_autotune(void* ctx, int psi_index, gs_tune* out) {
       if (rep == 1) HIPCHK(hipEventRecord(e[0], c->stream));
       if (pass1)
         launch_corr_pass1(L, c->tiles.as<float2>(), c->tspec.as<float2>(),
-                          c->maps_blk.as<MapRef>(), nb, c->G[0].as<float2>(),
+                          c->maps_blk.as<MapRef>(), nb, ylo, yhi, c->G[0].as<float2>(),
                           c->psum.as<float2>(), c->twL.as<float2>(), c->stream, v);
       else
         launch_corr_pass2(L, c->G[0].as<float2>(), nb, lo, hi, c->part.as<CorrPartial>(),

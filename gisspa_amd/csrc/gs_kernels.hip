@@ -317,10 +317,20 @@ constexpr int corr_lds2() {
   return (Geo<L>::RB * exec_stride_split<L / 2, Geo<L>::RB, WG, MAP_J, V>() + 1) / 2;
 }
 
+// Rows of the intermediate that pass 2 reads: those of the row blocks
+// blo / RB .. bhi / RB (the other blocks hold border pixels only)
+template <int L>
+__host__ __device__ __forceinline__ void corr_rows(int blo, int bhi, int* ylo, int* yhi) {
+  constexpr int RB = Geo<L>::RB;
+  *ylo = (blo / RB) * RB;
+  *yhi = (bhi / RB) * RB + RB - 1;
+}
+
 // Pass 1 of map m, column block cb: P = tile * conj(template) on the paired
 // columns, inverse column transforms, C2R pre-processing, blocked store.
 template <int L, bool COH = false, int V = 0>
 __device__ __forceinline__ void corr_pass1_body(float2* lds, float* red_q, int cb, int m, int gm,
+                                                int ylo, int yhi,
                                                 const float2* __restrict__ tiles,
                                                 const float2* __restrict__ tspec,
                                                 const MapRef* __restrict__ maps,
@@ -363,10 +373,18 @@ __device__ __forceinline__ void corr_pass1_body(float2* lds, float* red_q, int c
   // M-point complex FFT of Z').
   // Every slot is written: the slots pass 2 never reads (column M, the
   // second copy of column M/2, units past M/2) only hold unused values.
+  // Rows outside ylo..yhi are not read by pass 2 (their row blocks hold
+  // border pixels only): output slot r of the last radix pass holds rows
+  // r NS .. r NS + NS - 1 of every thread, so a slot wholly outside is
+  // skipped by the whole workgroup (a uniform branch; rows of a slot that
+  // straddles the bound are written).
+  constexpr Plan PL1 = variant_plan(L, CB, WG, MAP_B, V);
+  constexpr int NSL1 = L / PL1.r[PL1.count - 1];
   const int cl = threadIdx.x % CB;  // the slot this lane stores (MAP_B)
   float2 wk = twL[pair_col<L>(cb, cl)];  // column index <= M < L
   wk.y = -wk.y;
-  auto store = [&](int c, int y, float2 v) {
+  auto store = [&](int c, int y, float2 v, int, int r) {
+    if (r * NSL1 > yhi || r * NSL1 + NSL1 - 1 < ylo) return;
     const float2 pv = xchg<CB>(v);
     if constexpr (COH) store_coh(gb + y * CB + c, c2r_pre(v, pv, wk));
     else store_nt(gb + y * CB + c, c2r_pre(v, pv, wk));
@@ -401,6 +419,7 @@ template <int L, int V>
 __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ tiles,
                                                    const float2* __restrict__ tspec,
                                                    const MapRef* __restrict__ maps, int nmaps,
+                                                   int ylo, int yhi,
                                                    float2* __restrict__ Gout,
                                                    float2* __restrict__ psum,
                                                    const float2* __restrict__ twL) {
@@ -408,7 +427,8 @@ __global__ __launch_bounds__(WG) void k_corr_pass1(const float2* __restrict__ ti
   __shared__ float red_q[WG / 64];
   int cb, m;
   if (!corr_pass1_item<L>(blockIdx.x, nmaps, &cb, &m)) return;
-  corr_pass1_body<L, false, V>(lds, red_q, cb, m, m, tiles, tspec, maps, Gout, psum, twL);
+  corr_pass1_body<L, false, V>(lds, red_q, cb, m, m, ylo, yhi, tiles, tspec, maps, Gout, psum,
+                               twL);
 }
 
 // (-1)^(kx+ky) sgn on half spectra [t][kx][ky], in place
@@ -668,9 +688,10 @@ __global__ __launch_bounds__(WG) void k_corr_fused(
   }
   const int w = gi * 8 + x;
   if (kind == 0) {
-    int cb, m;
+    int cb, m, ylo, yhi;
     if (!corr_pass1_item<L>(w, nmaps1, &cb, &m)) return;
-    corr_pass1_body<L>(lds, red_f, cb, m, m, tiles, tspec, maps1, G1, psum1, twL);
+    corr_rows<L>(blo, bhi, &ylo, &yhi);
+    corr_pass1_body<L>(lds, red_f, cb, m, m, ylo, yhi, tiles, tspec, maps1, G1, psum1, twL);
   } else {
     const int m = w / nparts, rb = w % nparts;
     if (m >= nmaps2) return;
@@ -753,7 +774,10 @@ __global__ __launch_bounds__(WG) void k_corr_stream(
     const int m = 8 * r + x, cb = k;
     if (m >= nmaps || cb >= NCB) return;
     if (m >= 8 * ring) wait_count(done + (m - 8 * ring), NP, err, wait_ticks);
-    corr_pass1_body<L, true>(lds, red_f, cb, m, (r % ring) * 8 + x, tiles, tspec, maps, Gring,
+    int ylo, yhi;
+    corr_rows<L>(blo, bhi, &ylo, &yhi);
+    corr_pass1_body<L, true>(lds, red_f, cb, m, (r % ring) * 8 + x, ylo, yhi, tiles, tspec, maps,
+                             Gring,
                              psum, twL);
     signal_count(ready + m);
   } else {
@@ -1353,11 +1377,12 @@ constexpr bool has_variants(int L) { return L == 512 || L == 720; }
 
 template <int L>
 static void corr_pass1_t(const float2* tiles, const float2* tspec, const MapRef* maps, int nmaps,
-                         float2* G, float2* psum, const float2* twL, int v, hipStream_t s) {
+                         int ylo, int yhi, float2* G, float2* psum, const float2* twL, int v,
+                         hipStream_t s) {
   constexpr int NCB8 = (Geo<L>::NCB + 7) / 8;
   dim3 grid(8 * NCB8 * nmaps);
 #define P1(V_) hipLaunchKernelGGL((k_corr_pass1<L, V_>), grid, dim3(WG), 0, s, tiles, tspec, maps, \
-                                  nmaps, G, psum, twL)
+                                  nmaps, ylo, yhi, G, psum, twL)
   if constexpr (has_variants(L)) {
     switch (v) {
       case 1: P1(1); return;
@@ -1385,9 +1410,9 @@ static void corr_fused_t(const float2* tiles, const float2* tspec, const MapRef*
                      nmaps1, G1, psum1, ng1, G2, nmaps2, ng2, blo, bhi, part2, NP, twL, twM);
 }
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s,
-                       int variant) {
-  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, G, psum, twL, variant, s);
+                       int nmaps, int ylo, int yhi, float2* G, float2* psum, const float2* twL,
+                       hipStream_t s, int variant) {
+  DISPATCH(L, corr_pass1_t, tiles, tspec, maps, nmaps, ylo, yhi, G, psum, twL, variant, s);
 }
 
 int corr_variants(int L) { return has_variants(L) ? kPlanVariants : 1; }
@@ -1441,6 +1466,11 @@ static void border_range(int L, float lo, float hi, int* blo, int* bhi) {
   *blo = (int)ceilf(lo);
   *bhi = (int)floorf(hi);
   if (*bhi < *blo) { *blo = L + 1; *bhi = L + 1; }  // nothing valid
+}
+void corr_store_rows(int L, float lo, float hi, int* ylo, int* yhi) {
+  int blo, bhi;
+  border_range(L, lo, hi, &blo, &bhi);
+  GS_DISPATCH(L, corr_rows, blo, bhi, ylo, yhi);
 }
 void launch_corr_pass2(int L, const float2* G, int nmaps, float lo, float hi, CorrPartial* part,
                        const float2* twL, const float2* twM, hipStream_t s, int variant) {

@@ -107,8 +107,12 @@ void launch_rows_c2r_stream(int L, const float2* G, int nb, float* out, const fl
 // `variant` selects the FFT schedule (radix order) of the pass: 0 is the
 // tuned plan, 1 .. corr_variants(L) - 1 the alternatives gs_autotune times.
 void launch_corr_pass1(int L, const float2* tiles, const float2* tspec, const MapRef* maps,
-                       int nmaps, float2* G, float2* psum, const float2* twL, hipStream_t s,
-                       int variant = 0);
+                       int nmaps, int ylo, int yhi, float2* G, float2* psum, const float2* twL,
+                       hipStream_t s, int variant = 0);
+// Rows ylo..yhi of the intermediate that the peak search of pass 2 reads for
+// the border-exclusion bounds lo, hi (pass 1 may leave the others unwritten;
+// norm_type 1 reads every row, 0 .. L-1)
+void corr_store_rows(int L, float lo, float hi, int* ylo, int* yhi);
 // schedule variants compiled for window L (1 = the tuned plan only), and
 // the most any window has
 int corr_variants(int L);
