@@ -110,24 +110,24 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(RowSrc s, float2* __restrict__ 
                                                  const float2* __restrict__ twM) {
   using G = Geo<L>;
   constexpr int M = G::M, H = G::H, RB = G::RB;
-  __shared__ float2 lds[RB * M];
-  __shared__ float2 tw[M];
+  // the transform's split exchange, then the staging of its outputs (one
+  // component at a time, rows M + 1 floats apart) for the R2C pairing of
+  // Z[k] with Z[M - k]
+  constexpr int ZS = M + 1;
+  constexpr int LX = RB * exec_stride_split<M, RB, WG, MAP_J>();
+  __shared__ float lds[LX > RB * ZS ? LX : RB * ZS];
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * RB;
-  load_twiddles<M, WG>(tw, twM);
   float aux = 0.0f;
   if constexpr (SRC == SRC_PAD) aux = s.scale[b];
-  if constexpr (SRC == SRC_TILE) aux = (float)0;  // mean applied in double below
   double acc1 = 0, acc2 = 0;
-  for (int g = threadIdx.x; g < RB * M; g += WG) {
-    const int r = g / M, m = g - r * M;
+  // z[m] = (v(2m), v(2m+1)) of row y0 + r, formed in the first radix pass
+  auto load = [&](int r, int m) -> float2 {
     const int y = y0 + r;
     float v0 = 0.0f, v1 = 0.0f;
     if (y < L) {
       if constexpr (SRC == SRC_TILE) {
-        const float mean = (float)0;
-        (void)mean;
         const int ix = b % s.bxn, iy = b / s.bxn;
         const int gy = s.offy[iy] + y;
         const int gx0 = s.offx[ix] + 2 * m;
@@ -146,26 +146,69 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(RowSrc s, float2* __restrict__ 
       acc1 += (double)v0 + (double)v1;
       acc2 += (double)v0 * v0 + (double)v1 * v1;
     }
-    lds[r * M + m] = make_float2(v0, v1);
-  }
+    return make_float2(v0, v1);
+  };
+  constexpr Plan PL = split_plan(M, RB, WG, MAP_J, 0);
+  constexpr int RL = PL.r[PL.count - 1];  // radix of the last pass
+  constexpr int NSL = M / RL;             // its butterflies per row
+  constexpr int TOTL = RB * NSL;
+  constexpr int PERL = (TOTL + WG - 1) / WG;
+  float2 zv[PERL][RL];
+  auto store = [&](int, int, float2 z, int pr, int slot) { zv[pr][slot] = z; };
+  fft_exec_split<M, RB, WG, -1, MAP_J, MAP_J>(lds, twM, load, store);
   if constexpr (SRC == SRC_ROT || SRC == SRC_ROTB) {
-    block_sum2_d(acc1, acc2, red);
+    block_sum2_d(acc1, acc2, red);  // (its barriers also end the transform's LDS reads)
     if (threadIdx.x == 0) {
       s.partials[2 * ((long)b * gridDim.x + blockIdx.x)] = acc1;
       s.partials[2 * ((long)b * gridDim.x + blockIdx.x) + 1] = acc2;
     }
+  } else {
+    __syncthreads();  // the transform's last LDS reads end before the staging
+  }
+  // output (row, n) of round pr, slot sl: row = g / NSL, n = g % NSL + sl NSL,
+  // g = thread + pr WG (the last pass's butterfly mapping)
+  auto stage = [&](auto comp) {
+#pragma unroll
+    for (int pr = 0; pr < PERL; ++pr) {
+      const int g = threadIdx.x + pr * WG;
+      if (TOTL % WG == 0 || g < TOTL) {
+        const int r = g / NSL, j = g % NSL;
+#pragma unroll
+        for (int sl = 0; sl < RL; ++sl)
+          lds[r * ZS + j + sl * NSL] = decltype(comp)::value ? zv[pr][sl].y : zv[pr][sl].x;
+      }
+    }
+  };
+  // post-process to X[k], k = 0..M, and write out[b][k][y0 + r]: item
+  // g = k RB + r, so the RB rows of one k are consecutive threads
+  constexpr int NPOST = (RB * H + WG - 1) / WG;
+  float zre[NPOST][2];
+  stage(std::integral_constant<int, 0>{});
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NPOST; ++i) {
+    const int g = threadIdx.x + i * WG;
+    if (g < RB * H) {
+      const int k = g / RB, r = g - k * RB;
+      zre[i][0] = lds[r * ZS + (k % M)];
+      zre[i][1] = lds[r * ZS + ((M - k) % M)];
+    }
   }
   __syncthreads();
-  lds_fft<M, RB, WG, -1, M>(lds, tw);
-  // post-process to X[k], k = 0..M, and write out[b][k][y0 + r]
-  for (int g = threadIdx.x; g < RB * H; g += WG) {
-    const int k = g / RB, r = g - k * RB;
-    const int y = y0 + r;
-    if (y >= L) continue;
-    const float2 zk = lds[r * M + (k % M)];
-    const float2 zmk = lds[r * M + ((M - k) % M)];
-    const float2 wk = twL[k];
-    out[((long)b * H + k) * L + y] = r2c_post(zk, zmk, wk);
+  stage(std::integral_constant<int, 1>{});
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NPOST; ++i) {
+    const int g = threadIdx.x + i * WG;
+    if (g < RB * H) {
+      const int k = g / RB, r = g - k * RB;
+      const int y = y0 + r;
+      if (y < L) {
+        const float2 zk = make_float2(zre[i][0], lds[r * ZS + (k % M)]);
+        const float2 zmk = make_float2(zre[i][1], lds[r * ZS + ((M - k) % M)]);
+        out[((long)b * H + k) * L + y] = r2c_post(zk, zmk, twL[k]);
+      }
+    }
   }
 }
 
@@ -283,7 +326,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stream(const float2* __restrict
                                                         const float2* __restrict__ twM) {
   using G = Geo<L>;
   constexpr int M = G::M, HP = G::HP, RB = G::RB;
-  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  __shared__ float lds[RB * exec_stride_split<M, RB, WG, MAP_J>()];
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * RB;
   const float2* gb = Gin + (long)b * L * HP;
@@ -298,7 +341,7 @@ __global__ __launch_bounds__(WG) void k_rows_c2r_stream(const float2* __restrict
     const int y = y0 + r;
     if (y < L) *reinterpret_cast<float2*>(ob + (long)y * L + 2 * n) = z;
   };
-  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+  fft_exec_split<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
 }
 
 // ---------------------------------------------------------------------------
@@ -833,12 +876,12 @@ __global__ __launch_bounds__(WG) void k_norm1_acc(const float2* __restrict__ Gin
   using G = Geo<L>;
   constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
   constexpr long L2 = (long)L * L;
-  constexpr Plan PL = variant_plan(M, RB, WG, MAP_J, 0);
+  constexpr Plan PL = split_plan(M, RB, WG, MAP_J, 0);
   constexpr int RL = PL.r[PL.count - 1];   // radix of the last pass
   constexpr int NSL = M / RL;              // its butterflies per row
   constexpr int TOT = RB * NSL;
   constexpr int PER = (TOT + WG - 1) / WG;
-  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  __shared__ float lds[RB * exec_stride_split<M, RB, WG, MAP_J>()];
   const int y0 = blockIdx.x * RB;
   const int mfirst = blockIdx.y * nv;
   const int t = maps[mfirst].tile;
@@ -861,7 +904,7 @@ __global__ __launch_bounds__(WG) void k_norm1_acc(const float2* __restrict__ Gin
       acc2[p][r][0] += a * a;
       acc2[p][r][1] += b * b;
     };
-    fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+    fft_exec_split<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
     __syncthreads();  // the last pass's LDS reads end before the next map's first writes
   }
   // output (row, n) of round p, slot r: row = g / NSL, n = g % NSL + r * NSL

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(const float2* __restri
   constexpr int M = G::M, HP = G::HP, RB = G::RB;
   constexpr long L2 = (long)L * L;
   static_assert((L + RB - 1) / RB <= RPARTS, "partial slots");
-  __shared__ float2 lds[RB * exec_stride<M, RB, WG, MAP_J>()];
+  __shared__ float lds[RB * exec_stride_split<M, RB, WG, MAP_J>()];
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * RB;
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(const float2* __restri
       *reinterpret_cast<float2*>(ob + (long)y * L + 2 * n) = make_float2(a, c);
     }
   };
-  fft_exec<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+  fft_exec_split<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
   block_sum_n<2>(acc, red);
   if (threadIdx.x == 0) {
     part[((long)b * RPARTS + blockIdx.x) * 2] = acc[0];
