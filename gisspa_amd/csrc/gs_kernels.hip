@@ -867,8 +867,11 @@ __global__ void k_corr_finalize(const CorrPartial* __restrict__ part, int nparts
 // sums (FP32) are then added once to the tile's FP64 accumulators; batches
 // run in stream order and each (tile, row block) belongs to one workgroup, so
 // every accumulator sees a fixed order of additions.
+// 512 threads per workgroup: each thread keeps the sums of fewer outputs
+// (the accumulators, not the transform, set this kernel's registers)
+constexpr int WGA = 2 * WG;
 template <int L>
-__global__ __launch_bounds__(WG) void k_norm1_acc(const float2* __restrict__ Gin,
+__global__ __launch_bounds__(WGA) void k_norm1_acc(const float2* __restrict__ Gin,
                                                   const MapRef* __restrict__ maps, int nv,
                                                   double* __restrict__ s1,
                                                   double* __restrict__ s2,
@@ -876,12 +879,12 @@ __global__ __launch_bounds__(WG) void k_norm1_acc(const float2* __restrict__ Gin
   using G = Geo<L>;
   constexpr int M = G::M, RB = G::RB, CB = G::CB, NCB = G::NCB;
   constexpr long L2 = (long)L * L;
-  constexpr Plan PL = split_plan(M, RB, WG, MAP_J, 0);
+  constexpr Plan PL = split_plan(M, RB, WGA, MAP_J, 0);
   constexpr int RL = PL.r[PL.count - 1];   // radix of the last pass
   constexpr int NSL = M / RL;              // its butterflies per row
   constexpr int TOT = RB * NSL;
-  constexpr int PER = (TOT + WG - 1) / WG;
-  __shared__ float lds[RB * exec_stride_split<M, RB, WG, MAP_J>()];
+  constexpr int PER = (TOT + WGA - 1) / WGA;
+  __shared__ float lds[RB * exec_stride_split<M, RB, WGA, MAP_J>()];
   const int y0 = blockIdx.x * RB;
   const int mfirst = blockIdx.y * nv;
   const int t = maps[mfirst].tile;
@@ -904,16 +907,16 @@ __global__ __launch_bounds__(WG) void k_norm1_acc(const float2* __restrict__ Gin
       acc2[p][r][0] += a * a;
       acc2[p][r][1] += b * b;
     };
-    fft_exec_split<M, RB, WG, +1, MAP_J, MAP_J>(lds, twM, load, store);
+    fft_exec_split<M, RB, WGA, +1, MAP_J, MAP_J>(lds, twM, load, store);
     __syncthreads();  // the last pass's LDS reads end before the next map's first writes
   }
   // output (row, n) of round p, slot r: row = g / NSL, n = g % NSL + r * NSL
-  // with g = thread + p * WG (the last pass's butterfly mapping)
+  // with g = thread + p * WGA (the last pass's butterfly mapping)
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
-    const int g = threadIdx.x + p * WG;
+    const int g = threadIdx.x + p * WGA;
     const int y = y0 + g / NSL;
-    if ((TOT % WG != 0 && g >= TOT) || y >= L) continue;
+    if ((TOT % WGA != 0 && g >= TOT) || y >= L) continue;
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const long pix = t * L2 + (long)y * L + 2 * (g % NSL + r * NSL);
@@ -1625,7 +1628,7 @@ template <int L>
 static void norm1_acc_t(const float2* G, const MapRef* maps, int ntiles, int nv, double* s1,
                         double* s2, const float2* twM, hipStream_t s) {
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, ntiles);
-  hipLaunchKernelGGL(k_norm1_acc<L>, grid, dim3(WG), 0, s, G, maps, nv, s1, s2, twM);
+  hipLaunchKernelGGL(k_norm1_acc<L>, grid, dim3(WGA), 0, s, G, maps, nv, s1, s2, twM);
 }
 void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, int nv, double* s1,
                       double* s2, const float2* twM, hipStream_t s) {
