@@ -64,7 +64,7 @@ struct Ctx {
   // refinement batch buffers
   int ref_cap = 0;
   DevBuf r_cand, r_cs, r_A, r_gtmp, r_Tun, r_spec, r_rot, r_bscale, r_pre, r_gTb, r_gT, r_part,
-      r_sc, r_out;
+      r_sc, r_out, r_thr, r_skip;
   double half_count = 0;  // pixels of the x <= l/2 half plane inside radius l/2
   // micrograph batch: tile pool of every micrograph, tile descriptors
   std::vector<MicSlot> mics;
@@ -990,6 +990,7 @@ int ref_ensure(Ctx* c, int nb) {
   CHK(c->r_gT.ensure(nb * L * L * sizeof(float)));
   CHK(c->r_part.ensure(nb * (size_t)ref_parts() * 4 * sizeof(double)));
   CHK(c->r_sc.ensure(nb * sizeof(RefScal)));
+  CHK(c->r_skip.ensure(nb * sizeof(int)));
   c->ref_cap = nb;
   return GS_OK;
 }
@@ -1019,7 +1020,11 @@ int ref_cands(const Ctx* c, const gs_peak* pk, int n, std::vector<RefCand>& rc,
 
 // objective and gradient of n candidates: out[5 i ..] = score, d psi, d dx,
 // d dy, d defocus
-int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out) {
+// thr (may be null): a candidate whose score does not exceed thr[i] gets its
+// score only (zero gradient); the refinement rejects such trial steps and
+// never reads their gradient, so its adjoint is skipped on the device
+int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out,
+             const float* thr = nullptr) {
   CHK(ensure_white(c));
   hipStream_t s = c->stream;
   const int nb0 = std::min(n, kRefBatch);
@@ -1033,6 +1038,10 @@ int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out) {
   CHK(c->r_out.ensure((size_t)n * 5 * sizeof(float)));
   HIPCHK(hipMemcpyAsync(c->r_cand.p, rc, (size_t)n * sizeof(RefCand), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->r_cs.p, cs, (size_t)n * sizeof(double2), hipMemcpyHostToDevice, s));
+  if (thr) {
+    CHK(c->r_thr.ensure((size_t)n * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(c->r_thr.p, thr, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
+  }
   for (int b0 = 0; b0 < n; b0 += kRefBatch) {
     const int nb = std::min(kRefBatch, n - b0);
     RefArgs a{};
@@ -1057,6 +1066,8 @@ int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out) {
     a.part = c->r_part.as<double>();
     a.sc = c->r_sc.as<RefScal>();
     a.out = c->r_out.as<float>() + 5 * (size_t)b0;
+    a.thr = thr ? c->r_thr.as<float>() + b0 : nullptr;
+    a.skip = c->r_skip.as<int>();
     a.twL = c->twL.as<float2>();
     a.twM = c->twM.as<float2>();
     launch_ref_forward(c->L, a, s);
@@ -1596,6 +1607,7 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
   std::vector<int> act(n);
   for (int i = 0; i < n; ++i) act[i] = i;
   std::vector<gs_peak> tact;
+  std::vector<float> thr;
   for (int it = 0; it < iters && !act.empty(); ++it) {
     tact.resize(act.size());
     for (size_t a = 0; a < act.size(); ++a) {
@@ -1618,7 +1630,11 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
     }
     const int na = (int)act.size();
     CHK(ref_cands(c, tact.data(), na, rc, cs));
-    CHK(ref_eval(c, rc.data(), cs.data(), na, ft.data()));
+    // the step is kept only if the objective rises, so the gradient at a
+    // trial point is needed only there
+    thr.resize(na);
+    for (int a = 0; a < na; ++a) thr[a] = fc[5 * (size_t)act[a]];
+    CHK(ref_eval(c, rc.data(), cs.data(), na, ft.data(), thr.data()));
     size_t keep = 0;
     for (int a = 0; a < na; ++a) {
       const int i = act[a];

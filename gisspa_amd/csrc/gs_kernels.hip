@@ -51,6 +51,8 @@ struct RowSrc {
   const float* pre_b;
   // SRC_IMG: real images [b][L][L]
   const float* imgs;
+  // any source: rows of b with skip[b] != 0 are left alone (may be null)
+  const int* skip;
 };
 
 template <int L, int SRC>
@@ -119,6 +121,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(RowSrc s, float2* __restrict__ 
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * RB;
+  if (s.skip && s.skip[b]) return;  // the whole workgroup
   float aux = 0.0f;
   if constexpr (SRC == SRC_PAD) aux = s.scale[b];
   double acc1 = 0, acc2 = 0;
@@ -1358,15 +1361,16 @@ void launch_rows_r2c_rotb(int L, const float* treal, int nb, const double2* cs_b
 
 template <int L>
 static void rows_r2c_img_t(const float* imgs, int nb, float2* out, const float2* twL,
-                           const float2* twM, hipStream_t s) {
+                           const float2* twM, hipStream_t s, const int* skip) {
   RowSrc rs{};
   rs.imgs = imgs;
+  rs.skip = skip;
   dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
   hipLaunchKernelGGL((k_rows_r2c<L, SRC_IMG>), grid, dim3(WG), 0, s, rs, out, twL, twM);
 }
 void launch_rows_r2c_img(int L, const float* imgs, int nb, float2* out, const float2* twL,
-                         const float2* twM, hipStream_t s) {
-  DISPATCH(L, rows_r2c_img_t, imgs, nb, out, twL, twM, s);
+                         const float2* twM, hipStream_t s, const int* skip) {
+  DISPATCH(L, rows_r2c_img_t, imgs, nb, out, twL, twM, s, skip);
 }
 
 template <int L>

@@ -80,7 +80,7 @@ void launch_rows_r2c_rotb(int L, const float* treal, int nb, const double2* cs_b
                           const float2* twM, hipStream_t s);
 // Real rows -> half spectrum of real images imgs[b][L][L].
 void launch_rows_r2c_img(int L, const float* imgs, int nb, float2* out, const float2* twL,
-                         const float2* twM, hipStream_t s);
+                         const float2* twM, hipStream_t s, const int* skip = nullptr);
 
 // In-place column transform of [b][kx][y]; multiplies by scale (global) or
 // per-b scale from `bscale` (may be null); zero_dc zeroes element (0,0).
@@ -276,6 +276,9 @@ struct RefArgs {
   double* part;         // [nb][ref_parts()][4]
   RefScal* sc;          // [nb]
   float* out;           // [nb][5]: score, d psi, d dx, d dy, d defocus
+  const float* thr;     // [nb] or null: the adjoint runs only where the score
+                        // (as float) exceeds thr (a refinement step's test)
+  int* skip;            // [nb] work flags: set where the adjoint is skipped
   const float2* twL;
   const float2* twM;
 };

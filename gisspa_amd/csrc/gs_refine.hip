@@ -258,6 +258,7 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
                                                       const float2* __restrict__ spec,
                                                       const RefCand* __restrict__ cand,
                                                       const RefScal* __restrict__ sc,
+                                                      const int* __restrict__ skip,
                                                       float2* __restrict__ Gout,
                                                       double* __restrict__ part,
                                                       const float2* __restrict__ twL) {
@@ -268,6 +269,7 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
   __shared__ float2 ex[H], ey[L];
   __shared__ double red[WG / 64];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
+  if (skip[b]) return;  // the whole workgroup
   const RefCand cd = cand[b];
   phase_tables<L>(ex, ey, (double)cd.x0 + cd.dx, (double)cd.y0 + cd.dy);  // (ends in a barrier)
   const float2* I = tiles + (long)cd.ptile * H * L;
@@ -335,6 +337,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
                                                     const RefCand* __restrict__ cand,
                                                     WeightArgs w0,
                                                     const float4* __restrict__ wtab,
+                                                    const int* __restrict__ skip,
                                                     double* __restrict__ part,
                                                     const float2* __restrict__ twL) {
   using Gm = Geo<L>;
@@ -342,6 +345,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
   __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
   __shared__ double red[1 * (WG / 64)];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
+  if (skip[b]) return;  // the whole workgroup
   const RefCand cd = cand[b];
   const float2* base = rows + ((long)b * H + kx0) * L;
   const float2* Bj = white + (long)cd.view * H * L;
@@ -396,8 +400,11 @@ __global__ __launch_bounds__(WG) void k_ref_score(const float2* __restrict__ til
     for (int k = 0; k < 4; ++k) part[((long)b * RPARTS + blockIdx.x) * 4 + k] = acc[k];
 }
 
+// (and the skip flag of the adjoint: a trial whose score does not exceed its
+// threshold is rejected by the refinement, so its gradient is never used)
 __global__ void k_ref_score_fin(const double* __restrict__ part, int nb, int L,
-                                RefScal* __restrict__ sc) {
+                                RefScal* __restrict__ sc, const float* __restrict__ thr,
+                                int* __restrict__ skip) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double a[4] = {0, 0, 0, 0};
@@ -410,6 +417,7 @@ __global__ void k_ref_score_fin(const double* __restrict__ part, int nb, int L,
   const double k2pi = 2.0 * M_PI / L;
   sc[b].gx = sigma > 0 ? k2pi * a[2] / sigma : 0.0;
   sc[b].gy = sigma > 0 ? k2pi * a[3] / sigma : 0.0;
+  skip[b] = (thr && !((float)sc[b].score > thr[b])) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -426,6 +434,7 @@ __global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(const float2* __restri
                                                           const double2* __restrict__ cs_b,
                                                           const float* __restrict__ pre,
                                                           const RefScal* __restrict__ sc,
+                                                          const int* __restrict__ skip,
                                                           float dmask, float* __restrict__ gTb,
                                                           double* __restrict__ part,
                                                           const float2* __restrict__ twL,
@@ -438,6 +447,7 @@ __global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(const float2* __restri
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
   const int y0 = blockIdx.x * RB;
+  if (skip[b]) return;  // the whole workgroup
   const float2* gb = Gin + (long)b * L * HP;
   const double2 cs = cs_b[b];
   const float p = pre[b];
@@ -499,9 +509,11 @@ __global__ void k_ref_rot_fin(const double* __restrict__ part, int nb, int np,
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gTb,
                                                     const double2* __restrict__ cs_b, float dmask,
+                                                    const int* __restrict__ skip,
                                                     float* __restrict__ gT) {
   constexpr long L2 = (long)L * L;
   const int b = blockIdx.y;
+  if (skip[b]) return;
   const double2 cs = cs_b[b];
   const float* gsrc = gTb + b * L2;
   for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
@@ -511,9 +523,15 @@ __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gT
 }
 
 __global__ void k_ref_out(const double* __restrict__ part, int nb, RefScal* __restrict__ sc,
-                          float* __restrict__ out, int np, int stride) {
+                          const int* __restrict__ skip, float* __restrict__ out, int np,
+                          int stride) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
+  if (skip[b]) {  // the score only; no gradient was computed
+    out[5 * b + 0] = (float)sc[b].score;
+    out[5 * b + 1] = out[5 * b + 2] = out[5 * b + 3] = out[5 * b + 4] = 0.0f;
+    return;
+  }
   double d = 0;
   for (int i = 0; i < np; ++i) d += part[(long)b * stride + i];
   sc[b].ddf = d + sc[b].dmacc * sc[b].dm;
@@ -544,7 +562,8 @@ static void ref_forward_t(const RefArgs& a, hipStream_t s) {
                      (L + Geo<L>::RB - 1) / Geo<L>::RB, a.nb, (double)L * L, a.sc, a.bscale);
   launch_cols_fwd(L, a.spec, a.nb, 1.0f, a.bscale, 1, a.twL, s);
   hipLaunchKernelGGL((k_ref_score<L>), grid, blk, 0, s, a.tiles, a.spec, a.cand, a.part);
-  hipLaunchKernelGGL(k_ref_score_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, L, a.sc);
+  hipLaunchKernelGGL(k_ref_score_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, L, a.sc, a.thr,
+                     a.skip);
 }
 
 template <int L>
@@ -555,24 +574,24 @@ static void ref_backward_t(const RefArgs& a, hipStream_t s) {
   // d/dThat (with the Parseval sum of the normalisation adjoint) -> the
   // transposed intermediate of the C2R
   hipLaunchKernelGGL((k_ref_grad_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.tiles, a.spec, a.cand,
-                     a.sc, a.gtmp, a.part, a.twL);
+                     a.sc, a.skip, a.gtmp, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_norm_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, NCB,
                      (double)L * L, a.sc);
   // row C2R -> d/dRn -> d/dTb per pixel, d/dpsi and d/dm sums
   constexpr int NRB = (L + Geo<L>::RB - 1) / Geo<L>::RB;
   hipLaunchKernelGGL((k_ref_rows_c2r_grad<L>), dim3(NRB, a.nb), blk, 0, s, a.gtmp, a.Tun, a.cs,
-                     a.pre, a.sc, a.dmask, a.gTb, a.part, a.twL, a.twM);
+                     a.pre, a.sc, a.skip, a.dmask, a.gTb, a.part, a.twL, a.twM);
   hipLaunchKernelGGL(k_ref_rot_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, NRB, a.sc);
   // d/dT: the bilinear adjoint of d/dTb, gathered per source pixel (a
   // separate pass: gathered inside the row transform's loads it took 267 us
   // per batch of 64 at window 720 against 209 for the two kernels)
-  hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.gT);
+  hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.skip, a.gT);
   // d/dT -> d/dA: unnormalised R2C
-  launch_rows_r2c_img(L, a.gT, a.nb, a.A, a.twL, a.twM, s);
+  launch_rows_r2c_img(L, a.gT, a.nb, a.A, a.twL, a.twM, s, a.skip);
   hipLaunchKernelGGL((k_ref_cols_df<L>), dim3(NCB, a.nb), blk, 0, s, a.A, a.white, a.cand, a.w,
-                     a.wtab, a.part, a.twL);
-  hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.out, NCB,
-                     FSLOTS);
+                     a.wtab, a.skip, a.part, a.twL);
+  hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.skip, a.out,
+                     NCB, FSLOTS);
 }
 
 template <int L>
