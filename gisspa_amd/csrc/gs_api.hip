@@ -492,15 +492,18 @@ int rotate_templates(Ctx* c, int psi_index) {
   const int L = c->L;
   float cose, sine;
   psi_rotation(c, psi_index, &cose, &sine);
-  const int nrb = (L + c->geo.RB - 1) / c->geo.RB;
-  CHK(c->rot_part.ensure((size_t)c->N * nrb * 2 * sizeof(double)));
+  const float dmask = 1.5f * c->p.diameter;
+  // the masked templates are zero outside a band of rows around the centre:
+  // only the band is transformed by rows, and the columns read only it
+  const RowBand band = rot_row_band(L, dmask);
+  CHK(c->rot_part.ensure((size_t)c->N * band.nrb * 2 * sizeof(double)));
   CHK(c->bscale.ensure((size_t)c->N * sizeof(float)));
-  launch_rows_r2c_rot(L, c->treal.as<float>(), c->N, cose, sine, 1.5f * c->p.diameter,
+  launch_rows_r2c_rot(L, c->treal.as<float>(), c->N, cose, sine, dmask, band,
                       c->tspec.as<float2>(), c->rot_part.as<double>(), c->twL.as<float2>(),
                       c->twM.as<float2>(), c->stream);
-  launch_rot_scale(L, c->rot_part.as<double>(), c->N, c->bscale.as<float>(), c->stream);
+  launch_rot_scale(L, c->rot_part.as<double>(), c->N, band.nrb, c->bscale.as<float>(), c->stream);
   launch_cols_fwd(L, c->tspec.as<float2>(), c->N, 1.0f, c->bscale.as<float>(), 1,
-                  c->twL.as<float2>(), c->stream);
+                  c->twL.as<float2>(), c->stream, band);
   return GS_OK;
 }
 

@@ -53,6 +53,8 @@ struct RowSrc {
   const float* imgs;
   // any source: rows of b with skip[b] != 0 are left alone (may be null)
   const int* skip;
+  // first row block of the launch (row block blockIdx.x + rb0)
+  int rb0;
 };
 
 template <int L, int SRC>
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(RowSrc s, float2* __restrict__ 
   __shared__ float lds[LX > RB * ZS ? LX : RB * ZS];
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
-  const int y0 = blockIdx.x * RB;
+  const int y0 = (blockIdx.x + s.rb0) * RB;
   if (s.skip && s.skip[b]) return;  // the whole workgroup
   float aux = 0.0f;
   if constexpr (SRC == SRC_PAD) aux = s.scale[b];
@@ -221,9 +223,11 @@ __global__ __launch_bounds__(WG) void k_rows_r2c(RowSrc s, float2* __restrict__ 
 template <int L>
 __global__ __launch_bounds__(WG) void k_cols_fwd(float2* __restrict__ spec, float scale,
                                                  const float* __restrict__ bscale, int zero_dc,
+                                                 int ylo, int yhi,
                                                  const float2* __restrict__ twL) {
   // in place: every input of the workgroup's columns is read in the first
-  // radix pass, before the barrier that precedes the last pass's stores
+  // radix pass, before the barrier that precedes the last pass's stores;
+  // inputs outside rows ylo .. yhi are zero (not read)
   using G = Geo<L>;
   constexpr int H = G::H, CB = G::CB;
   __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(WG) void k_cols_fwd(float2* __restrict__ spec, floa
   const int ncol = min(CB, H - kx0);
   const float sc = bscale ? bscale[b] : scale;
   auto load = [&](int c, int y) -> float2 {
-    return c < ncol ? base[(long)c * L + y] : make_float2(0.f, 0.f);
+    return c < ncol && y >= ylo && y <= yhi ? base[(long)c * L + y] : make_float2(0.f, 0.f);
   };
   auto store = [&](int c, int y, float2 v) {
     if (c >= ncol) return;
@@ -321,17 +325,18 @@ __global__ __launch_bounds__(WG) void k_rows_c2r(const float2* __restrict__ Gin,
 // The same row C2R streamed: each thread reads X[k] and X[M - k] of its
 // inputs straight from the half spectra and pre-processes them on the way
 // into the first transform pass (the executor of the correlation's pass 2),
-// then writes its real pairs; no LDS staging of the spectra.
+// then writes its real pairs; no LDS staging of the spectra.  Row block
+// blockIdx.x + rb0.
 template <int L>
 __global__ __launch_bounds__(WG) void k_rows_c2r_stream(const float2* __restrict__ Gin,
-                                                        float* __restrict__ out,
+                                                        float* __restrict__ out, int rb0,
                                                         const float2* __restrict__ twL,
                                                         const float2* __restrict__ twM) {
   using G = Geo<L>;
   constexpr int M = G::M, HP = G::HP, RB = G::RB;
   __shared__ float lds[RB * exec_stride_split<M, RB, WG, MAP_J>()];
   const int b = blockIdx.y;
-  const int y0 = blockIdx.x * RB;
+  const int y0 = (blockIdx.x + rb0) * RB;
   const float2* gb = Gin + (long)b * L * HP;
   auto load = [&](int r, int k) -> float2 {
     const float2* row = gb + (long)min(y0 + r, L - 1) * HP;
@@ -1331,63 +1336,90 @@ void launch_rows_r2c_tiles(int L, const float* img, int nx, int ny, const int* o
 
 template <int L>
 static void rows_r2c_rot_t(const float* treal, int nb, float cose, float sine, float dmask,
-                           float2* out, double* partials, const float2* twL, const float2* twM,
-                           hipStream_t s) {
+                           RowBand band, float2* out, double* partials, const float2* twL,
+                           const float2* twM, hipStream_t s) {
   RowSrc rs{};
   rs.treal = treal; rs.cose = cose; rs.sine = sine; rs.dmask = dmask; rs.partials = partials;
-  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  rs.rb0 = band.rb0;
+  dim3 grid(band.nrb, nb);
   hipLaunchKernelGGL((k_rows_r2c<L, SRC_ROT>), grid, dim3(WG), 0, s, rs, out, twL, twM);
 }
 void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float sine, float dmask,
-                         float2* out, double* partials, const float2* twL, const float2* twM,
-                         hipStream_t s) {
-  DISPATCH(L, rows_r2c_rot_t, treal, nb, cose, sine, dmask, out, partials, twL, twM, s);
+                         RowBand band, float2* out, double* partials, const float2* twL,
+                         const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_r2c_rot_t, treal, nb, cose, sine, dmask, band, out, partials, twL, twM, s);
 }
 
 template <int L>
 static void rows_r2c_rotb_t(const float* treal, int nb, const double2* cs_b, const float* pre_b,
-                            float dmask, float2* out, double* partials, const float2* twL,
-                            const float2* twM, hipStream_t s) {
+                            float dmask, RowBand band, float2* out, double* partials,
+                            const float2* twL, const float2* twM, hipStream_t s) {
   RowSrc rs{};
   rs.treal = treal; rs.cs_b = cs_b; rs.pre_b = pre_b; rs.dmask = dmask; rs.partials = partials;
-  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  rs.rb0 = band.rb0;
+  dim3 grid(band.nrb, nb);
   hipLaunchKernelGGL((k_rows_r2c<L, SRC_ROTB>), grid, dim3(WG), 0, s, rs, out, twL, twM);
 }
 void launch_rows_r2c_rotb(int L, const float* treal, int nb, const double2* cs_b, const float* pre_b,
-                          float dmask, float2* out, double* partials, const float2* twL,
-                          const float2* twM, hipStream_t s) {
-  DISPATCH(L, rows_r2c_rotb_t, treal, nb, cs_b, pre_b, dmask, out, partials, twL, twM, s);
+                          float dmask, RowBand band, float2* out, double* partials,
+                          const float2* twL, const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_r2c_rotb_t, treal, nb, cs_b, pre_b, dmask, band, out, partials, twL, twM, s);
 }
 
 template <int L>
-static void rows_r2c_img_t(const float* imgs, int nb, float2* out, const float2* twL,
-                           const float2* twM, hipStream_t s, const int* skip) {
+static void rows_r2c_img_t(const float* imgs, int nb, RowBand band, float2* out,
+                           const float2* twL, const float2* twM, hipStream_t s,
+                           const int* skip) {
   RowSrc rs{};
   rs.imgs = imgs;
   rs.skip = skip;
-  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
+  rs.rb0 = band.rb0;
+  dim3 grid(band.nrb, nb);
   hipLaunchKernelGGL((k_rows_r2c<L, SRC_IMG>), grid, dim3(WG), 0, s, rs, out, twL, twM);
 }
-void launch_rows_r2c_img(int L, const float* imgs, int nb, float2* out, const float2* twL,
-                         const float2* twM, hipStream_t s, const int* skip) {
-  DISPATCH(L, rows_r2c_img_t, imgs, nb, out, twL, twM, s, skip);
+void launch_rows_r2c_img(int L, const float* imgs, int nb, RowBand band, float2* out,
+                         const float2* twL, const float2* twM, hipStream_t s, const int* skip) {
+  DISPATCH(L, rows_r2c_img_t, imgs, nb, band, out, twL, twM, s, skip);
 }
 
 template <int L>
 static void cols_fwd_t(float2* spec, int nb, float scale, const float* bscale, int zero_dc,
-                       const float2* twL, hipStream_t s) {
+                       RowBand band, const float2* twL, hipStream_t s) {
   dim3 grid((Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB, nb);
-  hipLaunchKernelGGL((k_cols_fwd<L>), grid, dim3(WG), 0, s, spec, scale, bscale, zero_dc, twL);
+  hipLaunchKernelGGL((k_cols_fwd<L>), grid, dim3(WG), 0, s, spec, scale, bscale, zero_dc,
+                     band.ylo(L), band.yhi(L), twL);
 }
 void launch_cols_fwd(int L, float2* spec, int nb, float scale, const float* bscale, int zero_dc,
-                     const float2* twL, hipStream_t s) {
-  DISPATCH(L, cols_fwd_t, spec, nb, scale, bscale, zero_dc, twL, s);
+                     const float2* twL, hipStream_t s, RowBand band) {
+  if (band.nrb == 0) band = full_band(L);
+  DISPATCH(L, cols_fwd_t, spec, nb, scale, bscale, zero_dc, band, twL, s);
 }
 
-void launch_rot_scale(int L, const double* partials, int nb, float* bscale, hipStream_t s) {
-  const int nparts = (L + geometry_of(L).RB - 1) / geometry_of(L).RB;
+void launch_rot_scale(int L, const double* partials, int nb, int nparts, float* bscale,
+                      hipStream_t s) {
   hipLaunchKernelGGL(k_rot_scale, dim3((nb + 255) / 256), dim3(256), 0, s, partials, nparts, nb,
                      (double)L * L, bscale);
+}
+
+RowBand full_band(int L) {
+  const int RB = geometry_of(L).RB;
+  RowBand b;
+  b.RB = RB;
+  b.rb0 = 0;
+  b.nrb = (L + RB - 1) / RB;
+  return b;
+}
+
+RowBand rot_row_band(int L, float dmask) {
+  RowBand b = full_band(L);
+  const float h = 0.5f * dmask;
+  // NaN or a mask wider than the window: every row
+  if (!(h < (float)L)) return b;
+  const int hw = h >= 0.0f ? (int)std::ceil(h) + 20 : 20;
+  const int y0 = std::max(0, L / 2 - hw), y1 = std::min(L - 1, L / 2 + hw);
+  b.rb0 = y0 / b.RB;
+  b.nrb = y1 / b.RB - b.rb0 + 1;
+  return b;
 }
 
 template <int L>
@@ -1407,14 +1439,14 @@ static void rows_c2r_t(const float2* G, int nb, float* out, const float2* twL, c
   hipLaunchKernelGGL((k_rows_c2r<L>), grid, dim3(WG), 0, s, G, out, twL, twM);
 }
 template <int L>
-static void rows_c2r_stream_t(const float2* G, int nb, float* out, const float2* twL,
-                              const float2* twM, hipStream_t s) {
-  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nb);
-  hipLaunchKernelGGL((k_rows_c2r_stream<L>), grid, dim3(WG), 0, s, G, out, twL, twM);
+static void rows_c2r_stream_t(const float2* G, int nb, RowBand band, float* out,
+                              const float2* twL, const float2* twM, hipStream_t s) {
+  dim3 grid(band.nrb, nb);
+  hipLaunchKernelGGL((k_rows_c2r_stream<L>), grid, dim3(WG), 0, s, G, out, band.rb0, twL, twM);
 }
-void launch_rows_c2r_stream(int L, const float2* G, int nb, float* out, const float2* twL,
-                            const float2* twM, hipStream_t s) {
-  DISPATCH(L, rows_c2r_stream_t, G, nb, out, twL, twM, s);
+void launch_rows_c2r_stream(int L, const float2* G, int nb, RowBand band, float* out,
+                            const float2* twL, const float2* twM, hipStream_t s) {
+  DISPATCH(L, rows_c2r_stream_t, G, nb, band, out, twL, twM, s);
 }
 void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* twL,
                      const float2* twM, hipStream_t s) {

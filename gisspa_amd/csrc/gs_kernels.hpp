@@ -6,6 +6,7 @@
 // of the 2-D inverse: [y][kx] with a padded row pitch HP.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 
 namespace gs {
@@ -59,6 +60,21 @@ Geometry geometry_of(int L);
 
 // ---- launches (all asynchronous on `stream`) ----
 
+// A range of whole row blocks (RB rows each) of an L x L image: blocks
+// rb0 .. rb0 + nrb - 1, rows ylo() .. yhi().
+struct RowBand {
+  int rb0 = 0, nrb = 0, RB = 0;
+  int ylo(int) const { return rb0 * RB; }
+  int yhi(int L) const { return std::min(L, (rb0 + nrb) * RB) - 1; }
+};
+RowBand full_band(int L);
+// The rows of a rotated template masked at diameter dmask (the picker's
+// apply_mask, GPU_func.cu:229-247: zero beyond radius dmask / 2 + 12 of the
+// window centre) that can be non-zero or be read by the bilinear rotation and
+// its adjoint gather (which reach at most 5 rows further out), with 20 rows
+// of margin; every row for a NaN diameter or a mask wider than the window.
+RowBand rot_row_band(int L, float dmask);
+
 // Real rows -> half spectrum [b][kx][y] (unscaled), from padded templates:
 // out[b] = R2C rows of pad(src[b], box bx x by centred) * scale[b]
 void launch_rows_r2c_pad(int L, const float* src, int bx, int by, const float* scale, int nb,
@@ -69,26 +85,32 @@ void launch_rows_r2c_tiles(int L, const float* img, int nx, int ny, const int* o
                            const int* offy, int bxn, int ntiles, const double* mean,
                            float2* out, const float2* twL, const float2* twM, hipStream_t s);
 // Real rows -> half spectrum from rotated + masked real templates, with
-// per-(template, row block) partial sums of value and value^2.
+// per-(template, row block) partial sums of value and value^2 (band.nrb per
+// template).  Only the rows of `band` are transformed and written: the band
+// must hold every row the mask leaves non-zero (rot_row_band).
 void launch_rows_r2c_rot(int L, const float* treal, int nb, float cose, float sine, float dmask,
-                         float2* out, double* partials, const float2* twL, const float2* twM,
-                         hipStream_t s);
+                         RowBand band, float2* out, double* partials, const float2* twL,
+                         const float2* twM, hipStream_t s);
 // Same with one angle per b (cs_b[b] = (cos, sin), coordinates in double
 // precision) and the interpolated value multiplied by pre_b[b].
 void launch_rows_r2c_rotb(int L, const float* treal, int nb, const double2* cs_b, const float* pre_b,
-                          float dmask, float2* out, double* partials, const float2* twL,
-                          const float2* twM, hipStream_t s);
-// Real rows -> half spectrum of real images imgs[b][L][L].
-void launch_rows_r2c_img(int L, const float* imgs, int nb, float2* out, const float2* twL,
-                         const float2* twM, hipStream_t s, const int* skip = nullptr);
+                          float dmask, RowBand band, float2* out, double* partials,
+                          const float2* twL, const float2* twM, hipStream_t s);
+// Real rows -> half spectrum of real images imgs[b][L][L], the rows of `band`.
+void launch_rows_r2c_img(int L, const float* imgs, int nb, RowBand band, float2* out,
+                         const float2* twL, const float2* twM, hipStream_t s,
+                         const int* skip = nullptr);
 
 // In-place column transform of [b][kx][y]; multiplies by scale (global) or
 // per-b scale from `bscale` (may be null); zero_dc zeroes element (0,0).
+// Inputs outside the rows of `band` are taken as zero (every row when
+// band.nrb == 0).
 void launch_cols_fwd(int L, float2* spec, int nb, float scale, const float* bscale, int zero_dc,
-                     const float2* twL, hipStream_t s);
+                     const float2* twL, hipStream_t s, RowBand band = RowBand());
 // Per-template normalisation scale 1/(L^2 sd) of the rotated templates from
-// the row partials (main.cu:682-707 semantics).
-void launch_rot_scale(int L, const double* partials, int nb, float* bscale, hipStream_t s);
+// the nparts row partials of each (main.cu:682-707 semantics).
+void launch_rot_scale(int L, const double* partials, int nb, int nparts, float* bscale,
+                      hipStream_t s);
 // Column inverse of [b][kx][y] written row-major to G[b][y][kx] (pitch HP).
 void launch_cols_inv_T(int L, const float2* spec, int nb, float2* G, const float2* twL,
                        hipStream_t s);
@@ -97,8 +119,9 @@ void launch_rows_c2r(int L, const float2* G, int nb, float* out, const float2* t
                      const float2* twM, hipStream_t s);
 // the same transform with the spectra read and pre-processed on the way into
 // the first pass (no LDS staging); equal to rounding
-void launch_rows_c2r_stream(int L, const float2* G, int nb, float* out, const float2* twL,
-                            const float2* twM, hipStream_t s);
+// (the rows of `band` only)
+void launch_rows_c2r_stream(int L, const float2* G, int nb, RowBand band, float* out,
+                            const float2* twL, const float2* twM, hipStream_t s);
 
 // Correlation pass 1: P = tile[t] * conj(tmpl[J]) (the tile spectra carry
 // (-1)^(kx+ky) sgn), column inverse and C2R pre-processing, written to the

@@ -197,12 +197,14 @@ constexpr int ncol_blocks() { return (Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB; 
 // Forward weighting fused into the inverse column pass: A = f(k; defocus) B_j
 // is formed while loading (never stored), with the partials of sum |A|^2 and
 // sum |B|^2 f f' (over the radial bins the normaliser counts), then the
-// inverse column transform writes the transposed intermediate of the C2R.
+// inverse column transform writes the transposed intermediate of the C2R,
+// rows ylo .. yhi only (the rows the rotation reads).
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict__ white,
                                                         const RefCand* __restrict__ cand,
                                                         WeightArgs w0,
                                                         const float4* __restrict__ wtab,
+                                                        int ylo, int yhi,
                                                         float2* __restrict__ Gout,
                                                         double* __restrict__ part,
                                                         const float2* __restrict__ twL) {
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
   };
   float2* gb = Gout + (long)b * L * HP;
   auto store = [&](int c, int y, float2 v) {
-    if (c < ncol) gb[(long)y * HP + kx0 + c] = v;
+    if (c < ncol && y >= ylo && y <= yhi) gb[(long)y * HP + kx0 + c] = v;
   };
   fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
   block_sum_n<2>(acc, red);
@@ -252,13 +254,15 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
 // The normalisation adjoint needs sum_x g Rn with g = d/dRn; by Parseval it
 // is sum_half w Re(G conj(That)) (w = 2 between the columns 0 and L/2, 1 on
 // them), accumulated here from the values the C2R receives; sum_x g is
-// Re G(0, 0) = 0, as the DC carries no gradient.
+// Re G(0, 0) = 0, as the DC carries no gradient.  Rows ylo .. yhi of the
+// intermediate are written (d/dRn is used inside the mask only).
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__ tiles,
                                                       const float2* __restrict__ spec,
                                                       const RefCand* __restrict__ cand,
                                                       const RefScal* __restrict__ sc,
                                                       const int* __restrict__ skip,
+                                                      int ylo, int yhi,
                                                       float2* __restrict__ Gout,
                                                       double* __restrict__ part,
                                                       const float2* __restrict__ twL) {
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
   };
   float2* gb = Gout + (long)b * L * HP;
   auto store = [&](int c, int y, float2 v) {
-    if (c < ncol) gb[(long)y * HP + kx0 + c] = v;
+    if (c < ncol && y >= ylo && y <= yhi) gb[(long)y * HP + kx0 + c] = v;
   };
   fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
   double acc[1] = {(double)accf};
@@ -331,6 +335,7 @@ __global__ void k_ref_norm_fin(const double* __restrict__ part, int nb, int ncb,
 // Backward: forward column transform of the row R2C of d/dT (d/dA), reduced
 // on the spot to the defocus partial sum_half w Re(conj(d/dA) f' B) of
 // d score / d defocus through A = f(defocus) B, so d/dA is never stored.
+// The row transforms hold rows ylo .. yhi (d/dT is zero outside them).
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ rows,
                                                     const float2* __restrict__ white,
@@ -338,6 +343,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
                                                     WeightArgs w0,
                                                     const float4* __restrict__ wtab,
                                                     const int* __restrict__ skip,
+                                                    int ylo, int yhi,
                                                     double* __restrict__ part,
                                                     const float2* __restrict__ twL) {
   using Gm = Geo<L>;
@@ -352,7 +358,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
   const int ncol = min(CB, H - kx0);
   const float sa = sqrtf(1.0f - w0.ampconst * w0.ampconst);
   auto load = [&](int c, int ky) -> float2 {
-    return c < ncol ? base[(long)c * L + ky] : make_float2(0.f, 0.f);
+    return c < ncol && ky >= ylo && ky <= yhi ? base[(long)c * L + ky] : make_float2(0.f, 0.f);
   };
   // the last radix pass hands each output d/dA(kx, ky) to the reduction
   double acc[1] = {0};
@@ -427,9 +433,10 @@ __global__ void k_ref_score_fin(const double* __restrict__ part, int nb, int L,
 // k_rows_c2r_stream) whose store applies the normalisation and mask adjoint
 // per pixel: d/dRm = (g - mean g - Rn mean(g Rn)) / sd, d/d(Tb p) = d/dRm
 // mask, written as d/dTb; it also accumulates the sums for d/dpsi (bilinear
-// slopes times d(x2, y2)/dpsi) and d/dm, one partial per row block.
+// slopes times d(x2, y2)/dpsi) and d/dm, one partial per row block (row
+// block blockIdx.x + rb0; outside the launched blocks d/dTb is zero).
 template <int L>
-__global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(const float2* __restrict__ Gin,
+__global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(int rb0, const float2* __restrict__ Gin,
                                                           const float* __restrict__ Tun,
                                                           const double2* __restrict__ cs_b,
                                                           const float* __restrict__ pre,
@@ -446,7 +453,7 @@ __global__ __launch_bounds__(WG) void k_ref_rows_c2r_grad(const float2* __restri
   __shared__ float lds[RB * exec_stride_split<M, RB, WG, MAP_J>()];
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y;
-  const int y0 = blockIdx.x * RB;
+  const int y0 = (blockIdx.x + rb0) * RB;
   if (skip[b]) return;  // the whole workgroup
   const float2* gb = Gin + (long)b * L * HP;
   const double2 cs = cs_b[b];
@@ -505,18 +512,21 @@ __global__ void k_ref_rot_fin(const double* __restrict__ part, int nb, int np,
   sc[b].dmacc = -dpre / (sc[b].m * sc[b].m);
 }
 
-// the bilinear adjoint of d/dTb (rot_adj_gather) for every source pixel
+// the bilinear adjoint of d/dTb (rot_adj_gather) for every source pixel of
+// rows ylo .. yhi (zero beyond the mask's reach, which the rows hold)
 template <int L>
 __global__ __launch_bounds__(WG) void k_ref_rot_adj(const float* __restrict__ gTb,
                                                     const double2* __restrict__ cs_b, float dmask,
-                                                    const int* __restrict__ skip,
+                                                    const int* __restrict__ skip, int ylo, int yhi,
                                                     float* __restrict__ gT) {
   constexpr long L2 = (long)L * L;
   const int b = blockIdx.y;
   if (skip[b]) return;
   const double2 cs = cs_b[b];
   const float* gsrc = gTb + b * L2;
-  for (long i = (long)blockIdx.x * WG + threadIdx.x; i < L2; i += (long)RPARTS * WG) {
+  const long iend = (long)(yhi + 1) * L;
+  for (long i = (long)ylo * L + (long)blockIdx.x * WG + threadIdx.x; i < iend;
+       i += (long)RPARTS * WG) {
     const int qy = (int)(i / L), qx = (int)(i - (long)qy * L);
     gT[b * L2 + i] = rot_adj_gather<L>(gsrc, qx, qy, cs, dmask);
   }
@@ -552,15 +562,21 @@ static void ref_forward_t(const RefArgs& a, hipStream_t s) {
   const dim3 grid(RPARTS, a.nb), blk(WG);
   const int small = (a.nb + 63) / 64;
   constexpr int NCB = ncol_blocks<L>();
+  // the masked template, its rotation source and the adjoint's images are
+  // zero or unread outside a band of rows: the row transforms run on the
+  // band, the column transforms write or read only its rows
+  const RowBand band = rot_row_band(L, a.dmask);
+  const int ylo = band.ylo(L), yhi = band.yhi(L);
   hipLaunchKernelGGL((k_ref_weight_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.white, a.cand, a.w,
-                     a.wtab, a.gtmp, a.part, a.twL);
+                     a.wtab, ylo, yhi, a.gtmp, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_norm_m, dim3(small), dim3(64), 0, s, a.part, a.nb, a.count, a.sc,
                      a.pre, NCB, FSLOTS);
-  launch_rows_c2r_stream(L, a.gtmp, a.nb, a.Tun, a.twL, a.twM, s);
-  launch_rows_r2c_rotb(L, a.Tun, a.nb, a.cs, a.pre, a.dmask, a.spec, a.rot_part, a.twL, a.twM, s);
-  hipLaunchKernelGGL(k_ref_rot_stats, dim3(small), dim3(64), 0, s, a.rot_part,
-                     (L + Geo<L>::RB - 1) / Geo<L>::RB, a.nb, (double)L * L, a.sc, a.bscale);
-  launch_cols_fwd(L, a.spec, a.nb, 1.0f, a.bscale, 1, a.twL, s);
+  launch_rows_c2r_stream(L, a.gtmp, a.nb, band, a.Tun, a.twL, a.twM, s);
+  launch_rows_r2c_rotb(L, a.Tun, a.nb, a.cs, a.pre, a.dmask, band, a.spec, a.rot_part, a.twL,
+                       a.twM, s);
+  hipLaunchKernelGGL(k_ref_rot_stats, dim3(small), dim3(64), 0, s, a.rot_part, band.nrb, a.nb,
+                     (double)L * L, a.sc, a.bscale);
+  launch_cols_fwd(L, a.spec, a.nb, 1.0f, a.bscale, 1, a.twL, s, band);
   hipLaunchKernelGGL((k_ref_score<L>), grid, blk, 0, s, a.tiles, a.spec, a.cand, a.part);
   hipLaunchKernelGGL(k_ref_score_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, L, a.sc, a.thr,
                      a.skip);
@@ -571,25 +587,27 @@ static void ref_backward_t(const RefArgs& a, hipStream_t s) {
   const dim3 grid(RPARTS, a.nb), blk(WG);
   const int small = (a.nb + 63) / 64;
   constexpr int NCB = ncol_blocks<L>();
+  const RowBand band = rot_row_band(L, a.dmask);  // as in the forward pass
+  const int ylo = band.ylo(L), yhi = band.yhi(L);
   // d/dThat (with the Parseval sum of the normalisation adjoint) -> the
   // transposed intermediate of the C2R
   hipLaunchKernelGGL((k_ref_grad_cols<L>), dim3(NCB, a.nb), blk, 0, s, a.tiles, a.spec, a.cand,
-                     a.sc, a.skip, a.gtmp, a.part, a.twL);
+                     a.sc, a.skip, ylo, yhi, a.gtmp, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_norm_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, NCB,
                      (double)L * L, a.sc);
   // row C2R -> d/dRn -> d/dTb per pixel, d/dpsi and d/dm sums
-  constexpr int NRB = (L + Geo<L>::RB - 1) / Geo<L>::RB;
-  hipLaunchKernelGGL((k_ref_rows_c2r_grad<L>), dim3(NRB, a.nb), blk, 0, s, a.gtmp, a.Tun, a.cs,
-                     a.pre, a.sc, a.skip, a.dmask, a.gTb, a.part, a.twL, a.twM);
-  hipLaunchKernelGGL(k_ref_rot_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, NRB, a.sc);
+  hipLaunchKernelGGL((k_ref_rows_c2r_grad<L>), dim3(band.nrb, a.nb), blk, 0, s, band.rb0, a.gtmp,
+                     a.Tun, a.cs, a.pre, a.sc, a.skip, a.dmask, a.gTb, a.part, a.twL, a.twM);
+  hipLaunchKernelGGL(k_ref_rot_fin, dim3(small), dim3(64), 0, s, a.part, a.nb, band.nrb, a.sc);
   // d/dT: the bilinear adjoint of d/dTb, gathered per source pixel (a
   // separate pass: gathered inside the row transform's loads it took 267 us
   // per batch of 64 at window 720 against 209 for the two kernels)
-  hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.skip, a.gT);
+  hipLaunchKernelGGL((k_ref_rot_adj<L>), grid, blk, 0, s, a.gTb, a.cs, a.dmask, a.skip, ylo, yhi,
+                     a.gT);
   // d/dT -> d/dA: unnormalised R2C
-  launch_rows_r2c_img(L, a.gT, a.nb, a.A, a.twL, a.twM, s, a.skip);
+  launch_rows_r2c_img(L, a.gT, a.nb, band, a.A, a.twL, a.twM, s, a.skip);
   hipLaunchKernelGGL((k_ref_cols_df<L>), dim3(NCB, a.nb), blk, 0, s, a.A, a.white, a.cand, a.w,
-                     a.wtab, a.skip, a.part, a.twL);
+                     a.wtab, a.skip, ylo, yhi, a.part, a.twL);
   hipLaunchKernelGGL(k_ref_out, dim3(small), dim3(64), 0, s, a.part, a.nb, a.sc, a.skip, a.out,
                      NCB, FSLOTS);
 }
