@@ -137,3 +137,45 @@ def test_full_size_phase_flip(gpu, full_case):
         got = pk.mapstats().reshape(ref.shape)
     _compare(got, ref, "5760x4092 / window 720, phase_flip 1")
     _compare_picks(picks, orc.select_norm0(p, ref, 0, 5760, 4092))
+
+
+def test_all_views_fused_against_plain(gpu):
+    """Every one of the 1649 views of the reference's euler_C1_step5.lst over
+    the 88 tiles of the full-size micrograph at two in-plane steps (180 and
+    185 degrees; 290,224 maps): the fused correlation against the
+    stage-by-stage mode (GS_MODE_PLAIN: products, complex column and row
+    transforms and the peak search as separate kernels, the reference's stage
+    order), scores within 1e-4.  A map whose two peak positions differ is a
+    near-tie between two FP32 computations; each such map is recomputed by
+    the FP64 oracle, whose position the fused map must give."""
+    import gisspa_amd
+    wl = synth.full_workload(seed=0, eulers=_cases.c1_step5_views())
+    tm = wl.templates
+    image, df = wl.micrographs[0]
+    params = wl.params
+    assert tm.shape == (1649, 180, 180)
+    with gisspa_amd.Picker(0, **params) as pk:
+        pk.set_templates(tm, df)
+        pk.set_micrograph(image, df)
+        fused_picks = pk.pick(36, 38)
+        fused = pk.mapstats().reshape(2, 88, 1649)
+        pk.set_mode(gisspa_amd.GS_MODE_PLAIN)
+        pk.set_micrograph(image, df)  # the plain mode keeps its own tile spectra
+        plain_picks = pk.pick(36, 38)
+        plain = pk.mapstats().reshape(2, 88, 1649)
+    rel = np.abs(fused["score"].astype(np.float64) - plain["score"]) / np.maximum(
+        np.abs(plain["score"].astype(np.float64)), 1.0)
+    diff = np.argwhere(fused["pos"] != plain["pos"])
+    print(f"all views: {fused.size} maps, fused vs plain score max rel err {rel.max():.2e}, "
+          f"position differences {len(diff)}, picks {len(fused_picks)} / {len(plain_picks)}")
+    assert rel.max() < 1e-4
+    assert len(diff) <= 8
+    p = orc.params(720, **params)
+    for s, t, v in diff:
+        ref = orc.pick_norm0(p, orc.ctf(df), tm[v:v + 1], image, 36 + s, 37 + s,
+                             nthreads=NTHREADS)
+        assert fused["pos"][s, t, v] == ref.reshape(88)["pos"][t]
+    if len(diff) == 0:
+        assert len(fused_picks) == len(plain_picks)
+        for k in ("psi_index", "view", "cx", "cy"):
+            assert np.array_equal(fused_picks[k], plain_picks[k])
