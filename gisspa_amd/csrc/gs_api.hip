@@ -567,10 +567,12 @@ int ensure_corr_buffers(Ctx* c) {
   const long tile_bytes = (long)(L / 2 + 1) * L * (long)sizeof(float2);
   const int tb = c->tile_block > 0 ? c->tile_block
                                    : (int)std::max(1l, (192l << 20) / tile_bytes);
-  // norm_type 1 batches: nv views of every tile; by default about 4 GiB of
-  // intermediate (23 views at L = 720), so that the per-batch flush of the
-  // FP64 accumulators is shared by many maps
-  const long nb1 = c->batch_set ? nb : std::max(1l, (4096l << 20) / per_map);
+  // norm_type 1 batches: nv views of every tile; by default about 8 GiB of
+  // intermediate per stream (46 views at L = 720), so that the per-batch
+  // flush of the FP64 accumulators is shared by many maps (206 views of the
+  // full-size micrograph: 23 / 46 / 69 / 103 views per batch measured
+  // 104.0 / 106.7 / 98.6 / 92.6 micrographs/hour)
+  const long nb1 = c->batch_set ? nb : std::max(1l, (8192l << 20) / per_map);
   const int nv1 = (int)std::max(1l, std::min((long)c->N, nb1 / c->T));
   if (c->maps_T != c->T || c->maps_N != c->N || c->maps_tb != tb || c->maps_nv != nv1) {
     std::vector<MapRef> mr, mb;

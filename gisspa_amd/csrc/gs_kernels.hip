@@ -976,6 +976,7 @@ __global__ __launch_bounds__(WG) void k_norm1_zrun(const float2* __restrict__ Gi
   __shared__ unsigned long long rk[NRUN];
   const int m = blockIdx.y;
   const int y0 = blockIdx.x * RB;
+  if (y0 >= L) return;  // the grid's padding to a multiple of 8 row blocks
   const long p0 = (long)y0 * L, p1 = (long)min(y0 + RB, L) * L;
   const int k0 = (int)(p0 >> 10), k1 = (int)((p1 - 1) >> 10);
   for (int i = threadIdx.x; i <= k1 - k0; i += WG) rk[i] = 0ull;
@@ -1673,7 +1674,11 @@ void launch_norm1_acc(int L, const float2* G, const MapRef* maps, int ntiles, in
 template <int L>
 static void norm1_zrun_t(const float2* G, const MapRef* maps, int nmaps, const void* norm,
                          unsigned long long* keys, const float2* twM, hipStream_t s) {
-  dim3 grid((L + Geo<L>::RB - 1) / Geo<L>::RB, nmaps);
+  // row blocks padded to a multiple of 8: workgroups are dealt to the 8 XCDs
+  // round-robin, so XCD x then runs the row blocks x, x + 8, ... of every map
+  // and its L2 holds only those rows of the tile's per-pixel mean and 1/sd,
+  // which the tile's other views read again
+  dim3 grid(((L + Geo<L>::RB - 1) / Geo<L>::RB + 7) / 8 * 8, nmaps);
   hipLaunchKernelGGL(k_norm1_zrun<L>, grid, dim3(WG), 0, s, G, maps,
                      static_cast<const float4*>(norm), keys, twM);
 }

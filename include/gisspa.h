@@ -350,8 +350,8 @@ int gs_set_profiling(void* ctx, int enable);
 /* Number of CC-map evaluations one gs_pick launch batch holds (tuning; 0
  * restores the defaults).  norm_type 0: default about 1.5 GiB of
  * intermediate per stream.  norm_type 1: batches are groups of
- * max(1, maps_per_batch / tiles) views over all tiles; default about 4 GiB
- * of intermediate. */
+ * max(1, maps_per_batch / tiles) views over all tiles; default about 8 GiB
+ * of intermediate per stream. */
 int gs_set_batch(void* ctx, int maps_per_batch);
 /* Correlation schedule (norm_type 0, fused mode):
  *   GS_PIPE_STREAM: one launch per psi; map m runs on XCD m % 8, which
