@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per CC map at window 720 from rocprofv3 FETCH_SIZE (x2.0) and
 # WRITE_SIZE (x1.0) counters, scaled by a 1 GiB copy kernel's readings
 # (profiles/corr_720_full_workload.md)
-TRAFFIC_PER_MAP_720 = {"k_corr_pass1": 4.262e6, "k_corr_pass2": 1.933e6}
+TRAFFIC_PER_MAP_720 = {"k_corr_pass1": 4.084e6, "k_corr_pass2": 1.933e6}
 
 
 def log(*a):
@@ -682,6 +682,10 @@ def main():
                     help="time rocFFT's inverse 2D transforms of the same size (0 = skip)")
     ap.add_argument("--project", type=int, default=1,
                     help="time the template projection of the density (0 = skip)")
+    ap.add_argument("--time-kernels", type=int, default=0, metavar="REPS",
+                    help="only time each correlation pass alone (gs_time_kernels, REPS launches "
+                         "of each, one at a time on one stream) for counter runs, then exit "
+                         "without a JSON line (0 = the normal run)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--cache", default=os.environ.get("TMPDIR", "/tmp"))
@@ -721,6 +725,14 @@ def main():
     t = time.time()
     pk.set_templates(templates, defocus)
     log(f"[bench] template prep {time.time() - t:.2f}s")
+    if args.time_kernels:
+        pk.set_micrograph_device(d_img, nx, ny, defocus)
+        k1, k2, nb = pk.time_kernels(0, args.time_kernels)
+        log(f"[bench] time_kernels: {nb} maps per launch: pass 1 {k1:.3f} ms "
+            f"({k1 / nb * 1e3:.3f} us/map), pass 2 {k2:.3f} ms ({k2 / nb * 1e3:.3f} us/map)")
+        pk.close()
+        hip.free(d_img)
+        return
     npsi = 72 if args.psi_limit <= 0 else min(72, args.psi_limit)
     tune = None
     pipeline = args.pipeline
