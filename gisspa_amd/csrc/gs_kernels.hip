@@ -230,7 +230,7 @@ __global__ __launch_bounds__(WG) void k_cols_fwd(float2* __restrict__ spec, floa
   // inputs outside rows ylo .. yhi are zero (not read)
   using G = Geo<L>;
   constexpr int H = G::H, CB = G::CB;
-  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_J>()];
   const int b = blockIdx.y;
   const int kx0 = blockIdx.x * CB;
   float2* base = spec + ((long)b * H + kx0) * L;
@@ -245,7 +245,9 @@ __global__ __launch_bounds__(WG) void k_cols_fwd(float2* __restrict__ spec, floa
     if (zero_dc && kx0 == 0 && c == 0 && y == 0) v = make_float2(0.f, 0.f);
     base[(long)c * L + y] = v;
   };
-  fft_exec_split<L, CB, WG, -1, MAP_J, MAP_B>(lds, twL, load, store);
+  // the last pass maps consecutive lanes to consecutive rows of a column, so
+  // the stores of a wave cover contiguous runs of the column-major layout
+  fft_exec_split<L, CB, WG, -1, MAP_J, MAP_J>(lds, twL, load, store);
 }
 
 // inverse columns, transposed write to G[b][y][kx] (pitch HP)

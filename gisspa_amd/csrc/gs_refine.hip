@@ -348,7 +348,7 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
                                                     const float2* __restrict__ twL) {
   using Gm = Geo<L>;
   constexpr int H = Gm::H, CB = Gm::CB;
-  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_J>()];
   __shared__ double red[1 * (WG / 64)];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   if (skip[b]) return;  // the whole workgroup
@@ -371,7 +371,9 @@ __global__ __launch_bounds__(WG) void k_ref_cols_df(const float2* __restrict__ r
     const float wh = (kx == 0 || kx == L / 2) ? 1.0f : 2.0f;
     acc[0] += (double)(wh * wd.fd * (ga.x * bv.x + ga.y * bv.y));
   };
-  fft_exec_split<L, CB, WG, -1, MAP_J, MAP_B>(lds, twL, load, store);
+  // consecutive lanes hold consecutive rows of a column in the last pass, so
+  // the table and spectrum reads of the reduction are contiguous
+  fft_exec_split<L, CB, WG, -1, MAP_J, MAP_J>(lds, twL, load, store);
   block_sum_n<1>(acc, red);
   if (threadIdx.x == 0) part[(long)b * FSLOTS + blockIdx.x] = acc[0];
 }
