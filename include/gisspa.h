@@ -244,10 +244,11 @@ int gs_time_kernels(void* ctx, int psi_index, int reps, double* out_ms, int* map
 int gs_adjoint(void* ctx, const gs_peak* peaks, int n, gs_grad* out);
 /* Local refinement of n candidates in place: up to `iters` steps of
  * gradient ascent on the objective of gs_adjoint (step length adapted per
- * candidate: x1.25 after an accepted step, x0.5 after a rejected one; psi
- * within one phistep, the shift within 1.5 pixels and the defocus within
- * 0.5 um of the start); a candidate stops once its step is below 0.05 degree
- * / pixel (0.005 um).  out (may be null) receives the final objective and
+ * candidate from a parabola through the start and the trial: its maximum,
+ * within 1-2x the step after an accepted trial and 0.1-0.5x after a
+ * rejected one; psi within one phistep, the shift within 1.5 pixels and the
+ * defocus within 0.5 um of the start); a candidate stops once its step is
+ * below 0.05 degree / pixel (0.005 um).  out (may be null) receives the final objective and
  * gradient. */
 int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out);
 
