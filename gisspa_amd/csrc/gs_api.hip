@@ -1611,9 +1611,13 @@ int gs_refine(void* ctx, gs_peak* peaks, int n, int iters, gs_grad* out) {
   std::vector<double> step(n, 0.5);
   // a candidate whose step has shrunk below kMinStep (0.05 degree / pixel /
   // 0.005 um) has converged and leaves the evaluated set.  On the top 10,000
-  // picks of the full-size micrograph (20 steps): 0.01 -> 198,261
-  // evaluations, mean objective 19.9422; 0.05 -> 159,967, 19.9417; 0.1 ->
-  // 134,883, 19.9400 (start 18.3470)
+  // picks of the full-size micrograph (20 steps, steps x1.25 / x0.5): 0.01 ->
+  // 198,261 evaluations, mean objective 19.9422; 0.05 -> 159,967, 19.9417;
+  // 0.1 -> 134,883, 19.9400 (start 18.3470).  With the parabola steps, on
+  // the top 7,616 picks of 24 in-plane steps: 0.03 / 0.05 / 0.08 / 0.1 ->
+  // 105,169 / 96,191 / 87,134 / 82,473 evaluations, mean objective 18.2328 /
+  // 18.2324 / 18.2311 / 18.2303; from 0.08 on some candidates stop without
+  // any gain
   constexpr double kMinStep = 0.05;
   std::vector<int> act(n);
   for (int i = 0; i < n; ++i) act[i] = i;
