@@ -179,3 +179,40 @@ def test_all_views_fused_against_plain(gpu):
         assert len(fused_picks) == len(plain_picks)
         for k in ("psi_index", "view", "cx", "cy"):
             assert np.array_equal(fused_picks[k], plain_picks[k])
+
+
+def test_norm_type1_at_window_720(gpu):
+    """norm_type 1 with the full-size shape's window 720 (the accumulation's
+    512-thread 30 x 12 row transforms and the z-run kernel at that size): a
+    1260 x 1260 micrograph of 4 tiles, 4 of the reference's views x 8
+    in-plane steps (per-pixel mean and sd over 32 maps), threshold 4.  Picks
+    must match the oracle's record for record; a record whose oracle z-score
+    lies within 1e-4 (relative) of the threshold may be missing on either
+    side."""
+    import gisspa_amd
+    dens = synth.blob_density(60, 180, 1)
+    views = _cases.c1_step5_views()
+    tm = synth.project_templates(dens, 180, views[::103][:4])
+    mg = synth.make_micrograph(1260, 1260, dens, views, 20, 0.05, 3, apix=1.0,
+                               defocus_um=1.8, ampconst=0.1, box=180, psi_step=5.0)
+    params = dict(angpix=1.0, phistep=45.0, kk=3, energy=300, cs=2.7, Highres=8, Lowres=100,
+                  diameter=135.0, threshold=4.0, window_size=720, overlap=180, ampconst=0.1,
+                  norm_type=1)
+    p = orc.params(720, **params)
+    ref = orc.pick_norm1(p, orc.ctf(1.8), tm, mg.image, nthreads=NTHREADS)
+    with gisspa_amd.Picker(0, **params) as pk:
+        pk.set_templates(tm, 1.8)
+        pk.set_micrograph(mg.image, 1.8)
+        assert pk.info().n_tiles == 4
+        got = pk.pick()
+    thr = params["threshold"]
+    key = lambda r: (int(r["psi_index"]), int(r["view"]), float(r["cx"]), float(r["cy"]))
+    rk = {key(r): float(r["score"]) for r in ref}
+    gk = {key(r): float(r["score"]) for r in got}
+    print("norm1 at window 720: picks gpu", len(got), "oracle", len(ref))
+    assert len(ref) > 10
+    for k in set(rk) ^ set(gk):
+        s = rk.get(k, gk.get(k))
+        assert abs(s - thr) <= 1e-4 * thr, (k, s)
+    for k in set(rk) & set(gk):
+        assert abs(rk[k] - gk[k]) <= 1e-4 * max(1.0, abs(rk[k])), (k, rk[k], gk[k])
