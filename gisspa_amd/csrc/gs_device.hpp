@@ -234,11 +234,13 @@ __device__ __forceinline__ float rot_adj_gather(const float* g, int qx, int qy, 
       if (gv == 0.0f) continue;
       const float xf = (float)(x - L / 2), yf = (float)(y - L / 2);
       // cheap FP32 pre-test: output x's cell corner is floor(x2), floor(y2),
-      // and the cell holds q only if x2 is within (qx - 1, qx + 1] (same in
-      // y); the FP32 coordinates are off by far less than the 0.25 margin
+      // and the cell holds q only if x2 lies in [qx - 1, qx + 1] (same in y;
+      // the edge rules clamp inside that range); the FP32 coordinates differ
+      // from the FP64 ones by at most about 1e-4 pixel (|x|, |y| <= L/2,
+      // relative rounding 2^-24 per operation), far inside the 2e-3 margin
       const float x2f = (float)cs.x * xf + (float)cs.y * yf + (float)(L / 2);
       const float y2f = -(float)cs.y * xf + (float)cs.x * yf + (float)(L / 2);
-      if (fabsf(x2f - (float)qx) > 1.25f || fabsf(y2f - (float)qy) > 1.25f) continue;
+      if (fabsf(x2f - (float)qx) > 1.002f || fabsf(y2f - (float)qy) > 1.002f) continue;
       const double x2 = cs.x * xf + cs.y * yf + (double)(L / 2);
       const double y2 = -cs.y * xf + cs.x * yf + (double)(L / 2);
       if (x2 < 0 || x2 > L - 1.0 || y2 < 0 || y2 > L - 1.0) continue;
