@@ -193,6 +193,13 @@ __device__ __forceinline__ void phase_tables(float2* ex, float2* ey, double ux, 
 constexpr int FSLOTS = 2 * RPARTS;
 template <int L>
 constexpr int ncol_blocks() { return (Geo<L>::H + Geo<L>::CB - 1) / Geo<L>::CB; }
+// FFT schedule of the inverse column kernels that compute per-cell work in
+// their loads (weighting, gradient): the first radix pass does that work,
+// so at window 720 the 24 x 30 schedule (240 of 256 threads in the first
+// pass) instead of the tuned 30 x 24 (192): 7,616 refinements 0.796 instead
+// of 0.832 s
+template <int L>
+constexpr int inv_cols_variant() { return L == 720 ? 1 : 0; }
 
 // Forward weighting fused into the inverse column pass: A = f(k; defocus) B_j
 // is formed while loading (never stored), with the partials of sum |A|^2 and
@@ -211,7 +218,7 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
   using Gm = Geo<L>;
   constexpr int H = Gm::H, HP = Gm::HP, CB = Gm::CB;
   static_assert(ncol_blocks<L>() <= FSLOTS, "partial slots");
-  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B, inv_cols_variant<L>()>()];
   __shared__ double red[2 * (WG / 64)];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
   const RefCand cd = cand[b];
@@ -236,7 +243,7 @@ __global__ __launch_bounds__(WG) void k_ref_weight_cols(const float2* __restrict
   auto store = [&](int c, int y, float2 v) {
     if (c < ncol && y >= ylo && y <= yhi) gb[(long)y * HP + kx0 + c] = v;
   };
-  fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
+  fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B, inv_cols_variant<L>()>(lds, twL, load, store);
   block_sum_n<2>(acc, red);
   if (threadIdx.x == 0) {
     part[((long)b * FSLOTS + blockIdx.x) * 2] = acc[0];
@@ -269,7 +276,7 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
   using Gm = Geo<L>;
   constexpr int H = Gm::H, HP = Gm::HP, CB = Gm::CB;
   static_assert(ncol_blocks<L>() <= FSLOTS, "partial slots");
-  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B>()];
+  __shared__ float lds[CB * exec_stride_split<L, CB, WG, MAP_B, inv_cols_variant<L>()>()];
   __shared__ float2 ex[H], ey[L];
   __shared__ double red[WG / 64];
   const int b = blockIdx.y, kx0 = blockIdx.x * CB;
@@ -314,7 +321,7 @@ __global__ __launch_bounds__(WG) void k_ref_grad_cols(const float2* __restrict__
   auto store = [&](int c, int y, float2 v) {
     if (c < ncol && y >= ylo && y <= yhi) gb[(long)y * HP + kx0 + c] = v;
   };
-  fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B>(lds, twL, load, store);
+  fft_exec_split<L, CB, WG, +1, MAP_J, MAP_B, inv_cols_variant<L>()>(lds, twL, load, store);
   double acc[1] = {(double)accf};
   block_sum_n<1>(acc, red);
   if (threadIdx.x == 0) part[(long)b * FSLOTS + blockIdx.x] = acc[0];
