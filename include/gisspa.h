@@ -328,7 +328,8 @@ typedef struct gs_tune {
 /* Choose the correlation schedule on the device (no counterpart in the
  * reference, whose cuFFT plans are fixed, main.cu:256-308): the FFT schedule
  * of each correlation pass among the compiled variants (radix orders; timed
- * alone on one batch of in-plane step psi_index), then the tile block (the
+ * alone on one batch of in-plane step psi_index; the tuned variant 0 stays
+ * unless another is more than 2 % faster), then the tile block (the
  * one in use or all tiles, when the tiles span more than one block), the
  * batch size (half, the current, double) and, unless gs_set_pipeline chose
  * one, the pipeline (GS_PIPE_SERIAL, GS_PIPE_LANES, GS_PIPE_LAUNCH), timed
