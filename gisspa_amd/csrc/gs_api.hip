@@ -971,12 +971,19 @@ int plain_psi(Ctx* c, int ip, MapStat* out) {
 // ---------------------------------------------------------------------------
 // refinement
 // ---------------------------------------------------------------------------
-// candidates per forward/backward launch sequence (about 12.5 MB of work
-// buffers each at window 720): on the top 10,000 picks of the full-size
-// micrograph (168,271 evaluations) 64 / 128 / 256 / 512 / 1024 ran 60.3 /
-// 62.9 / 66.5 / 68.6 / 69.5 k evaluations per second; more candidates per
-// launch fill the GPU better and pay the per-kernel ramp and tail less often
-constexpr int kRefBatch = 512;
+// candidates per forward/backward launch sequence: as many as about 16 GiB
+// of work buffers hold, 64 to 2048 (12.5 MB each at window 720: 1374).  On
+// the top 10,000 picks of the full-size micrograph (168,271 evaluations)
+// 64 / 128 / 256 / 512 / 1024 ran 60.3 / 62.9 / 66.5 / 68.6 / 69.5 k
+// evaluations per second, and on the top 7,616 picks of 24 in-plane steps
+// 512 / 1024 / 2048 took 0.797 / 0.785 / 0.777 s; more candidates per launch
+// fill the GPU better and pay the per-kernel ramp and tail less often
+int ref_batch(const Ctx* c) {
+  const size_t L = c->L, H = c->geo.H, HP = c->geo.HP;
+  const size_t per = 2 * H * L * sizeof(float2) + L * HP * sizeof(float2) +
+                     3 * L * L * sizeof(float);
+  return (int)std::min<size_t>(2048, std::max<size_t>(64, (16ull << 30) / per));
+}
 
 // work buffers of one batch of nb candidates (the per-candidate inputs and
 // outputs of a whole call are sized by ref_eval)
@@ -1032,7 +1039,8 @@ int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out,
              const float* thr = nullptr) {
   CHK(ensure_white(c));
   hipStream_t s = c->stream;
-  const int nb0 = std::min(n, kRefBatch);
+  const int rb = ref_batch(c);
+  const int nb0 = std::min(n, rb);
   CHK(ref_ensure(c, nb0));
   // every candidate's inputs go over once and every output comes back once;
   // the batches run back to back on the stream (no host round trip between
@@ -1047,8 +1055,8 @@ int ref_eval(Ctx* c, const RefCand* rc, const double2* cs, int n, float* out,
     CHK(c->r_thr.ensure((size_t)n * sizeof(float)));
     HIPCHK(hipMemcpyAsync(c->r_thr.p, thr, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s));
   }
-  for (int b0 = 0; b0 < n; b0 += kRefBatch) {
-    const int nb = std::min(kRefBatch, n - b0);
+  for (int b0 = 0; b0 < n; b0 += rb) {
+    const int nb = std::min(rb, n - b0);
     RefArgs a{};
     a.nb = nb;
     a.white = c->twhite.as<float2>();
